@@ -1,0 +1,171 @@
+"""Throughput of the MI355X step on the 100,000-cuboid pile.
+
+Workload: 100,000 cuboids of 0.25 m edge (density 1000 kg/m^3) in a 50x40x50
+jittered, yawed lattice with 0.27 m spacing dropped 0.5 m onto the ground
+plane; dt = 1/60 s, gravity -9.81, friction 0.6, restitution 0, 8 solver
+iterations. One step is one whole tick (integration, broadphase, narrowphase,
+contact solve, snapshot) replayed as a captured hipGraph with inputs resident in
+HBM. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 600] [--warmup 60]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(n, c, s, p, k, a, iterations):
+    """Compulsory HBM bytes per tick and kernel group: each input read once and
+    each output written once (per-unit figures in DESIGN.md)."""
+    return {
+        "integrate": 280 * n,
+        "morton": 136 * c + 8 * (s - c),
+        "sort": 16 * s,
+        "lbvh": 24 * (c - 1) + 72 * c + 112 * (c - 1),
+        "traverse": 40 * c + 72 * (2 * c - 1) + 8 * p,
+        "narrowphase": 200 * p + 64 * k,
+        "prepare": 340 * k,
+        "solve": iterations * (330 * a + 112 * n),
+    }
+
+
+LAUNCHES = {"integrate": 1, "morton": 2, "lbvh": 2, "traverse": 1, "narrowphase": 1, "prepare": 1}
+
+
+def cpu_baseline(scene, friction, iterations, budget_s=15.0, max_ticks=40):
+    from tests._oracle import Oracle
+
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    ora = Oracle(scene, iterations=iterations, friction=friction, threads=threads)
+    t0 = time.perf_counter()
+    ticks = 0
+    while ticks < max_ticks and time.perf_counter() - t0 < budget_s:
+        ticks += 1
+        ora.step(ticks)
+    dt = time.perf_counter() - t0
+    return {"value": ticks / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 C++ restatement, OpenMP {threads} threads, ticks 1..{ticks} of the same "
+                      f"100k-cuboid pile ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--profile-ticks", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mechanic_amd as mh
+
+    friction, iterations = 0.6, 8
+    scene = mh.named_pile("pile_100k")
+    cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=iterations, use_graph=True)
+
+    # Warm-up on a throwaway handle (code objects, clocks, graph path).
+    warm = mh.GpuPhysics.new_with_config(scene, cfg)
+    for t in range(1, args.warmup + 1):
+        warm.dispatch_tick(t)
+    warm.synchronize()
+    warm.close()
+
+    gpu = mh.GpuPhysics.new_with_config(scene, cfg)
+    gpu.dispatch_ticks(1, 1)  # captures the tick graph
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.dispatch_ticks(2, args.steps)
+    gpu.synchronize()
+    wall = time.perf_counter() - t0
+    steps = args.steps
+    readback = gpu.read_last_tick()
+
+    # Event-bracketed (uncaptured) ticks continuing the same run: ms per kernel group.
+    groups = ["integrate_ms", "morton_ms", "sort_ms", "lbvh_ms", "traverse_ms", "narrowphase_ms", "prepare_ms",
+              "solve_ms"]
+    acc = {g: 0.0 for g in groups}
+    total = 0.0
+    counts = []
+    first = steps + 2
+    for i in range(args.profile_ticks):
+        t = gpu.profile_tick(first + i)
+        r = gpu.read_last_tick()
+        counts.append((r.pair_count, r.contact_count, r.active_contact_count))
+        for g in groups:
+            acc[g] += t[g]
+        total += t["total_ms"]
+    npt = max(args.profile_ticks, 1)
+    phase_ms = {g[:-3]: acc[g] / npt for g in groups}
+    p = sum(c[0] for c in counts) / npt
+    k = sum(c[1] for c in counts) / npt
+    a = sum(c[2] for c in counts) / npt
+    n = scene.body_count
+    c = len(scene.colliders)
+    s = 1 << (c - 1).bit_length()
+    algo = algorithmic_bytes(n, c, s, p, k, a, iterations)
+    dominant = max(phase_ms, key=phase_ms.get)
+    dom_ms = phase_ms[dominant]
+    achieved = algo[dominant] / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    per_phase_gbs = {g: (algo[g] / (phase_ms[g] * 1e-3) / 1e9 if phase_ms[g] > 0 else 0.0) for g in phase_ms}
+
+    # Reference-style synchronous engine TPS: dispatch + diagnostics readback per tick.
+    sync_ticks = 60
+    t1 = time.perf_counter()
+    base = first + args.profile_ticks
+    for i in range(sync_ticks):
+        gpu.dispatch_tick(base + i)
+        gpu.read_last_tick()
+    engine_tps = sync_ticks / (time.perf_counter() - t1)
+    gpu.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(scene, friction, iterations)
+
+    value = steps / wall
+    out = {
+        "metric": "steps/sec at 100k bodies + ms/phase",
+        "value": round(value, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * wall / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 jittered cuboid lattice)",
+        "config": {"workload": "pile_100k: 100,000 cuboids 0.25 m, 50x40x50 lattice 0.27 m spacing, "
+                               "jitter 0.02 m, uniform yaw, 0.5 m drop, seed 1, friction 0.6, 8 iterations, "
+                               f"ticks 2..{steps + 1} after the drop",
+                   "bodies": n, "solver_iterations": iterations, "friction": friction, "parallelism": "1 GPU"},
+        "phase_ms": {g: round(v, 4) for g, v in phase_ms.items()},
+        "phase_gbs": {g: round(v, 1) for g, v in per_phase_gbs.items()},
+        "profiled_tick_ms": round(total / npt, 4),
+        "counts": {"pairs": round(p), "contacts": round(k), "active_contacts": round(a)},
+        "error_flags": readback.error_flags,
+        "engine_tps_sync": round(engine_tps, 2),
+        "roofline": {"bound": "hbm", "kernel_group": dominant, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,298 @@
+/*
+ * mechanic_hip.h — C ABI of the MI355X (gfx950) rigid-body step backend.
+ *
+ * This library stands in for `mechanic_gpu::GpuPhysics`
+ * (crates/mechanic-gpu/src/device.rs:199-1663) behind a flat C interface:
+ * plain pointers and sizes, no HIP, torch or wgpu types in any signature.
+ *
+ * Row layouts mirror the reference's fixed-size Pod rows byte for byte
+ * (crates/mechanic-gpu/src/abi.rs:16-296, size asserts abi.rs:298-315), and the
+ * error-flag bits and capacities are the reference's (abi.rs:3-13,
+ * crates/mechanic-gpu/src/lib.rs:29-51).
+ *
+ * Threading: a handle is not re-entrant; separate handles are independent.
+ * All work of one handle is ordered on one HIP stream owned by the handle.
+ */
+#ifndef MECHANIC_HIP_H
+#define MECHANIC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_API __attribute__((visibility("default")))
+
+/* ---- flags and capacities (abi.rs:3-13, lib.rs:29-51) ------------------- */
+
+#define MH_PAIR_OVERFLOW_FLAG (1u << 0)              /* abi.rs:4 */
+#define MH_INVALID_NUMERIC_FLAG (1u << 1)            /* abi.rs:7 */
+#define MH_CONSTRAINT_NON_CONVERGENCE_FLAG (1u << 2) /* abi.rs:10 */
+#define MH_MANIFOLD_OVERFLOW_FLAG (1u << 3)          /* abi.rs:13 */
+
+#define MH_MAX_BODIES 131072u         /* lib.rs:36 */
+#define MH_MAX_BEARINGS 262144u       /* lib.rs:39 */
+#define MH_MAX_COLLIDERS 131072u      /* lib.rs:42 */
+#define MH_MAX_CONTACT_PAIRS 2097152u /* lib.rs:45 */
+#define MH_SNAPSHOT_RING_SIZE 3u      /* lib.rs:51 */
+#define MH_INVALID_INDEX 0xffffffffu
+
+#define MH_DRIVE_MODE_PASSIVE 0u /* abi.rs:256 */
+#define MH_DRIVE_MODE_SPEED 1u   /* abi.rs:259 */
+#define MH_DRIVE_MODE_ANGLE 2u   /* abi.rs:262 */
+
+/* ---- status codes ---------------------------------------------------------
+ * One code per variant of GpuPhysicsError (device.rs:130-173),
+ * GpuImpulseError (device.rs:175-189) and GpuReadbackError (device.rs:113-128),
+ * plus codes for problems a Rust caller would meet as panics or wgpu errors. */
+enum {
+  MH_OK = 0,
+  MH_ERR_BODY_CAPACITY = 1,             /* GpuPhysicsError::BodyCapacity */
+  MH_ERR_BEARING_CAPACITY = 2,          /* GpuPhysicsError::BearingCapacity */
+  MH_ERR_COLLIDER_CAPACITY = 3,         /* GpuPhysicsError::ColliderCapacity */
+  MH_ERR_DRIVE_STATE_COUNT = 4,         /* GpuPhysicsError::DriveStateCount */
+  MH_ERR_COORDINATE_STATE_COUNT = 5,    /* GpuPhysicsError::CoordinateStateCount */
+  MH_ERR_BODY_INDEX_OUT_OF_RANGE = 6,   /* GpuImpulseError::BodyIndexOutOfRange */
+  MH_ERR_NON_FINITE = 7,                /* GpuImpulseError::NonFinite */
+  MH_ERR_INVALID_SNAPSHOT_SLOT = 8,     /* GpuReadbackError::InvalidSnapshotSlot */
+  MH_ERR_DEVICE = 9,                    /* GpuReadbackError::DevicePoll (a HIP error) */
+  MH_ERR_INVALID_ARGUMENT = 10,         /* null pointer, short buffer, bad config */
+  MH_ERR_NO_DEVICE = 11,                /* no HIP device visible */
+  MH_ERR_UNSUPPORTED = 12               /* a feature this build does not run */
+};
+
+/* ---- rows (abi.rs) ------------------------------------------------------- */
+
+typedef struct { /* GpuTickConfig, abi.rs:18-51 (64 B) */
+  uint32_t body_count, tick_index, snapshot_slot, collider_count;
+  float delta_seconds, gravity_y, linear_damping, angular_damping;
+  uint32_t bearing_count, suppression_count, pair_capacity, flags;
+  uint32_t hash_capacity, solver_iterations, reserved_a, reserved_b;
+} mh_tick_config;
+
+typedef struct { /* GpuDiagnostics, abi.rs:56-73 (32 B) */
+  uint32_t error_flags, pair_count, contact_count;
+  uint32_t max_anchor_micrometers, max_axis_microdegrees;
+  uint32_t active_contact_count, reserved_b, reserved_c;
+} mh_diagnostics;
+
+typedef struct { float position[4]; float rotation[4]; } mh_transform; /* abi.rs:78-83 */
+typedef struct { float linear[4]; float angular[4]; } mh_velocity;    /* abi.rs:88-93 */
+
+typedef struct { /* GpuMass, abi.rs:98-107 */
+  float inverse_mass[4], inverse_inertia_x[4], inverse_inertia_y[4], inverse_inertia_z[4];
+} mh_mass;
+
+typedef struct { /* GpuSpatialInertia, abi.rs:112-121 */
+  float mass[4], inertia_x[4], inertia_y[4], inertia_z[4];
+} mh_spatial_inertia;
+
+typedef struct { /* GpuCollider, abi.rs:126-135 */
+  float local_center[4];   /* w: full-cylinder segment-centre radius */
+  float local_rotation[4]; /* xyzw */
+  float half_extents[4];   /* w: full-cylinder outer radius */
+  uint32_t metadata[4];    /* compound, source slot, source generation, ground role */
+} mh_collider;
+
+typedef struct { /* GpuBearing, abi.rs:140-151 */
+  float local_anchor_a[4], local_anchor_b[4], local_axis_a[4], local_axis_b[4];
+  uint32_t metadata[4]; /* compound a, compound b, coordinate or MH_INVALID_INDEX, flags */
+} mh_bearing;
+
+typedef struct { uint32_t collider_a, collider_b; } mh_pair; /* abi.rs:156-161 */
+
+typedef struct { /* GpuContact, abi.rs:166-175 */
+  uint32_t metadata[4];         /* body a, body b, manifold slot, point count */
+  float normal_penetration[4];  /* normal a->b, depth */
+  float arm_a_impulse[4];       /* lever arm from body a, accumulated normal impulse */
+  float arm_b[4];               /* lever arm from body b, w: near-face axes / -1 analytic */
+} mh_contact;
+
+typedef struct { /* GpuPersistentManifold, abi.rs:180-187 */
+  uint32_t pair_tick[4];       /* collider low, collider high, last tick, point count */
+  float normal_penetration[4];
+  float point_impulse[4];      /* arm from body a, accumulated normal impulse */
+} mh_persistent_manifold;
+
+typedef struct { /* GpuMechanismBody, abi.rs:191-201 */
+  uint32_t metadata[4];  /* parent body, tree bearing row, direction, root flag */
+  uint32_t traversal[4]; /* component, depth, preorder, postorder */
+  float bind_relative_position[4];
+  float bind_relative_rotation[4];
+} mh_mechanism_body;
+
+typedef struct { float angle, angular_velocity; } mh_mechanism_coordinate; /* abi.rs:206-211 */
+
+typedef struct { /* GpuMechanismDrive, abi.rs:220-238 */
+  uint32_t mode;
+  float max_acceleration, max_speed, target_speed, target_angle, min_angle, max_angle, padding;
+} mh_mechanism_drive;
+
+typedef struct { float position[4]; float rotation[4]; uint32_t metadata[4]; } mh_link_state; /* abi.rs:280-288 */
+typedef struct { uint32_t metadata[4]; } mh_contraction_node; /* abi.rs:292-296 */
+
+/* ---- scene upload (what GpuPhysics::new_with_config lowers, device.rs:424-523) */
+
+typedef struct {
+  uint32_t body_count;
+  uint32_t collider_count;
+  uint32_t bearing_count;     /* articulated rows; must be 0 in this build */
+  uint32_t suppression_count;
+  const mh_transform* poses;          /* root pose per body (device.rs:429-441) */
+  const float* inverse_masses;        /* per body (device.rs:443-447) */
+  const mh_mass* masses;              /* body-frame inverse mass/inertia (device.rs:454-466) */
+  const mh_spatial_inertia* inertias; /* optional, may be NULL (device.rs:467-479) */
+  const mh_collider* colliders;       /* device.rs:480-499 */
+  const mh_bearing* bearings;         /* device.rs:500-515 */
+  const mh_pair* suppressed_pairs;    /* sorted (low, high) body pairs, device.rs:516-523 */
+  const uint32_t* body_components;    /* NULL: every body its own component (device.rs:448-453) */
+  const uint32_t* root_flags;         /* NULL: every body a root (device.rs:1699-1704) */
+} mh_creation;
+
+/* GpuPhysicsConfig (device.rs:22-41) plus the constants the reference hard-codes,
+ * exposed so the BASELINE scenes (friction 0.6) can run. Zero-initialised fields
+ * take the reference value; call mh_config_default() for an explicit default. */
+typedef struct {
+  uint32_t collisions_enabled;        /* device.rs:26, default 1 */
+  uint32_t mechanism_self_collisions; /* device.rs:28, default 1 */
+  uint32_t solver_iterations;         /* device.rs:30, default 8 */
+  float friction;                     /* CONTACT_FRICTION collision.wgsl:66, default 0.05 */
+  uint32_t pair_capacity;             /* MAX_CONTACT_PAIRS lib.rs:45 (power of two) */
+  uint32_t warm_start_mode;           /* 0 = persistent-manifold cache (reference) */
+  uint32_t use_graph;                 /* 1 = replay each tick as a captured hipGraph */
+  uint32_t reserved;
+} mh_config;
+
+/* GpuTickReadback (device.rs:74-92) with GpuKernelTimings (device.rs:95-111). */
+typedef struct {
+  double gpu_tick_ms;   /* sum of the phase timings below; -1 when not profiled */
+  double integration_ms, mechanism_ms, broadphase_ms, narrowphase_ms;
+  double contact_solver_ms, bearings_ms, snapshot_ms;
+  uint32_t error_flags, pair_count, contact_count, active_contact_count;
+  float anchor_residual_meters, axis_residual_degrees;
+  uint64_t tick_index;
+  uint32_t snapshot_slot, has_timings;
+} mh_tick_readback;
+
+/* Per-kernel-group timings of an event-bracketed (uncaptured) tick. */
+typedef struct {
+  double integrate_ms, morton_ms, sort_ms, lbvh_ms, traverse_ms;
+  double narrowphase_ms, prepare_ms, solve_ms, persist_ms, snapshot_ms, total_ms;
+} mh_phase_timings;
+
+typedef struct {
+  int code;
+  uint64_t required, capacity, provided;
+  uint32_t body_index, body_count;
+  char message[160];
+} mh_error_detail;
+
+typedef struct mh_physics mh_physics;
+
+MH_API mh_config mh_config_default(void);
+MH_API const char* mh_status_string(int code);
+MH_API int mh_device_count(int* count);
+
+/* GpuPhysics::new_with_config (device.rs:399-811). */
+MH_API int mh_physics_create(const mh_creation* creation, const mh_config* config,
+                             mh_physics** out, mh_error_detail* error);
+MH_API void mh_physics_destroy(mh_physics* physics);
+/* GpuPhysics::body_count (device.rs:1582). */
+MH_API uint32_t mh_physics_body_count(const mh_physics* physics);
+/* GpuPhysics::has_gpu_timestamps (device.rs:1573): always 1 (HIP events). */
+MH_API int mh_physics_has_gpu_timestamps(const mh_physics* physics);
+/* GpuPhysics::apply_impulse (device.rs:821-856). */
+MH_API int mh_physics_apply_impulse(mh_physics* physics, uint32_t body, const float world_point[3],
+                                    const float impulse[3], mh_error_detail* error);
+/* GpuPhysics::dispatch_tick (device.rs:859-961): enqueues one tick, returns at once. */
+MH_API int mh_physics_dispatch_tick(mh_physics* physics, uint64_t tick_index, uint8_t* snapshot_slot);
+/* `count` consecutive dispatch_tick calls starting at `first_tick`. */
+MH_API int mh_physics_dispatch_ticks(mh_physics* physics, uint64_t first_tick, uint32_t count);
+/* Uncaptured tick with HIP events between kernel groups; fills `timings`. */
+MH_API int mh_physics_profile_tick(mh_physics* physics, uint64_t tick_index, mh_phase_timings* timings);
+/* GpuPhysics::read_last_tick (device.rs:1439-1524): waits for the stream. */
+MH_API int mh_physics_read_last_tick(mh_physics* physics, mh_tick_readback* out);
+/* GpuPhysics::read_snapshot_transforms (device.rs:1534-1570). */
+MH_API int mh_physics_read_snapshot(mh_physics* physics, uint8_t slot, mh_transform* out, size_t capacity);
+/* GpuPhysics::snapshot (device.rs:1578-1580): device pointers of one ring slot. */
+MH_API int mh_physics_snapshot_device_ptrs(mh_physics* physics, uint8_t slot, const void** positions,
+                                           const void** rotations);
+/* Body state read/write (the reference's raw buffer getters, device.rs:1582-1620). */
+MH_API int mh_physics_read_bodies(mh_physics* physics, mh_transform* poses, mh_velocity* velocities,
+                                  size_t capacity);
+MH_API int mh_physics_write_bodies(mh_physics* physics, const mh_transform* poses,
+                                   const mh_velocity* velocities, size_t count);
+/* Raw read of an internal buffer for parity tests (see MH_BUFFER_*). */
+MH_API int mh_physics_read_buffer(mh_physics* physics, int buffer, void* dst, size_t bytes, size_t* written);
+MH_API int mh_physics_synchronize(mh_physics* physics);
+/* Runs one uncaptured tick up to a phase boundary, for phase parity tests:
+ * stop 1 = after integration, 2 = after the broadphase, 3 = after contact
+ * generation, 0 = whole tick. A stopped tick leaves the handle mid-tick. */
+MH_API int mh_physics_debug_tick(mh_physics* physics, uint64_t tick_index, int stop);
+/* GpuPhysics::write_mechanism_drives / initialize_mechanism_coordinates
+ * (device.rs:1622-1662); the articulated path is not built yet: they validate the
+ * row count like the reference and return MH_ERR_UNSUPPORTED for non-empty input. */
+MH_API int mh_physics_write_drives(mh_physics* physics, const mh_mechanism_drive* drives, size_t count,
+                                   mh_error_detail* error);
+MH_API int mh_physics_init_coordinates(mh_physics* physics, const mh_mechanism_coordinate* coordinates,
+                                       size_t count, mh_error_detail* error);
+
+enum {
+  MH_BUFFER_MORTON_KEYS = 1,     /* u64 (code << 32 | collider) per sort slot, sorted after a tick */
+  MH_BUFFER_COLLIDER_AABBS = 2,  /* float4 min, float4 max per collider */
+  MH_BUFFER_NODE_CHILDREN = 3,   /* uint2 per internal node, reference encoding (leaf = C-1+i) */
+  MH_BUFFER_NODE_AABBS = 4,      /* float4 min, float4 max per node (internal then leaves) */
+  MH_BUFFER_PAIRS = 5,           /* mh_pair per broadphase pair (unordered) */
+  MH_BUFFER_CONTACTS = 6,        /* mh_contact per contact (unordered) */
+  MH_BUFFER_DIAGNOSTICS = 7,     /* mh_diagnostics of the last tick */
+  MH_BUFFER_WORLD_MASSES = 8,    /* 3 x float4 per body */
+  MH_BUFFER_MANIFOLDS = 9,       /* mh_persistent_manifold per contact of the last tick */
+  MH_BUFFER_NODE_PARENTS = 10    /* u32 parent+1 per node, reference encoding */
+};
+
+/* ---- synthetic scenes (seeded; SplitMix64, u = (x >> 40) * 2^-24) ------- */
+
+typedef struct {
+  uint32_t nx, ny, nz;          /* lattice extent along x, y, z */
+  float edge;                   /* cuboid edge (m) */
+  float spacing;                /* lattice spacing (m) */
+  float drop_height;            /* lowest faces above y = 0 (m) */
+  float jitter;                 /* uniform +-jitter per axis (m) */
+  float density;                /* kg / m^3 */
+  uint64_t seed;
+  uint32_t yaw;                 /* 1: uniform yaw in [0, 2pi) */
+  uint32_t reserved;
+} mh_pile_params;
+
+/* Fills body_count = nx*ny*nz rows of a jittered cuboid lattice. Every body is
+ * one compound with one cuboid collider (root rotation identity, yaw carried by
+ * the collider's local rotation, as the reference's compile step lowers a
+ * rotated part: crates/mechanic-core/src/compile.rs:333-334, 889-895). */
+MH_API int mh_scene_cuboid_pile(const mh_pile_params* params, mh_transform* poses, float* inverse_masses,
+                                mh_mass* masses, mh_spatial_inertia* inertias, mh_collider* colliders,
+                                size_t capacity);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+static_assert(sizeof(mh_tick_config) == 64, "abi.rs:299");
+static_assert(sizeof(mh_diagnostics) == 32, "abi.rs:300");
+static_assert(sizeof(mh_transform) == 32, "abi.rs:301");
+static_assert(sizeof(mh_velocity) == 32, "abi.rs:302");
+static_assert(sizeof(mh_mass) == 64, "abi.rs:303");
+static_assert(sizeof(mh_spatial_inertia) == 64, "abi.rs:304");
+static_assert(sizeof(mh_collider) == 64, "abi.rs:305");
+static_assert(sizeof(mh_bearing) == 80, "abi.rs:306");
+static_assert(sizeof(mh_pair) == 8, "abi.rs:307");
+static_assert(sizeof(mh_contact) == 64, "abi.rs:308");
+static_assert(sizeof(mh_persistent_manifold) == 48, "abi.rs:309");
+static_assert(sizeof(mh_mechanism_body) == 64, "abi.rs:310");
+static_assert(sizeof(mh_mechanism_coordinate) == 8, "abi.rs:311");
+static_assert(sizeof(mh_mechanism_drive) == 32, "abi.rs:312");
+static_assert(sizeof(mh_link_state) == 48, "abi.rs:313");
+static_assert(sizeof(mh_contraction_node) == 16, "abi.rs:314");
+#endif
+
+#endif /* MECHANIC_HIP_H */

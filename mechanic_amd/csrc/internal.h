@@ -1,0 +1,111 @@
+// Device-side state shared by the step kernels and the host runtime.
+//
+// All per-body and per-collider state is structure-of-arrays: one array per
+// field, each element a 16-byte vector (or a scalar), so a wavefront touches
+// one contiguous 1 KiB run per field and every load is a dwordx4.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mh {
+
+constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kPosMask = 0x7fffffffu;
+constexpr uint32_t kBlock = 256;
+
+// Diagnostics words (abi.rs:56-73) plus private counters after them.
+enum DiagWord : uint32_t {
+  kDiagFlags = 0,
+  kDiagPairs = 1,
+  kDiagContacts = 2,
+  kDiagAnchor = 3,
+  kDiagAxis = 4,
+  kDiagActive = 5,
+  kDiagWords = 8,
+};
+
+// Tick state kept on the device so a captured tick can be replayed unchanged.
+struct TickState {
+  uint64_t tick_index;  // tick being run; its low 32 bits are tick_index of abi.rs:21
+  uint64_t pad;
+};
+
+struct Dev {
+  // ---- sizes and constants ----
+  uint32_t body_count;
+  uint32_t collider_count;
+  uint32_t sort_count;
+  uint32_t suppression_count;
+  uint32_t pair_capacity;
+  uint32_t hash_mask;
+  uint32_t solver_iterations;
+  uint32_t self_collisions;
+  float delta_seconds;
+  float gravity_y;
+  float linear_damping;
+  float angular_damping;
+  float friction;
+  uint32_t pad0;
+
+  // ---- bodies ----
+  float4* pos;
+  float4* rot;
+  float4* lin;
+  float4* ang;
+  const float* inv_mass;
+  const float4* masses;   // 4 per body: inverse mass, inverse inertia columns (body frame)
+  float4* world_masses;   // 3 per body: world inverse inertia columns, inverse mass in [0].w
+  const uint32_t* root_flags;
+  const uint32_t* components;
+
+  // ---- colliders ----
+  const float4* colliders;  // 4 per collider: centre, rotation, half extents, metadata bits
+  const uint2* suppressed;
+
+  // ---- broadphase ----
+  float4* aabbs;          // 2 per collider: min, max
+  uint64_t* keys;         // sort_count Morton entries (code << 32 | collider)
+  float4* inodes;         // 2 per internal node: (min, left child) (max, escape)
+  float4* leaves;         // 2 per leaf: (min, collider) (max, escape)
+  uint2* children;        // reference encoding of node_children (lbvh.wgsl:275)
+  uint32_t* parents;      // reference encoding of node_parents (parent + 1)
+  uint32_t* visits;       // refit arrival counters
+  uint2* pairs;
+
+  // ---- contacts and solver ----
+  float4* contacts;       // 4 per contact, mh_contact layout
+  uint32_t* active;       // contact index per active contact
+  float4* rows;           // 4 per active contact: solver rows
+  int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas
+
+  // ---- persistent manifolds (ping-pong by tick parity) ----
+  uint64_t* htab[2];      // open-addressing index: key -> manifold row
+  float4* mrows[2];       // 3 per contact: pair_tick bits, normal/penetration, point/impulse
+
+  // ---- publication ----
+  float4* snap_pos[3];
+  float4* snap_rot[3];
+
+  // ---- diagnostics ----
+  uint32_t* diag[2];      // kDiagWords each, by tick parity
+  TickState* tick;
+};
+
+// Launchers (one per kernel group), all on `stream`.
+void launch_set_tick(hipStream_t s, const Dev& d, uint64_t tick);
+void launch_end_tick(hipStream_t s, const Dev& d);
+void launch_integrate(hipStream_t s, const Dev& d, bool collisions, bool snapshot_now);
+void launch_snapshot(hipStream_t s, const Dev& d);
+void launch_external_impulse(hipStream_t s, const Dev& d, float4 point, float4 impulse, uint32_t body);
+void launch_clear_table(hipStream_t s, const Dev& d);
+void launch_morton(hipStream_t s, const Dev& d);
+void launch_sort(hipStream_t s, const Dev& d);
+void launch_lbvh(hipStream_t s, const Dev& d);
+void launch_traverse(hipStream_t s, const Dev& d);
+void launch_narrowphase(hipStream_t s, const Dev& d);
+void launch_prepare(hipStream_t s, const Dev& d);
+void launch_solve(hipStream_t s, const Dev& d);
+
+}  // namespace mh

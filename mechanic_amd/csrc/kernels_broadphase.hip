@@ -1,0 +1,416 @@
+// Broadphase: Morton keys and AABBs (with analytic ground contacts fused in),
+// an LDS-staged bitonic sort, Karras LBVH topology, an atomic bottom-up refit
+// and a stackless pair traversal with wavefront-ballot-compacted output.
+#include "internal.h"
+#include "mh_geometry.h"
+#include "mh_math.h"
+
+namespace mh {
+
+__device__ __forceinline__ uint64_t tick_now(const Dev& d) {
+  return *reinterpret_cast<const volatile uint64_t*>(&d.tick->tick_index);
+}
+
+// expand_morton_bits / morton_code (lbvh.wgsl:84-98).
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+  v &= 0x000003ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t quantize(float c) {
+  return (uint32_t)fmn(fmx(floorf(c + 512.0f), 0.0f), 1023.0f);
+}
+__device__ __forceinline__ uint32_t morton_code(v3 c) {
+  return expand_bits(quantize(c.x)) | (expand_bits(quantize(c.y)) << 1) | (expand_bits(quantize(c.z)) << 2);
+}
+
+// compute_morton (lbvh.wgsl:108-130) + generate_ground_contacts
+// (collision.wgsl:622-664). One thread per sort slot; slots past the collider
+// count get the padding key (0xffffffff, slot).
+__global__ __launch_bounds__(256) void k_morton(Dev d) {
+  const uint64_t tick = tick_now(d);
+  uint32_t* diag = d.diag[tick & 1];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.sort_count) return;
+  if (i >= d.collider_count) {
+    d.keys[i] = (0xffffffffull << 32) | i;
+    return;
+  }
+  ColliderPose p = collider_pose(d, i);
+  v3 we = (vabs(p.axes[0]) * p.half.x + vabs(p.axes[1]) * p.half.y) + vabs(p.axes[2]) * p.half.z;
+  d.aabbs[2 * (size_t)i] = f4(p.center - we, 0.0f);
+  d.aabbs[2 * (size_t)i + 1] = f4(p.center + we, 0.0f);
+  d.keys[i] = ((uint64_t)morton_code(p.center) << 32) | i;
+
+  // Ground plane y = 0.
+  const v3 down = mk(0.0f, -1.0f, 0.0f);
+  v3 support = support_point(p, down);
+  bool emit = true;
+  if (p.role != 0u) {
+    uint32_t count = full_cylinder_contact_count(p, down);
+    if (p.role > count) emit = false;
+    else support = full_cylinder_support_point(p, down, p.role);
+  }
+  float bottom = support.y;
+  if (bottom > 1.0e-5f) emit = false;
+  uint32_t row = wave_append(&diag[kDiagContacts], emit);
+  if (!emit) return;
+  if (row < d.pair_capacity) {
+    float4 bp = d.pos[p.body];
+    store_contact(d, row, make_uint4(p.body, kInvalid, i, kInvalid),
+                  make_float4(0.0f, -1.0f, 0.0f, fmx(-bottom, 0.0f)),
+                  f4(support - xyz(bp), 0.0f),
+                  make_float4(0.0f, 0.0f, 0.0f, p.role != 0u ? -1.0f : 0.0f));
+  } else {
+    atomicOr(&diag[kDiagFlags], 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bitonic sort of sort_count u64 keys, ascending. Keys are unique, so the result
+// equals the reference's order (lbvh.wgsl:100-106, 132-217).
+//
+// A tile of kTile keys lives in one workgroup: 8 keys per thread in registers
+// (blocked: thread t holds tile positions 8t..8t+7). Exchanges with j < 8 stay
+// in registers, 8 <= j < 512 go through wavefront shuffles (partner lane
+// t ^ j/8 is in the same 64-lane wavefront), and j >= 512 through LDS.
+// Distances of a tile or more are global multi-level butterfly kernels.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPer = 8;
+constexpr uint32_t kTile = 2048;
+
+__device__ __forceinline__ void cas(uint64_t& a, uint64_t& b, bool ascending) {
+  bool swap = ascending ? (a > b) : (a < b);
+  uint64_t lo = swap ? b : a;
+  uint64_t hi = swap ? a : b;
+  a = lo;
+  b = hi;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = (uint32_t)__shfl_xor((int)lo, m);
+  hi = (uint32_t)__shfl_xor((int)hi, m);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// One merge level at distance j inside a tile, keys in registers.
+// `base` is the global position of this thread's first key.
+__device__ __forceinline__ void tile_level(uint64_t (&r)[kPer], uint64_t* lds, uint32_t tid, uint32_t nthreads,
+                                           uint32_t tile_base, uint32_t base, uint32_t k, uint32_t j) {
+  if (j < kPer) {
+#pragma unroll
+    for (uint32_t e = 0; e < kPer; ++e) {
+      uint32_t f = e ^ j;
+      if (f > e) cas(r[e], r[f], ((base + e) & k) == 0);
+    }
+  } else if (j < kPer * 64) {
+    int m = (int)(j / kPer);
+    bool lower = (tid & (uint32_t)m) == 0;
+#pragma unroll
+    for (uint32_t e = 0; e < kPer; ++e) {
+      uint64_t other = shfl_xor64(r[e], m);
+      bool ascending = ((base + e) & k) == 0;
+      uint64_t mn = r[e] < other ? r[e] : other;
+      uint64_t mx = r[e] < other ? other : r[e];
+      r[e] = (lower == ascending) ? mn : mx;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t e = 0; e < kPer; ++e) lds[tid * kPer + e] = r[e];
+    __syncthreads();
+    uint32_t pairs = nthreads * kPer / 2;
+    for (uint32_t q = tid; q < pairs; q += nthreads) {
+      uint32_t lo = ((q / j) * 2 * j) + (q % j);
+      uint32_t hi = lo + j;
+      uint64_t a = lds[lo], b = lds[hi];
+      cas(a, b, ((tile_base + lo) & k) == 0);
+      lds[lo] = a;
+      lds[hi] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t e = 0; e < kPer; ++e) r[e] = lds[tid * kPer + e];
+    __syncthreads();
+  }
+}
+
+// Full bitonic sort of each tile (stages k = 2 .. tile), alternating directions by
+// the global position so neighbouring tiles form bitonic sequences.
+__global__ __launch_bounds__(256) void k_sort_tile(uint64_t* keys, uint32_t tile) {
+  __shared__ uint64_t lds[kTile];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthreads = blockDim.x;
+  const uint32_t tile_base = blockIdx.x * tile;
+  const uint32_t base = tile_base + tid * kPer;
+  uint64_t r[kPer];
+#pragma unroll
+  for (uint32_t e = 0; e < kPer; ++e) r[e] = keys[base + e];
+  for (uint32_t k = 2; k <= tile; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) tile_level(r, lds, tid, nthreads, tile_base, base, k, j);
+#pragma unroll
+  for (uint32_t e = 0; e < kPer; ++e) keys[base + e] = r[e];
+}
+
+// Finishing levels j = tile/2 .. 1 of stage k inside each tile.
+__global__ __launch_bounds__(256) void k_sort_merge(uint64_t* keys, uint32_t tile, uint32_t k) {
+  __shared__ uint64_t lds[kTile];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthreads = blockDim.x;
+  const uint32_t tile_base = blockIdx.x * tile;
+  const uint32_t base = tile_base + tid * kPer;
+  uint64_t r[kPer];
+#pragma unroll
+  for (uint32_t e = 0; e < kPer; ++e) r[e] = keys[base + e];
+  for (uint32_t j = tile >> 1; j > 0; j >>= 1) tile_level(r, lds, tid, nthreads, tile_base, base, k, j);
+#pragma unroll
+  for (uint32_t e = 0; e < kPer; ++e) keys[base + e] = r[e];
+}
+
+// `levels` consecutive levels j_hi, j_hi/2, ... of stage k: each thread owns the
+// 2^levels keys i0 + l * j_lo and runs the butterflies in registers.
+template <int L>
+__global__ __launch_bounds__(256) void k_sort_global(uint64_t* keys, uint32_t count, uint32_t k, uint32_t j_hi) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = 1u << L;
+  if (t >= (count >> L)) return;
+  const uint32_t j_lo = j_hi >> (L - 1);
+  const uint32_t lj = 31u - __clz(j_lo);
+  const uint32_t i0 = ((t >> lj) << (lj + L)) | (t & (j_lo - 1));
+  const bool ascending = (i0 & k) == 0;
+  uint64_t r[1 << L];
+#pragma unroll
+  for (uint32_t l = 0; l < n; ++l) r[l] = keys[i0 + l * j_lo];
+#pragma unroll
+  for (int lev = L - 1; lev >= 0; --lev) {
+    const uint32_t s = 1u << lev;
+#pragma unroll
+    for (uint32_t l = 0; l < n; ++l)
+      if ((l & s) == 0) cas(r[l], r[l | s], ascending);
+  }
+#pragma unroll
+  for (uint32_t l = 0; l < n; ++l) keys[i0 + l * j_lo] = r[l];
+}
+
+// ---------------------------------------------------------------------------
+// LBVH (Karras 2012) over the sorted keys, lbvh.wgsl:219-319.
+// Internal node n covers a contiguous range of sorted leaves; its record holds
+// (min, left child) and (max, escape). A child or escape is encoded as a sorted
+// position with kLeafBit set for leaves; the escape of a node whose range ends
+// at b is the node right after its subtree in depth-first order (the right
+// child of the node split at b), or the sentinel C after the last leaf.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int prefix(const uint64_t* keys, int c, int a, int b) {
+  if (b < 0 || b >= c) return -1;
+  return __clzll((long long)(keys[a] ^ keys[b]));
+}
+
+// Escape of a node whose range ends at sorted position b (C = leaf count).
+__device__ __forceinline__ uint32_t escape_after(const uint64_t* keys, int c, int b) {
+  if (b >= c - 1) return (uint32_t)c;
+  int n = b + 1;
+  if (n <= c - 2) {
+    // Internal node n spans [n, ...] iff its direction is +1.
+    if (prefix(keys, c, n, n + 1) > prefix(keys, c, n, n - 1)) return (uint32_t)n;
+  }
+  return (uint32_t)n | kLeafBit;
+}
+
+// build_topology (lbvh.wgsl:233-278).
+__global__ __launch_bounds__(256) void k_topology(Dev d) {
+  const int c = (int)d.collider_count;
+  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= 1 || node >= (uint32_t)(c - 1)) return;
+  const uint64_t* keys = d.keys;
+  const int i = (int)node;
+  const int dir = prefix(keys, c, i, i + 1) > prefix(keys, c, i, i - 1) ? 1 : -1;
+  const int min_prefix = prefix(keys, c, i, i - dir);
+  int max_len = 2;
+  while (prefix(keys, c, i, i + max_len * dir) > min_prefix) max_len *= 2;
+  int len = 0;
+  for (int step = max_len / 2; step >= 1; step /= 2)
+    if (prefix(keys, c, i, i + (len + step) * dir) > min_prefix) len += step;
+  const int other = i + len * dir;
+  const int node_prefix = prefix(keys, c, i, other);
+  int split_off = 0;
+  int step = (len + 1) / 2;
+  while (step >= 1) {
+    if (prefix(keys, c, i, i + (split_off + step) * dir) > node_prefix) split_off += step;
+    if (step == 1) break;
+    step = (step + 1) / 2;
+  }
+  const int split = i + split_off * dir + (dir < 0 ? dir : 0);
+  const int first = i < other ? i : other;
+  const int last = i < other ? other : i;
+  const uint32_t leaf_base = (uint32_t)c - 1u;
+  const uint32_t left_ref = split == first ? leaf_base + (uint32_t)split : (uint32_t)split;
+  const uint32_t right_ref = split + 1 == last ? leaf_base + (uint32_t)(split + 1) : (uint32_t)(split + 1);
+  d.children[node] = make_uint2(left_ref, right_ref);
+  d.parents[left_ref] = node + 1u;
+  d.parents[right_ref] = node + 1u;
+  if (node == 0) d.parents[0] = 0u;
+  d.visits[node] = 0u;
+  const uint32_t left_enc = split == first ? ((uint32_t)split | kLeafBit) : (uint32_t)split;
+  d.inodes[2 * (size_t)node].w = __uint_as_float(left_enc);
+  d.inodes[2 * (size_t)node + 1].w = __uint_as_float(escape_after(keys, c, last));
+}
+
+// prepare_leaves + build_bounds (lbvh.wgsl:280-319). The release/acquire pair
+// around the arrival counter publishes a child's box to the thread (in any
+// workgroup, on any XCD) that merges it.
+__global__ __launch_bounds__(256) void k_refit(Dev d) {
+  const uint64_t tick = tick_now(d);
+  uint32_t* diag = d.diag[tick & 1];
+  const int c = (int)d.collider_count;
+  const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= (uint32_t)c) return;
+  const uint32_t collider = (uint32_t)d.keys[leaf];
+  float4 mn = d.aabbs[2 * (size_t)collider];
+  float4 mx = d.aabbs[2 * (size_t)collider + 1];
+  d.leaves[2 * (size_t)leaf] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(collider));
+  d.leaves[2 * (size_t)leaf + 1] = make_float4(mx.x, mx.y, mx.z, __uint_as_float(escape_after(d.keys, c, (int)leaf)));
+  if (c <= 1) return;
+  const uint32_t leaf_base = (uint32_t)c - 1u;
+  uint32_t current = leaf_base + leaf;
+  // A Karras tree over unique 64-bit keys is at most 64 levels deep.
+  for (int level = 0; level < 128; ++level) {
+    uint32_t encoded = d.parents[current];
+    if (encoded == 0u) return;
+    uint32_t parent = encoded - 1u;
+    __threadfence();
+    uint32_t arrival = atomicAdd(&d.visits[parent], 1u);
+    if (arrival == 0u) return;
+    if (arrival > 1u) {
+      atomicOr(&diag[kDiagFlags], 1u);
+      return;
+    }
+    __threadfence();
+    uint2 ch = d.children[parent];
+    const float4* a = ch.x >= leaf_base ? d.leaves + 2 * (size_t)(ch.x - leaf_base) : d.inodes + 2 * (size_t)ch.x;
+    const float4* b = ch.y >= leaf_base ? d.leaves + 2 * (size_t)(ch.y - leaf_base) : d.inodes + 2 * (size_t)ch.y;
+    float4 amn = a[0], amx = a[1], bmn = b[0], bmx = b[1];
+    float4* p = d.inodes + 2 * (size_t)parent;
+    float lw = p[0].w, ew = p[1].w;
+    p[0] = make_float4(fmn(amn.x, bmn.x), fmn(amn.y, bmn.y), fmn(amn.z, bmn.z), lw);
+    p[1] = make_float4(fmx(amx.x, bmx.x), fmx(amx.y, bmx.y), fmx(amx.z, bmx.z), ew);
+    current = parent;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair traversal (lbvh.wgsl:321-398). One thread per sorted leaf p walks the
+// tree without a stack, following escapes, and only visits leaves right of p:
+// each unordered overlapping pair is found once and written as
+// (lower collider, higher collider), the set the reference's per-leaf stack
+// traversal emits.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool suppressed(const Dev& d, uint32_t body_a, uint32_t body_b) {
+  uint32_t lo = body_a < body_b ? body_a : body_b;
+  uint32_t hi = body_a < body_b ? body_b : body_a;
+  uint32_t left = 0, right = d.suppression_count;
+  while (left < right) {
+    uint32_t mid = left + (right - left) / 2;
+    uint2 cand = d.suppressed[mid];
+    if (cand.x < lo || (cand.x == lo && cand.y < hi)) left = mid + 1;
+    else right = mid;
+  }
+  return left < d.suppression_count && d.suppressed[left].x == lo && d.suppressed[left].y == hi;
+}
+
+__device__ __forceinline__ bool overlaps(float4 qmn, float4 qmx, float4 mn, float4 mx) {
+  return qmn.x <= mx.x && qmn.y <= mx.y && qmn.z <= mx.z && mn.x <= qmx.x && mn.y <= qmx.y && mn.z <= qmx.z;
+}
+
+__device__ __forceinline__ uint32_t collider_body(const Dev& d, uint32_t c) {
+  return __float_as_uint(d.colliders[4 * (size_t)c + 3].x);
+}
+
+__global__ __launch_bounds__(256) void k_traverse(Dev d) {
+  const uint64_t tick = tick_now(d);
+  uint32_t* diag = d.diag[tick & 1];
+  const uint32_t c = d.collider_count;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= c) return;
+  const float4 qmn = d.leaves[2 * (size_t)p];
+  const float4 qmx = d.leaves[2 * (size_t)p + 1];
+  const uint32_t self = __float_as_uint(qmn.w);
+  const uint32_t self_body = collider_body(d, self);
+  uint32_t cur = c > 1 ? 0u : (0u | kLeafBit);
+  // Every step moves forward in depth-first order, so 2C-1 steps visit every
+  // node once; the bound turns a corrupted tree into a flag instead of a hang.
+  uint32_t steps_left = 2u * c + 1u;
+  while (cur != c) {
+    if (steps_left-- == 0u) {
+      atomicOr(&diag[kDiagFlags], 1u);
+      break;
+    }
+    const uint32_t pos = cur & kPosMask;
+    bool emit = false;
+    uint32_t other = 0;
+    if (cur & kLeafBit) {
+      const float4 mn = d.leaves[2 * (size_t)pos];
+      const float4 mx = d.leaves[2 * (size_t)pos + 1];
+      if (pos > p && overlaps(qmn, qmx, mn, mx)) {
+        other = __float_as_uint(mn.w);
+        uint32_t other_body = collider_body(d, other);
+        emit = other_body != self_body && !(d.suppression_count != 0 && suppressed(d, self_body, other_body));
+      }
+      cur = __float_as_uint(mx.w);
+    } else {
+      const float4 mn = d.inodes[2 * (size_t)pos];
+      const float4 mx = d.inodes[2 * (size_t)pos + 1];
+      const uint32_t esc = __float_as_uint(mx.w);
+      if ((esc & kPosMask) <= p + 1u) cur = esc;
+      else if (overlaps(qmn, qmx, mn, mx)) cur = __float_as_uint(mn.w);
+      else cur = esc;
+    }
+    uint32_t row = wave_append(&diag[kDiagPairs], emit);
+    if (emit) {
+      if (row < d.pair_capacity) {
+        d.pairs[row] = make_uint2(self < other ? self : other, self < other ? other : self);
+      } else {
+        atomicOr(&diag[kDiagFlags], 1u);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+void launch_morton(hipStream_t s, const Dev& d) {
+  k_morton<<<blocks_for(d.sort_count), kBlock, 0, s>>>(d);
+}
+
+void launch_sort(hipStream_t s, const Dev& d) {
+  const uint32_t n = d.sort_count;  // power of two, >= 256
+  const uint32_t tile = n < kTile ? n : kTile;
+  k_sort_tile<<<n / tile, tile / kPer, 0, s>>>(d.keys, tile);
+  for (uint32_t k = tile * 2; k <= n; k <<= 1) {
+    uint32_t j = k >> 1;
+    while (j >= tile) {
+      uint32_t span = 31u - (uint32_t)__builtin_clz(j) - (31u - (uint32_t)__builtin_clz(tile)) + 1u;
+      uint32_t levels = span < 3u ? span : 3u;
+      uint32_t threads = n >> levels;
+      if (levels == 3) k_sort_global<3><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
+      else if (levels == 2) k_sort_global<2><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
+      else k_sort_global<1><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
+      j >>= levels;
+    }
+    k_sort_merge<<<n / tile, tile / kPer, 0, s>>>(d.keys, tile, k);
+  }
+}
+
+void launch_lbvh(hipStream_t s, const Dev& d) {
+  if (d.collider_count > 1) k_topology<<<blocks_for(d.collider_count - 1), kBlock, 0, s>>>(d);
+  k_refit<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
+}
+
+void launch_traverse(hipStream_t s, const Dev& d) {
+  k_traverse<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
+}
+
+}  // namespace mh

@@ -1,0 +1,298 @@
+// Contact preparation (persistent-manifold warm start) and the relaxed projected
+// Jacobi impulse solver with Q20 fixed-point velocity deltas,
+// collision.wgsl:488-538, 674-906, 1050-1088.
+//
+// The manifold cache is keyed by the ordered collider pair. Each tick inserts
+// the pairs it prepares into a fresh open-addressing index (by tick parity) and
+// looks pairs up in the index of the previous tick, so whether a contact finds
+// its cached impulse depends only on the previous tick's contact set, never on
+// the order in which threads ran.
+#include "internal.h"
+#include "mh_geometry.h"
+#include "mh_math.h"
+
+namespace mh {
+
+constexpr float kFixedScale = 1048576.0f;  // collision.wgsl:63
+constexpr float kRelaxation = 0.125f;      // collision.wgsl:64
+constexpr float kWarmStartScale = 0.5f;    // collision.wgsl:65
+constexpr float kSlop = 0.001f;            // collision.wgsl:67
+constexpr float kMaxCorrection = 1.0f;     // collision.wgsl:68
+constexpr float kNormalAlignment = 0.98f;  // collision.wgsl:69
+constexpr float kPointMovement = 0.02f;    // collision.wgsl:70
+constexpr uint32_t kMaxProbes = 256u;      // collision.wgsl:74
+
+__device__ __forceinline__ uint64_t tick_sv(const Dev& d) {
+  return *reinterpret_cast<const volatile uint64_t*>(&d.tick->tick_index);
+}
+
+__device__ __forceinline__ uint32_t read_count(uint32_t* word, uint32_t cap) {
+  uint32_t v = *reinterpret_cast<volatile uint32_t*>(word);
+  return v < cap ? v : cap;
+}
+
+// pair_hash (collision.wgsl:488-497).
+__device__ __forceinline__ uint32_t pair_hash(uint32_t a, uint32_t b) {
+  uint32_t v = a * 0x9e3779b9u;
+  v ^= b * 0x85ebca6bu;
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  v ^= v >> 16;
+  return v;
+}
+
+// 42-bit key of an ordered collider pair (ground: high = MH_INVALID_INDEX).
+__device__ __forceinline__ uint64_t pair_key(uint32_t low, uint32_t high) {
+  uint64_t hi = high == kInvalid ? 0ull : (uint64_t)high + 1ull;
+  return ((uint64_t)(low + 1u) << 21) | hi;
+}
+
+__device__ __forceinline__ v3 inverse_inertia(const float4* wm, v3 v) {
+  return (xyz(wm[0]) * v.x + xyz(wm[1]) * v.y) + xyz(wm[2]) * v.z;
+}
+
+// impulse_denominator (collision.wgsl:116-132).
+__device__ __forceinline__ float impulse_denominator(const float4* wa, const float4* wb, bool has_b, v3 arm_a,
+                                                     v3 arm_b, v3 direction) {
+  float result = wa[0].w;
+  v3 angular_a = cross(inverse_inertia(wa, cross(arm_a, direction)), arm_a);
+  result += dot(angular_a, direction);
+  if (has_b) {
+    result += wb[0].w;
+    v3 angular_b = cross(inverse_inertia(wb, cross(arm_b, direction)), arm_b);
+    result += dot(angular_b, direction);
+  }
+  return result;
+}
+
+__device__ __forceinline__ v3 contact_velocity(const Dev& d, uint32_t body, v3 arm) {
+  return xyz(d.lin[body]) + cross(xyz(d.ang[body]), arm);
+}
+
+// prepare_contacts (collision.wgsl:674-736) with the deterministic manifold index.
+__global__ __launch_bounds__(256) void k_prepare(Dev d) {
+  const uint64_t tick64 = tick_sv(d);
+  const uint32_t tick = (uint32_t)tick64;
+  uint32_t* diag = d.diag[tick64 & 1];
+  uint64_t* table_now = d.htab[tick64 & 1];
+  const uint64_t* table_prev = d.htab[(tick64 + 1) & 1];
+  float4* rows_now = d.mrows[tick64 & 1];
+  const float4* rows_prev = d.mrows[(tick64 + 1) & 1];
+  const uint32_t count = read_count(&diag[kDiagContacts], d.pair_capacity);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    const uint32_t ci = base + threadIdx.x;
+    bool active = false;
+    float4 r0, r1, r2, r3;
+    if (ci < count) {
+      float4* c = d.contacts + 4 * (size_t)ci;
+      const float4 m = c[0];
+      const uint32_t body_a = __float_as_uint(m.x), body_b = __float_as_uint(m.y);
+      const uint32_t col_a = __float_as_uint(m.z), col_b = __float_as_uint(m.w);
+      const float4 n_pen = c[1];
+      const float4 arm_a4 = c[2];
+      const float4 arm_b4 = c[3];
+      const v3 arm_a = xyz(arm_a4), arm_b = xyz(arm_b4), normal = xyz(n_pen);
+      const bool has_b = body_b != kInvalid;
+      const float4* wa = d.world_masses + 3 * (size_t)body_a;
+      const float4* wb = d.world_masses + 3 * (size_t)(has_b ? body_b : body_a);
+      float inverse_b = 0.0f;
+      v3 velocity_b = mk(0.0f, 0.0f, 0.0f);
+      if (has_b) {
+        inverse_b = wb[0].w;
+        velocity_b = contact_velocity(d, body_b, arm_b);
+      }
+      if (wa[0].w + inverse_b <= 0.0f) goto done;
+      {
+        const v3 relative = velocity_b - contact_velocity(d, body_a, arm_a);
+        const float normal_speed = dot(relative, normal);
+        if (n_pen.w <= 1.0e-6f && normal_speed >= 0.0f) {
+          c[0] = make_float4(m.x, m.y, __uint_as_float(kInvalid), __uint_as_float(0u));
+          goto done;
+        }
+        const uint32_t low = col_a < col_b ? col_a : col_b;
+        const uint32_t high = col_a < col_b ? col_b : col_a;
+        const uint64_t key = pair_key(low, high);
+        const uint32_t mask = d.hash_mask;
+        const uint32_t start = pair_hash(low, high) & mask;
+        // Look the pair up among the previous tick's manifolds.
+        float cached_impulse = 0.0f;
+        for (uint32_t probe = 0; probe < kMaxProbes; ++probe) {
+          uint64_t e = table_prev[(start + probe) & mask];
+          if (e == 0ull) break;
+          if ((e >> 22) == key) {
+            const float4* row = rows_prev + 3 * (size_t)(e & 0x3fffffull);
+            float4 pt = row[0], cn = row[1], cp = row[2];
+            bool matches = __float_as_uint(pt.z) == tick - 1u && dot(xyz(cn), normal) >= kNormalAlignment &&
+                           length(xyz(cp) - arm_a) <= kPointMovement;
+            cached_impulse = matches ? cp.w : 0.0f;
+            break;
+          }
+        }
+        // Insert the pair into this tick's index.
+        const uint64_t entry = (key << 22) | (uint64_t)ci;
+        bool inserted = false;
+        for (uint32_t probe = 0; probe < kMaxProbes; ++probe) {
+          unsigned long long* slot = reinterpret_cast<unsigned long long*>(&table_now[(start + probe) & mask]);
+          if (atomicCAS(slot, 0ull, (unsigned long long)entry) == 0ull) {
+            inserted = true;
+            break;
+          }
+        }
+        if (!inserted) {
+          atomicOr(&diag[kDiagFlags], 8u);
+          goto done;
+        }
+        c[0] = make_float4(m.x, m.y, __uint_as_float(ci), __uint_as_float(1u));
+        c[2] = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, cached_impulse);
+        float4* mr = rows_now + 3 * (size_t)ci;
+        mr[0] = make_float4(__uint_as_float(low), __uint_as_float(high), __uint_as_float(tick), __uint_as_float(1u));
+        mr[1] = n_pen;
+        mr[2] = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, cached_impulse);
+        if (n_pen.w > 1.0e-5f || normal_speed < -1.0e-5f || cached_impulse > 1.0e-6f) {
+          active = true;
+          r0 = n_pen;
+          r1 = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, cached_impulse);
+          r2 = arm_b4;
+          float denom = impulse_denominator(wa, wb, has_b, arm_a, arm_b, normal);
+          r3 = make_float4(m.x, m.y, __uint_as_float(ci), denom);
+        }
+      }
+    }
+  done:
+    uint32_t a = wave_append(&diag[kDiagActive], active);
+    if (active) {
+      d.active[a] = __float_as_uint(r3.z);
+      float4* row = d.rows + 4 * (size_t)a;
+      row[0] = r0;
+      row[1] = r1;
+      row[2] = r2;
+      row[3] = r3;
+    }
+  }
+}
+
+// add_velocity_delta (collision.wgsl:764-772).
+__device__ __forceinline__ void add_delta(const Dev& d, uint32_t body, v3 linear, v3 angular) {
+  int32_t* p = d.deltas + 6 * (size_t)body;
+  atomicAdd(&p[0], round_to_i32(linear.x * kFixedScale));
+  atomicAdd(&p[1], round_to_i32(linear.y * kFixedScale));
+  atomicAdd(&p[2], round_to_i32(linear.z * kFixedScale));
+  atomicAdd(&p[3], round_to_i32(angular.x * kFixedScale));
+  atomicAdd(&p[4], round_to_i32(angular.y * kFixedScale));
+  atomicAdd(&p[5], round_to_i32(angular.z * kFixedScale));
+}
+
+// warm_start (collision.wgsl:774-839) when first, solve_accumulate
+// (collision.wgsl:841-906) otherwise; the last pass also runs persist_contacts
+// (collision.wgsl:1071-1088).
+__global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  float4* rows_now = d.mrows[tick64 & 1];
+  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
+  const float friction = d.friction;
+  for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < count; ai += gridDim.x * blockDim.x) {
+    float4* row = d.rows + 4 * (size_t)ai;
+    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+    const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+    const bool has_b = body_b != kInvalid;
+    const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
+    const float penetration = r0.w;
+    const bool analytic = r2.w < 0.0f;
+    const float denominator = r3.w;
+    float lambda = r1.w;
+    const float4* wa = d.world_masses + 3 * (size_t)body_a;
+    const float4* wb = d.world_masses + 3 * (size_t)(has_b ? body_b : body_a);
+    v3 velocity_b = mk(0.0f, 0.0f, 0.0f);
+    if (has_b) velocity_b = contact_velocity(d, body_b, arm_b);
+    if (denominator > 0.0f) {
+      const v3 relative = velocity_b - contact_velocity(d, body_a, arm_a);
+      const float normal_speed = dot(relative, normal);
+      if (penetration <= 1.0e-5f && normal_speed >= -1.0e-5f && lambda <= 1.0e-6f) {
+        lambda = 0.0f;
+      } else {
+        const float recovery = analytic ? 1.0f : 0.2f;
+        const float bias = fmn((fmx(penetration - kSlop, 0.0f) * recovery) / d.delta_seconds, kMaxCorrection);
+        const float step = (kRelaxation * (-normal_speed + bias)) / denominator;
+        v3 impulse;
+        float bound;
+        if (first) {
+          const float warmed = lambda * (analytic ? 1.0f : kWarmStartScale);
+          const float accumulated = fmx(warmed + step, 0.0f);
+          lambda = accumulated;
+          impulse = normal * accumulated;
+          bound = accumulated * friction;
+        } else {
+          const float previous = fmx(lambda, 0.0f);
+          const float accumulated = fmx(previous + step, 0.0f);
+          const float delta = accumulated - previous;
+          lambda = accumulated;
+          impulse = normal * delta;
+          bound = fabsf(delta) * friction;
+        }
+        const v3 tangent_velocity = relative - normal * normal_speed;
+        const float tangent_speed = length(tangent_velocity);
+        if (tangent_speed > 1.0e-6f) {
+          const v3 tangent = tangent_velocity / tangent_speed;
+          const float tangent_denominator = impulse_denominator(wa, wb, has_b, arm_a, arm_b, tangent);
+          const float friction_impulse = fmn(tangent_speed / tangent_denominator, bound);
+          impulse = impulse - tangent * friction_impulse;
+        }
+        const v3 neg = -impulse;
+        add_delta(d, body_a, neg * wa[0].w, inverse_inertia(wa, cross(arm_a, neg)));
+        if (has_b) add_delta(d, body_b, impulse * wb[0].w, inverse_inertia(wb, cross(arm_b, impulse)));
+      }
+    }
+    row[1].w = lambda;
+    if (last) {
+      const uint32_t ci = __float_as_uint(r3.z);
+      d.contacts[4 * (size_t)ci + 2].w = lambda;
+      rows_now[3 * (size_t)ci + 2].w = lambda;
+    }
+  }
+}
+
+// solve_apply (collision.wgsl:1050-1069); skipped when no contact is active
+// (finalize_active_contacts, collision.wgsl:744).
+__global__ __launch_bounds__(256) void k_apply(Dev d) {
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.body_count) return;
+  int32_t* p = d.deltas + 6 * (size_t)b;
+  const int2 d01 = *reinterpret_cast<int2*>(p);
+  const int2 d23 = *reinterpret_cast<int2*>(p + 2);
+  const int2 d45 = *reinterpret_cast<int2*>(p + 4);
+  *reinterpret_cast<int2*>(p) = make_int2(0, 0);
+  *reinterpret_cast<int2*>(p + 2) = make_int2(0, 0);
+  *reinterpret_cast<int2*>(p + 4) = make_int2(0, 0);
+  const v3 lin_delta = mk((float)d01.x, (float)d01.y, (float)d23.x) / kFixedScale;
+  const v3 ang_delta = mk((float)d23.y, (float)d45.x, (float)d45.y) / kFixedScale;
+  d.lin[b] = f4(xyz(d.lin[b]) + lin_delta, 0.0f);
+  d.ang[b] = f4(xyz(d.ang[b]) + ang_delta, 0.0f);
+}
+
+static inline uint32_t grid_for(uint32_t n, uint32_t cap) {
+  uint32_t b = (n + kBlock - 1) / kBlock;
+  return b < cap ? b : cap;
+}
+
+void launch_prepare(hipStream_t s, const Dev& d) {
+  k_prepare<<<grid_for(d.pair_capacity, 2048), kBlock, 0, s>>>(d);
+}
+
+void launch_solve(hipStream_t s, const Dev& d) {
+  const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
+  const uint32_t solve_grid = grid_for(d.pair_capacity, 2048);
+  const uint32_t body_grid = (d.body_count + kBlock - 1) / kBlock;
+  for (uint32_t it = 0; it < iterations; ++it) {
+    k_solve<<<solve_grid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, it + 1 == iterations ? 1 : 0);
+    if (body_grid) k_apply<<<body_grid, kBlock, 0, s>>>(d);
+  }
+}
+
+}  // namespace mh

@@ -1,0 +1,633 @@
+// Host runtime behind the C ABI: owns the device buffers and one HIP stream per
+// handle, encodes a tick as a fixed kernel sequence and replays it as a captured
+// hipGraph. Mirrors GpuPhysics (crates/mechanic-gpu/src/device.rs:199-1663).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/mechanic_hip.h"
+#include "internal.h"
+
+using namespace mh;
+
+namespace {
+
+struct Timing {
+  mh_phase_timings t;
+  bool valid = false;
+};
+
+}  // namespace
+
+struct mh_physics {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Dev d{};
+  std::vector<void*> allocations;
+  mh_config config{};
+  bool collisions = false;
+  bool graph_enabled = false;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  uint64_t expected_tick = 0;
+  bool tick_known = false;
+  uint64_t last_tick = 0;
+  bool has_last_tick = false;
+  uint32_t* host_diag = nullptr;  // pinned
+  hipEvent_t events[10] = {};
+  Timing timing;
+};
+
+namespace {
+
+int fail(mh_error_detail* e, int code, const char* msg) {
+  if (e) {
+    e->code = code;
+    std::snprintf(e->message, sizeof(e->message), "%s", msg);
+  }
+  return code;
+}
+
+int hip_fail(mh_error_detail* e, hipError_t err, const char* what) {
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(err));
+  return fail(e, MH_ERR_DEVICE, buf);
+}
+
+#define HIP_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t err_ = (expr);                          \
+    if (err_ != hipSuccess) return hip_fail(error, err_, #expr); \
+  } while (0)
+
+#define HIP_TRY_NOERR(expr)                 \
+  do {                                      \
+    hipError_t err_ = (expr);               \
+    if (err_ != hipSuccess) return MH_ERR_DEVICE; \
+  } while (0)
+
+template <typename T>
+int alloc(mh_physics* p, T** out, size_t count, mh_error_detail* error) {
+  if (count == 0) count = 1;
+  void* ptr = nullptr;
+  HIP_TRY(hipMalloc(&ptr, count * sizeof(T)));
+  HIP_TRY(hipMemset(ptr, 0, count * sizeof(T)));
+  p->allocations.push_back(ptr);
+  *out = static_cast<T*>(ptr);
+  return MH_OK;
+}
+
+template <typename T>
+int upload(mh_physics* p, T* dst, const void* src, size_t bytes, mh_error_detail* error) {
+  if (bytes == 0 || src == nullptr) return MH_OK;
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  (void)p;
+  return MH_OK;
+}
+
+uint32_t next_pow2(uint32_t v) {
+  uint32_t r = 1;
+  while (r < v) r <<= 1;
+  return r;
+}
+
+// Enqueues one tick. `stop` ends it early for phase tests (1: after
+// integration, 2: after the broadphase, 3: after contact generation).
+void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0) {
+  const Dev& d = p->d;
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
+  mark(0);
+  launch_integrate(s, d, p->collisions, true);
+  mark(1);
+  if (p->collisions && stop != 1) {
+    launch_clear_table(s, d);
+    launch_morton(s, d);
+    mark(2);
+    launch_sort(s, d);
+    mark(3);
+    launch_lbvh(s, d);
+    mark(4);
+    launch_traverse(s, d);
+    mark(5);
+    if (stop == 2) return;
+    launch_narrowphase(s, d);
+    mark(6);
+    if (stop == 3) return;
+    launch_prepare(s, d);
+    mark(7);
+    launch_solve(s, d);
+    mark(8);
+  } else {
+    if (stop != 0) return;
+    for (int i = 2; i <= 8; ++i) mark(i);
+  }
+  launch_end_tick(s, d);
+  mark(9);
+}
+
+int begin_tick(mh_physics* p, uint64_t tick) {
+  if (!p->tick_known || p->expected_tick != tick) {
+    launch_set_tick(p->stream, p->d, tick);
+    HIP_TRY_NOERR(hipGetLastError());
+  }
+  p->expected_tick = tick + 1;
+  p->tick_known = true;
+  p->last_tick = tick;
+  p->has_last_tick = true;
+  return MH_OK;
+}
+
+int ensure_graph(mh_physics* p) {
+  if (p->graph_exec) return MH_OK;
+  HIP_TRY_NOERR(hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed));
+  encode_tick(p, p->stream, nullptr);
+  hipGraph_t g = nullptr;
+  HIP_TRY_NOERR(hipStreamEndCapture(p->stream, &g));
+  p->graph = g;
+  HIP_TRY_NOERR(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+  return MH_OK;
+}
+
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+}  // namespace
+
+extern "C" {
+
+mh_config mh_config_default(void) {
+  mh_config c{};
+  c.collisions_enabled = 1;
+  c.mechanism_self_collisions = 1;
+  c.solver_iterations = 8;
+  c.friction = 0.05f;
+  c.pair_capacity = MH_MAX_CONTACT_PAIRS;
+  c.warm_start_mode = 0;
+  c.use_graph = 1;
+  return c;
+}
+
+const char* mh_status_string(int code) {
+  switch (code) {
+    case MH_OK: return "ok";
+    case MH_ERR_BODY_CAPACITY: return "body capacity exceeded";
+    case MH_ERR_BEARING_CAPACITY: return "bearing capacity exceeded";
+    case MH_ERR_COLLIDER_CAPACITY: return "collider capacity exceeded";
+    case MH_ERR_DRIVE_STATE_COUNT: return "drive state row count mismatch";
+    case MH_ERR_COORDINATE_STATE_COUNT: return "coordinate state row count mismatch";
+    case MH_ERR_BODY_INDEX_OUT_OF_RANGE: return "body index out of range";
+    case MH_ERR_NON_FINITE: return "non-finite impulse";
+    case MH_ERR_INVALID_SNAPSHOT_SLOT: return "invalid snapshot slot";
+    case MH_ERR_DEVICE: return "HIP device error";
+    case MH_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case MH_ERR_NO_DEVICE: return "no HIP device";
+    case MH_ERR_UNSUPPORTED: return "unsupported in this build";
+    default: return "unknown status";
+  }
+}
+
+int mh_device_count(int* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (count) *count = (e == hipSuccess) ? n : 0;
+  return e == hipSuccess ? MH_OK : MH_ERR_NO_DEVICE;
+}
+
+int mh_physics_create(const mh_creation* creation, const mh_config* config_in, mh_physics** out,
+                      mh_error_detail* error) {
+  if (error) std::memset(error, 0, sizeof(*error));
+  if (!creation || !out) return fail(error, MH_ERR_INVALID_ARGUMENT, "null creation or output");
+  *out = nullptr;
+  // Capacity checks before any device work (device.rs:405-422).
+  if (creation->body_count > MH_MAX_BODIES) {
+    if (error) { error->required = creation->body_count; error->capacity = MH_MAX_BODIES; }
+    return fail(error, MH_ERR_BODY_CAPACITY, "scene exceeds the body capacity");
+  }
+  if (creation->bearing_count > MH_MAX_BEARINGS) {
+    if (error) { error->required = creation->bearing_count; error->capacity = MH_MAX_BEARINGS; }
+    return fail(error, MH_ERR_BEARING_CAPACITY, "scene exceeds the bearing capacity");
+  }
+  if (creation->collider_count > MH_MAX_COLLIDERS) {
+    if (error) { error->required = creation->collider_count; error->capacity = MH_MAX_COLLIDERS; }
+    return fail(error, MH_ERR_COLLIDER_CAPACITY, "scene exceeds the collider capacity");
+  }
+  if (creation->bearing_count > 0)
+    return fail(error, MH_ERR_UNSUPPORTED, "articulated bodies are not supported by this build");
+  const uint32_t n = creation->body_count, c = creation->collider_count;
+  if ((n > 0 && (!creation->poses || !creation->inverse_masses || !creation->masses)) ||
+      (c > 0 && !creation->colliders) || (creation->suppression_count > 0 && !creation->suppressed_pairs))
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "missing row arrays");
+  for (uint32_t i = 0; i < c; ++i)
+    if (creation->colliders[i].metadata[0] >= n)
+      return fail(error, MH_ERR_INVALID_ARGUMENT, "collider refers to a body outside the scene");
+  mh_config cfg = config_in ? *config_in : mh_config_default();
+  if (cfg.solver_iterations == 0) cfg.solver_iterations = 8;
+  if (cfg.pair_capacity == 0) cfg.pair_capacity = MH_MAX_CONTACT_PAIRS;
+  if (!(cfg.friction >= 0.0f) || !std::isfinite(cfg.friction))
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "friction must be finite and non-negative");
+  if ((cfg.pair_capacity & (cfg.pair_capacity - 1)) != 0 || cfg.pair_capacity < 256 ||
+      cfg.pair_capacity > (1u << 22))
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "pair_capacity must be a power of two in [256, 2^22]");
+
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0)
+    return fail(error, MH_ERR_NO_DEVICE, "no HIP device visible");
+
+  mh_physics* p = new mh_physics();
+  p->config = cfg;
+  p->collisions = cfg.collisions_enabled != 0 && c > 0;
+  p->graph_enabled = cfg.use_graph != 0;
+  HIP_TRY(hipGetDevice(&p->device));
+  int rc = MH_OK;
+#define ALLOC(ptr, count)                            \
+  do {                                               \
+    rc = alloc(p, &(ptr), (count), error);           \
+    if (rc != MH_OK) { mh_physics_destroy(p); return rc; } \
+  } while (0)
+#define UPLOAD(dst, src, bytes)                          \
+  do {                                                   \
+    rc = upload(p, (dst), (src), (bytes), error);        \
+    if (rc != MH_OK) { mh_physics_destroy(p); return rc; } \
+  } while (0)
+
+  hipError_t se = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (se != hipSuccess) { delete p; return hip_fail(error, se, "hipStreamCreateWithFlags"); }
+  for (auto& e : p->events) (void)hipEventCreate(&e);
+
+  Dev& d = p->d;
+  d.body_count = n;
+  d.collider_count = c;
+  d.sort_count = next_pow2(c < 256 ? 256 : c);
+  d.suppression_count = creation->suppression_count;
+  d.pair_capacity = cfg.pair_capacity;
+  d.hash_mask = cfg.pair_capacity - 1;
+  d.solver_iterations = cfg.solver_iterations;
+  d.self_collisions = cfg.mechanism_self_collisions ? 1u : 0u;
+  d.delta_seconds = 1.0f / 60.0f;  // FIXED_DT_SECONDS, lib.rs:33
+  d.gravity_y = -9.81f;            // device.rs:872
+  d.linear_damping = 0.999f;       // device.rs:873
+  d.angular_damping = 0.98f;       // device.rs:874
+  d.friction = cfg.friction;
+
+  float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *inodes, *leaves, *contacts, *rows;
+  float* inv_mass;
+  uint32_t *roots, *comps, *parents, *visits, *active, *diag;
+  uint2 *supp, *children, *pairs;
+  uint64_t *keys, *h0, *h1;
+  int32_t* deltas;
+  float4 *m0, *m1;
+  TickState* tick;
+  ALLOC(pos, n); ALLOC(rot, n); ALLOC(lin, n); ALLOC(ang, n);
+  ALLOC(inv_mass, n); ALLOC(masses, 4 * (size_t)n); ALLOC(wm, 3 * (size_t)n);
+  ALLOC(roots, n); ALLOC(comps, n);
+  ALLOC(cols, 4 * (size_t)c); ALLOC(supp, creation->suppression_count);
+  ALLOC(aabbs, 2 * (size_t)c); ALLOC(keys, d.sort_count);
+  ALLOC(inodes, 2 * (size_t)(c > 1 ? c - 1 : 1)); ALLOC(leaves, 2 * (size_t)c);
+  ALLOC(children, c > 1 ? c - 1 : 1); ALLOC(parents, c > 0 ? 2 * (size_t)c - 1 : 1);
+  ALLOC(visits, c > 1 ? c - 1 : 1);
+  ALLOC(pairs, cfg.pair_capacity);
+  ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
+  ALLOC(active, cfg.pair_capacity);
+  ALLOC(rows, 4 * (size_t)cfg.pair_capacity);
+  ALLOC(deltas, 6 * (size_t)n);
+  ALLOC(h0, cfg.pair_capacity); ALLOC(h1, cfg.pair_capacity);
+  ALLOC(m0, 3 * (size_t)cfg.pair_capacity); ALLOC(m1, 3 * (size_t)cfg.pair_capacity);
+  for (int s = 0; s < 3; ++s) {
+    ALLOC(d.snap_pos[s], n);
+    ALLOC(d.snap_rot[s], n);
+  }
+  ALLOC(diag, 2 * kDiagWords);
+  ALLOC(tick, 1);
+  d.pos = pos; d.rot = rot; d.lin = lin; d.ang = ang;
+  d.inv_mass = inv_mass; d.masses = masses; d.world_masses = wm;
+  d.root_flags = roots; d.components = comps;
+  d.colliders = cols; d.suppressed = supp;
+  d.aabbs = aabbs; d.keys = keys; d.inodes = inodes; d.leaves = leaves;
+  d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
+  d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
+  d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;
+  d.diag[0] = diag; d.diag[1] = diag + kDiagWords;
+  d.tick = tick;
+
+  // Lowering (device.rs:424-523): poses split into position/rotation planes,
+  // zero velocities, inverse masses, mass rows, colliders, suppression pairs.
+  if (n > 0) {
+    std::vector<float4> hp(n), hr(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      const float* q = creation->poses[i].position;
+      const float* r = creation->poses[i].rotation;
+      hp[i] = make_float4(q[0], q[1], q[2], 0.0f);
+      hr[i] = make_float4(r[0], r[1], r[2], r[3]);
+    }
+    UPLOAD(pos, hp.data(), n * sizeof(float4));
+    UPLOAD(rot, hr.data(), n * sizeof(float4));
+    UPLOAD(inv_mass, creation->inverse_masses, n * sizeof(float));
+    UPLOAD(masses, creation->masses, n * sizeof(mh_mass));
+    std::vector<uint32_t> hroots(n, 1u), hcomps(n);
+    if (creation->root_flags)
+      for (uint32_t i = 0; i < n; ++i) hroots[i] = creation->root_flags[i];
+    for (uint32_t i = 0; i < n; ++i) hcomps[i] = creation->body_components ? creation->body_components[i] : i;
+    UPLOAD(roots, hroots.data(), n * sizeof(uint32_t));
+    UPLOAD(comps, hcomps.data(), n * sizeof(uint32_t));
+  }
+  UPLOAD(cols, creation->colliders, (size_t)c * sizeof(mh_collider));
+  UPLOAD(supp, creation->suppressed_pairs, (size_t)creation->suppression_count * sizeof(mh_pair));
+  hipError_t he = hipHostMalloc(reinterpret_cast<void**>(&p->host_diag), kDiagWords * sizeof(uint32_t),
+                                hipHostMallocDefault);
+  if (he != hipSuccess) { mh_physics_destroy(p); return hip_fail(error, he, "hipHostMalloc"); }
+  he = hipDeviceSynchronize();
+  if (he != hipSuccess) { mh_physics_destroy(p); return hip_fail(error, he, "upload"); }
+  *out = p;
+  return MH_OK;
+#undef ALLOC
+#undef UPLOAD
+}
+
+void mh_physics_destroy(mh_physics* p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (auto& e : p->events)
+    if (e) (void)hipEventDestroy(e);
+  for (void* a : p->allocations) (void)hipFree(a);
+  if (p->host_diag) (void)hipHostFree(p->host_diag);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+uint32_t mh_physics_body_count(const mh_physics* p) { return p ? p->d.body_count : 0; }
+
+int mh_physics_has_gpu_timestamps(const mh_physics* p) { return p ? 1 : 0; }
+
+int mh_physics_apply_impulse(mh_physics* p, uint32_t body, const float world_point[3], const float impulse[3],
+                             mh_error_detail* error) {
+  if (error) std::memset(error, 0, sizeof(*error));
+  if (!p || !world_point || !impulse) return fail(error, MH_ERR_INVALID_ARGUMENT, "null argument");
+  if (body >= p->d.body_count) {
+    if (error) { error->body_index = body; error->body_count = p->d.body_count; }
+    return fail(error, MH_ERR_BODY_INDEX_OUT_OF_RANGE, "body index outside the uploaded body count");
+  }
+  if (!finite3(world_point) || !finite3(impulse))
+    return fail(error, MH_ERR_NON_FINITE, "external impulse point and vector must be finite");
+  launch_external_impulse(p->stream, p->d, make_float4(world_point[0], world_point[1], world_point[2], 0.0f),
+                          make_float4(impulse[0], impulse[1], impulse[2], 0.0f), body);
+  HIP_TRY(hipGetLastError());
+  return MH_OK;
+}
+
+int mh_physics_dispatch_tick(mh_physics* p, uint64_t tick, uint8_t* slot) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  int rc = begin_tick(p, tick);
+  if (rc != MH_OK) return rc;
+  if (p->graph_enabled) {
+    rc = ensure_graph(p);
+    if (rc != MH_OK) return rc;
+    HIP_TRY_NOERR(hipGraphLaunch(p->graph_exec, p->stream));
+  } else {
+    encode_tick(p, p->stream, nullptr);
+    HIP_TRY_NOERR(hipGetLastError());
+  }
+  if (slot) *slot = (uint8_t)(tick % 3u);
+  return MH_OK;
+}
+
+int mh_physics_dispatch_ticks(mh_physics* p, uint64_t first_tick, uint32_t count) {
+  for (uint32_t i = 0; i < count; ++i) {
+    int rc = mh_physics_dispatch_tick(p, first_tick + i, nullptr);
+    if (rc != MH_OK) return rc;
+  }
+  return MH_OK;
+}
+
+int mh_physics_profile_tick(mh_physics* p, uint64_t tick, mh_phase_timings* out) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  int rc = begin_tick(p, tick);
+  if (rc != MH_OK) return rc;
+  encode_tick(p, p->stream, p->events);
+  HIP_TRY_NOERR(hipGetLastError());
+  HIP_TRY_NOERR(hipEventSynchronize(p->events[9]));
+  float ms[9];
+  for (int i = 0; i < 9; ++i) HIP_TRY_NOERR(hipEventElapsedTime(&ms[i], p->events[i], p->events[i + 1]));
+  mh_phase_timings t{};
+  t.integrate_ms = ms[0];
+  t.morton_ms = ms[1];
+  t.sort_ms = ms[2];
+  t.lbvh_ms = ms[3];
+  t.traverse_ms = ms[4];
+  t.narrowphase_ms = ms[5];
+  t.prepare_ms = ms[6];
+  t.solve_ms = ms[7];
+  t.persist_ms = 0.0;  // persist is fused into the last solver pass
+  t.snapshot_ms = 0.0; // publication is fused into integration for unarticulated scenes
+  float total = 0.0f;
+  HIP_TRY_NOERR(hipEventElapsedTime(&total, p->events[0], p->events[9]));
+  t.total_ms = total;
+  p->timing.t = t;
+  p->timing.valid = true;
+  if (out) *out = t;
+  return MH_OK;
+}
+
+int mh_physics_debug_tick(mh_physics* p, uint64_t tick, int stop) {
+  if (!p || stop < 0 || stop > 3) return MH_ERR_INVALID_ARGUMENT;
+  int rc = begin_tick(p, tick);
+  if (rc != MH_OK) return rc;
+  encode_tick(p, p->stream, nullptr, stop);
+  HIP_TRY_NOERR(hipGetLastError());
+  // The device tick did not advance: the next tick must set it again.
+  if (stop != 0) p->tick_known = false;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  return MH_OK;
+}
+
+int mh_physics_synchronize(mh_physics* p) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  return MH_OK;
+}
+
+int mh_physics_read_last_tick(mh_physics* p, mh_tick_readback* out) {
+  if (!p || !out) return MH_ERR_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  const uint64_t tick = p->has_last_tick ? p->last_tick : 0;
+  HIP_TRY_NOERR(hipMemcpyAsync(p->host_diag, p->d.diag[tick & 1], kDiagWords * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  out->error_flags = p->host_diag[kDiagFlags];
+  out->pair_count = p->host_diag[kDiagPairs];
+  out->contact_count = p->host_diag[kDiagContacts];
+  out->active_contact_count = p->host_diag[kDiagActive];
+  out->anchor_residual_meters = (float)p->host_diag[kDiagAnchor] * 1.0e-6f;
+  out->axis_residual_degrees = (float)p->host_diag[kDiagAxis] * 1.0e-6f;
+  out->tick_index = tick;
+  out->snapshot_slot = (uint32_t)(tick % 3u);
+  if (p->timing.valid) {
+    const mh_phase_timings& t = p->timing.t;
+    out->has_timings = 1;
+    out->integration_ms = t.integrate_ms;
+    out->broadphase_ms = t.morton_ms + t.sort_ms + t.lbvh_ms + t.traverse_ms;
+    out->narrowphase_ms = t.narrowphase_ms;
+    out->contact_solver_ms = t.prepare_ms + t.solve_ms + t.persist_ms;
+    out->snapshot_ms = t.snapshot_ms;
+    out->gpu_tick_ms = out->integration_ms + out->mechanism_ms + out->broadphase_ms + out->narrowphase_ms +
+                       out->contact_solver_ms + out->bearings_ms + out->snapshot_ms;
+  } else {
+    out->gpu_tick_ms = -1.0;
+  }
+  return MH_OK;
+}
+
+int mh_physics_read_snapshot(mh_physics* p, uint8_t slot, mh_transform* out, size_t capacity) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  if (slot >= MH_SNAPSHOT_RING_SIZE) return MH_ERR_INVALID_SNAPSHOT_SLOT;
+  const uint32_t n = p->d.body_count;
+  if (!out || capacity < n) return MH_ERR_INVALID_ARGUMENT;
+  std::vector<float4> hp(n), hr(n);
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  if (n) {
+    HIP_TRY_NOERR(hipMemcpy(hp.data(), p->d.snap_pos[slot], n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_TRY_NOERR(hipMemcpy(hr.data(), p->d.snap_rot[slot], n * sizeof(float4), hipMemcpyDeviceToHost));
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    std::memcpy(out[i].position, &hp[i], 16);
+    std::memcpy(out[i].rotation, &hr[i], 16);
+  }
+  return MH_OK;
+}
+
+int mh_physics_snapshot_device_ptrs(mh_physics* p, uint8_t slot, const void** positions, const void** rotations) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  if (slot >= MH_SNAPSHOT_RING_SIZE) return MH_ERR_INVALID_SNAPSHOT_SLOT;
+  if (positions) *positions = p->d.snap_pos[slot];
+  if (rotations) *rotations = p->d.snap_rot[slot];
+  return MH_OK;
+}
+
+int mh_physics_read_bodies(mh_physics* p, mh_transform* poses, mh_velocity* velocities, size_t capacity) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  const uint32_t n = p->d.body_count;
+  if (capacity < n) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  std::vector<float4> a(n), b(n);
+  if (poses && n) {
+    HIP_TRY_NOERR(hipMemcpy(a.data(), p->d.pos, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_TRY_NOERR(hipMemcpy(b.data(), p->d.rot, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+      std::memcpy(poses[i].position, &a[i], 16);
+      std::memcpy(poses[i].rotation, &b[i], 16);
+    }
+  }
+  if (velocities && n) {
+    HIP_TRY_NOERR(hipMemcpy(a.data(), p->d.lin, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_TRY_NOERR(hipMemcpy(b.data(), p->d.ang, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+      std::memcpy(velocities[i].linear, &a[i], 16);
+      std::memcpy(velocities[i].angular, &b[i], 16);
+    }
+  }
+  return MH_OK;
+}
+
+int mh_physics_write_bodies(mh_physics* p, const mh_transform* poses, const mh_velocity* velocities, size_t count) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  const uint32_t n = p->d.body_count;
+  if (count != n) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  std::vector<float4> a(n), b(n);
+  if (poses && n) {
+    for (uint32_t i = 0; i < n; ++i) {
+      std::memcpy(&a[i], poses[i].position, 16);
+      std::memcpy(&b[i], poses[i].rotation, 16);
+    }
+    HIP_TRY_NOERR(hipMemcpy(p->d.pos, a.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY_NOERR(hipMemcpy(p->d.rot, b.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  if (velocities && n) {
+    for (uint32_t i = 0; i < n; ++i) {
+      std::memcpy(&a[i], velocities[i].linear, 16);
+      std::memcpy(&b[i], velocities[i].angular, 16);
+    }
+    HIP_TRY_NOERR(hipMemcpy(p->d.lin, a.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY_NOERR(hipMemcpy(p->d.ang, b.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  return MH_OK;
+}
+
+int mh_physics_read_buffer(mh_physics* p, int buffer, void* dst, size_t bytes, size_t* written) {
+  if (!p || !dst) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  const Dev& d = p->d;
+  const uint64_t tick = p->has_last_tick ? p->last_tick : 0;
+  const void* src = nullptr;
+  size_t size = 0;
+  uint32_t diag_host[kDiagWords] = {};
+  HIP_TRY_NOERR(hipMemcpy(diag_host, d.diag[tick & 1], sizeof(diag_host), hipMemcpyDeviceToHost));
+  auto capped = [&](uint32_t v) { return (size_t)(v < d.pair_capacity ? v : d.pair_capacity); };
+  const uint32_t c = d.collider_count;
+  std::vector<float4> node_aabbs;
+  switch (buffer) {
+    case MH_BUFFER_MORTON_KEYS: src = d.keys; size = (size_t)d.sort_count * 8; break;
+    case MH_BUFFER_COLLIDER_AABBS: src = d.aabbs; size = (size_t)c * 32; break;
+    case MH_BUFFER_NODE_CHILDREN: src = d.children; size = c > 1 ? (size_t)(c - 1) * 8 : 0; break;
+    case MH_BUFFER_NODE_PARENTS: src = d.parents; size = c > 0 ? (size_t)(2 * c - 1) * 4 : 0; break;
+    case MH_BUFFER_PAIRS: src = d.pairs; size = capped(diag_host[kDiagPairs]) * 8; break;
+    case MH_BUFFER_CONTACTS: src = d.contacts; size = capped(diag_host[kDiagContacts]) * 64; break;
+    case MH_BUFFER_DIAGNOSTICS: src = d.diag[tick & 1]; size = 32; break;
+    case MH_BUFFER_WORLD_MASSES: src = d.world_masses; size = (size_t)d.body_count * 48; break;
+    case MH_BUFFER_MANIFOLDS: src = d.mrows[tick & 1]; size = capped(diag_host[kDiagContacts]) * 48; break;
+    case MH_BUFFER_NODE_AABBS: {
+      // Reference layout (lbvh.wgsl:19, 287): internal nodes then leaves, min/max with w = 0.
+      size_t nodes = c > 0 ? 2 * (size_t)c - 1 : 0;
+      size = nodes * 32;
+      if (bytes < size) return MH_ERR_INVALID_ARGUMENT;
+      std::vector<float4> in(2 * (size_t)(c > 1 ? c - 1 : 0)), lv(2 * (size_t)c);
+      if (!in.empty())
+        HIP_TRY_NOERR(hipMemcpy(in.data(), d.inodes, in.size() * sizeof(float4), hipMemcpyDeviceToHost));
+      if (!lv.empty())
+        HIP_TRY_NOERR(hipMemcpy(lv.data(), d.leaves, lv.size() * sizeof(float4), hipMemcpyDeviceToHost));
+      float4* o = static_cast<float4*>(dst);
+      size_t k = 0;
+      for (size_t i = 0; i < in.size(); ++i) { o[k] = in[i]; o[k].w = 0.0f; ++k; }
+      for (size_t i = 0; i < lv.size(); ++i) { o[k] = lv[i]; o[k].w = 0.0f; ++k; }
+      if (written) *written = size;
+      return MH_OK;
+    }
+    default: return MH_ERR_INVALID_ARGUMENT;
+  }
+  if (bytes < size) return MH_ERR_INVALID_ARGUMENT;
+  if (size) HIP_TRY_NOERR(hipMemcpy(dst, src, size, hipMemcpyDeviceToHost));
+  if (written) *written = size;
+  return MH_OK;
+}
+
+int mh_physics_write_drives(mh_physics* p, const mh_mechanism_drive* drives, size_t count, mh_error_detail* error) {
+  if (error) std::memset(error, 0, sizeof(*error));
+  if (!p) return fail(error, MH_ERR_INVALID_ARGUMENT, "null handle");
+  (void)drives;
+  // This build uploads no tree bearings, so the compiled coordinate count is 0 (device.rs:1622-1644).
+  if (count != 0) {
+    if (error) { error->provided = count; error->required = 0; }
+    return fail(error, MH_ERR_DRIVE_STATE_COUNT, "drive state row count does not match the scene");
+  }
+  return MH_OK;
+}
+
+int mh_physics_init_coordinates(mh_physics* p, const mh_mechanism_coordinate* coordinates, size_t count,
+                                mh_error_detail* error) {
+  if (error) std::memset(error, 0, sizeof(*error));
+  if (!p) return fail(error, MH_ERR_INVALID_ARGUMENT, "null handle");
+  (void)coordinates;
+  if (count != 0) {
+    if (error) { error->provided = count; error->required = 0; }
+    return fail(error, MH_ERR_COORDINATE_STATE_COUNT, "coordinate state row count does not match the scene");
+  }
+  return MH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,249 @@
+"""Parity of the HIP step with the CPU oracle, through the C ABI.
+
+Integer and index results (Morton keys, sorted order, LBVH topology, pair set)
+must be bit-exact. Floating-point results are compared bit-exactly as well: the
+kernels and the oracle evaluate the same IEEE-754 binary32 operations in the
+same order (no contraction), so any difference is a defect. The north-star
+tolerances (1e-5 m manifolds, 1e-5 relative velocities) are asserted too, as
+the documented bar.
+"""
+import numpy as np
+import pytest
+
+import mechanic_amd as mh
+from mechanic_amd import _lib
+from tests._oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+INVALID = 0xFFFFFFFF
+
+
+def make(scene, **cfg):
+    iterations = cfg.pop("solver_iterations", 8)
+    friction = cfg.pop("friction", 0.05)
+    collisions = cfg.pop("collisions_enabled", True)
+    pair_capacity = cfg.pop("pair_capacity", _lib.MAX_CONTACT_PAIRS)
+    use_graph = cfg.pop("use_graph", True)
+    assert not cfg
+    gpu = mh.GpuPhysics.new_with_config(scene, mh.GpuPhysicsConfig(
+        collisions_enabled=collisions, solver_iterations=iterations, friction=friction,
+        pair_capacity=pair_capacity, use_graph=use_graph))
+    ora = Oracle(scene, iterations=iterations, friction=friction, pair_capacity=pair_capacity,
+                 collisions=collisions)
+    return gpu, ora
+
+
+def assert_bodies_equal(gpu, ora, what):
+    gp, gv = gpu.read_bodies()
+    op, ov = ora.bodies()
+    for name, g, o in (("poses", gp, op), ("velocities", gv, ov)):
+        diff = np.nonzero(g.view(np.uint32) != o.view(np.uint32))
+        if diff[0].size:
+            i = diff[0][0]
+            raise AssertionError(f"{what}: {name} differ at body {i}: gpu {g[i]} oracle {o[i]} "
+                                 f"({diff[0].size} words)")
+
+
+def gpu_array(gpu, buf, nbytes, dtype, width):
+    raw = gpu.read_buffer(buf, nbytes)
+    return np.frombuffer(raw, dtype=dtype).reshape(-1, width) if width else np.frombuffer(raw, dtype=dtype)
+
+
+def sort_rows(rows):
+    rows = np.asarray(rows)
+    keys = rows.view(np.uint32) if rows.dtype == np.float32 else rows
+    order = np.lexsort(tuple(keys[:, k] for k in range(keys.shape[1] - 1, -1, -1)))
+    return rows[order]
+
+
+def test_integration_only_300_ticks_bit_exact():
+    scene = mh.named_pile("pile_1k")
+    gpu, ora = make(scene, collisions_enabled=False)
+    for tick in range(1, 301):
+        slot = gpu.dispatch_tick(tick)
+        ora.step(tick)
+        if tick % 50 == 0:
+            assert_bodies_equal(gpu, ora, f"tick {tick}")
+            snap = gpu.read_snapshot_transforms(slot)
+            op, _ = ora.bodies()
+            assert np.array_equal(snap.view(np.uint32), op.view(np.uint32))
+    assert gpu.read_last_tick().error_flags == 0
+
+
+def compare_broadphase(gpu, ora, scene):
+    c = len(scene.colliders)
+    s = max(256, 1 << (c - 1).bit_length())
+    keys = gpu_array(gpu, _lib.BUFFER_MORTON_KEYS, s * 8, np.uint64, 0)
+    okeys = ora.keys()
+    assert np.array_equal(keys, okeys), "sorted Morton entries differ"
+    assert np.all(np.diff(keys.astype(np.uint64)) > 0)
+    aabbs = gpu_array(gpu, _lib.BUFFER_COLLIDER_AABBS, c * 32, np.float32, 8)
+    assert np.array_equal(aabbs.view(np.uint32), ora.aabbs().view(np.uint32)), "collider AABBs differ"
+    if c > 1:
+        ch = gpu_array(gpu, _lib.BUFFER_NODE_CHILDREN, (c - 1) * 8, np.uint32, 2)
+        assert np.array_equal(ch, ora.children()), "LBVH children differ"
+        par = gpu_array(gpu, _lib.BUFFER_NODE_PARENTS, (2 * c - 1) * 4, np.uint32, 0)
+        assert np.array_equal(par, ora.parents()), "LBVH parents differ"
+    nodes = gpu_array(gpu, _lib.BUFFER_NODE_AABBS, (2 * c - 1) * 32, np.float32, 8)
+    assert np.array_equal(nodes.view(np.uint32), ora.aabbs(nodes=True).view(np.uint32)), "node AABBs differ"
+    diag = np.frombuffer(gpu.read_buffer(_lib.BUFFER_DIAGNOSTICS, 32), np.uint32)
+    odiag = ora.diag()
+    assert diag[1] == odiag[1], f"pair counts {diag[1]} vs {odiag[1]}"
+    pairs = gpu_array(gpu, _lib.BUFFER_PAIRS, int(diag[1]) * 8, np.uint32, 2)
+    assert np.array_equal(sort_rows(pairs), sort_rows(ora.pairs())), "pair sets differ"
+    return int(diag[1])
+
+
+@pytest.mark.parametrize("name,ticks", [("pile_1k", 1), ("pile_1k", 90)])
+def test_broadphase_bit_exact(name, ticks):
+    scene = mh.named_pile(name)
+    gpu, ora = make(scene, use_graph=False)
+    for tick in range(1, ticks):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+    gpu.debug_tick(ticks, 2)
+    ora.step_partial(ticks, 2)
+    compare_broadphase(gpu, ora, scene)
+
+
+def contact_key(rows):
+    u = rows.view(np.uint32)
+    return u
+
+
+def compare_contacts(gpu, ora):
+    diag = np.frombuffer(gpu.read_buffer(_lib.BUFFER_DIAGNOSTICS, 32), np.uint32)
+    odiag = ora.diag()
+    assert diag[2] == odiag[2], f"contact counts {diag[2]} vs {odiag[2]}"
+    k = int(diag[2])
+    g = gpu_array(gpu, _lib.BUFFER_CONTACTS, k * 64, np.float32, 16)
+    o = ora.contacts()
+    gs, os_ = sort_rows(g.view(np.uint32)), sort_rows(o.view(np.uint32))
+    # Manifold geometry within the 1e-5 m bar, then bit-exact.
+    gf, of = gs.view(np.float32), os_.view(np.float32)
+    assert np.array_equal(gs[:, :4], os_[:, :4]), "contact bodies/colliders differ"
+    assert np.max(np.abs(gf[:, 4:] - of[:, 4:]), initial=0.0) <= 1e-5
+    bad = np.nonzero(np.any(gs != os_, axis=1))[0]
+    assert bad.size == 0, f"contacts differ in bits: gpu {gf[bad[0]]} oracle {of[bad[0]]}"
+    return k
+
+
+@pytest.mark.parametrize("ticks", [1, 45, 120])
+def test_contacts_bit_exact(ticks):
+    scene = mh.named_pile("pile_1k")
+    gpu, ora = make(scene, use_graph=False)
+    for tick in range(1, ticks):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+    gpu.debug_tick(ticks, 3)
+    ora.step_partial(ticks, 3)
+    compare_contacts(gpu, ora)
+
+
+@pytest.mark.parametrize("friction,iterations", [(0.05, 8), (0.6, 8), (0.6, 3), (0.05, 1)])
+def test_full_step_pile_1k_bit_exact(friction, iterations):
+    scene = mh.named_pile("pile_1k")
+    gpu, ora = make(scene, friction=friction, solver_iterations=iterations)
+    for tick in range(1, 301):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+        if tick % 25 == 0 or tick < 4:
+            assert_bodies_equal(gpu, ora, f"tick {tick}")
+            r = gpu.read_last_tick()
+            od = ora.diag()
+            assert (r.error_flags, r.pair_count, r.contact_count, r.active_contact_count) == (
+                od[0], od[1], od[2], od[5]), f"diagnostics differ at tick {tick}"
+    assert gpu.read_last_tick().active_contact_count > 0
+
+
+def test_graph_and_direct_launches_agree_and_repeat():
+    scene = mh.named_pile("pile_1k")
+    a = mh.GpuPhysics.new_with_config(scene, mh.GpuPhysicsConfig(use_graph=True))
+    b = mh.GpuPhysics.new_with_config(scene, mh.GpuPhysicsConfig(use_graph=False))
+    c = mh.GpuPhysics.new_with_config(scene, mh.GpuPhysicsConfig(use_graph=True))
+    for tick in range(1, 121):
+        a.dispatch_tick(tick)
+        b.dispatch_tick(tick)
+        c.dispatch_tick(tick)
+    pa, va = a.read_bodies()
+    pb, vb = b.read_bodies()
+    pc, vc = c.read_bodies()
+    assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+    assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+    assert np.array_equal(pa.view(np.uint32), pc.view(np.uint32))
+    assert np.array_equal(va.view(np.uint32), vc.view(np.uint32))
+
+
+def test_edge_scenes_single_and_static_bodies():
+    # One body: no internal nodes, no pairs; it lands on the ground.
+    one = mh.boxes([[0.0, 0.3, 0.0]], [0.125, 0.125, 0.125])
+    gpu, ora = make(one)
+    for tick in range(1, 61):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+    assert_bodies_equal(gpu, ora, "single body")
+    # A static body under a dynamic one: the static one never moves.
+    two = mh.boxes([[0.0, 0.125, 0.0], [0.02, 0.4, 0.01]], [0.125, 0.125, 0.125],
+                   inverse_mass=[0.0, 0.064], inverse_inertia=[0.0, 6.144])
+    gpu, ora = make(two)
+    for tick in range(1, 91):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+    assert_bodies_equal(gpu, ora, "static + dynamic")
+    p, _ = gpu.read_bodies()
+    assert p[0, 1] == np.float32(0.125)
+
+
+def test_uneven_counts_and_tiny_pair_capacity_flags_overflow():
+    scene = mh.cuboid_pile(7, 9, 5, seed=3, spacing=0.2)  # 315 bodies, overlapping start
+    gpu, ora = make(scene, pair_capacity=256)
+    for tick in range(1, 4):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+    r = gpu.read_last_tick()
+    assert r.error_flags & mh.PAIR_OVERFLOW_FLAG
+    assert r.pair_count == ora.diag()[1]
+
+
+def test_non_finite_state_raises_invalid_numeric():
+    scene = mh.named_pile("pile_1k")
+    gpu, ora = make(scene)
+    gpu.dispatch_tick(1)
+    ora.step(1)
+    p, v = gpu.read_bodies()
+    v[5, 0] = np.nan
+    gpu.write_bodies(p, v)
+    ora.set_bodies(p, v)
+    gpu.dispatch_tick(2)
+    ora.step(2)
+    r = gpu.read_last_tick()
+    assert r.error_flags & mh.INVALID_NUMERIC_FLAG
+    assert ora.diag()[0] & mh.INVALID_NUMERIC_FLAG
+
+
+def test_external_impulse_matches_oracle():
+    scene = mh.boxes([[0.0, 0.125, 0.0]], [0.5, 0.5, 0.5], inverse_mass=0.001, inverse_inertia=0.006)
+    gpu, ora = make(scene)
+    gpu.apply_impulse(0, [0.0, 0.4, 0.5], [500.0, 0.0, 0.0])
+    ora.apply_impulse(0, [0.0, 0.4, 0.5], [500.0, 0.0, 0.0])
+    gpu.dispatch_tick(1)
+    ora.step(1)
+    assert_bodies_equal(gpu, ora, "impulse")
+    p, v = gpu.read_bodies()
+    # device.rs:4144-4176: the body moves along +x and turns about -z... here about -y/z sign of the arm.
+    assert v[0, 0] > 0.0
+    with pytest.raises(mh.GpuImpulseError):
+        gpu.apply_impulse(3, [0, 0, 0], [1, 0, 0])
+    with pytest.raises(mh.GpuImpulseError):
+        gpu.apply_impulse(0, [0, np.inf, 0], [1, 0, 0])
+
+
+def test_pile_100k_first_tick_broadphase_and_contacts():
+    scene = mh.named_pile("pile_100k")
+    gpu, ora = make(scene, use_graph=False)
+    ora.set_threads(8)
+    gpu.debug_tick(1, 3)
+    ora.step_partial(1, 3)
+    compare_broadphase(gpu, ora, scene)
+    compare_contacts(gpu, ora)

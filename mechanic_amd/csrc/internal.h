@@ -13,7 +13,13 @@ namespace mh {
 constexpr uint32_t kInvalid = 0xffffffffu;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kPosMask = 0x7fffffffu;
+constexpr uint32_t kBucketBit = 0x40000000u;  // internal node whose leaves share one Morton code
 constexpr uint32_t kBlock = 256;
+// Contact sides a body can gather without atomics per tick; further sides
+// accumulate through the atomic overflow path.
+constexpr uint32_t kMaxSlots = 32;
+constexpr uint32_t kOverflowA = 0x40000000u;  // solver row flag bits in the contact-index word
+constexpr uint32_t kOverflowB = 0x80000000u;
 
 // Diagnostics words (abi.rs:56-73) plus private counters after them.
 enum DiagWord : uint32_t {
@@ -70,6 +76,8 @@ struct Dev {
   float4* inodes;         // 2 per internal node: (min, left child) (max, escape)
   float4* leaves;         // 2 per leaf: (min, collider) (max, escape)
   uint2* children;        // reference encoding of node_children (lbvh.wgsl:275)
+  uint2* ranges;          // sorted leaf range (first, last) per internal node
+  uint32_t* leaf_body;    // body of the collider at each sorted leaf
   uint32_t* parents;      // reference encoding of node_parents (parent + 1)
   uint32_t* visits;       // refit arrival counters
   uint2* pairs;
@@ -78,7 +86,10 @@ struct Dev {
   float4* contacts;       // 4 per contact, mh_contact layout
   uint32_t* active;       // contact index per active contact
   float4* rows;           // 4 per active contact: solver rows
-  int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas
+  int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
+  int4* cdelta;           // 2 int4 per contact side: that side's Q20 deltas of the last pass
+  uint32_t* slot_count;   // contact sides per body this tick
+  uint32_t* incid;        // kMaxSlots x body_count (slot-major): contact side = 2 * active + side
 
   // ---- persistent manifolds (ping-pong by tick parity) ----
   uint64_t* htab[2];      // open-addressing index: key -> manifold row

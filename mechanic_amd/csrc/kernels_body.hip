@@ -92,6 +92,7 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d, int collisions, int sn
   }
   (void)stored;
   if (collisions) {
+    d.slot_count[i] = 0u;
     const float4* m = d.masses + 4 * (size_t)i;
     v3 cx = world_inverse_inertia(rotation, m, mk(1.0f, 0.0f, 0.0f));
     v3 cy = world_inverse_inertia(rotation, m, mk(0.0f, 1.0f, 0.0f));

@@ -198,10 +198,13 @@ __global__ __launch_bounds__(256) void k_sort_global(uint64_t* keys, uint32_t co
 // ---------------------------------------------------------------------------
 // LBVH (Karras 2012) over the sorted keys, lbvh.wgsl:219-319.
 // Internal node n covers a contiguous range of sorted leaves; its record holds
-// (min, left child) and (max, escape). A child or escape is encoded as a sorted
+// (min, left word) and (max, escape). A child or escape is encoded as a sorted
 // position with kLeafBit set for leaves; the escape of a node whose range ends
 // at b is the node right after its subtree in depth-first order (the right
 // child of the node split at b), or the sentinel C after the last leaf.
+// A node whose leaves all share one Morton code is a bucket: its left word is
+// its first leaf with kBucketBit, and traversal scans its leaves directly
+// instead of descending the subtree the reference splits by collider index.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int prefix(const uint64_t* keys, int c, int a, int b) {
   if (b < 0 || b >= c) return -1;
@@ -249,18 +252,22 @@ __global__ __launch_bounds__(256) void k_topology(Dev d) {
   const uint32_t left_ref = split == first ? leaf_base + (uint32_t)split : (uint32_t)split;
   const uint32_t right_ref = split + 1 == last ? leaf_base + (uint32_t)(split + 1) : (uint32_t)(split + 1);
   d.children[node] = make_uint2(left_ref, right_ref);
+  d.ranges[node] = make_uint2((uint32_t)first, (uint32_t)last);
   d.parents[left_ref] = node + 1u;
   d.parents[right_ref] = node + 1u;
   if (node == 0) d.parents[0] = 0u;
   d.visits[node] = 0u;
+  const bool bucket = (keys[first] >> 32) == (keys[last] >> 32);
   const uint32_t left_enc = split == first ? ((uint32_t)split | kLeafBit) : (uint32_t)split;
-  d.inodes[2 * (size_t)node].w = __uint_as_float(left_enc);
+  const uint32_t left_word = bucket ? ((uint32_t)first | kBucketBit) : left_enc;
+  d.inodes[2 * (size_t)node].w = __uint_as_float(left_word);
   d.inodes[2 * (size_t)node + 1].w = __uint_as_float(escape_after(keys, c, last));
 }
 
-// prepare_leaves + build_bounds (lbvh.wgsl:280-319). The release/acquire pair
-// around the arrival counter publishes a child's box to the thread (in any
-// workgroup, on any XCD) that merges it.
+// prepare_leaves + build_bounds (lbvh.wgsl:280-319). The second thread to reach
+// a node merges its children's boxes. When the node's leaves all belong to this
+// workgroup, both arrivals come from it and a workgroup-scope release/acquire
+// suffices; above that, agent-scope fences publish boxes across CUs and XCDs.
 __global__ __launch_bounds__(256) void k_refit(Dev d) {
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
@@ -272,7 +279,10 @@ __global__ __launch_bounds__(256) void k_refit(Dev d) {
   float4 mx = d.aabbs[2 * (size_t)collider + 1];
   d.leaves[2 * (size_t)leaf] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(collider));
   d.leaves[2 * (size_t)leaf + 1] = make_float4(mx.x, mx.y, mx.z, __uint_as_float(escape_after(d.keys, c, (int)leaf)));
+  d.leaf_body[leaf] = __float_as_uint(d.colliders[4 * (size_t)collider + 3].x);
   if (c <= 1) return;
+  const uint32_t block_first = blockIdx.x * blockDim.x;
+  const uint32_t block_last = min(block_first + blockDim.x, (uint32_t)c) - 1u;
   const uint32_t leaf_base = (uint32_t)c - 1u;
   uint32_t current = leaf_base + leaf;
   // A Karras tree over unique 64-bit keys is at most 64 levels deep.
@@ -280,14 +290,23 @@ __global__ __launch_bounds__(256) void k_refit(Dev d) {
     uint32_t encoded = d.parents[current];
     if (encoded == 0u) return;
     uint32_t parent = encoded - 1u;
-    __threadfence();
-    uint32_t arrival = atomicAdd(&d.visits[parent], 1u);
+    const uint2 range = d.ranges[parent];
+    const bool local = range.x >= block_first && range.y <= block_last;
+    uint32_t arrival;
+    if (local) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      arrival = __hip_atomic_fetch_add(&d.visits[parent], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+      __threadfence();
+      arrival = atomicAdd(&d.visits[parent], 1u);
+      __threadfence();
+    }
     if (arrival == 0u) return;
     if (arrival > 1u) {
       atomicOr(&diag[kDiagFlags], 1u);
       return;
     }
-    __threadfence();
     uint2 ch = d.children[parent];
     const float4* a = ch.x >= leaf_base ? d.leaves + 2 * (size_t)(ch.x - leaf_base) : d.inodes + 2 * (size_t)ch.x;
     const float4* b = ch.y >= leaf_base ? d.leaves + 2 * (size_t)(ch.y - leaf_base) : d.inodes + 2 * (size_t)ch.y;
@@ -301,11 +320,16 @@ __global__ __launch_bounds__(256) void k_refit(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
-// Pair traversal (lbvh.wgsl:321-398). One thread per sorted leaf p walks the
-// tree without a stack, following escapes, and only visits leaves right of p:
-// each unordered overlapping pair is found once and written as
-// (lower collider, higher collider), the set the reference's per-leaf stack
-// traversal emits.
+// Pair traversal (lbvh.wgsl:321-398). A wavefront walks the tree as one packet
+// of 64 consecutive sorted leaves (queries), without a stack, following escapes:
+// it descends into a node when any of its queries overlaps the node and the node
+// holds leaves right of that query. At a bucket it scans the bucket's leaves,
+// 64 at a time, each lane testing its query against every candidate broadcast
+// from the lane that loaded it. A lane only pairs with leaves right of its own,
+// so each unordered overlapping pair is found once and written as
+// (lower collider, higher collider): the set the reference's per-leaf stack
+// traversal emits. Pairs are staged in LDS per wavefront and flushed with one
+// global atomic per batch.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool suppressed(const Dev& d, uint32_t body_a, uint32_t body_b) {
   uint32_t lo = body_a < body_b ? body_a : body_b;
@@ -324,58 +348,122 @@ __device__ __forceinline__ bool overlaps(float4 qmn, float4 qmx, float4 mn, floa
   return qmn.x <= mx.x && qmn.y <= mx.y && qmn.z <= mx.z && mn.x <= qmx.x && mn.y <= qmx.y && mn.z <= qmx.z;
 }
 
-__device__ __forceinline__ uint32_t collider_body(const Dev& d, uint32_t c) {
-  return __float_as_uint(d.colliders[4 * (size_t)c + 3].x);
+__device__ __forceinline__ float bcast(float v, uint32_t lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+__device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+constexpr uint32_t kStage = 512;  // staged pairs per wavefront
+
+struct PairStage {
+  uint2* buf;
+  uint32_t count;  // wave-uniform
+};
+
+__device__ __forceinline__ void flush_pairs(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&diag[kDiagPairs], st.count);
+  base = bcast(base, 0u);
+  for (uint32_t i = lane; i < st.count; i += 64u) {
+    const uint32_t row = base + i;
+    if (row < d.pair_capacity) d.pairs[row] = st.buf[i];
+  }
+  if (lane == 0 && base + st.count > d.pair_capacity) atomicOr(&diag[kDiagFlags], 1u);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+  st.count = 0;
 }
 
 __global__ __launch_bounds__(256) void k_traverse(Dev d) {
+  __shared__ uint2 stage_mem[4][kStage];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t c = d.collider_count;
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= c) return;
-  const float4 qmn = d.leaves[2 * (size_t)p];
-  const float4 qmx = d.leaves[2 * (size_t)p + 1];
-  const uint32_t self = __float_as_uint(qmn.w);
-  const uint32_t self_body = collider_body(d, self);
-  uint32_t cur = c > 1 ? 0u : (0u | kLeafBit);
-  // Every step moves forward in depth-first order, so 2C-1 steps visit every
-  // node once; the bound turns a corrupted tree into a flag instead of a hang.
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave_first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  if (wave_first >= c) return;  // whole wavefront past the last leaf
+  const uint32_t p = wave_first + lane;
+  const bool valid = p < c;
+  float4 qmn = make_float4(0.f, 0.f, 0.f, 0.f), qmx = qmn;
+  uint32_t self = 0, self_body = 0;
+  if (valid) {
+    qmn = d.leaves[2 * (size_t)p];
+    qmx = d.leaves[2 * (size_t)p + 1];
+    self = __float_as_uint(qmn.w);
+    self_body = d.leaf_body[p];
+  }
+  PairStage st{stage_mem[threadIdx.x >> 6], 0u};
+  const bool check_suppression = d.suppression_count != 0;
+
+  uint32_t cur = 0u;  // the root (c >= 2)
   uint32_t steps_left = 2u * c + 1u;
   while (cur != c) {
     if (steps_left-- == 0u) {
-      atomicOr(&diag[kDiagFlags], 1u);
+      if (lane == 0) atomicOr(&diag[kDiagFlags], 1u);
       break;
     }
     const uint32_t pos = cur & kPosMask;
-    bool emit = false;
-    uint32_t other = 0;
+    uint32_t first, last, next;
     if (cur & kLeafBit) {
-      const float4 mn = d.leaves[2 * (size_t)pos];
-      const float4 mx = d.leaves[2 * (size_t)pos + 1];
-      if (pos > p && overlaps(qmn, qmx, mn, mx)) {
-        other = __float_as_uint(mn.w);
-        uint32_t other_body = collider_body(d, other);
-        emit = other_body != self_body && !(d.suppression_count != 0 && suppressed(d, self_body, other_body));
-      }
-      cur = __float_as_uint(mx.w);
+      first = last = pos;
+      next = __float_as_uint(d.leaves[2 * (size_t)pos + 1].w);
     } else {
       const float4 mn = d.inodes[2 * (size_t)pos];
       const float4 mx = d.inodes[2 * (size_t)pos + 1];
+      const uint32_t left_word = __float_as_uint(mn.w);
       const uint32_t esc = __float_as_uint(mx.w);
-      if ((esc & kPosMask) <= p + 1u) cur = esc;
-      else if (overlaps(qmn, qmx, mn, mx)) cur = __float_as_uint(mn.w);
-      else cur = esc;
+      const uint32_t range_last = (esc & kPosMask) - 1u;
+      const bool want = valid && range_last > p && overlaps(qmn, qmx, mn, mx);
+      if (__ballot(want) == 0ull) {
+        cur = esc;
+        continue;
+      }
+      if (!(left_word & kBucketBit)) {
+        cur = left_word;
+        continue;
+      }
+      first = left_word & (kBucketBit - 1u);
+      last = range_last;
+      next = esc;
     }
-    uint32_t row = wave_append(&diag[kDiagPairs], emit);
-    if (emit) {
-      if (row < d.pair_capacity) {
-        d.pairs[row] = make_uint2(self < other ? self : other, self < other ? other : self);
-      } else {
-        atomicOr(&diag[kDiagFlags], 1u);
+    // Scan leaves [max(first, wave_first + 1), last] against the packet.
+    const uint32_t start = first > wave_first ? first : wave_first + 1u;
+    for (uint32_t base = start; base <= last; base += 64u) {
+      const uint32_t q = base + lane;
+      float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
+      uint32_t cbody = 0;
+      if (q <= last) {
+        cmn = d.leaves[2 * (size_t)q];
+        cmx = d.leaves[2 * (size_t)q + 1];
+        cbody = d.leaf_body[q];
+      }
+      const uint32_t n = last - base + 1u < 64u ? last - base + 1u : 64u;
+      for (uint32_t j = 0; j < n; ++j) {
+        const float4 bmn = make_float4(bcast(cmn.x, j), bcast(cmn.y, j), bcast(cmn.z, j), 0.0f);
+        const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
+        const uint32_t other = bcast(__float_as_uint(cmn.w), j);
+        const uint32_t other_body = bcast(cbody, j);
+        bool hit = valid && base + j > p && overlaps(qmn, qmx, bmn, bmx) && other_body != self_body;
+        if (hit && check_suppression) hit = !suppressed(d, self_body, other_body);
+        const uint64_t mask = __ballot(hit);
+        if (mask != 0ull) {
+          const uint32_t k = (uint32_t)__popcll(mask);
+          if (st.count + k > kStage) flush_pairs(d, diag, st, lane);
+          if (hit) {
+            const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            st.buf[st.count + rank] = make_uint2(self < other ? self : other, self < other ? other : self);
+          }
+          st.count += k;
+        }
       }
     }
+    cur = next;
   }
+  if (st.count) flush_pairs(d, diag, st, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -410,6 +498,7 @@ void launch_lbvh(hipStream_t s, const Dev& d) {
 }
 
 void launch_traverse(hipStream_t s, const Dev& d) {
+  if (d.collider_count < 2) return;
   k_traverse<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
 }
 

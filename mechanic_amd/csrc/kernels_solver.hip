@@ -164,25 +164,56 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   done:
     uint32_t a = wave_append(&diag[kDiagActive], active);
     if (active) {
-      d.active[a] = __float_as_uint(r3.z);
+      const uint32_t ci = __float_as_uint(r3.z);
+      const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+      uint32_t flags = 0;
+      // Register both contact sides with their bodies for the gather.
+      uint32_t sa = atomicAdd(&d.slot_count[body_a], 1u);
+      if (sa < kMaxSlots) d.incid[(size_t)sa * d.body_count + body_a] = 2u * a;
+      else flags |= kOverflowA;
+      if (body_b != kInvalid) {
+        uint32_t sb = atomicAdd(&d.slot_count[body_b], 1u);
+        if (sb < kMaxSlots) d.incid[(size_t)sb * d.body_count + body_b] = 2u * a + 1u;
+        else flags |= kOverflowB;
+      }
+      d.active[a] = ci;
       float4* row = d.rows + 4 * (size_t)a;
       row[0] = r0;
       row[1] = r1;
       row[2] = r2;
-      row[3] = r3;
+      row[3] = make_float4(r3.x, r3.y, __uint_as_float(ci | flags), r3.w);
     }
   }
 }
 
-// add_velocity_delta (collision.wgsl:764-772).
-__device__ __forceinline__ void add_delta(const Dev& d, uint32_t body, v3 linear, v3 angular) {
-  int32_t* p = d.deltas + 6 * (size_t)body;
-  atomicAdd(&p[0], round_to_i32(linear.x * kFixedScale));
-  atomicAdd(&p[1], round_to_i32(linear.y * kFixedScale));
-  atomicAdd(&p[2], round_to_i32(linear.z * kFixedScale));
-  atomicAdd(&p[3], round_to_i32(angular.x * kFixedScale));
-  atomicAdd(&p[4], round_to_i32(angular.y * kFixedScale));
-  atomicAdd(&p[5], round_to_i32(angular.z * kFixedScale));
+// The Q20 deltas of add_velocity_delta (collision.wgsl:764-772). Instead of six
+// atomics per body, a contact side stores its six integers and the body's
+// gather sums them; integer sums do not depend on order, so the result is the
+// same as the atomic accumulation. Sides past kMaxSlots use the atomics.
+__device__ __forceinline__ void put_side(const Dev& d, uint32_t side, bool overflow, uint32_t body, v3 linear,
+                                         v3 angular) {
+  const int32_t l0 = round_to_i32(linear.x * kFixedScale), l1 = round_to_i32(linear.y * kFixedScale),
+                l2 = round_to_i32(linear.z * kFixedScale), a0 = round_to_i32(angular.x * kFixedScale),
+                a1 = round_to_i32(angular.y * kFixedScale), a2 = round_to_i32(angular.z * kFixedScale);
+  if (!overflow) {
+    d.cdelta[2 * (size_t)side] = make_int4(l0, l1, l2, a0);
+    d.cdelta[2 * (size_t)side + 1] = make_int4(a1, a2, 0, 0);
+  } else {
+    int32_t* p = d.deltas + 6 * (size_t)body;
+    atomicAdd(&p[0], l0);
+    atomicAdd(&p[1], l1);
+    atomicAdd(&p[2], l2);
+    atomicAdd(&p[3], a0);
+    atomicAdd(&p[4], a1);
+    atomicAdd(&p[5], a2);
+  }
+}
+
+__device__ __forceinline__ void zero_side(const Dev& d, uint32_t side, bool overflow) {
+  if (!overflow) {
+    d.cdelta[2 * (size_t)side] = make_int4(0, 0, 0, 0);
+    d.cdelta[2 * (size_t)side + 1] = make_int4(0, 0, 0, 0);
+  }
 }
 
 // warm_start (collision.wgsl:774-839) when first, solve_accumulate
@@ -198,8 +229,11 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
     float4* row = d.rows + 4 * (size_t)ai;
     const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
     const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+    const uint32_t word = __float_as_uint(r3.z);
+    const bool over_a = (word & kOverflowA) != 0u, over_b = (word & kOverflowB) != 0u;
     const bool has_b = body_b != kInvalid;
     const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
+    bool wrote = false;
     const float penetration = r0.w;
     const bool analytic = r2.w < 0.0f;
     const float denominator = r3.w;
@@ -242,13 +276,18 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
           impulse = impulse - tangent * friction_impulse;
         }
         const v3 neg = -impulse;
-        add_delta(d, body_a, neg * wa[0].w, inverse_inertia(wa, cross(arm_a, neg)));
-        if (has_b) add_delta(d, body_b, impulse * wb[0].w, inverse_inertia(wb, cross(arm_b, impulse)));
+        put_side(d, 2u * ai, over_a, body_a, neg * wa[0].w, inverse_inertia(wa, cross(arm_a, neg)));
+        if (has_b) put_side(d, 2u * ai + 1u, over_b, body_b, impulse * wb[0].w, inverse_inertia(wb, cross(arm_b, impulse)));
+        wrote = true;
       }
+    }
+    if (!wrote) {
+      zero_side(d, 2u * ai, over_a);
+      if (has_b) zero_side(d, 2u * ai + 1u, over_b);
     }
     row[1].w = lambda;
     if (last) {
-      const uint32_t ci = __float_as_uint(r3.z);
+      const uint32_t ci = word & ~(kOverflowA | kOverflowB);
       d.contacts[4 * (size_t)ci + 2].w = lambda;
       rows_now[3 * (size_t)ci + 2].w = lambda;
     }
@@ -263,15 +302,33 @@ __global__ __launch_bounds__(256) void k_apply(Dev d) {
   if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= d.body_count) return;
-  int32_t* p = d.deltas + 6 * (size_t)b;
-  const int2 d01 = *reinterpret_cast<int2*>(p);
-  const int2 d23 = *reinterpret_cast<int2*>(p + 2);
-  const int2 d45 = *reinterpret_cast<int2*>(p + 4);
-  *reinterpret_cast<int2*>(p) = make_int2(0, 0);
-  *reinterpret_cast<int2*>(p + 2) = make_int2(0, 0);
-  *reinterpret_cast<int2*>(p + 4) = make_int2(0, 0);
-  const v3 lin_delta = mk((float)d01.x, (float)d01.y, (float)d23.x) / kFixedScale;
-  const v3 ang_delta = mk((float)d23.y, (float)d45.x, (float)d45.y) / kFixedScale;
+  // Wrapping 32-bit sums, the arithmetic of atomicAdd on i32.
+  uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+  const uint32_t count = d.slot_count[b];
+  const uint32_t n = count < kMaxSlots ? count : kMaxSlots;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t side = d.incid[(size_t)k * d.body_count + b];
+    const int4 x = d.cdelta[2 * (size_t)side];
+    const int4 y = d.cdelta[2 * (size_t)side + 1];
+    s0 += (uint32_t)x.x;
+    s1 += (uint32_t)x.y;
+    s2 += (uint32_t)x.z;
+    s3 += (uint32_t)x.w;
+    s4 += (uint32_t)y.x;
+    s5 += (uint32_t)y.y;
+  }
+  if (count > kMaxSlots) {
+    int32_t* p = d.deltas + 6 * (size_t)b;
+    s0 += (uint32_t)p[0];
+    s1 += (uint32_t)p[1];
+    s2 += (uint32_t)p[2];
+    s3 += (uint32_t)p[3];
+    s4 += (uint32_t)p[4];
+    s5 += (uint32_t)p[5];
+    for (int k = 0; k < 6; ++k) p[k] = 0;
+  }
+  const v3 lin_delta = mk((float)(int32_t)s0, (float)(int32_t)s1, (float)(int32_t)s2) / kFixedScale;
+  const v3 ang_delta = mk((float)(int32_t)s3, (float)(int32_t)s4, (float)(int32_t)s5) / kFixedScale;
   d.lin[b] = f4(xyz(d.lin[b]) + lin_delta, 0.0f);
   d.ang[b] = f4(xyz(d.ang[b]) + ang_delta, 0.0f);
 }

@@ -276,9 +276,12 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *inodes, *leaves, *contacts, *rows;
   float* inv_mass;
   uint32_t *roots, *comps, *parents, *visits, *active, *diag;
-  uint2 *supp, *children, *pairs;
+  uint2 *supp, *children, *pairs, *ranges;
+  uint32_t* leaf_body;
   uint64_t *keys, *h0, *h1;
   int32_t* deltas;
+  int4* cdelta;
+  uint32_t *slot_count, *incid;
   float4 *m0, *m1;
   TickState* tick;
   ALLOC(pos, n); ALLOC(rot, n); ALLOC(lin, n); ALLOC(ang, n);
@@ -288,12 +291,16 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   ALLOC(aabbs, 2 * (size_t)c); ALLOC(keys, d.sort_count);
   ALLOC(inodes, 2 * (size_t)(c > 1 ? c - 1 : 1)); ALLOC(leaves, 2 * (size_t)c);
   ALLOC(children, c > 1 ? c - 1 : 1); ALLOC(parents, c > 0 ? 2 * (size_t)c - 1 : 1);
+  ALLOC(ranges, c > 1 ? c - 1 : 1); ALLOC(leaf_body, c);
   ALLOC(visits, c > 1 ? c - 1 : 1);
   ALLOC(pairs, cfg.pair_capacity);
   ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
   ALLOC(active, cfg.pair_capacity);
   ALLOC(rows, 4 * (size_t)cfg.pair_capacity);
   ALLOC(deltas, 6 * (size_t)n);
+  ALLOC(cdelta, 4 * (size_t)cfg.pair_capacity);
+  ALLOC(slot_count, n);
+  ALLOC(incid, (size_t)kMaxSlots * n);
   ALLOC(h0, cfg.pair_capacity); ALLOC(h1, cfg.pair_capacity);
   ALLOC(m0, 3 * (size_t)cfg.pair_capacity); ALLOC(m1, 3 * (size_t)cfg.pair_capacity);
   for (int s = 0; s < 3; ++s) {
@@ -308,7 +315,9 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.colliders = cols; d.suppressed = supp;
   d.aabbs = aabbs; d.keys = keys; d.inodes = inodes; d.leaves = leaves;
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
+  d.ranges = ranges; d.leaf_body = leaf_body;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
+  d.cdelta = cdelta; d.slot_count = slot_count; d.incid = incid;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;
   d.diag[0] = diag; d.diag[1] = diag + kDiagWords;
   d.tick = tick;

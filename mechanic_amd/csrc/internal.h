@@ -72,6 +72,7 @@ struct Dev {
 
   // ---- broadphase ----
   float4* aabbs;          // 2 per collider: min, max
+  float4* cposes;         // 5 per collider: (centre, hx) (rotation) (axis x, hy) (axis y, hz) (axis z, body)
   uint64_t* keys;         // sort_count Morton entries (code << 32 | collider)
   float4* inodes;         // 2 per internal node: (min, left child) (max, escape)
   float4* leaves;         // 2 per leaf: (min, collider) (max, escape)

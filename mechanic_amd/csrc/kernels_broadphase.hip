@@ -40,6 +40,7 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
     return;
   }
   ColliderPose p = collider_pose(d, i);
+  store_pose(d, i, p);
   v3 we = (vabs(p.axes[0]) * p.half.x + vabs(p.axes[1]) * p.half.y) + vabs(p.axes[2]) * p.half.z;
   d.aabbs[2 * (size_t)i] = f4(p.center - we, 0.0f);
   d.aabbs[2 * (size_t)i + 1] = f4(p.center + we, 0.0f);

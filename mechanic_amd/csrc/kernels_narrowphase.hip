@@ -115,8 +115,8 @@ __global__ __launch_bounds__(256) void k_narrowphase(Dev d) {
     float4 n_pen, arm_a, arm_b;
     if (index < count) {
       const uint2 pair = d.pairs[index];
-      const ColliderPose a = collider_pose(d, pair.x);
-      const ColliderPose b = collider_pose(d, pair.y);
+      const ColliderPose a = load_pose(d, pair.x);
+      const ColliderPose b = load_pose(d, pair.y);
       bool skip = !d.self_collisions && d.components[a.body] == d.components[b.body];
       if (!skip) {
         Sat sat = obb_sat(a, b);

@@ -42,6 +42,34 @@ __device__ __forceinline__ ColliderPose collider_pose(const Dev& d, uint32_t c) 
   return p;
 }
 
+// The world pose the Morton kernel stored for collider c this tick; the same
+// bits collider_pose() computes (cylinder ground data excluded).
+__device__ __forceinline__ void store_pose(const Dev& d, uint32_t c, const ColliderPose& p) {
+  float4* o = d.cposes + 5 * (size_t)c;
+  o[0] = f4(p.center, p.half.x);
+  o[1] = p.rotation;
+  o[2] = f4(p.axes[0], p.half.y);
+  o[3] = f4(p.axes[1], p.half.z);
+  o[4] = f4(p.axes[2], __uint_as_float(p.body));
+}
+
+__device__ __forceinline__ ColliderPose load_pose(const Dev& d, uint32_t c) {
+  const float4* o = d.cposes + 5 * (size_t)c;
+  const float4 a = o[0], b = o[1], x = o[2], y = o[3], z = o[4];
+  ColliderPose p;
+  p.center = xyz(a);
+  p.rotation = b;
+  p.axes[0] = xyz(x);
+  p.axes[1] = xyz(y);
+  p.axes[2] = xyz(z);
+  p.half = mk(a.w, x.w, y.w);
+  p.half_w = 0.0f;
+  p.center_w = 0.0f;
+  p.body = __float_as_uint(z.w);
+  p.role = 0u;
+  return p;
+}
+
 __device__ __forceinline__ float comp(v3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
 // collider_support_point (collision.wgsl:175-191).

@@ -273,7 +273,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.angular_damping = 0.98f;       // device.rs:874
   d.friction = cfg.friction;
 
-  float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *inodes, *leaves, *contacts, *rows;
+  float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
   float* inv_mass;
   uint32_t *roots, *comps, *parents, *visits, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
@@ -288,7 +288,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   ALLOC(inv_mass, n); ALLOC(masses, 4 * (size_t)n); ALLOC(wm, 3 * (size_t)n);
   ALLOC(roots, n); ALLOC(comps, n);
   ALLOC(cols, 4 * (size_t)c); ALLOC(supp, creation->suppression_count);
-  ALLOC(aabbs, 2 * (size_t)c); ALLOC(keys, d.sort_count);
+  ALLOC(aabbs, 2 * (size_t)c); ALLOC(cposes, 5 * (size_t)c); ALLOC(keys, d.sort_count);
   ALLOC(inodes, 2 * (size_t)(c > 1 ? c - 1 : 1)); ALLOC(leaves, 2 * (size_t)c);
   ALLOC(children, c > 1 ? c - 1 : 1); ALLOC(parents, c > 0 ? 2 * (size_t)c - 1 : 1);
   ALLOC(ranges, c > 1 ? c - 1 : 1); ALLOC(leaf_body, c);
@@ -313,7 +313,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.inv_mass = inv_mass; d.masses = masses; d.world_masses = wm;
   d.root_flags = roots; d.components = comps;
   d.colliders = cols; d.suppressed = supp;
-  d.aabbs = aabbs; d.keys = keys; d.inodes = inodes; d.leaves = leaves;
+  d.aabbs = aabbs; d.cposes = cposes; d.keys = keys; d.inodes = inodes; d.leaves = leaves;
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
   d.ranges = ranges; d.leaf_body = leaf_body;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;

@@ -32,6 +32,13 @@ enum DiagWord : uint32_t {
   kDiagWords = 8,
 };
 
+// Private per-tick counters (cleared by the topology kernel).
+enum CounterWord : uint32_t {
+  kCountFrontier = 0,
+  kCountWork = 1,
+  kCounterWords = 16,
+};
+
 // Tick state kept on the device so a captured tick can be replayed unchanged.
 struct TickState {
   uint64_t tick_index;  // tick being run; its low 32 bits are tick_index of abi.rs:21
@@ -79,6 +86,10 @@ struct Dev {
   uint2* children;        // reference encoding of node_children (lbvh.wgsl:275)
   uint2* ranges;          // sorted leaf range (first, last) per internal node
   uint32_t* leaf_body;    // body of the collider at each sorted leaf
+  uint32_t* frontier;     // nodes completed inside a workgroup whose parent is not (refit)
+  uint4* work;            // traversal work items: (packet first leaf, first, last, 0)
+  uint32_t work_capacity;
+  uint32_t* counters;     // per-tick private counters, see CounterWord
   uint32_t* parents;      // reference encoding of node_parents (parent + 1)
   uint32_t* visits;       // refit arrival counters
   uint2* pairs;

@@ -74,14 +74,15 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
 // Bitonic sort of sort_count u64 keys, ascending. Keys are unique, so the result
 // equals the reference's order (lbvh.wgsl:100-106, 132-217).
 //
-// A tile of kTile keys lives in one workgroup: 8 keys per thread in registers
+// A tile of up to kTile keys lives in one workgroup (1024 threads, 64 KiB of
+// LDS): 8 keys per thread in registers
 // (blocked: thread t holds tile positions 8t..8t+7). Exchanges with j < 8 stay
 // in registers, 8 <= j < 512 go through wavefront shuffles (partner lane
 // t ^ j/8 is in the same 64-lane wavefront), and j >= 512 through LDS.
 // Distances of a tile or more are global multi-level butterfly kernels.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kPer = 8;
-constexpr uint32_t kTile = 2048;
+constexpr uint32_t kTile = 8192;
 
 __device__ __forceinline__ void cas(uint64_t& a, uint64_t& b, bool ascending) {
   bool swap = ascending ? (a > b) : (a < b);
@@ -141,7 +142,7 @@ __device__ __forceinline__ void tile_level(uint64_t (&r)[kPer], uint64_t* lds, u
 
 // Full bitonic sort of each tile (stages k = 2 .. tile), alternating directions by
 // the global position so neighbouring tiles form bitonic sequences.
-__global__ __launch_bounds__(256) void k_sort_tile(uint64_t* keys, uint32_t tile) {
+__global__ __launch_bounds__(1024) void k_sort_tile(uint64_t* keys, uint32_t tile) {
   __shared__ uint64_t lds[kTile];
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(256) void k_sort_tile(uint64_t* keys, uint32_t tile
 }
 
 // Finishing levels j = tile/2 .. 1 of stage k inside each tile.
-__global__ __launch_bounds__(256) void k_sort_merge(uint64_t* keys, uint32_t tile, uint32_t k) {
+__global__ __launch_bounds__(1024) void k_sort_merge(uint64_t* keys, uint32_t tile, uint32_t k) {
   __shared__ uint64_t lds[kTile];
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
@@ -257,6 +258,7 @@ __global__ __launch_bounds__(256) void k_topology(Dev d) {
   d.parents[left_ref] = node + 1u;
   d.parents[right_ref] = node + 1u;
   if (node == 0) d.parents[0] = 0u;
+  if (node < kCounterWords) d.counters[node] = 0u;
   d.visits[node] = 0u;
   const bool bucket = (keys[first] >> 32) == (keys[last] >> 32);
   const uint32_t left_enc = split == first ? ((uint32_t)split | kLeafBit) : (uint32_t)split;
@@ -265,10 +267,31 @@ __global__ __launch_bounds__(256) void k_topology(Dev d) {
   d.inodes[2 * (size_t)node + 1].w = __uint_as_float(escape_after(keys, c, last));
 }
 
-// prepare_leaves + build_bounds (lbvh.wgsl:280-319). The second thread to reach
-// a node merges its children's boxes. When the node's leaves all belong to this
-// workgroup, both arrivals come from it and a workgroup-scope release/acquire
-// suffices; above that, agent-scope fences publish boxes across CUs and XCDs.
+// prepare_leaves + build_bounds (lbvh.wgsl:280-319), in two launches. The
+// second thread to reach a node merges its children's boxes. k_refit climbs
+// while the node's leaves lie inside this workgroup, so both arrivals come from
+// it and workgroup-scope release/acquire orders the boxes; a node completed
+// there whose parent spans workgroups goes on the frontier list. k_refit_top
+// then finishes the tree above the frontier in a single workgroup, again with
+// workgroup-scope ordering only (the launch boundary publishes k_refit's boxes).
+__device__ __forceinline__ void merge_children(const Dev& d, uint32_t parent, uint32_t leaf_base) {
+  uint2 ch = d.children[parent];
+  const float4* a = ch.x >= leaf_base ? d.leaves + 2 * (size_t)(ch.x - leaf_base) : d.inodes + 2 * (size_t)ch.x;
+  const float4* b = ch.y >= leaf_base ? d.leaves + 2 * (size_t)(ch.y - leaf_base) : d.inodes + 2 * (size_t)ch.y;
+  float4 amn = a[0], amx = a[1], bmn = b[0], bmx = b[1];
+  float4* p = d.inodes + 2 * (size_t)parent;
+  float lw = p[0].w, ew = p[1].w;
+  p[0] = make_float4(fmn(amn.x, bmn.x), fmn(amn.y, bmn.y), fmn(amn.z, bmn.z), lw);
+  p[1] = make_float4(fmx(amx.x, bmx.x), fmx(amx.y, bmx.y), fmx(amx.z, bmx.z), ew);
+}
+
+__device__ __forceinline__ uint32_t arrive_workgroup(uint32_t* counter) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  uint32_t arrival = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return arrival;
+}
+
 __global__ __launch_bounds__(256) void k_refit(Dev d) {
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
@@ -288,35 +311,48 @@ __global__ __launch_bounds__(256) void k_refit(Dev d) {
   uint32_t current = leaf_base + leaf;
   // A Karras tree over unique 64-bit keys is at most 64 levels deep.
   for (int level = 0; level < 128; ++level) {
-    uint32_t encoded = d.parents[current];
+    const uint32_t encoded = d.parents[current];
     if (encoded == 0u) return;
-    uint32_t parent = encoded - 1u;
+    const uint32_t parent = encoded - 1u;
     const uint2 range = d.ranges[parent];
-    const bool local = range.x >= block_first && range.y <= block_last;
-    uint32_t arrival;
-    if (local) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      arrival = __hip_atomic_fetch_add(&d.visits[parent], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    } else {
-      __threadfence();
-      arrival = atomicAdd(&d.visits[parent], 1u);
-      __threadfence();
+    if (!(range.x >= block_first && range.y <= block_last)) {
+      uint32_t slot = wave_append(&d.counters[kCountFrontier], true);
+      d.frontier[slot] = current;
+      return;
     }
+    const uint32_t arrival = arrive_workgroup(&d.visits[parent]);
     if (arrival == 0u) return;
     if (arrival > 1u) {
       atomicOr(&diag[kDiagFlags], 1u);
       return;
     }
-    uint2 ch = d.children[parent];
-    const float4* a = ch.x >= leaf_base ? d.leaves + 2 * (size_t)(ch.x - leaf_base) : d.inodes + 2 * (size_t)ch.x;
-    const float4* b = ch.y >= leaf_base ? d.leaves + 2 * (size_t)(ch.y - leaf_base) : d.inodes + 2 * (size_t)ch.y;
-    float4 amn = a[0], amx = a[1], bmn = b[0], bmx = b[1];
-    float4* p = d.inodes + 2 * (size_t)parent;
-    float lw = p[0].w, ew = p[1].w;
-    p[0] = make_float4(fmn(amn.x, bmn.x), fmn(amn.y, bmn.y), fmn(amn.z, bmn.z), lw);
-    p[1] = make_float4(fmx(amx.x, bmx.x), fmx(amx.y, bmx.y), fmx(amx.z, bmx.z), ew);
+    merge_children(d, parent, leaf_base);
     current = parent;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_refit_top(Dev d) {
+  const uint64_t tick = tick_now(d);
+  uint32_t* diag = d.diag[tick & 1];
+  const uint32_t c = d.collider_count;
+  if (c <= 1) return;
+  const uint32_t leaf_base = c - 1u;
+  const uint32_t n = *reinterpret_cast<volatile uint32_t*>(&d.counters[kCountFrontier]);
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint32_t current = d.frontier[i];
+    for (int level = 0; level < 128; ++level) {
+      const uint32_t encoded = d.parents[current];
+      if (encoded == 0u) break;
+      const uint32_t parent = encoded - 1u;
+      const uint32_t arrival = arrive_workgroup(&d.visits[parent]);
+      if (arrival == 0u) break;
+      if (arrival > 1u) {
+        atomicOr(&diag[kDiagFlags], 1u);
+        break;
+      }
+      merge_children(d, parent, leaf_base);
+      current = parent;
+    }
   }
 }
 
@@ -379,26 +415,106 @@ __device__ __forceinline__ void flush_pairs(const Dev& d, uint32_t* diag, PairSt
   st.count = 0;
 }
 
+// One packet of queries: the wavefront's 64 consecutive sorted leaves.
+struct Packet {
+  uint32_t first;  // sorted position of lane 0's leaf
+  uint32_t p;      // this lane's leaf
+  bool valid;
+  float4 qmn, qmx;
+  uint32_t self, self_body;
+};
+
+__device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint32_t lane) {
+  Packet k;
+  k.first = first;
+  k.p = first + lane;
+  k.valid = k.p < d.collider_count;
+  k.qmn = make_float4(0.f, 0.f, 0.f, 0.f);
+  k.qmx = k.qmn;
+  k.self = 0;
+  k.self_body = 0;
+  if (k.valid) {
+    k.qmn = d.leaves[2 * (size_t)k.p];
+    k.qmx = d.leaves[2 * (size_t)k.p + 1];
+    k.self = __float_as_uint(k.qmn.w);
+    k.self_body = d.leaf_body[k.p];
+  }
+  return k;
+}
+
+// Tests every lane's query against the leaves [start, last], 64 at a time: each
+// lane loads one candidate and all lanes test it after a v_readlane broadcast.
+__device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
+                                           const Packet& k, uint32_t start, uint32_t last) {
+  const bool check_suppression = d.suppression_count != 0;
+  for (uint32_t base = start; base <= last; base += 64u) {
+    const uint32_t q = base + lane;
+    float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
+    uint32_t cbody = 0;
+    if (q <= last) {
+      cmn = d.leaves[2 * (size_t)q];
+      cmx = d.leaves[2 * (size_t)q + 1];
+      cbody = d.leaf_body[q];
+    }
+    const uint32_t n = last - base + 1u < 64u ? last - base + 1u : 64u;
+    for (uint32_t j = 0; j < n; ++j) {
+      const float4 bmn = make_float4(bcast(cmn.x, j), bcast(cmn.y, j), bcast(cmn.z, j), 0.0f);
+      const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
+      const uint32_t other = bcast(__float_as_uint(cmn.w), j);
+      const uint32_t other_body = bcast(cbody, j);
+      bool hit = k.valid && base + j > k.p && overlaps(k.qmn, k.qmx, bmn, bmx) && other_body != k.self_body;
+      if (hit && check_suppression) hit = !suppressed(d, k.self_body, other_body);
+      const uint64_t mask = __ballot(hit);
+      if (mask != 0ull) {
+        const uint32_t cnt = (uint32_t)__popcll(mask);
+        if (st.count + cnt > kStage) flush_pairs(d, diag, st, lane);
+        if (hit) {
+          const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+          st.buf[st.count + rank] = make_uint2(k.self < other ? k.self : other, k.self < other ? other : k.self);
+        }
+        st.count += cnt;
+      }
+    }
+  }
+}
+
+constexpr uint32_t kItemStage = 64;  // staged work items per wavefront
+
+// Phase 1: the packet walks the tree and records the leaf ranges (buckets and
+// single leaves) it must scan as work items. If the work list is full, the
+// range is scanned here instead.
 __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
+  __shared__ uint4 item_mem[4][kItemStage];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t c = d.collider_count;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave_first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
   if (wave_first >= c) return;  // whole wavefront past the last leaf
-  const uint32_t p = wave_first + lane;
-  const bool valid = p < c;
-  float4 qmn = make_float4(0.f, 0.f, 0.f, 0.f), qmx = qmn;
-  uint32_t self = 0, self_body = 0;
-  if (valid) {
-    qmn = d.leaves[2 * (size_t)p];
-    qmx = d.leaves[2 * (size_t)p + 1];
-    self = __float_as_uint(qmn.w);
-    self_body = d.leaf_body[p];
-  }
+  const Packet k = load_packet(d, wave_first, lane);
   PairStage st{stage_mem[threadIdx.x >> 6], 0u};
-  const bool check_suppression = d.suppression_count != 0;
+  uint4* items = item_mem[threadIdx.x >> 6];
+  uint32_t n_items = 0;  // wave-uniform
+
+  auto flush_items = [&]() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&d.counters[kCountWork], n_items);
+    base = bcast(base, 0u);
+    if (lane < n_items && base + lane < d.work_capacity) d.work[base + lane] = items[lane];
+    // Items past the list's capacity are scanned by this wavefront.
+    for (uint32_t i = 0; i < n_items; ++i) {
+      if (base + i >= d.work_capacity) {
+        const uint4 it = items[i];
+        scan_range(d, diag, st, lane, k, it.y, it.z);
+      }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    n_items = 0;
+  };
 
   uint32_t cur = 0u;  // the root (c >= 2)
   uint32_t steps_left = 2u * c + 1u;
@@ -418,7 +534,7 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
       const uint32_t left_word = __float_as_uint(mn.w);
       const uint32_t esc = __float_as_uint(mx.w);
       const uint32_t range_last = (esc & kPosMask) - 1u;
-      const bool want = valid && range_last > p && overlaps(qmn, qmx, mn, mx);
+      const bool want = k.valid && range_last > k.p && overlaps(k.qmn, k.qmx, mn, mx);
       if (__ballot(want) == 0ull) {
         cur = esc;
         continue;
@@ -431,38 +547,43 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
       last = range_last;
       next = esc;
     }
-    // Scan leaves [max(first, wave_first + 1), last] against the packet.
     const uint32_t start = first > wave_first ? first : wave_first + 1u;
-    for (uint32_t base = start; base <= last; base += 64u) {
-      const uint32_t q = base + lane;
-      float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
-      uint32_t cbody = 0;
-      if (q <= last) {
-        cmn = d.leaves[2 * (size_t)q];
-        cmx = d.leaves[2 * (size_t)q + 1];
-        cbody = d.leaf_body[q];
-      }
-      const uint32_t n = last - base + 1u < 64u ? last - base + 1u : 64u;
-      for (uint32_t j = 0; j < n; ++j) {
-        const float4 bmn = make_float4(bcast(cmn.x, j), bcast(cmn.y, j), bcast(cmn.z, j), 0.0f);
-        const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
-        const uint32_t other = bcast(__float_as_uint(cmn.w), j);
-        const uint32_t other_body = bcast(cbody, j);
-        bool hit = valid && base + j > p && overlaps(qmn, qmx, bmn, bmx) && other_body != self_body;
-        if (hit && check_suppression) hit = !suppressed(d, self_body, other_body);
-        const uint64_t mask = __ballot(hit);
-        if (mask != 0ull) {
-          const uint32_t k = (uint32_t)__popcll(mask);
-          if (st.count + k > kStage) flush_pairs(d, diag, st, lane);
-          if (hit) {
-            const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-            st.buf[st.count + rank] = make_uint2(self < other ? self : other, self < other ? other : self);
-          }
-          st.count += k;
-        }
-      }
+    if (start <= last) {
+      if (lane == 0) items[n_items] = make_uint4(wave_first, start, last, 0u);
+      n_items += 1;
+      if (n_items == kItemStage) flush_items();
     }
     cur = next;
+  }
+  if (n_items) flush_items();
+  if (st.count) flush_pairs(d, diag, st, lane);
+}
+
+// Phase 2: every wavefront takes a contiguous share of the work items and scans
+// each range against the item's packet.
+__global__ __launch_bounds__(256) void k_scan(Dev d) {
+  __shared__ uint2 stage_mem[4][kStage];
+  const uint64_t tick = tick_now(d);
+  uint32_t* diag = d.diag[tick & 1];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t requested = *reinterpret_cast<volatile uint32_t*>(&d.counters[kCountWork]);
+  const uint32_t count = requested < d.work_capacity ? requested : d.work_capacity;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t share = (count + waves - 1u) / waves;
+  const uint32_t begin = wave * share;
+  const uint32_t end = begin + share < count ? begin + share : count;
+  if (begin >= end) return;
+  PairStage st{stage_mem[threadIdx.x >> 6], 0u};
+  uint32_t packet_first = kInvalid;
+  Packet k{};
+  for (uint32_t i = begin; i < end; ++i) {
+    const uint4 it = d.work[i];
+    if (it.x != packet_first) {
+      packet_first = it.x;
+      k = load_packet(d, packet_first, lane);
+    }
+    scan_range(d, diag, st, lane, k, it.y, it.z);
   }
   if (st.count) flush_pairs(d, diag, st, lane);
 }
@@ -482,9 +603,10 @@ void launch_sort(hipStream_t s, const Dev& d) {
     uint32_t j = k >> 1;
     while (j >= tile) {
       uint32_t span = 31u - (uint32_t)__builtin_clz(j) - (31u - (uint32_t)__builtin_clz(tile)) + 1u;
-      uint32_t levels = span < 3u ? span : 3u;
+      uint32_t levels = span < 4u ? span : 4u;
       uint32_t threads = n >> levels;
-      if (levels == 3) k_sort_global<3><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
+      if (levels == 4) k_sort_global<4><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
+      else if (levels == 3) k_sort_global<3><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
       else if (levels == 2) k_sort_global<2><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
       else k_sort_global<1><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
       j >>= levels;
@@ -496,11 +618,13 @@ void launch_sort(hipStream_t s, const Dev& d) {
 void launch_lbvh(hipStream_t s, const Dev& d) {
   if (d.collider_count > 1) k_topology<<<blocks_for(d.collider_count - 1), kBlock, 0, s>>>(d);
   k_refit<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
+  if (d.collider_count > 1) k_refit_top<<<1, 1024, 0, s>>>(d);
 }
 
 void launch_traverse(hipStream_t s, const Dev& d) {
   if (d.collider_count < 2) return;
   k_traverse<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
+  k_scan<<<2048, kBlock, 0, s>>>(d);
 }
 
 }  // namespace mh

@@ -277,7 +277,8 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   float* inv_mass;
   uint32_t *roots, *comps, *parents, *visits, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
-  uint32_t* leaf_body;
+  uint32_t *leaf_body, *frontier, *counters;
+  uint4* work;
   uint64_t *keys, *h0, *h1;
   int32_t* deltas;
   int4* cdelta;
@@ -292,6 +293,10 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   ALLOC(inodes, 2 * (size_t)(c > 1 ? c - 1 : 1)); ALLOC(leaves, 2 * (size_t)c);
   ALLOC(children, c > 1 ? c - 1 : 1); ALLOC(parents, c > 0 ? 2 * (size_t)c - 1 : 1);
   ALLOC(ranges, c > 1 ? c - 1 : 1); ALLOC(leaf_body, c);
+  ALLOC(frontier, 2 * (size_t)c); ALLOC(counters, kCounterWords);
+  const uint32_t packets = (c + 63) / 64;
+  d.work_capacity = 64 * packets < 4096 ? 4096 : 64 * packets;
+  ALLOC(work, d.work_capacity);
   ALLOC(visits, c > 1 ? c - 1 : 1);
   ALLOC(pairs, cfg.pair_capacity);
   ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
@@ -315,7 +320,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.colliders = cols; d.suppressed = supp;
   d.aabbs = aabbs; d.cposes = cposes; d.keys = keys; d.inodes = inodes; d.leaves = leaves;
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
-  d.ranges = ranges; d.leaf_body = leaf_body;
+  d.ranges = ranges; d.leaf_body = leaf_body; d.frontier = frontier; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
   d.cdelta = cdelta; d.slot_count = slot_count; d.incid = incid;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;

@@ -32,7 +32,7 @@ enum DiagWord : uint32_t {
   kDiagWords = 8,
 };
 
-// Private per-tick counters (cleared by the topology kernel).
+// Private per-tick counters (cleared by the Morton kernel).
 enum CounterWord : uint32_t {
   kCountFrontier = 0,
   kCountWork = 1,

@@ -34,6 +34,7 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kCounterWords) d.counters[i] = 0u;  // sort_count >= 256 threads always run
   if (i >= d.sort_count) return;
   if (i >= d.collider_count) {
     d.keys[i] = (0xffffffffull << 32) | i;
@@ -258,7 +259,6 @@ __global__ __launch_bounds__(256) void k_topology(Dev d) {
   d.parents[left_ref] = node + 1u;
   d.parents[right_ref] = node + 1u;
   if (node == 0) d.parents[0] = 0u;
-  if (node < kCounterWords) d.counters[node] = 0u;
   d.visits[node] = 0u;
   const bool bucket = (keys[first] >> 32) == (keys[last] >> 32);
   const uint32_t left_enc = split == first ? ((uint32_t)split | kLeafBit) : (uint32_t)split;

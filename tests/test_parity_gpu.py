@@ -247,3 +247,16 @@ def test_pile_100k_first_tick_broadphase_and_contacts():
     ora.step_partial(1, 3)
     compare_broadphase(gpu, ora, scene)
     compare_contacts(gpu, ora)
+
+
+@pytest.mark.parametrize("count", [2, 3, 5, 16, 17, 33])
+def test_small_stacks_many_ticks(count):
+    # Fewer colliders than the per-tick counters: every counter must still reset.
+    centers = [[0.01 * (i % 3), 0.125 + 0.26 * i, -0.01 * (i % 2)] for i in range(count)]
+    scene = mh.boxes(centers, [0.125] * 3, inverse_mass=0.064, inverse_inertia=6.144)
+    gpu, ora = make(scene, friction=0.6)
+    for tick in range(1, 241):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+        if tick % 60 == 0:
+            assert_bodies_equal(gpu, ora, f"{count} bodies, tick {tick}")

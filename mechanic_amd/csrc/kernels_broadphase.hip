@@ -31,34 +31,38 @@ __device__ __forceinline__ uint32_t morton_code(v3 c) {
 // (collision.wgsl:622-664). One thread per sort slot; slots past the collider
 // count get the padding key (0xffffffff, slot).
 __global__ __launch_bounds__(256) void k_morton(Dev d) {
+  __shared__ uint32_t lds_append[kBlock / 64 + 1];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kCounterWords) d.counters[i] = 0u;  // sort_count >= 256 threads always run
-  if (i >= d.sort_count) return;
-  if (i >= d.collider_count) {
-    d.keys[i] = (0xffffffffull << 32) | i;
-    return;
+  bool emit = false;
+  ColliderPose p;
+  v3 support;
+  float bottom = 0.0f;
+  if (i < d.sort_count && i >= d.collider_count) d.keys[i] = (0xffffffffull << 32) | i;
+  if (i < d.collider_count) {
+    p = collider_pose(d, i);
+    store_pose(d, i, p);
+    v3 we = (vabs(p.axes[0]) * p.half.x + vabs(p.axes[1]) * p.half.y) + vabs(p.axes[2]) * p.half.z;
+    d.aabbs[2 * (size_t)i] = f4(p.center - we, 0.0f);
+    d.aabbs[2 * (size_t)i + 1] = f4(p.center + we, 0.0f);
+    d.keys[i] = ((uint64_t)morton_code(p.center) << 32) | i;
+    // Ground plane y = 0.
+    const v3 down = mk(0.0f, -1.0f, 0.0f);
+    support = support_point(p, down);
+    emit = true;
+    if (p.role != 0u) {
+      uint32_t count = full_cylinder_contact_count(p, down);
+      if (p.role > count) emit = false;
+      else support = full_cylinder_support_point(p, down, p.role);
+    }
+    bottom = support.y;
+    if (bottom > 1.0e-5f) emit = false;
   }
-  ColliderPose p = collider_pose(d, i);
-  store_pose(d, i, p);
-  v3 we = (vabs(p.axes[0]) * p.half.x + vabs(p.axes[1]) * p.half.y) + vabs(p.axes[2]) * p.half.z;
-  d.aabbs[2 * (size_t)i] = f4(p.center - we, 0.0f);
-  d.aabbs[2 * (size_t)i + 1] = f4(p.center + we, 0.0f);
-  d.keys[i] = ((uint64_t)morton_code(p.center) << 32) | i;
-
-  // Ground plane y = 0.
-  const v3 down = mk(0.0f, -1.0f, 0.0f);
-  v3 support = support_point(p, down);
-  bool emit = true;
-  if (p.role != 0u) {
-    uint32_t count = full_cylinder_contact_count(p, down);
-    if (p.role > count) emit = false;
-    else support = full_cylinder_support_point(p, down, p.role);
-  }
-  float bottom = support.y;
-  if (bottom > 1.0e-5f) emit = false;
-  uint32_t row = wave_append(&diag[kDiagContacts], emit);
+  // The grid covers sort_count (a multiple of 256) exactly, so every thread of
+  // every block reaches the block-wide append.
+  uint32_t row = block_append(&diag[kDiagContacts], emit, lds_append);
   if (!emit) return;
   if (row < d.pair_capacity) {
     float4 bp = d.pos[p.body];
@@ -75,7 +79,7 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
 // Bitonic sort of sort_count u64 keys, ascending. Keys are unique, so the result
 // equals the reference's order (lbvh.wgsl:100-106, 132-217).
 //
-// A tile of up to kTile keys lives in one workgroup (1024 threads, 64 KiB of
+// A tile of up to kTile keys lives in one workgroup (256 threads, 16 KiB of
 // LDS): 8 keys per thread in registers
 // (blocked: thread t holds tile positions 8t..8t+7). Exchanges with j < 8 stay
 // in registers, 8 <= j < 512 go through wavefront shuffles (partner lane
@@ -83,7 +87,7 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
 // Distances of a tile or more are global multi-level butterfly kernels.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kPer = 8;
-constexpr uint32_t kTile = 8192;
+constexpr uint32_t kTile = 2048;
 
 __device__ __forceinline__ void cas(uint64_t& a, uint64_t& b, bool ascending) {
   bool swap = ascending ? (a > b) : (a < b);
@@ -143,7 +147,7 @@ __device__ __forceinline__ void tile_level(uint64_t (&r)[kPer], uint64_t* lds, u
 
 // Full bitonic sort of each tile (stages k = 2 .. tile), alternating directions by
 // the global position so neighbouring tiles form bitonic sequences.
-__global__ __launch_bounds__(1024) void k_sort_tile(uint64_t* keys, uint32_t tile) {
+__global__ __launch_bounds__(256) void k_sort_tile(uint64_t* keys, uint32_t tile) {
   __shared__ uint64_t lds[kTile];
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
@@ -159,7 +163,7 @@ __global__ __launch_bounds__(1024) void k_sort_tile(uint64_t* keys, uint32_t til
 }
 
 // Finishing levels j = tile/2 .. 1 of stage k inside each tile.
-__global__ __launch_bounds__(1024) void k_sort_merge(uint64_t* keys, uint32_t tile, uint32_t k) {
+__global__ __launch_bounds__(256) void k_sort_merge(uint64_t* keys, uint32_t tile, uint32_t k) {
   __shared__ uint64_t lds[kTile];
   const uint32_t tid = threadIdx.x;
   const uint32_t nthreads = blockDim.x;
@@ -422,7 +426,19 @@ struct Packet {
   bool valid;
   float4 qmn, qmx;
   uint32_t self, self_body;
+  float4 umn, umx;  // union box of the packet's queries (wave-uniform)
 };
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmn(v, __shfl_xor(v, m));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmx(v, __shfl_xor(v, m));
+  return v;
+}
 
 __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint32_t lane) {
   Packet k;
@@ -439,11 +455,18 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
     k.self = __float_as_uint(k.qmn.w);
     k.self_body = d.leaf_body[k.p];
   }
+  const float inf = __builtin_huge_valf();
+  k.umn = make_float4(wave_min(k.valid ? k.qmn.x : inf), wave_min(k.valid ? k.qmn.y : inf),
+                      wave_min(k.valid ? k.qmn.z : inf), 0.0f);
+  k.umx = make_float4(wave_max(k.valid ? k.qmx.x : -inf), wave_max(k.valid ? k.qmx.y : -inf),
+                      wave_max(k.valid ? k.qmx.z : -inf), 0.0f);
   return k;
 }
 
 // Tests every lane's query against the leaves [start, last], 64 at a time: each
-// lane loads one candidate and all lanes test it after a v_readlane broadcast.
+// lane loads one candidate, candidates that miss the packet's union box are
+// dropped by one ballot, and all lanes test each remaining candidate after a
+// v_readlane broadcast.
 __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
                                            const Packet& k, uint32_t start, uint32_t last) {
   const bool check_suppression = d.suppression_count != 0;
@@ -456,8 +479,10 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
       cmx = d.leaves[2 * (size_t)q + 1];
       cbody = d.leaf_body[q];
     }
-    const uint32_t n = last - base + 1u < 64u ? last - base + 1u : 64u;
-    for (uint32_t j = 0; j < n; ++j) {
+    uint64_t todo = __ballot(q <= last && overlaps(k.umn, k.umx, cmn, cmx));
+    while (todo != 0ull) {
+      const uint32_t j = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
+      todo &= todo - 1ull;
       const float4 bmn = make_float4(bcast(cmn.x, j), bcast(cmn.y, j), bcast(cmn.z, j), 0.0f);
       const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
       const uint32_t other = bcast(__float_as_uint(cmn.w), j);

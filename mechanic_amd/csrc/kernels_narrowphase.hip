@@ -103,6 +103,7 @@ __device__ __forceinline__ v3 contact_point(const ColliderPose& a, const Collide
 }
 
 __global__ __launch_bounds__(256) void k_narrowphase(Dev d) {
+  __shared__ uint32_t lds_append[kBlock / 64 + 1];
   const uint64_t tick = tick_np(d);
   uint32_t* diag = d.diag[tick & 1];
   uint32_t requested = *reinterpret_cast<volatile uint32_t*>(&diag[kDiagPairs]);
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256) void k_narrowphase(Dev d) {
         }
       }
     }
-    uint32_t row = wave_append(&diag[kDiagContacts], emit);
+    uint32_t row = block_append(&diag[kDiagContacts], emit, lds_append);
     if (emit) {
       if (row < d.pair_capacity) store_contact(d, row, meta, n_pen, arm_a, arm_b);
       else atomicOr(&diag[kDiagFlags], 1u);

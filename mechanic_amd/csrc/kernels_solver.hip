@@ -73,6 +73,7 @@ __device__ __forceinline__ v3 contact_velocity(const Dev& d, uint32_t body, v3 a
 
 // prepare_contacts (collision.wgsl:674-736) with the deterministic manifold index.
 __global__ __launch_bounds__(256) void k_prepare(Dev d) {
+  __shared__ uint32_t lds_append[kBlock / 64 + 1];
   const uint64_t tick64 = tick_sv(d);
   const uint32_t tick = (uint32_t)tick64;
   uint32_t* diag = d.diag[tick64 & 1];
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
       }
     }
   done:
-    uint32_t a = wave_append(&diag[kDiagActive], active);
+    uint32_t a = block_append(&diag[kDiagActive], active, lds_append);
     if (active) {
       const uint32_t ci = __float_as_uint(r3.z);
       const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);

@@ -130,6 +130,33 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   return pred ? base + rank : kInvalid;
 }
 
+// Appends one row per thread with `pred` set, one atomic per workgroup. Every
+// thread of the block must call it (block-uniform control flow); `lds` holds
+// kBlock / 64 + 1 words. Returns the row index (may be >= capacity), or
+// kInvalid without `pred`.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, uint32_t* lds) {
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t waves = blockDim.x >> 6;
+  const uint64_t mask = __ballot(pred);
+  const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) lds[wave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < waves; ++w) {
+      uint32_t n = lds[w];
+      lds[w] = total;
+      total += n;
+    }
+    lds[waves] = total ? atomicAdd(counter, total) : 0u;
+  }
+  __syncthreads();
+  const uint32_t row = lds[waves] + lds[wave] + rank;
+  __syncthreads();
+  return pred ? row : kInvalid;
+}
+
 __device__ __forceinline__ void store_contact(const Dev& d, uint32_t row, uint4 meta, float4 n_pen,
                                               float4 arm_a, float4 arm_b) {
   float4* c = d.contacts + 4 * (size_t)row;

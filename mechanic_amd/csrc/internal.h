@@ -116,6 +116,27 @@ struct Dev {
   TickState* tick;
 };
 
+// Work split for grid-stride kernels on gfx950: workgroups are dispatched to the
+// 8 XCDs round-robin by block index, so block b runs on XCD b % 8. Giving each
+// XCD a contiguous eighth of the (spatially ordered) items keeps the bodies it
+// touches in its own L2.
+constexpr uint32_t kXcds = 8;
+struct XcdSplit {
+  uint32_t begin, end;   // this XCD's item range
+  uint32_t first, step;  // first item of this block, block stride within the XCD
+};
+__device__ __forceinline__ XcdSplit xcd_split(uint32_t n) {
+  const uint32_t xcd = blockIdx.x % kXcds;
+  const uint32_t local = blockIdx.x / kXcds;
+  const uint32_t blocks = (gridDim.x - xcd + kXcds - 1) / kXcds;
+  XcdSplit s;
+  s.begin = (uint32_t)(((uint64_t)n * xcd) / kXcds);
+  s.end = (uint32_t)(((uint64_t)n * (xcd + 1)) / kXcds);
+  s.first = s.begin + local * blockDim.x;
+  s.step = blocks * blockDim.x;
+  return s;
+}
+
 // Launchers (one per kernel group), all on `stream`.
 void launch_set_tick(hipStream_t s, const Dev& d, uint64_t tick);
 void launch_end_tick(hipStream_t s, const Dev& d);

@@ -108,13 +108,13 @@ __global__ __launch_bounds__(256) void k_narrowphase(Dev d) {
   uint32_t* diag = d.diag[tick & 1];
   uint32_t requested = *reinterpret_cast<volatile uint32_t*>(&diag[kDiagPairs]);
   const uint32_t count = requested < d.pair_capacity ? requested : d.pair_capacity;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+  const XcdSplit xs = xcd_split(count);
+  for (uint32_t base = xs.first; base < xs.end; base += xs.step) {
     const uint32_t index = base + threadIdx.x;
     bool emit = false;
     uint4 meta;
     float4 n_pen, arm_a, arm_b;
-    if (index < count) {
+    if (index < xs.end) {
       const uint2 pair = d.pairs[index];
       const ColliderPose a = load_pose(d, pair.x);
       const ColliderPose b = load_pose(d, pair.y);

@@ -82,12 +82,12 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   float4* rows_now = d.mrows[tick64 & 1];
   const float4* rows_prev = d.mrows[(tick64 + 1) & 1];
   const uint32_t count = read_count(&diag[kDiagContacts], d.pair_capacity);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+  const XcdSplit xs = xcd_split(count);
+  for (uint32_t base = xs.first; base < xs.end; base += xs.step) {
     const uint32_t ci = base + threadIdx.x;
     bool active = false;
     float4 r0, r1, r2, r3;
-    if (ci < count) {
+    if (ci < xs.end) {
       float4* c = d.contacts + 4 * (size_t)ci;
       const float4 m = c[0];
       const uint32_t body_a = __float_as_uint(m.x), body_b = __float_as_uint(m.y);
@@ -226,7 +226,8 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
   float4* rows_now = d.mrows[tick64 & 1];
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
   const float friction = d.friction;
-  for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < count; ai += gridDim.x * blockDim.x) {
+  const XcdSplit xs = xcd_split(count);
+  for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
     float4* row = d.rows + 4 * (size_t)ai;
     const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
     const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
@@ -301,37 +302,38 @@ __global__ __launch_bounds__(256) void k_apply(Dev d) {
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= d.body_count) return;
-  // Wrapping 32-bit sums, the arithmetic of atomicAdd on i32.
-  uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-  const uint32_t count = d.slot_count[b];
-  const uint32_t n = count < kMaxSlots ? count : kMaxSlots;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t side = d.incid[(size_t)k * d.body_count + b];
-    const int4 x = d.cdelta[2 * (size_t)side];
-    const int4 y = d.cdelta[2 * (size_t)side + 1];
-    s0 += (uint32_t)x.x;
-    s1 += (uint32_t)x.y;
-    s2 += (uint32_t)x.z;
-    s3 += (uint32_t)x.w;
-    s4 += (uint32_t)y.x;
-    s5 += (uint32_t)y.y;
+  const XcdSplit xs = xcd_split(d.body_count);
+  for (uint32_t b = xs.first + threadIdx.x; b < xs.end; b += xs.step) {
+    // Wrapping 32-bit sums, the arithmetic of atomicAdd on i32.
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    const uint32_t count = d.slot_count[b];
+    const uint32_t n = count < kMaxSlots ? count : kMaxSlots;
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t side = d.incid[(size_t)k * d.body_count + b];
+      const int4 x = d.cdelta[2 * (size_t)side];
+      const int4 y = d.cdelta[2 * (size_t)side + 1];
+      s0 += (uint32_t)x.x;
+      s1 += (uint32_t)x.y;
+      s2 += (uint32_t)x.z;
+      s3 += (uint32_t)x.w;
+      s4 += (uint32_t)y.x;
+      s5 += (uint32_t)y.y;
+    }
+    if (count > kMaxSlots) {
+      int32_t* p = d.deltas + 6 * (size_t)b;
+      s0 += (uint32_t)p[0];
+      s1 += (uint32_t)p[1];
+      s2 += (uint32_t)p[2];
+      s3 += (uint32_t)p[3];
+      s4 += (uint32_t)p[4];
+      s5 += (uint32_t)p[5];
+      for (int k = 0; k < 6; ++k) p[k] = 0;
+    }
+    const v3 lin_delta = mk((float)(int32_t)s0, (float)(int32_t)s1, (float)(int32_t)s2) / kFixedScale;
+    const v3 ang_delta = mk((float)(int32_t)s3, (float)(int32_t)s4, (float)(int32_t)s5) / kFixedScale;
+    d.lin[b] = f4(xyz(d.lin[b]) + lin_delta, 0.0f);
+    d.ang[b] = f4(xyz(d.ang[b]) + ang_delta, 0.0f);
   }
-  if (count > kMaxSlots) {
-    int32_t* p = d.deltas + 6 * (size_t)b;
-    s0 += (uint32_t)p[0];
-    s1 += (uint32_t)p[1];
-    s2 += (uint32_t)p[2];
-    s3 += (uint32_t)p[3];
-    s4 += (uint32_t)p[4];
-    s5 += (uint32_t)p[5];
-    for (int k = 0; k < 6; ++k) p[k] = 0;
-  }
-  const v3 lin_delta = mk((float)(int32_t)s0, (float)(int32_t)s1, (float)(int32_t)s2) / kFixedScale;
-  const v3 ang_delta = mk((float)(int32_t)s3, (float)(int32_t)s4, (float)(int32_t)s5) / kFixedScale;
-  d.lin[b] = f4(xyz(d.lin[b]) + lin_delta, 0.0f);
-  d.ang[b] = f4(xyz(d.ang[b]) + ang_delta, 0.0f);
 }
 
 static inline uint32_t grid_for(uint32_t n, uint32_t cap) {
@@ -346,7 +348,7 @@ void launch_prepare(hipStream_t s, const Dev& d) {
 void launch_solve(hipStream_t s, const Dev& d) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t solve_grid = grid_for(d.pair_capacity, 2048);
-  const uint32_t body_grid = (d.body_count + kBlock - 1) / kBlock;
+  const uint32_t body_grid = d.body_count ? (d.body_count + kBlock - 1) / kBlock + kXcds : 0;
   for (uint32_t it = 0; it < iterations; ++it) {
     k_solve<<<solve_grid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, it + 1 == iterations ? 1 : 0);
     if (body_grid) k_apply<<<body_grid, kBlock, 0, s>>>(d);

@@ -1,0 +1,58 @@
+"""Summarises a rocprofv3 results database (rocpd SQLite) as text.
+
+    python -m mechanic_amd.rocprof_summary RESULTS.db [--top 30]
+
+Prints, per kernel: calls, average / total duration and, when the run
+collected hardware counters (--pmc), the average value of each counter per
+dispatch.
+"""
+from __future__ import annotations
+
+import argparse
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def kernel_stats(cur):
+    rows = cur.execute(
+        "select name, count(*), avg(end - start), sum(end - start) from kernels group by name "
+        "order by sum(end - start) desc").fetchall()
+    return rows
+
+
+def counter_stats(cur):
+    try:
+        rows = cur.execute(
+            "select kernel_name, counter_name, avg(value), count(*) from counters_collection "
+            "group by kernel_name, counter_name").fetchall()
+    except sqlite3.Error:
+        return {}
+    out = defaultdict(dict)
+    for kernel, counter, value, n in rows:
+        out[kernel][counter] = (value, n)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("db")
+    ap.add_argument("--top", type=int, default=30)
+    args = ap.parse_args(argv)
+    cur = sqlite3.connect(args.db).cursor()
+    counters = counter_stats(cur)
+    stats = kernel_stats(cur)
+    total = sum(r[3] for r in stats) or 1
+    print(f"{'kernel':64s} {'calls':>6s} {'avg_us':>9s} {'total_ms':>9s} {'pct':>6s}")
+    for name, n, avg, tot in stats[: args.top]:
+        print(f"{name[:64]:64s} {n:6d} {avg / 1e3:9.2f} {tot / 1e6:9.3f} {100 * tot / total:6.1f}")
+    if counters:
+        print("\ncounters (average per dispatch)")
+        for kernel in sorted(counters, key=lambda k: -sum(v[0] for v in counters[k].values())):
+            vals = ", ".join(f"{c}={v:.4g}" for c, (v, _) in sorted(counters[kernel].items()))
+            print(f"{kernel[:64]:64s} {vals}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -245,7 +245,7 @@ enum {
   MH_BUFFER_NODE_CHILDREN = 3,   /* uint2 per internal node, reference encoding (leaf = C-1+i) */
   MH_BUFFER_NODE_AABBS = 4,      /* float4 min, float4 max per node (internal then leaves) */
   MH_BUFFER_PAIRS = 5,           /* mh_pair per broadphase pair (unordered) */
-  MH_BUFFER_CONTACTS = 6,        /* mh_contact per contact (unordered) */
+  MH_BUFFER_CONTACTS = 6,        /* mh_contact per contact as generated (unordered); prepared state: MANIFOLDS */
   MH_BUFFER_DIAGNOSTICS = 7,     /* mh_diagnostics of the last tick */
   MH_BUFFER_WORLD_MASSES = 8,    /* 3 x float4 per body */
   MH_BUFFER_MANIFOLDS = 9,       /* mh_persistent_manifold per contact of the last tick */

@@ -84,6 +84,7 @@ __device__ __forceinline__ bool contains(const ColliderPose& p, v3 point) {
 __device__ __forceinline__ v3 contact_point(const ColliderPose& a, const ColliderPose& b, v3 normal) {
   v3 sum = mk(0.0f, 0.0f, 0.0f);
   float count = 0.0f;
+#pragma unroll
   for (uint32_t v = 0; v < 8; ++v) {
     v3 pa = vertex(a, v);
     if (contains(b, pa)) {

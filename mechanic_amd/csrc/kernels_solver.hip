@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
     bool active = false;
     float4 r0, r1, r2, r3;
     if (ci < xs.end) {
-      float4* c = d.contacts + 4 * (size_t)ci;
+      const float4* c = d.contacts + 4 * (size_t)ci;
       const float4 m = c[0];
       const uint32_t body_a = __float_as_uint(m.x), body_b = __float_as_uint(m.y);
       const uint32_t col_a = __float_as_uint(m.z), col_b = __float_as_uint(m.w);
@@ -109,10 +109,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
       {
         const v3 relative = velocity_b - contact_velocity(d, body_a, arm_a);
         const float normal_speed = dot(relative, normal);
-        if (n_pen.w <= 1.0e-6f && normal_speed >= 0.0f) {
-          c[0] = make_float4(m.x, m.y, __uint_as_float(kInvalid), __uint_as_float(0u));
-          goto done;
-        }
+        if (n_pen.w <= 1.0e-6f && normal_speed >= 0.0f) goto done;
         const uint32_t low = col_a < col_b ? col_a : col_b;
         const uint32_t high = col_a < col_b ? col_b : col_a;
         const uint64_t key = pair_key(low, high);
@@ -146,8 +143,6 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
           atomicOr(&diag[kDiagFlags], 8u);
           goto done;
         }
-        c[0] = make_float4(m.x, m.y, __uint_as_float(ci), __uint_as_float(1u));
-        c[2] = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, cached_impulse);
         float4* mr = rows_now + 3 * (size_t)ci;
         mr[0] = make_float4(__uint_as_float(low), __uint_as_float(high), __uint_as_float(tick), __uint_as_float(1u));
         mr[1] = n_pen;
@@ -178,6 +173,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
         else flags |= kOverflowB;
       }
       d.active[a] = ci;
+      d.lambda[a] = r1.w;
       float4* row = d.rows + 4 * (size_t)a;
       row[0] = r0;
       row[1] = r1;
@@ -228,7 +224,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
   const float friction = d.friction;
   const XcdSplit xs = xcd_split(count);
   for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
-    float4* row = d.rows + 4 * (size_t)ai;
+    const float4* row = d.rows + 4 * (size_t)ai;
     const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
     const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
     const uint32_t word = __float_as_uint(r3.z);
@@ -239,7 +235,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
     const float penetration = r0.w;
     const bool analytic = r2.w < 0.0f;
     const float denominator = r3.w;
-    float lambda = r1.w;
+    float lambda = d.lambda[ai];
     const float4* wa = d.world_masses + 3 * (size_t)body_a;
     const float4* wb = d.world_masses + 3 * (size_t)(has_b ? body_b : body_a);
     v3 velocity_b = mk(0.0f, 0.0f, 0.0f);
@@ -287,10 +283,9 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
       zero_side(d, 2u * ai, over_a);
       if (has_b) zero_side(d, 2u * ai + 1u, over_b);
     }
-    row[1].w = lambda;
+    d.lambda[ai] = lambda;
     if (last) {
       const uint32_t ci = word & ~(kOverflowA | kOverflowB);
-      d.contacts[4 * (size_t)ci + 2].w = lambda;
       rows_now[3 * (size_t)ci + 2].w = lambda;
     }
   }

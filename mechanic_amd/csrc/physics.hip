@@ -274,7 +274,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.friction = cfg.friction;
 
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
-  float* inv_mass;
+  float *inv_mass, *lambda;
   uint32_t *roots, *comps, *parents, *visits, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
   uint32_t *leaf_body, *frontier, *counters;
@@ -302,6 +302,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
   ALLOC(active, cfg.pair_capacity);
   ALLOC(rows, 4 * (size_t)cfg.pair_capacity);
+  ALLOC(lambda, cfg.pair_capacity);
   ALLOC(deltas, 6 * (size_t)n);
   ALLOC(cdelta, 4 * (size_t)cfg.pair_capacity);
   ALLOC(slot_count, n);
@@ -322,7 +323,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
   d.ranges = ranges; d.leaf_body = leaf_body; d.frontier = frontier; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
-  d.cdelta = cdelta; d.slot_count = slot_count; d.incid = incid;
+  d.cdelta = cdelta; d.slot_count = slot_count; d.incid = incid; d.lambda = lambda;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;
   d.diag[0] = diag; d.diag[1] = diag + kDiagWords;
   d.tick = tick;

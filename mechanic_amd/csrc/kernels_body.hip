@@ -84,6 +84,8 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d, int collisions, int sn
     stored = false;
     position = d.pos[i];
     rotation = d.rot[i];
+    linear = d.lin[i];
+    angular = d.ang[i];
   } else {
     d.pos[i] = position;
     d.rot[i] = rotation;
@@ -101,6 +103,11 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d, int collisions, int sn
     w[0] = f4(cx, m[0].x);
     w[1] = f4(cy, 0.0f);
     w[2] = f4(cz, 0.0f);
+    float4* sb = d.sbody + 4 * (size_t)i;
+    sb[0] = make_float4(linear.x, linear.y, linear.z, m[0].x);
+    sb[1] = make_float4(angular.x, angular.y, angular.z, cx.x);
+    sb[2] = make_float4(cx.y, cx.z, cy.x, cy.y);
+    sb[3] = make_float4(cy.z, cz.x, cz.y, cz.z);
   }
   if (snapshot_now) {
     uint32_t slot = (uint32_t)(tick % 3u);

@@ -49,26 +49,67 @@ __device__ __forceinline__ uint64_t pair_key(uint32_t low, uint32_t high) {
   return ((uint64_t)(low + 1u) << 21) | hi;
 }
 
-__device__ __forceinline__ v3 inverse_inertia(const float4* wm, v3 v) {
-  return (xyz(wm[0]) * v.x + xyz(wm[1]) * v.y) + xyz(wm[2]) * v.z;
+// Solver state of one body as one 64-byte record (sbody): (lin, inverse mass),
+// (ang, cx.x), (cx.y, cx.z, cy.x, cy.y), (cy.z, cz) where cx, cy, cz are the
+// world inverse-inertia columns of update_world_masses (collision.wgsl:144-162).
+struct SolverBody {
+  v3 lin, ang, cx, cy, cz;
+  float inv_mass;
+};
+
+__device__ __forceinline__ SolverBody load_body(const Dev& d, uint32_t b) {
+  const float4* r = d.sbody + 4 * (size_t)b;
+  const float4 a = r[0], c = r[1], e = r[2], f = r[3];
+  SolverBody s;
+  s.lin = xyz(a);
+  s.inv_mass = a.w;
+  s.ang = xyz(c);
+  s.cx = mk(c.w, e.x, e.y);
+  s.cy = mk(e.z, e.w, f.x);
+  s.cz = mk(f.y, f.z, f.w);
+  return s;
 }
 
-// impulse_denominator (collision.wgsl:116-132).
-__device__ __forceinline__ float impulse_denominator(const float4* wa, const float4* wb, bool has_b, v3 arm_a,
-                                                     v3 arm_b, v3 direction) {
-  float result = wa[0].w;
-  v3 angular_a = cross(inverse_inertia(wa, cross(arm_a, direction)), arm_a);
+// inverse_inertia (collision.wgsl:109-114).
+__device__ __forceinline__ v3 inv_inertia(const SolverBody& s, v3 v) {
+  return (s.cx * v.x + s.cy * v.y) + s.cz * v.z;
+}
+
+__device__ __forceinline__ float denom_of(const SolverBody& a, const SolverBody& b, bool has_b, v3 arm_a, v3 arm_b,
+                                          v3 direction) {
+  float result = a.inv_mass;
+  v3 angular_a = cross(inv_inertia(a, cross(arm_a, direction)), arm_a);
   result += dot(angular_a, direction);
   if (has_b) {
-    result += wb[0].w;
-    v3 angular_b = cross(inverse_inertia(wb, cross(arm_b, direction)), arm_b);
+    result += b.inv_mass;
+    v3 angular_b = cross(inv_inertia(b, cross(arm_b, direction)), arm_b);
     result += dot(angular_b, direction);
   }
   return result;
 }
 
-__device__ __forceinline__ v3 contact_velocity(const Dev& d, uint32_t body, v3 arm) {
-  return xyz(d.lin[body]) + cross(xyz(d.ang[body]), arm);
+// The six Q20 integers add_velocity_delta (collision.wgsl:764-772) adds for one
+// contact side: side A gets -impulse, side B +impulse.
+struct Q20 {
+  int32_t v[6];
+};
+__device__ __forceinline__ Q20 side_delta(const SolverBody& s, v3 arm, v3 impulse) {
+  const v3 linear = impulse * s.inv_mass;
+  const v3 angular = inv_inertia(s, cross(arm, impulse));
+  Q20 q;
+  q.v[0] = round_to_i32(linear.x * kFixedScale);
+  q.v[1] = round_to_i32(linear.y * kFixedScale);
+  q.v[2] = round_to_i32(linear.z * kFixedScale);
+  q.v[3] = round_to_i32(angular.x * kFixedScale);
+  q.v[4] = round_to_i32(angular.y * kFixedScale);
+  q.v[5] = round_to_i32(angular.z * kFixedScale);
+  return q;
+}
+
+__device__ __forceinline__ void add_overflow(const Dev& d, uint32_t body, const Q20& q) {
+  int32_t* p = d.deltas + 6 * (size_t)body;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) atomicAdd(&p[k], q.v[k]);
 }
 
 // prepare_contacts (collision.wgsl:674-736) with the deterministic manifold index.
@@ -97,17 +138,18 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
       const float4 arm_b4 = c[3];
       const v3 arm_a = xyz(arm_a4), arm_b = xyz(arm_b4), normal = xyz(n_pen);
       const bool has_b = body_b != kInvalid;
-      const float4* wa = d.world_masses + 3 * (size_t)body_a;
-      const float4* wb = d.world_masses + 3 * (size_t)(has_b ? body_b : body_a);
+      const SolverBody sa = load_body(d, body_a);
+      SolverBody sb = sa;
       float inverse_b = 0.0f;
       v3 velocity_b = mk(0.0f, 0.0f, 0.0f);
       if (has_b) {
-        inverse_b = wb[0].w;
-        velocity_b = contact_velocity(d, body_b, arm_b);
+        sb = load_body(d, body_b);
+        inverse_b = sb.inv_mass;
+        velocity_b = sb.lin + cross(sb.ang, arm_b);
       }
-      if (wa[0].w + inverse_b <= 0.0f) goto done;
+      if (sa.inv_mass + inverse_b <= 0.0f) goto done;
       {
-        const v3 relative = velocity_b - contact_velocity(d, body_a, arm_a);
+        const v3 relative = velocity_b - (sa.lin + cross(sa.ang, arm_a));
         const float normal_speed = dot(relative, normal);
         if (n_pen.w <= 1.0e-6f && normal_speed >= 0.0f) goto done;
         const uint32_t low = col_a < col_b ? col_a : col_b;
@@ -152,7 +194,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
           r0 = n_pen;
           r1 = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, cached_impulse);
           r2 = arm_b4;
-          float denom = impulse_denominator(wa, wb, has_b, arm_a, arm_b, normal);
+          float denom = denom_of(sa, sb, has_b, arm_a, arm_b, normal);
           r3 = make_float4(m.x, m.y, __uint_as_float(ci), denom);
         }
       }
@@ -173,7 +215,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
         else flags |= kOverflowB;
       }
       d.active[a] = ci;
-      d.lambda[a] = r1.w;
+      d.impulse[a] = make_float4(0.0f, 0.0f, 0.0f, r1.w);
       float4* row = d.rows + 4 * (size_t)a;
       row[0] = r0;
       row[1] = r1;
@@ -183,39 +225,11 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   }
 }
 
-// The Q20 deltas of add_velocity_delta (collision.wgsl:764-772). Instead of six
-// atomics per body, a contact side stores its six integers and the body's
-// gather sums them; integer sums do not depend on order, so the result is the
-// same as the atomic accumulation. Sides past kMaxSlots use the atomics.
-__device__ __forceinline__ void put_side(const Dev& d, uint32_t side, bool overflow, uint32_t body, v3 linear,
-                                         v3 angular) {
-  const int32_t l0 = round_to_i32(linear.x * kFixedScale), l1 = round_to_i32(linear.y * kFixedScale),
-                l2 = round_to_i32(linear.z * kFixedScale), a0 = round_to_i32(angular.x * kFixedScale),
-                a1 = round_to_i32(angular.y * kFixedScale), a2 = round_to_i32(angular.z * kFixedScale);
-  if (!overflow) {
-    d.cdelta[2 * (size_t)side] = make_int4(l0, l1, l2, a0);
-    d.cdelta[2 * (size_t)side + 1] = make_int4(a1, a2, 0, 0);
-  } else {
-    int32_t* p = d.deltas + 6 * (size_t)body;
-    atomicAdd(&p[0], l0);
-    atomicAdd(&p[1], l1);
-    atomicAdd(&p[2], l2);
-    atomicAdd(&p[3], a0);
-    atomicAdd(&p[4], a1);
-    atomicAdd(&p[5], a2);
-  }
-}
-
-__device__ __forceinline__ void zero_side(const Dev& d, uint32_t side, bool overflow) {
-  if (!overflow) {
-    d.cdelta[2 * (size_t)side] = make_int4(0, 0, 0, 0);
-    d.cdelta[2 * (size_t)side + 1] = make_int4(0, 0, 0, 0);
-  }
-}
-
 // warm_start (collision.wgsl:774-839) when first, solve_accumulate
 // (collision.wgsl:841-906) otherwise; the last pass also runs persist_contacts
-// (collision.wgsl:1071-1088).
+// (collision.wgsl:1071-1088). Each contact stores its impulse (zero when it
+// applies none) and accumulated impulse; the bodies' gather turns impulses into
+// the Q20 deltas. Sides past kMaxSlots add theirs with atomics here.
 __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
@@ -228,20 +242,23 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
     const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
     const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
     const uint32_t word = __float_as_uint(r3.z);
-    const bool over_a = (word & kOverflowA) != 0u, over_b = (word & kOverflowB) != 0u;
     const bool has_b = body_b != kInvalid;
     const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
-    bool wrote = false;
     const float penetration = r0.w;
     const bool analytic = r2.w < 0.0f;
     const float denominator = r3.w;
-    float lambda = d.lambda[ai];
-    const float4* wa = d.world_masses + 3 * (size_t)body_a;
-    const float4* wb = d.world_masses + 3 * (size_t)(has_b ? body_b : body_a);
+    float lambda = d.impulse[ai].w;
+    const SolverBody sa = load_body(d, body_a);
+    SolverBody sb = sa;
     v3 velocity_b = mk(0.0f, 0.0f, 0.0f);
-    if (has_b) velocity_b = contact_velocity(d, body_b, arm_b);
+    if (has_b) {
+      sb = load_body(d, body_b);
+      velocity_b = sb.lin + cross(sb.ang, arm_b);
+    }
+    v3 impulse = mk(0.0f, 0.0f, 0.0f);
+    bool applies = false;
     if (denominator > 0.0f) {
-      const v3 relative = velocity_b - contact_velocity(d, body_a, arm_a);
+      const v3 relative = velocity_b - (sa.lin + cross(sa.ang, arm_a));
       const float normal_speed = dot(relative, normal);
       if (penetration <= 1.0e-5f && normal_speed >= -1.0e-5f && lambda <= 1.0e-6f) {
         lambda = 0.0f;
@@ -249,7 +266,6 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
         const float recovery = analytic ? 1.0f : 0.2f;
         const float bias = fmn((fmx(penetration - kSlop, 0.0f) * recovery) / d.delta_seconds, kMaxCorrection);
         const float step = (kRelaxation * (-normal_speed + bias)) / denominator;
-        v3 impulse;
         float bound;
         if (first) {
           const float warmed = lambda * (analytic ? 1.0f : kWarmStartScale);
@@ -269,21 +285,18 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
         const float tangent_speed = length(tangent_velocity);
         if (tangent_speed > 1.0e-6f) {
           const v3 tangent = tangent_velocity / tangent_speed;
-          const float tangent_denominator = impulse_denominator(wa, wb, has_b, arm_a, arm_b, tangent);
+          const float tangent_denominator = denom_of(sa, sb, has_b, arm_a, arm_b, tangent);
           const float friction_impulse = fmn(tangent_speed / tangent_denominator, bound);
           impulse = impulse - tangent * friction_impulse;
         }
-        const v3 neg = -impulse;
-        put_side(d, 2u * ai, over_a, body_a, neg * wa[0].w, inverse_inertia(wa, cross(arm_a, neg)));
-        if (has_b) put_side(d, 2u * ai + 1u, over_b, body_b, impulse * wb[0].w, inverse_inertia(wb, cross(arm_b, impulse)));
-        wrote = true;
+        applies = true;
       }
     }
-    if (!wrote) {
-      zero_side(d, 2u * ai, over_a);
-      if (has_b) zero_side(d, 2u * ai + 1u, over_b);
+    d.impulse[ai] = f4(impulse, lambda);
+    if (applies) {
+      if (word & kOverflowA) add_overflow(d, body_a, side_delta(sa, arm_a, -impulse));
+      if (has_b && (word & kOverflowB)) add_overflow(d, body_b, side_delta(sb, arm_b, impulse));
     }
-    d.lambda[ai] = lambda;
     if (last) {
       const uint32_t ci = word & ~(kOverflowA | kOverflowB);
       rows_now[3 * (size_t)ci + 2].w = lambda;
@@ -292,42 +305,49 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
 }
 
 // solve_apply (collision.wgsl:1050-1069); skipped when no contact is active
-// (finalize_active_contacts, collision.wgsl:744).
-__global__ __launch_bounds__(256) void k_apply(Dev d) {
+// (finalize_active_contacts, collision.wgsl:744). Each body sums the Q20 deltas
+// of its contact sides (wrapping 32-bit sums: the arithmetic of atomicAdd on
+// i32, in any order) and adds them to its velocities. The last pass also writes
+// the body velocities back to the state arrays.
+__global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
   const XcdSplit xs = xcd_split(d.body_count);
   for (uint32_t b = xs.first + threadIdx.x; b < xs.end; b += xs.step) {
-    // Wrapping 32-bit sums, the arithmetic of atomicAdd on i32.
-    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    const SolverBody sb = load_body(d, b);
+    uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
     const uint32_t count = d.slot_count[b];
     const uint32_t n = count < kMaxSlots ? count : kMaxSlots;
     for (uint32_t k = 0; k < n; ++k) {
       const uint32_t side = d.incid[(size_t)k * d.body_count + b];
-      const int4 x = d.cdelta[2 * (size_t)side];
-      const int4 y = d.cdelta[2 * (size_t)side + 1];
-      s0 += (uint32_t)x.x;
-      s1 += (uint32_t)x.y;
-      s2 += (uint32_t)x.z;
-      s3 += (uint32_t)x.w;
-      s4 += (uint32_t)y.x;
-      s5 += (uint32_t)y.y;
+      const uint32_t a = side >> 1;
+      const float4 imp = d.impulse[a];
+      const float4 arm = d.rows[4 * (size_t)a + 1 + (side & 1u)];
+      const v3 j = (side & 1u) ? xyz(imp) : -xyz(imp);
+      const Q20 q = side_delta(sb, xyz(arm), j);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[c] += (uint32_t)q.v[c];
     }
     if (count > kMaxSlots) {
       int32_t* p = d.deltas + 6 * (size_t)b;
-      s0 += (uint32_t)p[0];
-      s1 += (uint32_t)p[1];
-      s2 += (uint32_t)p[2];
-      s3 += (uint32_t)p[3];
-      s4 += (uint32_t)p[4];
-      s5 += (uint32_t)p[5];
-      for (int k = 0; k < 6; ++k) p[k] = 0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        acc[c] += (uint32_t)p[c];
+        p[c] = 0;
+      }
     }
-    const v3 lin_delta = mk((float)(int32_t)s0, (float)(int32_t)s1, (float)(int32_t)s2) / kFixedScale;
-    const v3 ang_delta = mk((float)(int32_t)s3, (float)(int32_t)s4, (float)(int32_t)s5) / kFixedScale;
-    d.lin[b] = f4(xyz(d.lin[b]) + lin_delta, 0.0f);
-    d.ang[b] = f4(xyz(d.ang[b]) + ang_delta, 0.0f);
+    const v3 lin_delta = mk((float)(int32_t)acc[0], (float)(int32_t)acc[1], (float)(int32_t)acc[2]) / kFixedScale;
+    const v3 ang_delta = mk((float)(int32_t)acc[3], (float)(int32_t)acc[4], (float)(int32_t)acc[5]) / kFixedScale;
+    const v3 lin = sb.lin + lin_delta;
+    const v3 ang = sb.ang + ang_delta;
+    float4* r = d.sbody + 4 * (size_t)b;
+    r[0] = f4(lin, sb.inv_mass);
+    r[1] = f4(ang, sb.cx.x);
+    if (last) {
+      d.lin[b] = f4(lin, 0.0f);
+      d.ang[b] = f4(ang, 0.0f);
+    }
   }
 }
 
@@ -345,8 +365,9 @@ void launch_solve(hipStream_t s, const Dev& d) {
   const uint32_t solve_grid = grid_for(d.pair_capacity, 2048);
   const uint32_t body_grid = d.body_count ? (d.body_count + kBlock - 1) / kBlock + kXcds : 0;
   for (uint32_t it = 0; it < iterations; ++it) {
-    k_solve<<<solve_grid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, it + 1 == iterations ? 1 : 0);
-    if (body_grid) k_apply<<<body_grid, kBlock, 0, s>>>(d);
+    const int last = it + 1 == iterations ? 1 : 0;
+    k_solve<<<solve_grid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last);
+    if (body_grid) k_apply<<<body_grid, kBlock, 0, s>>>(d, last);
   }
 }
 

@@ -274,14 +274,14 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.friction = cfg.friction;
 
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
-  float *inv_mass, *lambda;
+  float* inv_mass;
+  float4 *impulse, *sbody;
   uint32_t *roots, *comps, *parents, *visits, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
   uint32_t *leaf_body, *frontier, *counters;
   uint4* work;
   uint64_t *keys, *h0, *h1;
   int32_t* deltas;
-  int4* cdelta;
   uint32_t *slot_count, *incid;
   float4 *m0, *m1;
   TickState* tick;
@@ -302,9 +302,9 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
   ALLOC(active, cfg.pair_capacity);
   ALLOC(rows, 4 * (size_t)cfg.pair_capacity);
-  ALLOC(lambda, cfg.pair_capacity);
+  ALLOC(impulse, cfg.pair_capacity);
+  ALLOC(sbody, 4 * (size_t)n);
   ALLOC(deltas, 6 * (size_t)n);
-  ALLOC(cdelta, 4 * (size_t)cfg.pair_capacity);
   ALLOC(slot_count, n);
   ALLOC(incid, (size_t)kMaxSlots * n);
   ALLOC(h0, cfg.pair_capacity); ALLOC(h1, cfg.pair_capacity);
@@ -323,7 +323,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
   d.ranges = ranges; d.leaf_body = leaf_body; d.frontier = frontier; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
-  d.cdelta = cdelta; d.slot_count = slot_count; d.incid = incid; d.lambda = lambda;
+ d.slot_count = slot_count; d.incid = incid; d.impulse = impulse; d.sbody = sbody;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;
   d.diag[0] = diag; d.diag[1] = diag + kDiagWords;
   d.tick = tick;

@@ -98,7 +98,8 @@ struct Dev {
   float4* contacts;       // 4 per contact, mh_contact layout
   uint32_t* active;       // contact index per active contact
   float4* rows;           // 4 per active contact: solver rows
-  float4* impulse;        // per active contact: last pass's impulse, accumulated normal impulse
+  float* lambda;          // accumulated normal impulse per active contact
+  int4* cdelta;           // 2 int4 per contact side: that side's Q20 deltas of the last pass
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)
   int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
   uint32_t* slot_count;   // contact sides per body this tick

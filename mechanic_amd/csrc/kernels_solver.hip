@@ -106,6 +106,11 @@ __device__ __forceinline__ Q20 side_delta(const SolverBody& s, v3 arm, v3 impuls
   return q;
 }
 
+__device__ __forceinline__ void put_side(const Dev& d, uint32_t side, const Q20& q) {
+  d.cdelta[2 * (size_t)side] = make_int4(q.v[0], q.v[1], q.v[2], q.v[3]);
+  d.cdelta[2 * (size_t)side + 1] = make_int4(q.v[4], q.v[5], 0, 0);
+}
+
 __device__ __forceinline__ void add_overflow(const Dev& d, uint32_t body, const Q20& q) {
   int32_t* p = d.deltas + 6 * (size_t)body;
 #pragma unroll
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
         else flags |= kOverflowB;
       }
       d.active[a] = ci;
-      d.impulse[a] = make_float4(0.0f, 0.0f, 0.0f, r1.w);
+      d.lambda[a] = r1.w;
       float4* row = d.rows + 4 * (size_t)a;
       row[0] = r0;
       row[1] = r1;
@@ -227,9 +232,8 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
 
 // warm_start (collision.wgsl:774-839) when first, solve_accumulate
 // (collision.wgsl:841-906) otherwise; the last pass also runs persist_contacts
-// (collision.wgsl:1071-1088). Each contact stores its impulse (zero when it
-// applies none) and accumulated impulse; the bodies' gather turns impulses into
-// the Q20 deltas. Sides past kMaxSlots add theirs with atomics here.
+// (collision.wgsl:1071-1088). Each contact side stores its six Q20 integers
+// for the bodies' gather; sides past kMaxSlots add theirs with atomics.
 __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
     const float penetration = r0.w;
     const bool analytic = r2.w < 0.0f;
     const float denominator = r3.w;
-    float lambda = d.impulse[ai].w;
+    float lambda = d.lambda[ai];
     const SolverBody sa = load_body(d, body_a);
     SolverBody sb = sa;
     v3 velocity_b = mk(0.0f, 0.0f, 0.0f);
@@ -292,10 +296,21 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
         applies = true;
       }
     }
-    d.impulse[ai] = f4(impulse, lambda);
-    if (applies) {
-      if (word & kOverflowA) add_overflow(d, body_a, side_delta(sa, arm_a, -impulse));
-      if (has_b && (word & kOverflowB)) add_overflow(d, body_b, side_delta(sb, arm_b, impulse));
+    d.lambda[ai] = lambda;
+    // A contact that applies no impulse adds zero deltas (collision.wgsl:793-804).
+    const Q20 qa = applies ? side_delta(sa, arm_a, -impulse) : Q20{};
+    if (word & kOverflowA) {
+      if (applies) add_overflow(d, body_a, qa);
+    } else {
+      put_side(d, 2u * ai, qa);
+    }
+    if (has_b) {
+      const Q20 qb = applies ? side_delta(sb, arm_b, impulse) : Q20{};
+      if (word & kOverflowB) {
+        if (applies) add_overflow(d, body_b, qb);
+      } else {
+        put_side(d, 2u * ai + 1u, qb);
+      }
     }
     if (last) {
       const uint32_t ci = word & ~(kOverflowA | kOverflowB);
@@ -306,7 +321,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
 
 // solve_apply (collision.wgsl:1050-1069); skipped when no contact is active
 // (finalize_active_contacts, collision.wgsl:744). Each body sums the Q20 deltas
-// of its contact sides (wrapping 32-bit sums: the arithmetic of atomicAdd on
+// its contact sides stored (wrapping 32-bit sums: the arithmetic of atomicAdd on
 // i32, in any order) and adds them to its velocities. The last pass also writes
 // the body velocities back to the state arrays.
 __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
@@ -321,13 +336,14 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
     const uint32_t n = count < kMaxSlots ? count : kMaxSlots;
     for (uint32_t k = 0; k < n; ++k) {
       const uint32_t side = d.incid[(size_t)k * d.body_count + b];
-      const uint32_t a = side >> 1;
-      const float4 imp = d.impulse[a];
-      const float4 arm = d.rows[4 * (size_t)a + 1 + (side & 1u)];
-      const v3 j = (side & 1u) ? xyz(imp) : -xyz(imp);
-      const Q20 q = side_delta(sb, xyz(arm), j);
-#pragma unroll
-      for (int c = 0; c < 6; ++c) acc[c] += (uint32_t)q.v[c];
+      const int4 x = d.cdelta[2 * (size_t)side];
+      const int4 y = d.cdelta[2 * (size_t)side + 1];
+      acc[0] += (uint32_t)x.x;
+      acc[1] += (uint32_t)x.y;
+      acc[2] += (uint32_t)x.z;
+      acc[3] += (uint32_t)x.w;
+      acc[4] += (uint32_t)y.x;
+      acc[5] += (uint32_t)y.y;
     }
     if (count > kMaxSlots) {
       int32_t* p = d.deltas + 6 * (size_t)b;

@@ -275,7 +275,9 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
 
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
   float* inv_mass;
-  float4 *impulse, *sbody;
+  float* lambda;
+  float4* sbody;
+  int4* cdelta;
   uint32_t *roots, *comps, *parents, *visits, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
   uint32_t *leaf_body, *frontier, *counters;
@@ -302,7 +304,8 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
   ALLOC(active, cfg.pair_capacity);
   ALLOC(rows, 4 * (size_t)cfg.pair_capacity);
-  ALLOC(impulse, cfg.pair_capacity);
+  ALLOC(lambda, cfg.pair_capacity);
+  ALLOC(cdelta, 4 * (size_t)cfg.pair_capacity);
   ALLOC(sbody, 4 * (size_t)n);
   ALLOC(deltas, 6 * (size_t)n);
   ALLOC(slot_count, n);
@@ -323,7 +326,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
   d.ranges = ranges; d.leaf_body = leaf_body; d.frontier = frontier; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
- d.slot_count = slot_count; d.incid = incid; d.impulse = impulse; d.sbody = sbody;
+ d.slot_count = slot_count; d.incid = incid; d.lambda = lambda; d.cdelta = cdelta; d.sbody = sbody;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;
   d.diag[0] = diag; d.diag[1] = diag + kDiagWords;
   d.tick = tick;

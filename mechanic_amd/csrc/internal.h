@@ -36,6 +36,7 @@ enum DiagWord : uint32_t {
 enum CounterWord : uint32_t {
   kCountFrontier = 0,
   kCountWork = 1,
+  kCountTop = 2,
   kCounterWords = 16,
 };
 
@@ -86,7 +87,8 @@ struct Dev {
   uint2* children;        // reference encoding of node_children (lbvh.wgsl:275)
   uint2* ranges;          // sorted leaf range (first, last) per internal node
   uint32_t* leaf_body;    // body of the collider at each sorted leaf
-  uint32_t* frontier;     // nodes completed inside a workgroup whose parent is not (refit)
+  uint32_t* top;          // internal nodes whose leaves span several refit workgroups
+  float4* block_box;      // 2 per refit workgroup: box of its 256 leaves
   uint4* work;            // traversal work items: (packet first leaf, first, last, 0)
   uint32_t work_capacity;
   uint32_t* counters;     // per-tick private counters, see CounterWord
@@ -103,7 +105,7 @@ struct Dev {
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)
   int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
   uint32_t* slot_count;   // contact sides per body this tick
-  uint32_t* incid;        // kMaxSlots x body_count (slot-major): contact side = 2 * active + side
+  uint32_t* incid;        // body_count x kMaxSlots (one 128-byte line per body): contact side = 2 * active + side
 
   // ---- persistent manifolds (ping-pong by tick parity) ----
   uint64_t* htab[2];      // open-addressing index: key -> manifold row

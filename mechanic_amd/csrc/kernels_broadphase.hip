@@ -230,10 +230,24 @@ __device__ __forceinline__ uint32_t escape_after(const uint64_t* keys, int c, in
 }
 
 // build_topology (lbvh.wgsl:233-278).
+__device__ __forceinline__ bool build_node_impl(Dev& d, uint32_t node);
+__device__ __forceinline__ bool build_node(Dev& d, uint32_t node) { return build_node_impl(d, node); }
+
 __global__ __launch_bounds__(256) void k_topology(Dev d) {
+  __shared__ uint32_t lds_append[kBlock / 64 + 1];
   const int c = (int)d.collider_count;
   const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c <= 1 || node >= (uint32_t)(c - 1)) return;
+  // The grid is exactly ceil((C-1)/256) blocks; every thread reaches the append.
+  bool top = false;
+  if (c > 1 && node < (uint32_t)(c - 1)) top = build_node(d, node);
+  const uint32_t slot = block_append(&d.counters[kCountTop], top, lds_append);
+  if (top) d.top[slot] = node;
+}
+
+// Builds internal node `node`; returns whether its leaves span several refit
+// workgroups (kBlock leaves each).
+__device__ __forceinline__ bool build_node_impl(Dev& d, uint32_t node) {
+  const int c = (int)d.collider_count;
   const uint64_t* keys = d.keys;
   const int i = (int)node;
   const int dir = prefix(keys, c, i, i + 1) > prefix(keys, c, i, i - 1) ? 1 : -1;
@@ -269,15 +283,19 @@ __global__ __launch_bounds__(256) void k_topology(Dev d) {
   const uint32_t left_word = bucket ? ((uint32_t)first | kBucketBit) : left_enc;
   d.inodes[2 * (size_t)node].w = __uint_as_float(left_word);
   d.inodes[2 * (size_t)node + 1].w = __uint_as_float(escape_after(keys, c, last));
+  return ((uint32_t)first / kBlock) != ((uint32_t)last / kBlock);
 }
 
-// prepare_leaves + build_bounds (lbvh.wgsl:280-319), in two launches. The
-// second thread to reach a node merges its children's boxes. k_refit climbs
-// while the node's leaves lie inside this workgroup, so both arrivals come from
-// it and workgroup-scope release/acquire orders the boxes; a node completed
-// there whose parent spans workgroups goes on the frontier list. k_refit_top
-// then finishes the tree above the frontier in a single workgroup, again with
-// workgroup-scope ordering only (the launch boundary publishes k_refit's boxes).
+// prepare_leaves + build_bounds (lbvh.wgsl:280-319), in two launches.
+// Box unions use min/max that order -0 below +0, so a union gives the same bits
+// in any grouping; the tree's pairwise merges and the range unions below agree.
+__device__ __forceinline__ float umin(float a, float b) {
+  return a < b ? a : (b < a ? b : (__builtin_signbit(a) ? a : b));
+}
+__device__ __forceinline__ float umax(float a, float b) {
+  return a > b ? a : (b > a ? b : (__builtin_signbit(a) ? b : a));
+}
+
 __device__ __forceinline__ void merge_children(const Dev& d, uint32_t parent, uint32_t leaf_base) {
   uint2 ch = d.children[parent];
   const float4* a = ch.x >= leaf_base ? d.leaves + 2 * (size_t)(ch.x - leaf_base) : d.inodes + 2 * (size_t)ch.x;
@@ -285,8 +303,8 @@ __device__ __forceinline__ void merge_children(const Dev& d, uint32_t parent, ui
   float4 amn = a[0], amx = a[1], bmn = b[0], bmx = b[1];
   float4* p = d.inodes + 2 * (size_t)parent;
   float lw = p[0].w, ew = p[1].w;
-  p[0] = make_float4(fmn(amn.x, bmn.x), fmn(amn.y, bmn.y), fmn(amn.z, bmn.z), lw);
-  p[1] = make_float4(fmx(amx.x, bmx.x), fmx(amx.y, bmx.y), fmx(amx.z, bmx.z), ew);
+  p[0] = make_float4(umin(amn.x, bmn.x), umin(amn.y, bmn.y), umin(amn.z, bmn.z), lw);
+  p[1] = make_float4(umax(amx.x, bmx.x), umax(amx.y, bmx.y), umax(amx.z, bmx.z), ew);
 }
 
 __device__ __forceinline__ uint32_t arrive_workgroup(uint32_t* counter) {
@@ -296,19 +314,68 @@ __device__ __forceinline__ uint32_t arrive_workgroup(uint32_t* counter) {
   return arrival;
 }
 
+struct Box {
+  float x0, y0, z0, x1, y1, z1;
+};
+__device__ __forceinline__ Box empty_box() {
+  const float inf = __builtin_huge_valf();
+  return Box{inf, inf, inf, -inf, -inf, -inf};
+}
+__device__ __forceinline__ void grow(Box& b, float4 mn, float4 mx) {
+  b.x0 = umin(b.x0, mn.x);
+  b.y0 = umin(b.y0, mn.y);
+  b.z0 = umin(b.z0, mn.z);
+  b.x1 = umax(b.x1, mx.x);
+  b.y1 = umax(b.y1, mx.y);
+  b.z1 = umax(b.z1, mx.z);
+}
+__device__ __forceinline__ Box wave_union(Box b) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    b.x0 = umin(b.x0, __shfl_xor(b.x0, m));
+    b.y0 = umin(b.y0, __shfl_xor(b.y0, m));
+    b.z0 = umin(b.z0, __shfl_xor(b.z0, m));
+    b.x1 = umax(b.x1, __shfl_xor(b.x1, m));
+    b.y1 = umax(b.y1, __shfl_xor(b.y1, m));
+    b.z1 = umax(b.z1, __shfl_xor(b.z1, m));
+  }
+  return b;
+}
+
+// Leaves, the box of each workgroup's kBlock leaves, and the nodes whose leaves
+// lie inside one workgroup: the second thread to reach such a node merges its
+// children with workgroup-scope ordering only.
 __global__ __launch_bounds__(256) void k_refit(Dev d) {
+  __shared__ Box lds_box[kBlock / 64];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const int c = (int)d.collider_count;
   const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
-  if (leaf >= (uint32_t)c) return;
-  const uint32_t collider = (uint32_t)d.keys[leaf];
-  float4 mn = d.aabbs[2 * (size_t)collider];
-  float4 mx = d.aabbs[2 * (size_t)collider + 1];
-  d.leaves[2 * (size_t)leaf] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(collider));
-  d.leaves[2 * (size_t)leaf + 1] = make_float4(mx.x, mx.y, mx.z, __uint_as_float(escape_after(d.keys, c, (int)leaf)));
-  d.leaf_body[leaf] = __float_as_uint(d.colliders[4 * (size_t)collider + 3].x);
-  if (c <= 1) return;
+  const bool valid = leaf < (uint32_t)c;
+  Box box = empty_box();
+  if (valid) {
+    const uint32_t collider = (uint32_t)d.keys[leaf];
+    float4 mn = d.aabbs[2 * (size_t)collider];
+    float4 mx = d.aabbs[2 * (size_t)collider + 1];
+    d.leaves[2 * (size_t)leaf] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(collider));
+    d.leaves[2 * (size_t)leaf + 1] =
+        make_float4(mx.x, mx.y, mx.z, __uint_as_float(escape_after(d.keys, c, (int)leaf)));
+    d.leaf_body[leaf] = __float_as_uint(d.colliders[4 * (size_t)collider + 3].x);
+    grow(box, mn, mx);
+  }
+  box = wave_union(box);
+  if ((threadIdx.x & 63u) == 0) lds_box[threadIdx.x >> 6] = box;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Box b = lds_box[0];
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+      grow(b, make_float4(lds_box[w].x0, lds_box[w].y0, lds_box[w].z0, 0.0f),
+           make_float4(lds_box[w].x1, lds_box[w].y1, lds_box[w].z1, 0.0f));
+    }
+    d.block_box[2 * (size_t)blockIdx.x] = make_float4(b.x0, b.y0, b.z0, 0.0f);
+    d.block_box[2 * (size_t)blockIdx.x + 1] = make_float4(b.x1, b.y1, b.z1, 0.0f);
+  }
+  if (!valid || c <= 1) return;
   const uint32_t block_first = blockIdx.x * blockDim.x;
   const uint32_t block_last = min(block_first + blockDim.x, (uint32_t)c) - 1u;
   const uint32_t leaf_base = (uint32_t)c - 1u;
@@ -319,11 +386,7 @@ __global__ __launch_bounds__(256) void k_refit(Dev d) {
     if (encoded == 0u) return;
     const uint32_t parent = encoded - 1u;
     const uint2 range = d.ranges[parent];
-    if (!(range.x >= block_first && range.y <= block_last)) {
-      uint32_t slot = wave_append(&d.counters[kCountFrontier], true);
-      d.frontier[slot] = current;
-      return;
-    }
+    if (!(range.x >= block_first && range.y <= block_last)) return;  // k_refit_top's node
     const uint32_t arrival = arrive_workgroup(&d.visits[parent]);
     if (arrival == 0u) return;
     if (arrival > 1u) {
@@ -335,27 +398,30 @@ __global__ __launch_bounds__(256) void k_refit(Dev d) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_refit_top(Dev d) {
-  const uint64_t tick = tick_now(d);
-  uint32_t* diag = d.diag[tick & 1];
-  const uint32_t c = d.collider_count;
-  if (c <= 1) return;
-  const uint32_t leaf_base = c - 1u;
-  const uint32_t n = *reinterpret_cast<volatile uint32_t*>(&d.counters[kCountFrontier]);
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    uint32_t current = d.frontier[i];
-    for (int level = 0; level < 128; ++level) {
-      const uint32_t encoded = d.parents[current];
-      if (encoded == 0u) break;
-      const uint32_t parent = encoded - 1u;
-      const uint32_t arrival = arrive_workgroup(&d.visits[parent]);
-      if (arrival == 0u) break;
-      if (arrival > 1u) {
-        atomicOr(&diag[kDiagFlags], 1u);
-        break;
-      }
-      merge_children(d, parent, leaf_base);
-      current = parent;
+// Nodes whose leaves span several workgroups (listed by k_topology): one
+// wavefront per node unions the leaf boxes of its partial first and last
+// workgroups and the boxes of the whole workgroups between them. No node
+// depends on another, so there is no climb and no cross-workgroup handoff.
+__global__ __launch_bounds__(256) void k_refit_top(Dev d) {
+  const uint32_t n = *reinterpret_cast<volatile uint32_t*>(&d.counters[kCountTop]);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+    const uint32_t node = d.top[i];
+    const uint2 range = d.ranges[node];
+    const uint32_t b0 = range.x / kBlock, b1 = range.y / kBlock;
+    Box box = empty_box();
+    const uint32_t head_end = (b0 + 1) * kBlock - 1u < range.y ? (b0 + 1) * kBlock - 1u : range.y;
+    for (uint32_t l = range.x + lane; l <= head_end; l += 64u) grow(box, d.leaves[2 * (size_t)l], d.leaves[2 * (size_t)l + 1]);
+    for (uint32_t b = b0 + 1 + lane; b < b1; b += 64u) grow(box, d.block_box[2 * (size_t)b], d.block_box[2 * (size_t)b + 1]);
+    if (b1 > b0)
+      for (uint32_t l = b1 * kBlock + lane; l <= range.y; l += 64u) grow(box, d.leaves[2 * (size_t)l], d.leaves[2 * (size_t)l + 1]);
+    box = wave_union(box);
+    if (lane == 0) {
+      float4* p = d.inodes + 2 * (size_t)node;
+      const float lw = p[0].w, ew = p[1].w;
+      p[0] = make_float4(box.x0, box.y0, box.z0, lw);
+      p[1] = make_float4(box.x1, box.y1, box.z1, ew);
     }
   }
 }
@@ -464,30 +530,38 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
 }
 
 // Tests every lane's query against the leaves [start, last], 64 at a time: each
-// lane loads one candidate, candidates that miss the packet's union box are
-// dropped by one ballot, and all lanes test each remaining candidate after a
-// v_readlane broadcast.
-__device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
-                                           const Packet& k, uint32_t start, uint32_t last) {
+// lane loads one candidate into the wavefront's LDS slot array, candidates that
+// miss the packet's union box are dropped by one ballot, and all lanes test each
+// remaining candidate with broadcast LDS reads.
+struct Candidate {
+  float4 mn;  // w: collider
+  float4 mx;  // w: body
+};
+
+__device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, Candidate* cand,
+                                           uint32_t lane, const Packet& k, uint32_t start, uint32_t last) {
   const bool check_suppression = d.suppression_count != 0;
   for (uint32_t base = start; base <= last; base += 64u) {
     const uint32_t q = base + lane;
-    float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
-    uint32_t cbody = 0;
+    bool keep = false;
     if (q <= last) {
-      cmn = d.leaves[2 * (size_t)q];
-      cmx = d.leaves[2 * (size_t)q + 1];
-      cbody = d.leaf_body[q];
+      Candidate c;
+      c.mn = d.leaves[2 * (size_t)q];
+      c.mx = d.leaves[2 * (size_t)q + 1];
+      c.mx.w = __uint_as_float(d.leaf_body[q]);
+      keep = overlaps(k.umn, k.umx, c.mn, c.mx);
+      cand[lane] = c;
     }
-    uint64_t todo = __ballot(q <= last && overlaps(k.umn, k.umx, cmn, cmx));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    uint64_t todo = __ballot(keep);
     while (todo != 0ull) {
       const uint32_t j = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
       todo &= todo - 1ull;
-      const float4 bmn = make_float4(bcast(cmn.x, j), bcast(cmn.y, j), bcast(cmn.z, j), 0.0f);
-      const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
-      const uint32_t other = bcast(__float_as_uint(cmn.w), j);
-      const uint32_t other_body = bcast(cbody, j);
-      bool hit = k.valid && base + j > k.p && overlaps(k.qmn, k.qmx, bmn, bmx) && other_body != k.self_body;
+      const Candidate c = cand[j];
+      const uint32_t other = __float_as_uint(c.mn.w);
+      const uint32_t other_body = __float_as_uint(c.mx.w);
+      bool hit = k.valid && base + j > k.p && overlaps(k.qmn, k.qmx, c.mn, c.mx) && other_body != k.self_body;
       if (hit && check_suppression) hit = !suppressed(d, k.self_body, other_body);
       const uint64_t mask = __ballot(hit);
       if (mask != 0ull) {
@@ -500,6 +574,8 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
         st.count += cnt;
       }
     }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -511,6 +587,7 @@ constexpr uint32_t kItemStage = 64;  // staged work items per wavefront
 __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
   __shared__ uint4 item_mem[4][kItemStage];
+  __shared__ Candidate cand_mem[4][64];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t c = d.collider_count;
@@ -533,7 +610,7 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
     for (uint32_t i = 0; i < n_items; ++i) {
       if (base + i >= d.work_capacity) {
         const uint4 it = items[i];
-        scan_range(d, diag, st, lane, k, it.y, it.z);
+        scan_range(d, diag, st, cand_mem[threadIdx.x >> 6], lane, k, it.y, it.z);
       }
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -588,6 +665,7 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
 // each range against the item's packet.
 __global__ __launch_bounds__(256) void k_scan(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
+  __shared__ Candidate cand_mem[4][64];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t lane = threadIdx.x & 63u;
@@ -608,7 +686,7 @@ __global__ __launch_bounds__(256) void k_scan(Dev d) {
       packet_first = it.x;
       k = load_packet(d, packet_first, lane);
     }
-    scan_range(d, diag, st, lane, k, it.y, it.z);
+    scan_range(d, diag, st, cand_mem[threadIdx.x >> 6], lane, k, it.y, it.z);
   }
   if (st.count) flush_pairs(d, diag, st, lane);
 }
@@ -643,7 +721,7 @@ void launch_sort(hipStream_t s, const Dev& d) {
 void launch_lbvh(hipStream_t s, const Dev& d) {
   if (d.collider_count > 1) k_topology<<<blocks_for(d.collider_count - 1), kBlock, 0, s>>>(d);
   k_refit<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
-  if (d.collider_count > 1) k_refit_top<<<1, 1024, 0, s>>>(d);
+  if (d.collider_count > 1) k_refit_top<<<256, kBlock, 0, s>>>(d);
 }
 
 void launch_traverse(hipStream_t s, const Dev& d) {

@@ -280,7 +280,8 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   int4* cdelta;
   uint32_t *roots, *comps, *parents, *visits, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
-  uint32_t *leaf_body, *frontier, *counters;
+  uint32_t *leaf_body, *top, *counters;
+  float4* block_box;
   uint4* work;
   uint64_t *keys, *h0, *h1;
   int32_t* deltas;
@@ -295,7 +296,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   ALLOC(inodes, 2 * (size_t)(c > 1 ? c - 1 : 1)); ALLOC(leaves, 2 * (size_t)c);
   ALLOC(children, c > 1 ? c - 1 : 1); ALLOC(parents, c > 0 ? 2 * (size_t)c - 1 : 1);
   ALLOC(ranges, c > 1 ? c - 1 : 1); ALLOC(leaf_body, c);
-  ALLOC(frontier, 2 * (size_t)c); ALLOC(counters, kCounterWords);
+  ALLOC(top, c); ALLOC(block_box, 2 * (size_t)((c + kBlock - 1) / kBlock)); ALLOC(counters, kCounterWords);
   const uint32_t packets = (c + 63) / 64;
   d.work_capacity = 64 * packets < 4096 ? 4096 : 64 * packets;
   ALLOC(work, d.work_capacity);
@@ -324,7 +325,7 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.colliders = cols; d.suppressed = supp;
   d.aabbs = aabbs; d.cposes = cposes; d.keys = keys; d.inodes = inodes; d.leaves = leaves;
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
-  d.ranges = ranges; d.leaf_body = leaf_body; d.frontier = frontier; d.work = work; d.counters = counters;
+  d.ranges = ranges; d.leaf_body = leaf_body; d.top = top; d.block_box = block_box; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
  d.slot_count = slot_count; d.incid = incid; d.lambda = lambda; d.cdelta = cdelta; d.sbody = sbody;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;

@@ -63,6 +63,10 @@ template <typename R> V3<R> cross(V3<R> a, V3<R> b) {
 template <typename R> R mx(R a, R b) { return a > b ? a : b; }
 template <typename R> R mn(R a, R b) { return a < b ? a : b; }
 template <typename R> R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+// Box-union min/max ordering -0 below +0, so a union does not depend on the
+// grouping (WGSL leaves min/max of signed zeros to the implementation).
+template <typename R> R umin(R a, R b) { return a < b ? a : (b < a ? b : (std::signbit(a) ? a : b)); }
+template <typename R> R umax(R a, R b) { return a > b ? a : (b > a ? b : (std::signbit(a) ? b : a)); }
 template <typename R> R inverse_sqrt(R x) { return R(1) / std::sqrt(x); }
 template <typename R> R sgn(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : R(0)); }
 template <typename R> V3<R> vabs(V3<R> a) { return {std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; }
@@ -364,8 +368,8 @@ template <typename R> void build_bounds(World<R>& w) {
       }
       const Aabb<R>& l = w.node_aabbs[w.children[parent].first];
       const Aabb<R>& r = w.node_aabbs[w.children[parent].second];
-      w.node_aabbs[parent] = {{mn(l.mn.x, r.mn.x), mn(l.mn.y, r.mn.y), mn(l.mn.z, r.mn.z)},
-                              {mx(l.mx.x, r.mx.x), mx(l.mx.y, r.mx.y), mx(l.mx.z, r.mx.z)}};
+      w.node_aabbs[parent] = {{umin(l.mn.x, r.mn.x), umin(l.mn.y, r.mn.y), umin(l.mn.z, r.mn.z)},
+                              {umax(l.mx.x, r.mx.x), umax(l.mx.y, r.mx.y), umax(l.mx.z, r.mx.z)}};
       current = parent;
     }
   }

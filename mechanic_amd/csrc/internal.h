@@ -105,7 +105,7 @@ struct Dev {
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)
   int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
   uint32_t* slot_count;   // contact sides per body this tick
-  uint32_t* incid;        // body_count x kMaxSlots (one 128-byte line per body): contact side = 2 * active + side
+  uint32_t* incid;        // kMaxSlots x body_count (slot-major): contact side = 2 * active + side
 
   // ---- persistent manifolds (ping-pong by tick parity) ----
   uint64_t* htab[2];      // open-addressing index: key -> manifold row

@@ -530,38 +530,30 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
 }
 
 // Tests every lane's query against the leaves [start, last], 64 at a time: each
-// lane loads one candidate into the wavefront's LDS slot array, candidates that
-// miss the packet's union box are dropped by one ballot, and all lanes test each
-// remaining candidate with broadcast LDS reads.
-struct Candidate {
-  float4 mn;  // w: collider
-  float4 mx;  // w: body
-};
-
-__device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, Candidate* cand,
-                                           uint32_t lane, const Packet& k, uint32_t start, uint32_t last) {
+// lane loads one candidate, candidates that miss the packet's union box are
+// dropped by one ballot, and all lanes test each remaining candidate after a
+// v_readlane broadcast.
+__device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
+                                           const Packet& k, uint32_t start, uint32_t last) {
   const bool check_suppression = d.suppression_count != 0;
   for (uint32_t base = start; base <= last; base += 64u) {
     const uint32_t q = base + lane;
-    bool keep = false;
+    float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
+    uint32_t cbody = 0;
     if (q <= last) {
-      Candidate c;
-      c.mn = d.leaves[2 * (size_t)q];
-      c.mx = d.leaves[2 * (size_t)q + 1];
-      c.mx.w = __uint_as_float(d.leaf_body[q]);
-      keep = overlaps(k.umn, k.umx, c.mn, c.mx);
-      cand[lane] = c;
+      cmn = d.leaves[2 * (size_t)q];
+      cmx = d.leaves[2 * (size_t)q + 1];
+      cbody = d.leaf_body[q];
     }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_wave_barrier();
-    uint64_t todo = __ballot(keep);
+    uint64_t todo = __ballot(q <= last && overlaps(k.umn, k.umx, cmn, cmx));
     while (todo != 0ull) {
       const uint32_t j = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
       todo &= todo - 1ull;
-      const Candidate c = cand[j];
-      const uint32_t other = __float_as_uint(c.mn.w);
-      const uint32_t other_body = __float_as_uint(c.mx.w);
-      bool hit = k.valid && base + j > k.p && overlaps(k.qmn, k.qmx, c.mn, c.mx) && other_body != k.self_body;
+      const float4 bmn = make_float4(bcast(cmn.x, j), bcast(cmn.y, j), bcast(cmn.z, j), 0.0f);
+      const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
+      const uint32_t other = bcast(__float_as_uint(cmn.w), j);
+      const uint32_t other_body = bcast(cbody, j);
+      bool hit = k.valid && base + j > k.p && overlaps(k.qmn, k.qmx, bmn, bmx) && other_body != k.self_body;
       if (hit && check_suppression) hit = !suppressed(d, k.self_body, other_body);
       const uint64_t mask = __ballot(hit);
       if (mask != 0ull) {
@@ -574,8 +566,6 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
         st.count += cnt;
       }
     }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -587,7 +577,6 @@ constexpr uint32_t kItemStage = 64;  // staged work items per wavefront
 __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
   __shared__ uint4 item_mem[4][kItemStage];
-  __shared__ Candidate cand_mem[4][64];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t c = d.collider_count;
@@ -610,7 +599,7 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
     for (uint32_t i = 0; i < n_items; ++i) {
       if (base + i >= d.work_capacity) {
         const uint4 it = items[i];
-        scan_range(d, diag, st, cand_mem[threadIdx.x >> 6], lane, k, it.y, it.z);
+        scan_range(d, diag, st, lane, k, it.y, it.z);
       }
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -665,7 +654,6 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
 // each range against the item's packet.
 __global__ __launch_bounds__(256) void k_scan(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
-  __shared__ Candidate cand_mem[4][64];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t lane = threadIdx.x & 63u;
@@ -686,7 +674,7 @@ __global__ __launch_bounds__(256) void k_scan(Dev d) {
       packet_first = it.x;
       k = load_packet(d, packet_first, lane);
     }
-    scan_range(d, diag, st, cand_mem[threadIdx.x >> 6], lane, k, it.y, it.z);
+    scan_range(d, diag, st, lane, k, it.y, it.z);
   }
   if (st.count) flush_pairs(d, diag, st, lane);
 }

@@ -212,11 +212,11 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
       uint32_t flags = 0;
       // Register both contact sides with their bodies for the gather.
       uint32_t sa = atomicAdd(&d.slot_count[body_a], 1u);
-      if (sa < kMaxSlots) d.incid[(size_t)body_a * kMaxSlots + sa] = 2u * a;
+      if (sa < kMaxSlots) d.incid[(size_t)sa * d.body_count + body_a] = 2u * a;
       else flags |= kOverflowA;
       if (body_b != kInvalid) {
         uint32_t sb = atomicAdd(&d.slot_count[body_b], 1u);
-        if (sb < kMaxSlots) d.incid[(size_t)body_b * kMaxSlots + sb] = 2u * a + 1u;
+        if (sb < kMaxSlots) d.incid[(size_t)sb * d.body_count + body_b] = 2u * a + 1u;
         else flags |= kOverflowB;
       }
       d.active[a] = ci;
@@ -334,9 +334,8 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
     uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
     const uint32_t count = d.slot_count[b];
     const uint32_t n = count < kMaxSlots ? count : kMaxSlots;
-    const uint32_t* slots = d.incid + (size_t)b * kMaxSlots;
     for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t side = slots[k];
+      const uint32_t side = d.incid[(size_t)k * d.body_count + b];
       const int4 x = d.cdelta[2 * (size_t)side];
       const int4 y = d.cdelta[2 * (size_t)side + 1];
       acc[0] += (uint32_t)x.x;

@@ -260,3 +260,24 @@ def test_small_stacks_many_ticks(count):
         ora.step(tick)
         if tick % 60 == 0:
             assert_bodies_equal(gpu, ora, f"{count} bodies, tick {tick}")
+
+
+def test_driver_publishes_and_stops_on_flags():
+    from mechanic_amd import runtime as rt
+    scene = mh.named_pile("pile_1k")
+    gpu = mh.GpuPhysics.new(scene)
+    runtime = rt.PhysicsRuntime()
+    runtime.load(scene)
+    runtime.start()
+    driver = rt.PhysicsDriver(gpu, runtime)
+    assert driver.update(250_000_000) == 15
+    p = runtime.published_state()
+    assert p.current_tick == 15 and p.current_slot == 0
+    snap = gpu.read_snapshot_transforms(p.current_slot)
+    pose, _ = gpu.read_bodies()
+    assert (snap == pose).all()
+    poses, vel = gpu.read_bodies()
+    vel[3, 1] = float("nan")
+    gpu.write_bodies(poses, vel)
+    driver.update(100_000_000)
+    assert runtime.failure() == rt.InvalidNumericState()

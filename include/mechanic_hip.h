@@ -160,10 +160,17 @@ typedef struct {
   uint32_t solver_iterations;         /* device.rs:30, default 8 */
   float friction;                     /* CONTACT_FRICTION collision.wgsl:66, default 0.05 */
   uint32_t pair_capacity;             /* MAX_CONTACT_PAIRS lib.rs:45 (power of two) */
-  uint32_t warm_start_mode;           /* 0 = persistent-manifold cache (reference) */
+  uint32_t warm_start_mode;           /* MH_WARM_START_*; default MH_WARM_START_CACHE */
   uint32_t use_graph;                 /* 1 = replay each tick as a captured hipGraph */
-  uint32_t reserved;
+  uint32_t residual;                  /* 1 = measure the contact residual after the solve */
 } mh_config;
+
+/* Warm start of the contact solver's first pass. */
+enum {
+  MH_WARM_START_CACHE = 0,   /* persistent-manifold cache, halved impulse (collision.wgsl:699-733, 783) */
+  MH_WARM_START_LOG = 1,     /* as CACHE, and records training rows (features, solved impulses) */
+  MH_WARM_START_LEARNED = 2  /* initial impulses predicted by the network set with mh_physics_set_mlp */
+};
 
 /* GpuTickReadback (device.rs:74-92) with GpuKernelTimings (device.rs:95-111). */
 typedef struct {
@@ -238,6 +245,57 @@ MH_API int mh_physics_write_drives(mh_physics* physics, const mh_mechanism_drive
                                    mh_error_detail* error);
 MH_API int mh_physics_init_coordinates(mh_physics* physics, const mh_mechanism_coordinate* coordinates,
                                        size_t count, mh_error_detail* error);
+
+/* ---- learned warm start (no counterpart in the reference) --------------
+ * A 16 -> 64 -> 64 -> 3 GELU network (tanh form) predicts, per active contact,
+ * the accumulated normal impulse and the friction impulse (two components in
+ * the contact's tangent basis) from 16 contact features. Inference inside a
+ * tick runs on MFMA in BF16 with FP32 accumulation; training runs on the GPU in
+ * FP32 (forward, backward, Adam). Parameter layout, floats: W1[16][64] b1[64]
+ * W2[64][64] b2[64] W3[64][3] b3[3] (5443), then input mean[16], 1/std[16],
+ * output mean[3], std[3] (5481 in all). */
+typedef struct mh_mlp mh_mlp;
+
+#define MH_MLP_FEATURES 16
+#define MH_MLP_OUTPUTS 3
+
+MH_API int mh_mlp_create(uint64_t seed, mh_mlp** out);
+MH_API void mh_mlp_destroy(mh_mlp* mlp);
+MH_API uint32_t mh_mlp_param_count(void);
+MH_API uint32_t mh_mlp_net_floats(void);
+MH_API int mh_mlp_get_net(mh_mlp* mlp, float* out, size_t count);
+MH_API int mh_mlp_set_net(mh_mlp* mlp, const float* net, size_t count);
+/* Uploads `rows` training rows (16 features, 3 targets each) and sets the
+ * normalisation to the dataset's per-column mean and standard deviation. */
+MH_API int mh_mlp_set_dataset(mh_mlp* mlp, const float* features, const float* targets, size_t rows);
+/* Mini-batch Adam (beta 0.9 / 0.999) over the dataset, shuffled per epoch by
+ * the handle's seeded generator; losses[e] = mean batch loss of epoch e. */
+MH_API int mh_mlp_train(mh_mlp* mlp, uint32_t epochs, uint32_t batch, float learning_rate, float* losses);
+/* Loss and gradient over the whole dataset (no update), FP32. */
+MH_API int mh_mlp_gradient(mh_mlp* mlp, float* gradient, float* loss);
+/* MFMA inference of host feature rows; out receives 3 floats per row. */
+MH_API int mh_mlp_predict(mh_mlp* mlp, const float* features, size_t rows, float* out);
+
+/* Attaches a network to a handle (NULL detaches). The network must outlive its
+ * use; weights changed by training are seen by the next tick. */
+MH_API int mh_physics_set_mlp(mh_physics* physics, const mh_mlp* mlp);
+/* MH_WARM_START_LOG handles: the last tick's training rows (features, then the
+ * solved normal impulse and friction impulse in the tangent basis). */
+MH_API int mh_physics_read_training_rows(mh_physics* physics, float* features, float* targets, size_t capacity,
+                                         size_t* rows);
+
+/* Normal complementarity residual of the last tick, per active contact
+ * r = |max(0, lambda + (bias - v_n) / d) - lambda| after the last pass. */
+typedef struct {
+  double rms;                 /* sqrt(mean r^2) (N s) */
+  double mean_penetration;    /* (m) */
+  double max_penetration;     /* (m) */
+  uint32_t active_contacts;
+  uint32_t reserved;
+} mh_contact_residual;
+
+/* Handles created with config.residual = 1. */
+MH_API int mh_physics_read_residual(mh_physics* physics, mh_contact_residual* out);
 
 enum {
   MH_BUFFER_MORTON_KEYS = 1,     /* u64 (code << 32 | collider) per sort slot, sorted after a tick */

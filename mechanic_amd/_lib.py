@@ -53,6 +53,12 @@ BUFFER_WORLD_MASSES = 8
 BUFFER_MANIFOLDS = 9
 BUFFER_NODE_PARENTS = 10
 
+WARM_START_CACHE = 0
+WARM_START_LOG = 1
+WARM_START_LEARNED = 2
+MLP_FEATURES = 16
+MLP_OUTPUTS = 3
+
 
 class TickConfig(ctypes.Structure):
     _fields_ = [(n, c_uint32) for n in ("body_count", "tick_index", "snapshot_slot", "collider_count")] + [
@@ -139,7 +145,12 @@ class Creation(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [("collisions_enabled", c_uint32), ("mechanism_self_collisions", c_uint32),
                 ("solver_iterations", c_uint32), ("friction", c_float), ("pair_capacity", c_uint32),
-                ("warm_start_mode", c_uint32), ("use_graph", c_uint32), ("reserved", c_uint32)]
+                ("warm_start_mode", c_uint32), ("use_graph", c_uint32), ("residual", c_uint32)]
+
+
+class ContactResidual(ctypes.Structure):
+    _fields_ = [("rms", c_double), ("mean_penetration", c_double), ("max_penetration", c_double),
+                ("active_contacts", c_uint32), ("reserved", c_uint32)]
 
 
 class TickReadback(ctypes.Structure):
@@ -213,6 +224,20 @@ def load() -> ctypes.CDLL:
         "mh_physics_debug_tick": (c_int, [H, c_uint64, c_int]),
         "mh_physics_write_drives": (c_int, [H, POINTER(MechanismDrive), c_size_t, POINTER(ErrorDetail)]),
         "mh_physics_init_coordinates": (c_int, [H, POINTER(MechanismCoordinate), c_size_t, POINTER(ErrorDetail)]),
+        "mh_mlp_create": (c_int, [c_uint64, POINTER(H)]),
+        "mh_mlp_destroy": (None, [H]),
+        "mh_mlp_param_count": (c_uint32, []),
+        "mh_mlp_net_floats": (c_uint32, []),
+        "mh_mlp_get_net": (c_int, [H, POINTER(c_float), c_size_t]),
+        "mh_mlp_set_net": (c_int, [H, POINTER(c_float), c_size_t]),
+        "mh_mlp_set_dataset": (c_int, [H, POINTER(c_float), POINTER(c_float), c_size_t]),
+        "mh_mlp_train": (c_int, [H, c_uint32, c_uint32, c_float, POINTER(c_float)]),
+        "mh_mlp_gradient": (c_int, [H, POINTER(c_float), POINTER(c_float)]),
+        "mh_mlp_predict": (c_int, [H, POINTER(c_float), c_size_t, POINTER(c_float)]),
+        "mh_physics_set_mlp": (c_int, [H, H]),
+        "mh_physics_read_training_rows": (c_int, [H, POINTER(c_float), POINTER(c_float), c_size_t,
+                                                  POINTER(c_size_t)]),
+        "mh_physics_read_residual": (c_int, [H, POINTER(ContactResidual)]),
         "mh_scene_cuboid_pile": (c_int, [POINTER(PileParams), POINTER(Transform), POINTER(c_float), POINTER(Mass),
                                          POINTER(SpatialInertia), POINTER(Collider), c_size_t]),
     }

@@ -20,6 +20,9 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kMaxSlots = 32;
 constexpr uint32_t kOverflowA = 0x40000000u;  // solver row flag bits in the contact-index word
 constexpr uint32_t kOverflowB = 0x80000000u;
+constexpr uint32_t kFeatures = 16;      // learned warm-start inputs per contact
+constexpr uint32_t kResidBlocks = 512;  // residual kernel grid
+enum LearnMode : uint32_t { kLearnOff = 0, kLearnLog = 1, kLearnInfer = 2 };
 
 // Diagnostics words (abi.rs:56-73) plus private counters after them.
 enum DiagWord : uint32_t {
@@ -111,6 +114,15 @@ struct Dev {
   uint64_t* htab[2];      // open-addressing index: key -> manifold row
   float4* mrows[2];       // 3 per contact: pair_tick bits, normal/penetration, point/impulse
 
+  // ---- learned warm start and contact residual ----
+  uint32_t learn_mode;    // kLearnOff, kLearnLog or kLearnInfer
+  uint32_t residual_on;
+  float* feat;            // kFeatures per active contact
+  float4* fsum;           // friction impulse applied over the passes, per active contact
+  float4* finit;          // predicted initial friction impulse, per active contact
+  double* resid;          // 4 per residual block: sum r^2, active contacts, sum penetration, max penetration
+  const float* mlp;       // network parameters (mlp.hip layout), or null
+
   // ---- publication ----
   float4* snap_pos[3];
   float4* snap_rot[3];
@@ -155,5 +167,37 @@ void launch_traverse(hipStream_t s, const Dev& d);
 void launch_narrowphase(hipStream_t s, const Dev& d);
 void launch_prepare(hipStream_t s, const Dev& d);
 void launch_solve(hipStream_t s, const Dev& d);
+void launch_features(hipStream_t s, const Dev& d);
+void launch_targets(hipStream_t s, const Dev& d);
+void launch_mlp_infer(hipStream_t s, const Dev& d);
+void launch_residual(hipStream_t s, const Dev& d);
+
+}  // namespace mh
+
+struct mh_mlp;
+
+namespace mh {
+
+// Device parameters of a learned warm-start network (mlp.hip).
+const float* mlp_device_net(const mh_mlp* m);
+
+// Tangent basis of a contact normal: t1 = normalize(n x e), t2 = n x t1, with e
+// the x axis unless n is within ~25 degrees of it (then the y axis).
+__device__ __forceinline__ void tangent_basis(float nx, float ny, float nz, float* t1, float* t2) {
+  float ex = 1.0f, ey = 0.0f;
+  if (fabsf(nx) > 0.9f) {
+    ex = 0.0f;
+    ey = 1.0f;
+  }
+  // n x e with e = (ex, ey, 0)
+  float ax = ny * 0.0f - nz * ey, ay = nz * ex - nx * 0.0f, az = nx * ey - ny * ex;
+  float inv = 1.0f / sqrtf((ax * ax + ay * ay) + az * az);
+  t1[0] = ax * inv;
+  t1[1] = ay * inv;
+  t1[2] = az * inv;
+  t2[0] = ny * t1[2] - nz * t1[1];
+  t2[1] = nz * t1[0] - nx * t1[2];
+  t2[2] = nx * t1[1] - ny * t1[0];
+}
 
 }  // namespace mh

@@ -10,6 +10,7 @@
 #include "internal.h"
 #include "mh_geometry.h"
 #include "mh_math.h"
+#include "mlp_net.h"
 
 namespace mh {
 
@@ -133,6 +134,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
     const uint32_t ci = base + threadIdx.x;
     bool active = false;
     float4 r0, r1, r2, r3;
+    float cached_lambda = 0.0f;
     if (ci < xs.end) {
       const float4* c = d.contacts + 4 * (size_t)ci;
       const float4 m = c[0];
@@ -164,6 +166,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
         const uint32_t start = pair_hash(low, high) & mask;
         // Look the pair up among the previous tick's manifolds.
         float cached_impulse = 0.0f;
+        bool hit = false;
         for (uint32_t probe = 0; probe < kMaxProbes; ++probe) {
           uint64_t e = table_prev[(start + probe) & mask];
           if (e == 0ull) break;
@@ -173,6 +176,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
             bool matches = __float_as_uint(pt.z) == tick - 1u && dot(xyz(cn), normal) >= kNormalAlignment &&
                            length(xyz(cp) - arm_a) <= kPointMovement;
             cached_impulse = matches ? cp.w : 0.0f;
+            hit = matches;
             break;
           }
         }
@@ -197,7 +201,8 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
         if (n_pen.w > 1.0e-5f || normal_speed < -1.0e-5f || cached_impulse > 1.0e-6f) {
           active = true;
           r0 = n_pen;
-          r1 = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, cached_impulse);
+          r1 = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, hit ? 1.0f : 0.0f);
+          cached_lambda = cached_impulse;
           r2 = arm_b4;
           float denom = denom_of(sa, sb, has_b, arm_a, arm_b, normal);
           r3 = make_float4(m.x, m.y, __uint_as_float(ci), denom);
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
         else flags |= kOverflowB;
       }
       d.active[a] = ci;
-      d.lambda[a] = r1.w;
+      d.lambda[a] = cached_lambda;
       float4* row = d.rows + 4 * (size_t)a;
       row[0] = r0;
       row[1] = r1;
@@ -272,7 +277,9 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
         const float step = (kRelaxation * (-normal_speed + bias)) / denominator;
         float bound;
         if (first) {
-          const float warmed = lambda * (analytic ? 1.0f : kWarmStartScale);
+          // A learned prediction replaces the halved cached impulse in full.
+          const float scale = (analytic || d.learn_mode == kLearnInfer) ? 1.0f : kWarmStartScale;
+          const float warmed = lambda * scale;
           const float accumulated = fmx(warmed + step, 0.0f);
           lambda = accumulated;
           impulse = normal * accumulated;
@@ -287,16 +294,30 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
         }
         const v3 tangent_velocity = relative - normal * normal_speed;
         const float tangent_speed = length(tangent_velocity);
-        if (tangent_speed > 1.0e-6f) {
+        v3 friction_vec = mk(0.0f, 0.0f, 0.0f);
+        if (first && d.learn_mode == kLearnInfer) {
+          // Learned warm start: the predicted friction impulse, inside the cone.
+          const float4 f = d.finit[ai];
+          const float mag = length(xyz(f));
+          const float cap = bound;
+          friction_vec = mag > cap && mag > 0.0f ? xyz(f) * (cap / mag) : xyz(f);
+          impulse = impulse + friction_vec;
+        } else if (tangent_speed > 1.0e-6f) {
           const v3 tangent = tangent_velocity / tangent_speed;
           const float tangent_denominator = denom_of(sa, sb, has_b, arm_a, arm_b, tangent);
           const float friction_impulse = fmn(tangent_speed / tangent_denominator, bound);
+          friction_vec = -(tangent * friction_impulse);
           impulse = impulse - tangent * friction_impulse;
+        }
+        if (d.learn_mode == kLearnLog) {
+          const float4 acc = first ? make_float4(0.f, 0.f, 0.f, 0.f) : d.fsum[ai];
+          d.fsum[ai] = make_float4(acc.x + friction_vec.x, acc.y + friction_vec.y, acc.z + friction_vec.z, 0.0f);
         }
         applies = true;
       }
     }
     d.lambda[ai] = lambda;
+    if (first && !applies && d.learn_mode == kLearnLog) d.fsum[ai] = make_float4(0.f, 0.f, 0.f, 0.f);
     // A contact that applies no impulse adds zero deltas (collision.wgsl:793-804).
     const Q20 qa = applies ? side_delta(sa, arm_a, -impulse) : Q20{};
     if (word & kOverflowA) {
@@ -367,6 +388,201 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
   }
 }
 
+// Inputs of the learned warm start for one active contact, from the state the
+// first solver pass sees: relative normal and tangential speed, penetration,
+// bias, both inverse masses, both inverse inertias along the contact frame
+// (n, t1, t2), the normal denominator, the cached impulse, whether the cache
+// matched, and whether the contact is against the ground. Also returns the
+// tangent basis.
+__device__ __forceinline__ void contact_features(const Dev& d, uint32_t ai, float f[kFeatures], v3& e1, v3& e2) {
+  const float4* row = d.rows + 4 * (size_t)ai;
+  const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+  const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+  const bool has_b = body_b != kInvalid;
+  const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
+  const SolverBody sa = load_body(d, body_a);
+  const SolverBody sb = has_b ? load_body(d, body_b) : sa;
+  const v3 vb = has_b ? sb.lin + cross(sb.ang, arm_b) : mk(0.0f, 0.0f, 0.0f);
+  const v3 relative = vb - (sa.lin + cross(sa.ang, arm_a));
+  const float normal_speed = dot(relative, normal);
+  const float tangent_speed = length(relative - normal * normal_speed);
+  const bool analytic = r2.w < 0.0f;
+  const float bias = fmn((fmx(r0.w - kSlop, 0.0f) * (analytic ? 1.0f : 0.2f)) / d.delta_seconds, kMaxCorrection);
+  float t1[3], t2[3];
+  tangent_basis(normal.x, normal.y, normal.z, t1, t2);
+  e1 = mk(t1[0], t1[1], t1[2]);
+  e2 = mk(t2[0], t2[1], t2[2]);
+  f[0] = normal_speed;
+  f[1] = tangent_speed;
+  f[2] = r0.w;
+  f[3] = bias;
+  f[4] = sa.inv_mass;
+  f[5] = has_b ? sb.inv_mass : 0.0f;
+  f[6] = dot(inv_inertia(sa, normal), normal);
+  f[7] = dot(inv_inertia(sa, e1), e1);
+  f[8] = dot(inv_inertia(sa, e2), e2);
+  f[9] = has_b ? dot(inv_inertia(sb, normal), normal) : 0.0f;
+  f[10] = has_b ? dot(inv_inertia(sb, e1), e1) : 0.0f;
+  f[11] = has_b ? dot(inv_inertia(sb, e2), e2) : 0.0f;
+  f[12] = r3.w;
+  f[13] = d.lambda[ai];
+  f[14] = r1.w;
+  f[15] = has_b ? 0.0f : 1.0f;
+}
+
+// Logging: the features of every active contact, row-major.
+__global__ __launch_bounds__(256) void k_features(Dev d) {
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
+  const XcdSplit xs = xcd_split(count);
+  for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
+    float f[kFeatures];
+    v3 e1, e2;
+    contact_features(d, ai, f, e1, e2);
+    float4* o = reinterpret_cast<float4*>(d.feat + (size_t)ai * kFeatures);
+#pragma unroll
+    for (int k = 0; k < kFeatures / 4; ++k) o[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+  }
+}
+
+// Logging: the training targets of every active contact after the last pass --
+// the accumulated normal impulse and the friction impulse applied over the
+// passes in the contact's tangent basis.
+__global__ __launch_bounds__(256) void k_targets(Dev d) {
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
+  const XcdSplit xs = xcd_split(count);
+  for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
+    const float4 n = d.rows[4 * (size_t)ai];
+    float t1[3], t2[3];
+    tangent_basis(n.x, n.y, n.z, t1, t2);
+    const v3 f = xyz(d.fsum[ai]);
+    d.finit[ai] = make_float4(d.lambda[ai], dot(f, mk(t1[0], t1[1], t1[2])), dot(f, mk(t2[0], t2[1], t2[2])), 0.0f);
+  }
+}
+
+// Inference: features in registers, staged through LDS into the MFMA operand
+// layout, the network on MFMA, and the predicted initial normal impulse and
+// friction impulse written for the first solver pass. Each wave takes 64
+// contacts at a time: lane l computes contact l's features, then four groups
+// of 16 contacts run through the network.
+__global__ __launch_bounds__(256) void k_learned(Dev d) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ float sfeat[kWaves][64][kFeatures + 1];
+  __shared__ float sout[kWaves][64][4];
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
+  MlpRegs m;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  mlp_load(d.mlp, lane, m);
+  const uint32_t chunks = (count + 63) / 64;
+  const uint32_t waves = gridDim.x * kWaves;
+  for (uint32_t chunk = blockIdx.x * kWaves + wave; chunk < chunks; chunk += waves) {
+    const uint32_t ai = chunk * 64 + lane;
+    v3 e1 = mk(0.f, 0.f, 0.f), e2 = e1;
+    float f[kFeatures];
+    if (ai < count) {
+      contact_features(d, ai, f, e1, e2);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kFeatures; ++k) f[k] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < kFeatures; ++k) sfeat[wave][lane][k] = f[k];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const uint32_t c = g * 16 + lane % 16, k0 = 4 * (lane / 16);
+      const float x[4] = {sfeat[wave][c][k0], sfeat[wave][c][k0 + 1], sfeat[wave][c][k0 + 2], sfeat[wave][c][k0 + 3]};
+      float y[3];
+      mlp_group(m, x, y);
+      if (lane < 16) {
+        sout[wave][c][0] = y[0];
+        sout[wave][c][1] = y[1];
+        sout[wave][c][2] = y[2];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (ai < count) {
+      const float y0 = sout[wave][lane][0], y1 = sout[wave][lane][1], y2 = sout[wave][lane][2];
+      d.lambda[ai] = fmx(y0, 0.0f);
+      d.finit[ai] = f4(e1 * y1 + e2 * y2, 0.0f);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Normal complementarity residual after the last pass, per active contact:
+// r = |max(0, lambda + (bias - v_n) / d) - lambda|, the impulse change an
+// unrelaxed projected step would still make (zero when the contact is either
+// separating with no impulse or holding exactly its bias speed). Per-block
+// partial sums in FP64; the host adds the blocks in order.
+__global__ __launch_bounds__(256) void k_residual(Dev d) {
+  __shared__ double part[4][kBlock / 64];
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
+  double sum_r2 = 0.0, n = 0.0, sum_pen = 0.0, max_pen = 0.0;
+  for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < count; ai += gridDim.x * blockDim.x) {
+    const float4* row = d.rows + 4 * (size_t)ai;
+    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+    const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+    const bool has_b = body_b != kInvalid;
+    const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
+    const SolverBody sa = load_body(d, body_a);
+    const v3 vb = has_b ? [&] { const SolverBody sb = load_body(d, body_b); return sb.lin + cross(sb.ang, arm_b); }()
+                        : mk(0.0f, 0.0f, 0.0f);
+    const float normal_speed = dot(vb - (sa.lin + cross(sa.ang, arm_a)), normal);
+    const bool analytic = r2.w < 0.0f;
+    const double bias = (double)fmn((fmx(r0.w - kSlop, 0.0f) * (analytic ? 1.0f : 0.2f)) / d.delta_seconds,
+                                    kMaxCorrection);
+    const double lambda = (double)d.lambda[ai];
+    double r = 0.0;
+    if (r3.w > 0.0f) {
+      double next = lambda + (bias - (double)normal_speed) / (double)r3.w;
+      if (next < 0.0) next = 0.0;
+      r = next - lambda;
+    }
+    sum_r2 += r * r;
+    n += 1.0;
+    sum_pen += (double)r0.w;
+    max_pen = (double)r0.w > max_pen ? (double)r0.w : max_pen;
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    sum_r2 += __shfl_xor(sum_r2, m);
+    n += __shfl_xor(n, m);
+    sum_pen += __shfl_xor(sum_pen, m);
+    const double o = __shfl_xor(max_pen, m);
+    max_pen = o > max_pen ? o : max_pen;
+  }
+  const uint32_t wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0) {
+    part[0][wave] = sum_r2;
+    part[1][wave] = n;
+    part[2][wave] = sum_pen;
+    part[3][wave] = max_pen;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0, c = 0.0, m = 0.0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+      a += part[0][w];
+      b += part[1][w];
+      c += part[2][w];
+      m = part[3][w] > m ? part[3][w] : m;
+    }
+    double* o = d.resid + 4 * (size_t)blockIdx.x;
+    o[0] = a;
+    o[1] = b;
+    o[2] = c;
+    o[3] = m;
+  }
+}
+
 static inline uint32_t grid_for(uint32_t n, uint32_t cap) {
   uint32_t b = (n + kBlock - 1) / kBlock;
   return b < cap ? b : cap;
@@ -386,5 +602,19 @@ void launch_solve(hipStream_t s, const Dev& d) {
     if (body_grid) k_apply<<<body_grid, kBlock, 0, s>>>(d, last);
   }
 }
+
+}  // namespace mh
+
+namespace mh {
+
+void launch_features(hipStream_t s, const Dev& d) {
+  k_features<<<grid_for(d.pair_capacity, 2048), kBlock, 0, s>>>(d);
+}
+
+void launch_targets(hipStream_t s, const Dev& d) { k_targets<<<grid_for(d.pair_capacity, 2048), kBlock, 0, s>>>(d); }
+
+void launch_mlp_infer(hipStream_t s, const Dev& d) { k_learned<<<1024, kBlock, 0, s>>>(d); }
+
+void launch_residual(hipStream_t s, const Dev& d) { k_residual<<<kResidBlocks, kBlock, 0, s>>>(d); }
 
 }  // namespace mh

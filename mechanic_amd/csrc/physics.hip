@@ -39,6 +39,7 @@ struct mh_physics {
   uint32_t* host_diag = nullptr;  // pinned
   hipEvent_t events[10] = {};
   Timing timing;
+  const mh_mlp* mlp = nullptr;
 };
 
 namespace {
@@ -119,8 +120,12 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0) {
     mark(6);
     if (stop == 3) return;
     launch_prepare(s, d);
+    if (d.learn_mode == kLearnLog) launch_features(s, d);
+    if (d.learn_mode == kLearnInfer) launch_mlp_infer(s, d);
     mark(7);
     launch_solve(s, d);
+    if (d.learn_mode == kLearnLog) launch_targets(s, d);
+    if (d.residual_on) launch_residual(s, d);
     mark(8);
   } else {
     if (stop != 0) return;
@@ -131,6 +136,7 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0) {
 }
 
 int begin_tick(mh_physics* p, uint64_t tick) {
+  if (p->d.learn_mode == kLearnInfer && p->d.mlp == nullptr) return MH_ERR_INVALID_ARGUMENT;
   if (!p->tick_known || p->expected_tick != tick) {
     launch_set_tick(p->stream, p->d, tick);
     HIP_TRY_NOERR(hipGetLastError());
@@ -229,6 +235,8 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   if (cfg.pair_capacity == 0) cfg.pair_capacity = MH_MAX_CONTACT_PAIRS;
   if (!(cfg.friction >= 0.0f) || !std::isfinite(cfg.friction))
     return fail(error, MH_ERR_INVALID_ARGUMENT, "friction must be finite and non-negative");
+  if (cfg.warm_start_mode > MH_WARM_START_LEARNED)
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "unknown warm_start_mode");
   if ((cfg.pair_capacity & (cfg.pair_capacity - 1)) != 0 || cfg.pair_capacity < 256 ||
       cfg.pair_capacity > (1u << 22))
     return fail(error, MH_ERR_INVALID_ARGUMENT, "pair_capacity must be a power of two in [256, 2^22]");
@@ -272,6 +280,8 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   d.linear_damping = 0.999f;       // device.rs:873
   d.angular_damping = 0.98f;       // device.rs:874
   d.friction = cfg.friction;
+  d.learn_mode = cfg.warm_start_mode;
+  d.residual_on = cfg.residual ? 1u : 0u;
 
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
   float* inv_mass;
@@ -319,6 +329,12 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   }
   ALLOC(diag, 2 * kDiagWords);
   ALLOC(tick, 1);
+  if (d.learn_mode != kLearnOff) {
+    ALLOC(d.feat, (size_t)kFeatures * cfg.pair_capacity);
+    ALLOC(d.fsum, cfg.pair_capacity);
+    ALLOC(d.finit, cfg.pair_capacity);
+  }
+  ALLOC(d.resid, 4 * (size_t)kResidBlocks);
   d.pos = pos; d.rot = rot; d.lin = lin; d.ang = ang;
   d.inv_mass = inv_mass; d.masses = masses; d.world_masses = wm;
   d.root_flags = roots; d.components = comps;
@@ -646,6 +662,62 @@ int mh_physics_init_coordinates(mh_physics* p, const mh_mechanism_coordinate* co
     if (error) { error->provided = count; error->required = 0; }
     return fail(error, MH_ERR_COORDINATE_STATE_COUNT, "coordinate state row count does not match the scene");
   }
+  return MH_OK;
+}
+
+int mh_physics_set_mlp(mh_physics* p, const mh_mlp* mlp) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  p->mlp = mlp;
+  p->d.mlp = mlp_device_net(mlp);
+  // Kernel arguments are captured by value: capture again with the new pointer.
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+  return MH_OK;
+}
+
+int mh_physics_read_training_rows(mh_physics* p, float* features, float* targets, size_t capacity, size_t* rows) {
+  if (!p || !features || !targets) return MH_ERR_INVALID_ARGUMENT;
+  if (p->d.learn_mode != kLearnLog) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  const uint64_t tick = p->has_last_tick ? p->last_tick : 0;
+  uint32_t diag_host[kDiagWords] = {};
+  HIP_TRY_NOERR(hipMemcpy(diag_host, p->d.diag[tick & 1], sizeof(diag_host), hipMemcpyDeviceToHost));
+  const size_t n = diag_host[kDiagActive] < p->d.pair_capacity ? diag_host[kDiagActive] : p->d.pair_capacity;
+  if (rows) *rows = n;
+  if (capacity < n) return MH_ERR_INVALID_ARGUMENT;
+  if (n == 0) return MH_OK;
+  HIP_TRY_NOERR(hipMemcpy(features, p->d.feat, n * kFeatures * sizeof(float), hipMemcpyDeviceToHost));
+  std::vector<float4> t(n);
+  HIP_TRY_NOERR(hipMemcpy(t.data(), p->d.finit, n * sizeof(float4), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) {
+    targets[3 * i] = t[i].x;
+    targets[3 * i + 1] = t[i].y;
+    targets[3 * i + 2] = t[i].z;
+  }
+  return MH_OK;
+}
+
+int mh_physics_read_residual(mh_physics* p, mh_contact_residual* out) {
+  if (!p || !out) return MH_ERR_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  if (!p->d.residual_on) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  std::vector<double> part(4 * (size_t)kResidBlocks);
+  HIP_TRY_NOERR(hipMemcpy(part.data(), p->d.resid, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double r2 = 0.0, n = 0.0, pen = 0.0, mx = 0.0;
+  for (uint32_t b = 0; b < kResidBlocks; ++b) {
+    r2 += part[4 * b];
+    n += part[4 * b + 1];
+    pen += part[4 * b + 2];
+    mx = part[4 * b + 3] > mx ? part[4 * b + 3] : mx;
+  }
+  out->active_contacts = (uint32_t)n;
+  out->rms = n > 0.0 ? std::sqrt(r2 / n) : 0.0;
+  out->mean_penetration = n > 0.0 ? pen / n : 0.0;
+  out->max_penetration = mx;
   return MH_OK;
 }
 

@@ -65,11 +65,13 @@ class GpuPhysicsConfig:
     friction: float = 0.05  # CONTACT_FRICTION, collision.wgsl:66
     pair_capacity: int = _lib.MAX_CONTACT_PAIRS
     use_graph: bool = True
+    warm_start: int = _lib.WARM_START_CACHE  # WARM_START_CACHE / _LOG / _LEARNED
+    residual: bool = False  # measure the contact residual after each tick's solve
 
     def to_c(self) -> _lib.Config:
         return _lib.Config(int(self.collisions_enabled), int(self.mechanism_self_collisions),
-                           int(self.solver_iterations), float(self.friction), int(self.pair_capacity), 0,
-                           int(self.use_graph), 0)
+                           int(self.solver_iterations), float(self.friction), int(self.pair_capacity),
+                           int(self.warm_start), int(self.use_graph), int(self.residual))
 
 
 @dataclass
@@ -291,6 +293,32 @@ class GpuPhysics:
         if rc in _PHYSICS_KINDS:
             raise GpuPhysicsError(_PHYSICS_KINDS[rc], err.required, err.capacity, err.provided, err.message.decode())
         _check(rc, "initialize_mechanism_coordinates")
+
+
+    # --- learned warm start and contact residual (no counterpart in the reference) ---
+    def set_mlp(self, mlp) -> None:
+        """Attaches a `WarmStartNet` (None detaches) for WARM_START_LEARNED handles."""
+        _check(_lib.load().mh_physics_set_mlp(self._h, mlp._h if mlp is not None else None), "set_mlp")
+        self._mlp = mlp  # keep the network alive while attached
+
+    def read_training_rows(self):
+        """(features (R, 16), targets (R, 3)) of the last tick of a WARM_START_LOG handle."""
+        lib = _lib.load()
+        rows = ctypes.c_size_t()
+        cap = max(int(self.config.pair_capacity), 1)
+        f = np.empty((cap, _lib.MLP_FEATURES), np.float32)
+        t = np.empty((cap, _lib.MLP_OUTPUTS), np.float32)
+        fp = f.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        tp = t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        _check(lib.mh_physics_read_training_rows(self._h, fp, tp, cap, ctypes.byref(rows)), "read_training_rows")
+        return f[: rows.value].copy(), t[: rows.value].copy()
+
+    def read_residual(self) -> dict:
+        """Contact residual of the last tick of a handle created with residual=True."""
+        r = _lib.ContactResidual()
+        _check(_lib.load().mh_physics_read_residual(self._h, ctypes.byref(r)), "read_residual")
+        return dict(rms=r.rms, mean_penetration=r.mean_penetration, max_penetration=r.max_penetration,
+                    active_contacts=r.active_contacts)
 
 
 def device_count() -> int:

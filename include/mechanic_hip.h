@@ -138,7 +138,7 @@ typedef struct { uint32_t metadata[4]; } mh_contraction_node; /* abi.rs:292-296 
 typedef struct {
   uint32_t body_count;
   uint32_t collider_count;
-  uint32_t bearing_count;     /* articulated rows; must be 0 in this build */
+  uint32_t bearing_count;     /* bearing rows (tree and loop-closure) */
   uint32_t suppression_count;
   const mh_transform* poses;          /* root pose per body (device.rs:429-441) */
   const float* inverse_masses;        /* per body (device.rs:443-447) */
@@ -149,6 +149,12 @@ typedef struct {
   const mh_pair* suppressed_pairs;    /* sorted (low, high) body pairs, device.rs:516-523 */
   const uint32_t* body_components;    /* NULL: every body its own component (device.rs:448-453) */
   const uint32_t* root_flags;         /* NULL: every body a root (device.rs:1699-1704) */
+  /* Articulated rows (device.rs:1684-1808); all NULL / 0 when bearing_count == 0. */
+  const mh_mechanism_body* mechanism_bodies; /* per body: parent, tree bearing, direction, root flag */
+  const uint32_t* preorder;                  /* bodies with every parent before its children */
+  const mh_mechanism_drive* drives;          /* per coordinate; NULL: every coordinate passive */
+  uint32_t coordinate_count;                 /* tree bearings (one coordinate each) */
+  uint32_t reserved;
 } mh_creation;
 
 /* GpuPhysicsConfig (device.rs:22-41) plus the constants the reference hard-codes,
@@ -187,6 +193,8 @@ typedef struct {
 typedef struct {
   double integrate_ms, morton_ms, sort_ms, lbvh_ms, traverse_ms;
   double narrowphase_ms, prepare_ms, solve_ms, persist_ms, snapshot_ms, total_ms;
+  double mechanism_ms;   /* bearing projection, coordinates, forward kinematics, closures */
+  double validation_ms;  /* articulated-state and bearing validation (with the snapshot) */
 } mh_phase_timings;
 
 typedef struct {
@@ -239,12 +247,15 @@ MH_API int mh_physics_synchronize(mh_physics* physics);
  * generation, 0 = whole tick. A stopped tick leaves the handle mid-tick. */
 MH_API int mh_physics_debug_tick(mh_physics* physics, uint64_t tick_index, int stop);
 /* GpuPhysics::write_mechanism_drives / initialize_mechanism_coordinates
- * (device.rs:1622-1662); the articulated path is not built yet: they validate the
- * row count like the reference and return MH_ERR_UNSUPPORTED for non-empty input. */
+ * (device.rs:1622-1662): replace every coordinate's drive row or state; the row
+ * count must equal the scene's coordinate count (MH_ERR_DRIVE_STATE_COUNT /
+ * MH_ERR_COORDINATE_STATE_COUNT otherwise). Ordered after the handle's queued ticks. */
 MH_API int mh_physics_write_drives(mh_physics* physics, const mh_mechanism_drive* drives, size_t count,
                                    mh_error_detail* error);
 MH_API int mh_physics_init_coordinates(mh_physics* physics, const mh_mechanism_coordinate* coordinates,
                                        size_t count, mh_error_detail* error);
+/* Current coordinate rows (angle, angular velocity), for tests and tools. */
+MH_API int mh_physics_read_coordinates(mh_physics* physics, mh_mechanism_coordinate* out, size_t capacity);
 
 /* ---- learned warm start (no counterpart in the reference) --------------
  * A 16 -> 64 -> 64 -> 3 GELU network (tanh form) predicts, per active contact,

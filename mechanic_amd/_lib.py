@@ -139,6 +139,8 @@ class Creation(ctypes.Structure):
         ("poses", c_void_p), ("inverse_masses", c_void_p), ("masses", c_void_p), ("inertias", c_void_p),
         ("colliders", c_void_p), ("bearings", c_void_p), ("suppressed_pairs", c_void_p),
         ("body_components", c_void_p), ("root_flags", c_void_p),
+        ("mechanism_bodies", c_void_p), ("preorder", c_void_p), ("drives", c_void_p),
+        ("coordinate_count", c_uint32), ("reserved", c_uint32),
     ]
 
 
@@ -164,7 +166,7 @@ class TickReadback(ctypes.Structure):
 class PhaseTimings(ctypes.Structure):
     _fields_ = [(n, c_double) for n in ("integrate_ms", "morton_ms", "sort_ms", "lbvh_ms", "traverse_ms",
                                         "narrowphase_ms", "prepare_ms", "solve_ms", "persist_ms", "snapshot_ms",
-                                        "total_ms")]
+                                        "total_ms", "mechanism_ms", "validation_ms")]
 
 
 class ErrorDetail(ctypes.Structure):
@@ -224,6 +226,7 @@ def load() -> ctypes.CDLL:
         "mh_physics_debug_tick": (c_int, [H, c_uint64, c_int]),
         "mh_physics_write_drives": (c_int, [H, POINTER(MechanismDrive), c_size_t, POINTER(ErrorDetail)]),
         "mh_physics_init_coordinates": (c_int, [H, POINTER(MechanismCoordinate), c_size_t, POINTER(ErrorDetail)]),
+        "mh_physics_read_coordinates": (c_int, [H, POINTER(MechanismCoordinate), c_size_t]),
         "mh_mlp_create": (c_int, [c_uint64, POINTER(H)]),
         "mh_mlp_destroy": (None, [H]),
         "mh_mlp_param_count": (c_uint32, []),

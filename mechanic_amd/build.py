@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmechanic_hip.so")
 SOURCES = ["physics.hip", "kernels_body.hip", "kernels_broadphase.hip", "kernels_narrowphase.hip",
-           "kernels_solver.hip", "mlp.hip", "scene.cpp"]
+           "kernels_solver.hip", "kernels_mechanism.hip", "mlp.hip",
+           "scene.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
 

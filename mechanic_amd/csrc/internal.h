@@ -123,6 +123,36 @@ struct Dev {
   double* resid;          // 4 per residual block: sum r^2, active contacts, sum penetration, max penetration
   const float* mlp;       // network parameters (mlp.hip layout), or null
 
+  // ---- articulated path (bearings, coordinates, forward kinematics, closures) ----
+  uint32_t bearing_count;
+  uint32_t coordinate_count;
+  uint32_t closure_count;
+  uint32_t mech_count;        // non-root mechanism bodies (mlist)
+  uint32_t bearing_body_count;// bodies some bearing references (blist)
+  uint32_t level_count;       // depth levels 1..max in mlist
+  uint32_t jump_rounds;       // pointer-jump rounds of one forward-kinematics pass
+  uint32_t has_dynamic_root;  // a root with inverse mass > 0 exists
+  const float4* bearings;     // 5 per bearing: anchor a, anchor b, axis a, axis b, metadata bits
+  const float4* mbodies;      // 4 per body (mh_mechanism_body): metadata, traversal, bind position, bind rotation
+  float2* coords;             // angle, angular velocity per coordinate
+  const float4* drives;       // 2 per coordinate (mh_mechanism_drive)
+  float4* links[2];           // 3 per body (mh_link_state): position, rotation, metadata bits
+  int32_t* mdeltas;           // 6 Q20 integers per body
+  const uint32_t* mlist;      // non-root mechanism bodies sorted by (depth, preorder)
+  const uint32_t* level_offset;  // level_count + 1 offsets into mlist
+  const uint32_t* blist;      // bodies referenced by bearings
+  const uint32_t* closure_rows;  // bearing row of each closure
+  const uint32_t* cpath_offset;  // 2 * closure_count + 1 offsets: branch a, branch b of each closure
+  const uint32_t* cpath;         // coordinates along each branch, walking to the root
+  const uint32_t* coord_offset;  // coordinate_count + 1 offsets into coord_entries
+  const uint32_t* coord_entries; // (closure << 1 | side) in (closure, side) order
+  const uint32_t* coord_body;    // child body of each coordinate's tree bearing
+  float4* cl_state;           // 5 per closure: anchor a, axis a, anchor b, axis b, (residual active flag)
+  float4* cl_product;         // 2 per closure: J d (position, axis)
+  float4* joints;             // 2 per coordinate: joint axis, joint anchor (component-root frame)
+  float* pcg;                 // 8 per coordinate: solution, residual, preconditioned, direction,
+                              // operator product, gradient, diagonal, unused
+
   // ---- publication ----
   float4* snap_pos[3];
   float4* snap_rot[3];
@@ -166,9 +196,20 @@ void launch_lbvh(hipStream_t s, const Dev& d);
 void launch_traverse(hipStream_t s, const Dev& d);
 void launch_narrowphase(hipStream_t s, const Dev& d);
 void launch_prepare(hipStream_t s, const Dev& d);
-void launch_solve(hipStream_t s, const Dev& d);
+void launch_solve(hipStream_t s, const Dev& d, int bearings);
 void launch_features(hipStream_t s, const Dev& d);
 void launch_targets(hipStream_t s, const Dev& d);
+// Articulated path (kernels_mechanism.hip). `in_contact`: velocities live in the
+// solver records during the contact solve; `write_back` also stores them to the
+// state arrays.
+void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back);
+void launch_advance_coordinates(hipStream_t s, const Dev& d);
+void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures);
+void launch_capture_coordinates(hipStream_t s, const Dev& d);
+void launch_reconstruct(hipStream_t s, const Dev& d);
+void launch_world_masses(hipStream_t s, const Dev& d);
+void launch_validate_snapshot(hipStream_t s, const Dev& d);
+void launch_validate_bearings(hipStream_t s, const Dev& d);
 void launch_mlp_infer(hipStream_t s, const Dev& d);
 void launch_residual(hipStream_t s, const Dev& d);
 

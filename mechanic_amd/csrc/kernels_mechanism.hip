@@ -1,0 +1,710 @@
+// Articulated path: bearing velocity projection, reduced coordinates, forward
+// kinematics by pointer jumping, loop-closure Newton/PCG, velocity
+// reconstruction and validation (articulated.wgsl, mechanism.wgsl,
+// closure.wgsl, bearings.wgsl; pass order device.rs:963-1216, 1370-1416).
+//
+// Design on MI355X:
+// * Bearing projection is a parallel Jacobi pass whose per-body deltas are Q20
+//   integers summed with atomics -- the same order-independent arithmetic as the
+//   reference -- applied to the bodies that some bearing references.
+// * Everything that walks the forest (forward kinematics, the closure Newton
+//   steps with their PCG solves, pose publication and the parent-to-child
+//   velocity reconstruction) runs in ONE workgroup of 1024 threads separated by
+//   workgroup barriers, instead of the reference's ~12 x 13 dispatches and its
+//   single-thread loops. Reconstruction proceeds level by level (all bodies of
+//   one depth in parallel), which gives the serial preorder loop's results: a
+//   body reads only its parent's final velocity.
+// * Closure sums are deterministic: every coordinate gathers its contributions
+//   from a CSR list in (closure, side) order -- the order in which the reference's
+//   serial PCG adds them -- and the PCG dot products are accumulated in FP64
+//   with a fixed reduction tree.
+#include "internal.h"
+#include "mh_math.h"
+
+namespace mh {
+
+namespace {
+
+constexpr float kFixedScaleM = 1048576.0f;       // articulated.wgsl:59
+constexpr float kBearingRelaxation = 0.5f;       // articulated.wgsl:62
+constexpr float kSleepSpeed = 0.005f;            // articulated.wgsl:63
+constexpr uint32_t kDrivePassive = 0u;           // articulated.wgsl:65
+constexpr uint32_t kDriveAngle = 2u;             // articulated.wgsl:66
+constexpr float kAngleDeadband = 0.0005f;        // articulated.wgsl:69
+constexpr uint32_t kFlagInvalid = 2u;            // INVALID_NUMERIC_FLAG
+constexpr uint32_t kFlagNonConvergence = 4u;     // CONSTRAINT_NON_CONVERGENCE_FLAG
+constexpr uint32_t kAnchorTolerance = 10u;       // closure.wgsl:69
+constexpr uint32_t kAxisTolerance = 1000u;       // closure.wgsl:70
+constexpr float kStepScale = 0.35f;              // closure.wgsl:72
+constexpr float kMaxStep = 0.2f;                 // closure.wgsl:73
+constexpr int kNewtonSteps = 12;                 // device.rs:977
+constexpr int kPcgIterations = 8;                // closure.wgsl:362
+constexpr uint32_t kPoseThreads = 1024;
+
+__device__ __forceinline__ uint64_t tick_m(const Dev& d) {
+  return *reinterpret_cast<const volatile uint64_t*>(&d.tick->tick_index);
+}
+__device__ __forceinline__ uint32_t* diag_now(const Dev& d) { return d.diag[tick_m(d) & 1]; }
+
+struct BearingRow {
+  v3 anchor_a, anchor_b, axis_a, axis_b;
+  uint32_t a, b, coordinate, closure;
+};
+
+__device__ __forceinline__ BearingRow load_bearing(const Dev& d, uint32_t i) {
+  const float4* r = d.bearings + 5 * (size_t)i;
+  BearingRow b;
+  b.anchor_a = xyz(r[0]);
+  b.anchor_b = xyz(r[1]);
+  b.axis_a = xyz(r[2]);
+  b.axis_b = xyz(r[3]);
+  const float4 m = r[4];
+  b.a = __float_as_uint(m.x);
+  b.b = __float_as_uint(m.y);
+  b.coordinate = __float_as_uint(m.z);
+  b.closure = __float_as_uint(m.w);
+  return b;
+}
+
+struct MechBody {
+  uint32_t parent, tree_bearing, direction, root;
+};
+__device__ __forceinline__ MechBody load_mbody(const Dev& d, uint32_t body) {
+  const float4 m = d.mbodies[4 * (size_t)body];
+  return MechBody{__float_as_uint(m.x), __float_as_uint(m.y), __float_as_uint(m.z), __float_as_uint(m.w)};
+}
+
+__device__ __forceinline__ v3 normalize_v(v3 v) { return v / length(v); }
+__device__ __forceinline__ float4 normalize_q(float4 q) {
+  const float l = sqrtf(dot4(q, q));
+  return make_float4(q.x / l, q.y / l, q.z / l, q.w / l);
+}
+__device__ __forceinline__ float clampf(float e, float lo, float hi) { return fmn(fmx(e, lo), hi); }
+__device__ __forceinline__ bool finite3(v3 v) {
+  const float lim = 3.402823e+38f;
+  return v.x == v.x && v.y == v.y && v.z == v.z && fabsf(v.x) < lim && fabsf(v.y) < lim && fabsf(v.z) < lim;
+}
+// u32(round(x)) with WGSL's saturating conversion.
+__device__ __forceinline__ uint32_t round_to_u32(float x) {
+  const float r = __builtin_rintf(x);
+  if (!(r > 0.0f)) return 0u;
+  if (r >= 4294967296.0f) return 0xffffffffu;
+  return (uint32_t)r;
+}
+
+// world_inverse_inertia (articulated.wgsl:97-105) from the body's rotation and
+// body-frame inverse inertia.
+__device__ __forceinline__ v3 wii(const Dev& d, uint32_t body, v3 v) {
+  const float4 rotation = d.rot[body];
+  const v3 local = quat_rotate(quat_conj(rotation), v);
+  const float4* m = d.masses + 4 * (size_t)body;
+  const v3 r = (xyz(m[1]) * local.x + xyz(m[2]) * local.y) + xyz(m[3]) * local.z;
+  return quat_rotate(rotation, r);
+}
+__device__ __forceinline__ float inv_mass_of(const Dev& d, uint32_t body) { return d.masses[4 * (size_t)body].x; }
+
+// Velocities: the state arrays, or the solver records while the contact solve runs.
+__device__ __forceinline__ v3 lin_of(const Dev& d, uint32_t b, int in_contact) {
+  return in_contact ? xyz(d.sbody[4 * (size_t)b]) : xyz(d.lin[b]);
+}
+__device__ __forceinline__ v3 ang_of(const Dev& d, uint32_t b, int in_contact) {
+  return in_contact ? xyz(d.sbody[4 * (size_t)b + 1]) : xyz(d.ang[b]);
+}
+
+__device__ __forceinline__ void add_mdelta(const Dev& d, uint32_t body, v3 linear, v3 angular) {
+  int32_t* p = d.mdeltas + 6 * (size_t)body;
+  atomicAdd(&p[0], round_to_i32(linear.x * kFixedScaleM));
+  atomicAdd(&p[1], round_to_i32(linear.y * kFixedScaleM));
+  atomicAdd(&p[2], round_to_i32(linear.z * kFixedScaleM));
+  atomicAdd(&p[3], round_to_i32(angular.x * kFixedScaleM));
+  atomicAdd(&p[4], round_to_i32(angular.y * kFixedScaleM));
+  atomicAdd(&p[5], round_to_i32(angular.z * kFixedScaleM));
+}
+
+// solve_linear_axis (articulated.wgsl:118-143).
+__device__ __forceinline__ void linear_axis(const Dev& d, uint32_t a, uint32_t b, v3 arm_a, v3 arm_b, v3 relative,
+                                            v3 direction) {
+  const v3 angular_a = cross(arm_a, direction);
+  const v3 angular_b = cross(arm_b, direction);
+  const float denominator =
+      ((inv_mass_of(d, a) + inv_mass_of(d, b)) + dot(angular_a, wii(d, a, angular_a))) + dot(angular_b, wii(d, b, angular_b));
+  if (denominator <= 1.0e-12f) return;
+  const v3 impulse = direction * ((kBearingRelaxation * dot(relative, direction)) / denominator);
+  add_mdelta(d, a, impulse * inv_mass_of(d, a), wii(d, a, cross(arm_a, impulse)));
+  add_mdelta(d, b, (-impulse) * inv_mass_of(d, b), wii(d, b, cross(arm_b, -impulse)));
+}
+
+// solve_angular_axis (articulated.wgsl:145-155).
+__device__ __forceinline__ void angular_axis(const Dev& d, uint32_t a, uint32_t b, v3 relative, v3 axis) {
+  const v3 inverse_a = wii(d, a, axis);
+  const v3 inverse_b = wii(d, b, axis);
+  const float denominator = dot(axis, inverse_a + inverse_b);
+  if (denominator <= 1.0e-12f) return;
+  const float impulse = (kBearingRelaxation * dot(relative, axis)) / denominator;
+  add_mdelta(d, a, mk(0.0f, 0.0f, 0.0f), inverse_a * impulse);
+  add_mdelta(d, b, mk(0.0f, 0.0f, 0.0f), (-inverse_b) * impulse);
+}
+
+// project_bearing_velocities (articulated.wgsl:281-305).
+__global__ __launch_bounds__(256) void k_bearing_project(Dev d, int in_contact) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.bearing_count) return;
+  const BearingRow r = load_bearing(d, i);
+  const float4 rot_a = d.rot[r.a], rot_b = d.rot[r.b];
+  const v3 arm_a = quat_rotate(rot_a, r.anchor_a);
+  const v3 arm_b = quat_rotate(rot_b, r.anchor_b);
+  const v3 ang_a = ang_of(d, r.a, in_contact), ang_b = ang_of(d, r.b, in_contact);
+  const v3 velocity_a = lin_of(d, r.a, in_contact) + cross(ang_a, arm_a);
+  const v3 velocity_b = lin_of(d, r.b, in_contact) + cross(ang_b, arm_b);
+  const v3 relative = velocity_b - velocity_a;
+  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(1.0f, 0.0f, 0.0f));
+  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 1.0f, 0.0f));
+  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 0.0f, 1.0f));
+  const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
+  const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
+  const v3 hinge = normalize_v(axis_a + axis_b);
+  const v3 helper = fabsf(hinge.x) > 0.8f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+  const v3 tangent_a = normalize_v(cross(hinge, helper));
+  const v3 tangent_b = cross(hinge, tangent_a);
+  const v3 relative_angular = ang_b - ang_a;
+  angular_axis(d, r.a, r.b, relative_angular, tangent_a);
+  angular_axis(d, r.a, r.b, relative_angular, tangent_b);
+}
+
+// apply_velocity_deltas (articulated.wgsl:307-326) over the bodies some bearing
+// references; every other body's delta is zero.
+__global__ __launch_bounds__(256) void k_bearing_apply(Dev d, int in_contact, int write_back) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.bearing_body_count) return;
+  const uint32_t body = d.blist[i];
+  int32_t* p = d.mdeltas + 6 * (size_t)body;
+  int32_t q[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) q[k] = atomicExch(&p[k], 0);
+  const v3 linear_delta = mk((float)q[0], (float)q[1], (float)q[2]) / kFixedScaleM;
+  const v3 angular_delta = mk((float)q[3], (float)q[4], (float)q[5]) / kFixedScaleM;
+  const v3 lin = lin_of(d, body, in_contact) + linear_delta;
+  const v3 ang = ang_of(d, body, in_contact) + angular_delta;
+  if (in_contact) {
+    float4* r = d.sbody + 4 * (size_t)body;
+    r[0] = f4(lin, r[0].w);
+    r[1] = f4(ang, r[1].w);
+  }
+  if (!in_contact || write_back) {
+    d.lin[body] = f4(lin, 0.0f);
+    d.ang[body] = f4(ang, 0.0f);
+  }
+}
+
+// permitted_axis / permitted_speed / stabilized_speed (articulated.wgsl:328-355).
+__device__ __forceinline__ v3 permitted_axis(const Dev& d, const MechBody& m, const BearingRow& r) {
+  const float4 rp = d.rot[m.parent];
+  return m.direction != 0u ? -normalize_v(quat_rotate(rp, r.axis_b)) : normalize_v(quat_rotate(rp, r.axis_a));
+}
+__device__ __forceinline__ float stabilized(v3 axis, float speed) {
+  const bool gravity_aligned = fabsf(axis.y) > 0.999f;
+  return gravity_aligned && fabsf(speed) < kSleepSpeed ? 0.0f : speed;
+}
+
+// advance_coordinates (articulated.wgsl:357-407).
+__global__ __launch_bounds__(256) void k_advance(Dev d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.mech_count) return;
+  const uint32_t body = d.mlist[i];
+  const MechBody m = load_mbody(d, body);
+  const BearingRow r = load_bearing(d, m.tree_bearing);
+  const uint32_t c = r.coordinate;
+  if (c == kInvalid) return;
+  const v3 axis = permitted_axis(d, m, r);
+  const float measured = dot(xyz(d.ang[body]) - xyz(d.ang[m.parent]), axis) * d.angular_damping;
+  const float4 d0 = d.drives[2 * (size_t)c], d1 = d.drives[2 * (size_t)c + 1];
+  const uint32_t mode = __float_as_uint(d0.x);
+  const float max_acceleration = d0.y, max_speed = d0.z, target_speed = d0.w;
+  const float target_angle = d1.x, min_angle = d1.y, max_angle = d1.z;
+  const float2 coordinate = d.coords[c];
+  float speed;
+  if (mode == kDrivePassive) {
+    speed = stabilized(axis, measured);
+  } else {
+    float desired = clampf(target_speed, -max_speed, max_speed);
+    if (mode == kDriveAngle) {
+      const float error = target_angle - coordinate.x;
+      const float brake = sqrtf((2.0f * max_acceleration) * fabsf(error));
+      desired = signf(error) * fmn(brake, max_speed);
+      if (fabsf(error) < kAngleDeadband) desired = 0.0f;
+    }
+    const float budget = max_acceleration * d.delta_seconds;
+    speed = measured + clampf(desired - measured, -budget, budget);
+  }
+  float angle = coordinate.x + speed * d.delta_seconds;
+  if (angle < min_angle) {
+    angle = min_angle;
+    speed = 0.0f;
+  } else if (angle > max_angle) {
+    angle = max_angle;
+    speed = 0.0f;
+  }
+  d.coords[c] = make_float2(angle, speed);
+}
+
+// capture_coordinates (articulated.wgsl:409-421).
+__global__ __launch_bounds__(256) void k_capture(Dev d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.mech_count) return;
+  const uint32_t body = d.mlist[i];
+  const MechBody m = load_mbody(d, body);
+  const BearingRow r = load_bearing(d, m.tree_bearing);
+  if (r.coordinate == kInvalid) return;
+  const v3 axis = permitted_axis(d, m, r);
+  const float speed = dot(xyz(d.ang[body]) - xyz(d.ang[m.parent]), axis);
+  d.coords[r.coordinate].y = stabilized(axis, speed);
+}
+
+// ---- forest walks in one workgroup -----------------------------------------
+
+struct Link {
+  float4 position, rotation;
+  uint32_t ancestor, resolved;
+};
+__device__ __forceinline__ Link load_link(const float4* links, uint32_t b) {
+  const float4* p = links + 3 * (size_t)b;
+  const float4 m = p[2];
+  return Link{p[0], p[1], __float_as_uint(m.x), __float_as_uint(m.y)};
+}
+__device__ __forceinline__ void store_link(float4* links, uint32_t b, const Link& l) {
+  float4* p = links + 3 * (size_t)b;
+  p[0] = l.position;
+  p[1] = l.rotation;
+  p[2] = make_float4(__uint_as_float(l.ancestor), __uint_as_float(l.resolved), 0.0f, 0.0f);
+}
+
+// axis_rotation (mechanism.wgsl:70-73).
+__device__ __forceinline__ float4 axis_rotation(v3 axis, float angle) {
+  const float half_angle = angle * 0.5f;
+  const v3 a = normalize_v(axis) * sinf(half_angle);
+  return f4(a, cosf(half_angle));
+}
+
+// prepare_links (mechanism.wgsl:75-107) for one non-root body.
+__device__ __forceinline__ void prepare_link(const Dev& d, uint32_t body) {
+  const MechBody m = load_mbody(d, body);
+  const BearingRow r = load_bearing(d, m.tree_bearing);
+  const float angle = d.coords[r.coordinate].x;
+  const float4 bind_rotation = d.mbodies[4 * (size_t)body + 3];
+  float4 rr;
+  v3 rp;
+  if (m.direction == 0u) {
+    rr = quat_multiply(axis_rotation(r.axis_a, angle), bind_rotation);
+    rp = r.anchor_a - quat_rotate(rr, r.anchor_b);
+  } else {
+    rr = quat_multiply(axis_rotation(r.axis_b, -angle), bind_rotation);
+    rp = r.anchor_b - quat_rotate(rr, r.anchor_a);
+  }
+  store_link(d.links[0], body, Link{f4(rp, 0.0f), rr, m.parent, 0u});
+}
+
+// jump_a_to_b / jump_b_to_a (mechanism.wgsl:121-147) with compose (109-119).
+__device__ __forceinline__ void jump_link(const float4* src, float4* dst, uint32_t body) {
+  const Link child = load_link(src, body);
+  if (child.resolved != 0u) {
+    store_link(dst, body, child);
+    return;
+  }
+  const Link parent = load_link(src, child.ancestor);
+  Link out;
+  out.position = f4(xyz(parent.position) + quat_rotate(parent.rotation, xyz(child.position)), 0.0f);
+  out.rotation = normalize_q(quat_multiply(parent.rotation, child.rotation));
+  out.ancestor = parent.ancestor;
+  out.resolved = parent.resolved;
+  store_link(dst, body, out);
+}
+
+__device__ void forward_kinematics(const Dev& d) {
+  for (uint32_t i = threadIdx.x; i < d.mech_count; i += blockDim.x) prepare_link(d, d.mlist[i]);
+  __syncthreads();
+  for (uint32_t round = 0; round < d.jump_rounds; ++round) {
+    const float4* src = d.links[round & 1u];
+    float4* dst = d.links[(round & 1u) ^ 1u];
+    for (uint32_t i = threadIdx.x; i < d.mech_count; i += blockDim.x) jump_link(src, dst, d.mlist[i]);
+    __syncthreads();
+  }
+}
+
+// reconstruct_body_velocities (articulated.wgsl:423-456), one depth level at a time.
+__device__ __forceinline__ void reconstruct_body(const Dev& d, uint32_t body) {
+  const MechBody m = load_mbody(d, body);
+  const BearingRow r = load_bearing(d, m.tree_bearing);
+  const float4 rp = d.rot[m.parent], rb = d.rot[body];
+  v3 axis, parent_anchor, child_anchor;
+  if (m.direction != 0u) {
+    axis = -normalize_v(quat_rotate(rp, r.axis_b));
+    parent_anchor = quat_rotate(rp, r.anchor_b);
+    child_anchor = quat_rotate(rb, r.anchor_a);
+  } else {
+    axis = normalize_v(quat_rotate(rp, r.axis_a));
+    parent_anchor = quat_rotate(rp, r.anchor_a);
+    child_anchor = quat_rotate(rb, r.anchor_b);
+  }
+  const float speed = d.coords[r.coordinate].y;
+  const v3 parent_ang = xyz(d.ang[m.parent]);
+  const v3 angular = parent_ang + axis * speed;
+  const v3 anchor_velocity = xyz(d.lin[m.parent]) + cross(parent_ang, parent_anchor);
+  d.lin[body] = f4(anchor_velocity - cross(angular, child_anchor), 0.0f);
+  d.ang[body] = f4(angular, 0.0f);
+}
+
+__device__ void reconstruct_levels(const Dev& d) {
+  for (uint32_t level = 0; level < d.level_count; ++level) {
+    const uint32_t first = d.level_offset[level], end = d.level_offset[level + 1];
+    for (uint32_t i = first + threadIdx.x; i < end; i += blockDim.x) reconstruct_body(d, d.mlist[i]);
+    __syncthreads();
+  }
+}
+
+// publish_a / publish_b (mechanism.wgsl:149-185).
+__device__ void publish(const Dev& d, const float4* links) {
+  for (uint32_t i = threadIdx.x; i < d.mech_count; i += blockDim.x) {
+    const uint32_t body = d.mlist[i];
+    const Link l = load_link(links, body);
+    const uint32_t root = l.ancestor;
+    if (body == root) continue;
+    const float4 root_rotation = d.rot[root];
+    d.pos[body] = f4(xyz(d.pos[root]) + quat_rotate(root_rotation, xyz(l.position)), 0.0f);
+    d.rot[body] = normalize_q(quat_multiply(root_rotation, l.rotation));
+  }
+  __syncthreads();
+}
+
+// Deterministic FP64 sum over the workgroup: per-thread partials, then a fixed
+// halving tree in LDS.
+__device__ double block_sum(double v, double* scratch) {
+  scratch[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t half = blockDim.x >> 1; half > 0; half >>= 1) {
+    if (threadIdx.x < half) scratch[threadIdx.x] += scratch[threadIdx.x + half];
+    __syncthreads();
+  }
+  const double total = scratch[0];
+  __syncthreads();
+  return total;
+}
+
+struct Endpoint {
+  v3 anchor, axis;
+};
+
+// Joint axis and anchor of a coordinate in its component root's frame
+// (closure.wgsl:172-185).
+__device__ __forceinline__ void joint_of(const Dev& d, const float4* links, uint32_t body, v3& axis, v3& anchor) {
+  const MechBody m = load_mbody(d, body);
+  const BearingRow r = load_bearing(d, m.tree_bearing);
+  const Link parent = load_link(links, m.parent);
+  if (m.direction == 0u) {
+    axis = normalize_v(quat_rotate(parent.rotation, r.axis_a));
+    anchor = xyz(parent.position) + quat_rotate(parent.rotation, r.anchor_a);
+  } else {
+    axis = -normalize_v(quat_rotate(parent.rotation, r.axis_b));
+    anchor = xyz(parent.position) + quat_rotate(parent.rotation, r.anchor_b);
+  }
+}
+
+// The closure Newton loop (device.rs:976-1020, closure.wgsl:233-427).
+__device__ void closure_newton(const Dev& d, const float4* links, uint32_t* diag) {
+  __shared__ uint32_t state[2];
+  __shared__ double scratch[kPoseThreads];
+  const uint32_t nc = d.closure_count, nq = d.coordinate_count;
+  for (int step = 0; step < kNewtonSteps; ++step) {
+    if (threadIdx.x == 0) {
+      state[0] = 0u;
+      state[1] = 0u;
+    }
+    __syncthreads();
+    // evaluate_closures: endpoints, residuals, tolerance maxima.
+    for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
+      const BearingRow r = load_bearing(d, d.closure_rows[k]);
+      const Link pa = load_link(links, r.a), pb = load_link(links, r.b);
+      const v3 anchor_a = xyz(pa.position) + quat_rotate(pa.rotation, r.anchor_a);
+      const v3 anchor_b = xyz(pb.position) + quat_rotate(pb.rotation, r.anchor_b);
+      const v3 axis_a = normalize_v(quat_rotate(pa.rotation, r.axis_a));
+      const v3 axis_b = normalize_v(quat_rotate(pb.rotation, r.axis_b));
+      const v3 res_pos = anchor_a - anchor_b, res_axis = axis_a - axis_b;
+      float4* s = d.cl_state + 6 * (size_t)k;
+      s[0] = f4(anchor_a, 0.0f);
+      s[1] = f4(axis_a, 0.0f);
+      s[2] = f4(anchor_b, 0.0f);
+      s[3] = f4(axis_b, 0.0f);
+      float active = 0.0f;
+      if (!finite3(res_pos) || !finite3(res_axis)) {
+        atomicOr(&diag[kDiagFlags], kFlagInvalid);
+        atomicMax(&state[0], 0xffffffffu);
+        atomicMax(&state[1], 0xffffffffu);
+      } else {
+        const uint32_t um = round_to_u32(length(res_pos) * 1000000.0f);
+        const float degrees = acosf(clampf(dot(axis_a, axis_b), -1.0f, 1.0f)) * 57.295779513f;
+        const uint32_t ud = round_to_u32(degrees * 1000000.0f);
+        atomicMax(&state[0], um);
+        atomicMax(&state[1], ud);
+        active = (um <= kAnchorTolerance && ud <= kAxisTolerance) ? 0.0f : 1.0f;
+      }
+      s[4] = f4(res_pos, active);
+      s[5] = f4(res_axis, 0.0f);
+    }
+    __syncthreads();
+    // finalize_closures: the rest of this step, and every later step, only
+    // while some closure is outside tolerance.
+    if (!(state[0] > kAnchorTolerance || state[1] > kAxisTolerance)) break;
+    // Joint axes and anchors of every coordinate.
+    for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
+      v3 axis, anchor;
+      joint_of(d, links, d.coord_body[c], axis, anchor);
+      d.joints[2 * (size_t)c] = f4(axis, 0.0f);
+      d.joints[2 * (size_t)c + 1] = f4(anchor, 0.0f);
+    }
+    __syncthreads();
+    // Gradient and Jacobi diagonal (accumulate_branch), then the PCG start.
+    double rz_part = 0.0;
+    for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
+      const v3 jaxis = xyz(d.joints[2 * (size_t)c]), janchor = xyz(d.joints[2 * (size_t)c + 1]);
+      double g = 0.0, dg = 0.0;
+      for (uint32_t e = d.coord_offset[c]; e < d.coord_offset[c + 1]; ++e) {
+        const uint32_t entry = d.coord_entries[e], k = entry >> 1, side = entry & 1u;
+        const float4* s = d.cl_state + 6 * (size_t)k;
+        if (s[4].w == 0.0f) continue;
+        const v3 end_anchor = xyz(s[side ? 2 : 0]), end_axis = xyz(s[side ? 3 : 1]);
+        const float sign = side ? -1.0f : 1.0f;
+        const v3 pd = cross(jaxis, end_anchor - janchor) * sign;
+        const v3 ad = cross(jaxis, end_axis) * sign;
+        const float gradient = dot(xyz(s[4]), pd) + 1.0f * dot(xyz(s[5]), ad);
+        const float diagonal = dot(pd, pd) + 1.0f * dot(ad, ad);
+        if (gradient == gradient && diagonal == diagonal) {
+          g += (double)gradient;
+          dg += (double)diagonal;
+        } else {
+          atomicOr(&diag[kDiagFlags], kFlagInvalid);
+        }
+      }
+      const float gf = (float)g, df = (float)dg;
+      const float residual = df > 1.0e-12f ? -gf : 0.0f;
+      const float pre = df > 1.0e-12f ? residual / df : 0.0f;
+      float* q = d.pcg + 8 * (size_t)c;
+      q[0] = 0.0f;
+      q[1] = residual;
+      q[2] = pre;
+      q[3] = pre;
+      q[4] = 0.0f;
+      q[5] = gf;
+      q[6] = df;
+      rz_part += (double)residual * (double)pre;
+    }
+    double rz = block_sum(rz_part, scratch);
+    for (int it = 0; it < kPcgIterations; ++it) {
+      // branch_product for every closure (closure.wgsl:188-220).
+      for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
+        const float4* s = d.cl_state + 6 * (size_t)k;
+        v3 pos_sum[2], axis_sum[2];
+        for (uint32_t side = 0; side < 2; ++side) {
+          const v3 end_anchor = xyz(s[side ? 2 : 0]), end_axis = xyz(s[side ? 3 : 1]);
+          const float sign = side ? -1.0f : 1.0f;
+          v3 pp = mk(0.0f, 0.0f, 0.0f), pa = pp;
+          for (uint32_t e = d.cpath_offset[2 * k + side]; e < d.cpath_offset[2 * k + side + 1]; ++e) {
+            const uint32_t c = d.cpath[e];
+            const v3 jaxis = xyz(d.joints[2 * (size_t)c]), janchor = xyz(d.joints[2 * (size_t)c + 1]);
+            const float direction = d.pcg[8 * (size_t)c + 3];
+            pp = pp + (cross(jaxis, end_anchor - janchor) * sign) * direction;
+            pa = pa + (cross(jaxis, end_axis) * sign) * direction;
+          }
+          pos_sum[side] = pp;
+          axis_sum[side] = pa;
+        }
+        d.cl_product[2 * (size_t)k] = f4(pos_sum[0] + pos_sum[1], 0.0f);
+        d.cl_product[2 * (size_t)k + 1] = f4(axis_sum[0] + axis_sum[1], 0.0f);
+      }
+      __syncthreads();
+      // branch_transpose gathered per coordinate (closure.wgsl:222-252, 365-381).
+      double dad_part = 0.0;
+      for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
+        const v3 jaxis = xyz(d.joints[2 * (size_t)c]), janchor = xyz(d.joints[2 * (size_t)c + 1]);
+        float op = 0.0f;
+        for (uint32_t e = d.coord_offset[c]; e < d.coord_offset[c + 1]; ++e) {
+          const uint32_t entry = d.coord_entries[e], k = entry >> 1, side = entry & 1u;
+          const float4* s = d.cl_state + 6 * (size_t)k;
+          const v3 end_anchor = xyz(s[side ? 2 : 0]), end_axis = xyz(s[side ? 3 : 1]);
+          const float sign = side ? -1.0f : 1.0f;
+          const v3 pd = cross(jaxis, end_anchor - janchor) * sign;
+          const v3 ad = cross(jaxis, end_axis) * sign;
+          op = op + (dot(pd, xyz(d.cl_product[2 * (size_t)k])) + 1.0f * dot(ad, xyz(d.cl_product[2 * (size_t)k + 1])));
+        }
+        float* q = d.pcg + 8 * (size_t)c;
+        q[4] = op;
+        dad_part += (double)q[3] * (double)op;
+      }
+      const double dad = block_sum(dad_part, scratch);
+      const float alpha = fabs(dad) > 1.0e-20 ? (float)(rz / dad) : 0.0f;
+      double rz_next_part = 0.0;
+      for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
+        float* q = d.pcg + 8 * (size_t)c;
+        q[0] = q[0] + alpha * q[3];
+        q[1] = q[1] - alpha * q[4];
+        const float df = q[6];
+        q[2] = df > 1.0e-12f ? q[1] / df : 0.0f;
+        rz_next_part += (double)q[1] * (double)q[2];
+      }
+      const double rz_next = block_sum(rz_next_part, scratch);
+      const float beta = fabs(rz) > 1.0e-20 ? (float)(rz_next / rz) : 0.0f;
+      for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
+        float* q = d.pcg + 8 * (size_t)c;
+        q[3] = q[2] + beta * q[3];
+      }
+      rz = rz_next;
+      __syncthreads();
+    }
+    // Apply the clamped Newton step (closure.wgsl:416-426).
+    for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
+      const float step_c = clampf(kStepScale * d.pcg[8 * (size_t)c], -kMaxStep, kMaxStep);
+      if (step_c == step_c) d.coords[c].x = d.coords[c].x + step_c;
+      else atomicOr(&diag[kDiagFlags], kFlagInvalid);
+    }
+    __syncthreads();
+    forward_kinematics(d);
+  }
+}
+
+// Forward kinematics, the closure Newton loop, pose publication and the
+// parent-to-child velocity reconstruction of encode_mechanism_passes
+// (device.rs:963-1048), in one workgroup.
+__global__ __launch_bounds__(kPoseThreads) void k_mechanism_pose(Dev d, int closures) {
+  uint32_t* diag = diag_now(d);
+  forward_kinematics(d);
+  const float4* links = d.links[d.jump_rounds & 1u];
+  if (closures) closure_newton(d, links, diag);
+  publish(d, links);
+  reconstruct_levels(d);
+}
+
+__global__ __launch_bounds__(kPoseThreads) void k_reconstruct(Dev d) { reconstruct_levels(d); }
+
+// update_world_masses (collision.wgsl:144-162) and the solver record, for ticks
+// whose poses and velocities change after integration.
+__device__ __forceinline__ v3 world_inverse_column(const float4* m, float4 rotation, v3 axis) {
+  const v3 local = quat_rotate(quat_conj(rotation), axis);
+  const v3 r = (xyz(m[1]) * local.x + xyz(m[2]) * local.y) + xyz(m[3]) * local.z;
+  return quat_rotate(rotation, r);
+}
+
+__global__ __launch_bounds__(256) void k_world_masses(Dev d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.body_count) return;
+  const float4 rotation = d.rot[i];
+  const float4* m = d.masses + 4 * (size_t)i;
+  const v3 cx = world_inverse_column(m, rotation, mk(1.0f, 0.0f, 0.0f));
+  const v3 cy = world_inverse_column(m, rotation, mk(0.0f, 1.0f, 0.0f));
+  const v3 cz = world_inverse_column(m, rotation, mk(0.0f, 0.0f, 1.0f));
+  float4* w = d.world_masses + 3 * (size_t)i;
+  w[0] = f4(cx, m[0].x);
+  w[1] = f4(cy, 0.0f);
+  w[2] = f4(cz, 0.0f);
+  const float4 linear = d.lin[i], angular = d.ang[i];
+  float4* sb = d.sbody + 4 * (size_t)i;
+  sb[0] = make_float4(linear.x, linear.y, linear.z, m[0].x);
+  sb[1] = make_float4(angular.x, angular.y, angular.z, cx.x);
+  sb[2] = make_float4(cx.y, cx.z, cy.x, cy.y);
+  sb[3] = make_float4(cy.z, cz.x, cz.y, cz.z);
+  d.slot_count[i] = 0u;
+}
+
+// validate_articulated_state (articulated.wgsl:458-484) fused with
+// publish_snapshot (snapshot.wgsl:6-14).
+__global__ __launch_bounds__(256) void k_validate_snapshot(Dev d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.body_count) return;
+  const uint64_t tick = tick_m(d);
+  const float4 position = d.pos[i], rotation = d.rot[i];
+  const float rotation_length = dot4(rotation, rotation);
+  bool valid = finite4(position) && finite4(rotation) && finite4(d.lin[i]) && finite4(d.ang[i]) &&
+               rotation_length > 0.5f && rotation_length < 1.5f;
+  const MechBody m = load_mbody(d, i);
+  if (m.root == 0u) {
+    const float2 s = d.coords[load_bearing(d, m.tree_bearing).coordinate];
+    valid = valid && s.x == s.x && fabsf(s.x) < 3.402823e+38f && s.y == s.y && fabsf(s.y) < 3.402823e+38f;
+  }
+  if (!valid) atomicOr(&d.diag[tick & 1][kDiagFlags], kFlagInvalid);
+  const uint32_t slot = (uint32_t)(tick % 3u);
+  d.snap_pos[slot][i] = position;
+  d.snap_rot[slot][i] = rotation;
+}
+
+// record_residual (bearings.wgsl:53-72).
+__device__ __forceinline__ void record_residual(uint32_t* diag, v3 anchor_a, v3 anchor_b, v3 axis_a, v3 axis_b,
+                                                bool closure) {
+  const uint32_t um = round_to_u32(length(anchor_a - anchor_b) * 1000000.0f);
+  const float degrees = atan2f(length(cross(axis_a, axis_b)), clampf(dot(axis_a, axis_b), -1.0f, 1.0f)) * 57.295779513f;
+  const uint32_t ud = round_to_u32(degrees * 1000000.0f);
+  atomicMax(&diag[kDiagAnchor], um);
+  atomicMax(&diag[kDiagAxis], ud);
+  if (closure && (um > kAnchorTolerance || ud > kAxisTolerance)) atomicOr(&diag[kDiagFlags], kFlagNonConvergence);
+}
+
+// validate_mechanism_bearings / validate_bearings (bearings.wgsl:74-102).
+__global__ __launch_bounds__(256) void k_validate_bearings(Dev d, int mechanism) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.bearing_count) return;
+  uint32_t* diag = diag_now(d);
+  const BearingRow r = load_bearing(d, i);
+  float4 pos_a, rot_a, pos_b, rot_b;
+  if (mechanism) {
+    const float4* links = d.links[d.jump_rounds & 1u];
+    const Link la = load_link(links, r.a), lb = load_link(links, r.b);
+    pos_a = la.position;
+    rot_a = la.rotation;
+    pos_b = lb.position;
+    rot_b = lb.rotation;
+  } else {
+    pos_a = d.pos[r.a];
+    rot_a = d.rot[r.a];
+    pos_b = d.pos[r.b];
+    rot_b = d.rot[r.b];
+  }
+  const v3 anchor_a = xyz(pos_a) + quat_rotate(rot_a, r.anchor_a);
+  const v3 anchor_b = xyz(pos_b) + quat_rotate(rot_b, r.anchor_b);
+  const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
+  const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
+  record_residual(diag, anchor_a, anchor_b, axis_a, axis_b, r.closure != 0u);
+}
+
+inline uint32_t blocks_for(uint32_t n) { return (n + 255u) / 256u; }
+
+}  // namespace
+
+void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back) {
+  if (d.bearing_count == 0) return;
+  k_bearing_project<<<blocks_for(d.bearing_count), 256, 0, s>>>(d, in_contact);
+  k_bearing_apply<<<blocks_for(d.bearing_body_count), 256, 0, s>>>(d, in_contact, write_back);
+}
+
+void launch_advance_coordinates(hipStream_t s, const Dev& d) {
+  if (d.mech_count) k_advance<<<blocks_for(d.mech_count), 256, 0, s>>>(d);
+}
+
+void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures) {
+  k_mechanism_pose<<<1, kPoseThreads, 0, s>>>(d, closures);
+}
+
+void launch_capture_coordinates(hipStream_t s, const Dev& d) {
+  if (d.mech_count) k_capture<<<blocks_for(d.mech_count), 256, 0, s>>>(d);
+}
+
+void launch_reconstruct(hipStream_t s, const Dev& d) { k_reconstruct<<<1, kPoseThreads, 0, s>>>(d); }
+
+void launch_world_masses(hipStream_t s, const Dev& d) {
+  if (d.body_count) k_world_masses<<<blocks_for(d.body_count), 256, 0, s>>>(d);
+}
+
+void launch_validate_snapshot(hipStream_t s, const Dev& d) {
+  if (d.body_count) k_validate_snapshot<<<blocks_for(d.body_count), 256, 0, s>>>(d);
+}
+
+void launch_validate_bearings(hipStream_t s, const Dev& d) {
+  if (d.bearing_count) k_validate_bearings<<<blocks_for(d.bearing_count), 256, 0, s>>>(d, d.mech_count > 0 ? 1 : 0);
+}
+
+}  // namespace mh

@@ -592,7 +592,10 @@ void launch_prepare(hipStream_t s, const Dev& d) {
   k_prepare<<<grid_for(d.pair_capacity, 2048), kBlock, 0, s>>>(d);
 }
 
-void launch_solve(hipStream_t s, const Dev& d) {
+// The contact passes of encode_collision_passes (device.rs:1351-1424); with
+// `bearings`, one bearing projection follows every pass (a mechanism with a
+// dynamic root, device.rs:1373-1416).
+void launch_solve(hipStream_t s, const Dev& d, int bearings) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t solve_grid = grid_for(d.pair_capacity, 2048);
   const uint32_t body_grid = d.body_count ? (d.body_count + kBlock - 1) / kBlock + kXcds : 0;
@@ -600,6 +603,7 @@ void launch_solve(hipStream_t s, const Dev& d) {
     const int last = it + 1 == iterations ? 1 : 0;
     k_solve<<<solve_grid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last);
     if (body_grid) k_apply<<<body_grid, kBlock, 0, s>>>(d, last);
+    if (bearings) launch_bearing_projection(s, d, 1, last);
   }
 }
 

@@ -20,6 +20,12 @@ struct Timing {
   bool valid = false;
 };
 
+// Event marks of a profiled tick.
+enum Mark {
+  kMarkStart, kMarkIntegrate, kMarkMechanism, kMarkMorton, kMarkSort, kMarkLbvh, kMarkTraverse, kMarkNarrow,
+  kMarkPrepare, kMarkSolve, kMarkValidate, kMarkEnd, kEvents
+};
+
 }  // namespace
 
 struct mh_physics {
@@ -37,9 +43,10 @@ struct mh_physics {
   uint64_t last_tick = 0;
   bool has_last_tick = false;
   uint32_t* host_diag = nullptr;  // pinned
-  hipEvent_t events[10] = {};
+  hipEvent_t events[kEvents] = {};
   Timing timing;
   const mh_mlp* mlp = nullptr;
+  bool mech_active = false;  // some body has a parent (device.rs:2232)
 };
 
 namespace {
@@ -95,44 +102,69 @@ uint32_t next_pow2(uint32_t v) {
   return r;
 }
 
-// Enqueues one tick. `stop` ends it early for phase tests (1: after
-// integration, 2: after the broadphase, 3: after contact generation).
+// Enqueues one tick in the pass order of dispatch_tick (device.rs:886-951).
+// `stop` ends it early for phase tests (1: after integration and the mechanism
+// passes, 2: after the broadphase, 3: after contact generation).
 void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0) {
   const Dev& d = p->d;
+  const bool mech = p->mech_active;
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], s);
   };
-  mark(0);
-  launch_integrate(s, d, p->collisions, true);
-  mark(1);
+  mark(kMarkStart);
+  // Unarticulated ticks fuse world masses and the snapshot into integration:
+  // no later kernel moves a body.
+  launch_integrate(s, d, p->collisions && !mech, !mech);
+  mark(kMarkIntegrate);
+  if (mech) {
+    // encode_mechanism_passes (device.rs:963-1048).
+    const uint32_t iterations = d.solver_iterations ? d.solver_iterations : 1u;
+    for (uint32_t i = 0; i < iterations; ++i) launch_bearing_projection(s, d, 0, 1);
+    launch_advance_coordinates(s, d);
+    launch_mechanism_pose(s, d, d.closure_count > 0 ? 1 : 0);
+  }
+  mark(kMarkMechanism);
   if (p->collisions && stop != 1) {
+    if (mech) launch_world_masses(s, d);
     launch_clear_table(s, d);
     launch_morton(s, d);
-    mark(2);
+    mark(kMarkMorton);
     launch_sort(s, d);
-    mark(3);
+    mark(kMarkSort);
     launch_lbvh(s, d);
-    mark(4);
+    mark(kMarkLbvh);
     launch_traverse(s, d);
-    mark(5);
+    mark(kMarkTraverse);
     if (stop == 2) return;
     launch_narrowphase(s, d);
-    mark(6);
+    mark(kMarkNarrow);
     if (stop == 3) return;
     launch_prepare(s, d);
     if (d.learn_mode == kLearnLog) launch_features(s, d);
     if (d.learn_mode == kLearnInfer) launch_mlp_infer(s, d);
-    mark(7);
-    launch_solve(s, d);
+    mark(kMarkPrepare);
+    launch_solve(s, d, mech && d.has_dynamic_root);
     if (d.learn_mode == kLearnLog) launch_targets(s, d);
     if (d.residual_on) launch_residual(s, d);
-    mark(8);
+    if (mech) {
+      // encode_post_contact_mechanism (device.rs:1128-1155).
+      if (!d.has_dynamic_root) {
+        const uint32_t iterations = d.solver_iterations ? d.solver_iterations : 1u;
+        for (uint32_t i = 0; i < iterations; ++i) launch_bearing_projection(s, d, 0, 1);
+      }
+      launch_capture_coordinates(s, d);
+      if (!d.has_dynamic_root) launch_reconstruct(s, d);
+    }
+    mark(kMarkSolve);
   } else {
     if (stop != 0) return;
-    for (int i = 2; i <= 8; ++i) mark(i);
+    for (int i = kMarkMorton; i <= kMarkSolve; ++i) mark(i);
   }
+  if (mech) launch_validate_snapshot(s, d);
+  if (d.bearing_count > 0) launch_validate_bearings(s, d);
+  mark(kMarkValidate);
   launch_end_tick(s, d);
-  mark(9);
+  mark(kMarkEnd);
 }
 
 int begin_tick(mh_physics* p, uint64_t tick) {
@@ -160,6 +192,201 @@ int ensure_graph(mh_physics* p) {
 }
 
 bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+// Host-side derivation of the forest lists the articulated kernels walk
+// (what create_mechanism_resources derives, device.rs:1684-2232).
+struct Forest {
+  std::vector<uint32_t> mlist, level_offset, blist, closure_rows, cpath_offset, cpath, coord_offset, coord_entries,
+      coord_body;
+  uint32_t max_depth = 0, jump_rounds = 0;
+  bool has_dynamic_root = false;
+};
+
+int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
+  const uint32_t n = c->body_count, nb = c->bearing_count, nq = c->coordinate_count;
+  if (!c->bearings || !c->mechanism_bodies || (n > 0 && !c->preorder))
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "bearing rows need mechanism_bodies and preorder");
+  for (uint32_t i = 0; i < nb; ++i) {
+    const uint32_t* m = c->bearings[i].metadata;
+    if (m[0] >= n || m[1] >= n || m[0] == m[1])
+      return fail(error, MH_ERR_INVALID_ARGUMENT, "bearing joins bodies outside the scene");
+    if (m[3] == 0 && m[2] >= nq) return fail(error, MH_ERR_INVALID_ARGUMENT, "tree bearing without a coordinate");
+  }
+  std::vector<uint32_t> parent(n), depth(n, 0), root(n, 1), preorder_index(n, 0), tree_row(n, 0);
+  for (uint32_t b = 0; b < n; ++b) {
+    const mh_mechanism_body& m = c->mechanism_bodies[b];
+    root[b] = m.metadata[3] != 0 ? 1u : 0u;
+    parent[b] = root[b] ? b : m.metadata[0];
+    tree_row[b] = m.metadata[1];
+    depth[b] = m.traversal[1];
+    if (!root[b] && (parent[b] >= n || tree_row[b] >= nb || c->bearings[tree_row[b]].metadata[3] != 0 ||
+                     depth[b] == 0 || depth[parent[b]] + 1 != depth[b]))
+      return fail(error, MH_ERR_INVALID_ARGUMENT, "inconsistent mechanism body row");
+    if (root[b] && c->inverse_masses[b] > 0.0f) f.has_dynamic_root = true;
+    f.max_depth = std::max(f.max_depth, root[b] ? 0u : depth[b]);
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    if (c->preorder[k] >= n) return fail(error, MH_ERR_INVALID_ARGUMENT, "preorder row outside the scene");
+    preorder_index[c->preorder[k]] = k;
+  }
+  // Non-root bodies by (depth, preorder) and the offsets of each depth level.
+  for (uint32_t b = 0; b < n; ++b)
+    if (!root[b]) f.mlist.push_back(b);
+  std::sort(f.mlist.begin(), f.mlist.end(), [&](uint32_t x, uint32_t y) {
+    return depth[x] != depth[y] ? depth[x] < depth[y] : preorder_index[x] < preorder_index[y];
+  });
+  f.level_offset.push_back(0);
+  for (uint32_t level = 1; level <= f.max_depth; ++level) {
+    uint32_t end = f.level_offset.back();
+    while (end < f.mlist.size() && depth[f.mlist[end]] == level) ++end;
+    f.level_offset.push_back(end);
+  }
+  // pointer_jump_rounds (device.rs:2103-2111).
+  uint32_t covered = 1;
+  while (covered < f.max_depth) {
+    covered *= 2;
+    ++f.jump_rounds;
+  }
+  if (f.max_depth > 0) f.jump_rounds = std::max(f.jump_rounds, 1u);
+  // Bodies any bearing touches.
+  std::vector<uint8_t> touched(n, 0);
+  for (uint32_t i = 0; i < nb; ++i) touched[c->bearings[i].metadata[0]] = touched[c->bearings[i].metadata[1]] = 1;
+  for (uint32_t b = 0; b < n; ++b)
+    if (touched[b]) f.blist.push_back(b);
+  // Coordinates and their child bodies.
+  f.coord_body.assign(nq, kInvalid);
+  for (uint32_t b = 0; b < n; ++b)
+    if (!root[b]) f.coord_body[c->bearings[tree_row[b]].metadata[2]] = b;
+  for (uint32_t q = 0; q < nq; ++q)
+    if (f.coord_body[q] == kInvalid) return fail(error, MH_ERR_INVALID_ARGUMENT, "coordinate without a tree body");
+  // Closure branches: coordinates from each endpoint body to its root, and
+  // the inverse lists in (closure, side) order.
+  std::vector<std::vector<uint32_t>> per_coord(nq);
+  f.cpath_offset.push_back(0);
+  for (uint32_t i = 0; i < nb; ++i) {
+    if (c->bearings[i].metadata[3] == 0) continue;
+    const uint32_t k = (uint32_t)f.closure_rows.size();
+    f.closure_rows.push_back(i);
+    for (uint32_t side = 0; side < 2; ++side) {
+      uint32_t body = c->bearings[i].metadata[side];
+      uint32_t guard = 0;
+      while (!root[body]) {
+        const uint32_t q = c->bearings[tree_row[body]].metadata[2];
+        f.cpath.push_back(q);
+        per_coord[q].push_back((k << 1) | side);
+        body = parent[body];
+        if (++guard > n) return fail(error, MH_ERR_INVALID_ARGUMENT, "mechanism parents form a cycle");
+      }
+      f.cpath_offset.push_back((uint32_t)f.cpath.size());
+    }
+  }
+  f.coord_offset.push_back(0);
+  for (uint32_t q = 0; q < nq; ++q) {
+    f.coord_entries.insert(f.coord_entries.end(), per_coord[q].begin(), per_coord[q].end());
+    f.coord_offset.push_back((uint32_t)f.coord_entries.size());
+  }
+  // The serial small-mechanism solvers (device.rs:1063-1075, 1370-1400) are not built.
+  if (f.max_depth > 0 && f.has_dynamic_root && nb <= 64)
+    return fail(error, MH_ERR_UNSUPPORTED,
+                "mechanisms with <= 64 bearings next to a dynamic root use the serial solver, not built");
+  return MH_OK;
+}
+
+
+template <typename T>
+int alloc_upload(mh_physics* p, T** out, const std::vector<T>& host, mh_error_detail* error) {
+  int rc = alloc(p, out, host.size(), error);
+  if (rc != MH_OK) return rc;
+  return upload(p, *out, host.data(), host.size() * sizeof(T), error);
+}
+
+int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error_detail* error) {
+  Dev& d = p->d;
+  const uint32_t n = c->body_count, nb = c->bearing_count, nq = c->coordinate_count;
+  d.bearing_count = nb;
+  d.coordinate_count = nq;
+  d.closure_count = (uint32_t)f.closure_rows.size();
+  d.mech_count = (uint32_t)f.mlist.size();
+  d.bearing_body_count = (uint32_t)f.blist.size();
+  d.level_count = f.max_depth;
+  d.jump_rounds = f.jump_rounds;
+  d.has_dynamic_root = f.has_dynamic_root ? 1u : 0u;
+  p->mech_active = f.max_depth > 0;
+  int rc;
+#define TRY(x)                  \
+  do {                          \
+    rc = (x);                   \
+    if (rc != MH_OK) return rc; \
+  } while (0)
+  float4 *bearings, *mbodies, *drives, *l0, *l1, *cl_state, *cl_product, *joints;
+  float2* coords;
+  int32_t* mdeltas;
+  float* pcg;
+  TRY(alloc(p, &bearings, 5 * (size_t)nb, error));
+  TRY(upload(p, bearings, c->bearings, (size_t)nb * sizeof(mh_bearing), error));
+  TRY(alloc(p, &mbodies, 4 * (size_t)n, error));
+  TRY(upload(p, mbodies, c->mechanism_bodies, (size_t)n * sizeof(mh_mechanism_body), error));
+  TRY(alloc(p, &coords, nq, error));
+  // Passive rows when no drives are given (device.rs:1799-1808).
+  std::vector<mh_mechanism_drive> dr(nq);
+  for (uint32_t q = 0; q < nq; ++q) {
+    if (c->drives) {
+      dr[q] = c->drives[q];
+    } else {
+      dr[q] = mh_mechanism_drive{MH_DRIVE_MODE_PASSIVE, 0.0f, 0.0f, 0.0f, 0.0f, -INFINITY, INFINITY, 0.0f};
+    }
+  }
+  TRY(alloc(p, &drives, 2 * (size_t)nq, error));
+  TRY(upload(p, drives, dr.data(), (size_t)nq * sizeof(mh_mechanism_drive), error));
+  // Links: every body starts as its own resolved root (prepare_links' root rows).
+  std::vector<float4> links(3 * (size_t)n);
+  for (uint32_t b = 0; b < n; ++b) {
+    links[3 * (size_t)b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    links[3 * (size_t)b + 1] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    uint32_t meta[4] = {b, 1u, 0u, 0u};
+    std::memcpy(&links[3 * (size_t)b + 2], meta, 16);
+  }
+  TRY(alloc_upload(p, &l0, links, error));
+  TRY(alloc_upload(p, &l1, links, error));
+  TRY(alloc(p, &mdeltas, 6 * (size_t)n, error));
+  uint32_t *mlist, *level_offset, *blist, *closure_rows, *cpath_offset, *cpath, *coord_offset, *coord_entries,
+      *coord_body;
+  TRY(alloc_upload(p, &mlist, f.mlist, error));
+  TRY(alloc_upload(p, &level_offset, f.level_offset, error));
+  TRY(alloc_upload(p, &blist, f.blist, error));
+  TRY(alloc_upload(p, &closure_rows, f.closure_rows, error));
+  TRY(alloc_upload(p, &cpath_offset, f.cpath_offset, error));
+  TRY(alloc_upload(p, &cpath, f.cpath, error));
+  TRY(alloc_upload(p, &coord_offset, f.coord_offset, error));
+  TRY(alloc_upload(p, &coord_entries, f.coord_entries, error));
+  TRY(alloc_upload(p, &coord_body, f.coord_body, error));
+  TRY(alloc(p, &cl_state, 6 * (size_t)d.closure_count, error));
+  TRY(alloc(p, &cl_product, 2 * (size_t)d.closure_count, error));
+  TRY(alloc(p, &joints, 2 * (size_t)nq, error));
+  TRY(alloc(p, &pcg, 8 * (size_t)nq, error));
+#undef TRY
+  d.bearings = bearings;
+  d.mbodies = mbodies;
+  d.coords = coords;
+  d.drives = drives;
+  d.links[0] = l0;
+  d.links[1] = l1;
+  d.mdeltas = mdeltas;
+  d.mlist = mlist;
+  d.level_offset = level_offset;
+  d.blist = blist;
+  d.closure_rows = closure_rows;
+  d.cpath_offset = cpath_offset;
+  d.cpath = cpath;
+  d.coord_offset = coord_offset;
+  d.coord_entries = coord_entries;
+  d.coord_body = coord_body;
+  d.cl_state = cl_state;
+  d.cl_product = cl_product;
+  d.joints = joints;
+  d.pcg = pcg;
+  return MH_OK;
+}
 
 }  // namespace
 
@@ -221,8 +448,11 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
     if (error) { error->required = creation->collider_count; error->capacity = MH_MAX_COLLIDERS; }
     return fail(error, MH_ERR_COLLIDER_CAPACITY, "scene exceeds the collider capacity");
   }
-  if (creation->bearing_count > 0)
-    return fail(error, MH_ERR_UNSUPPORTED, "articulated bodies are not supported by this build");
+  Forest forest;
+  if (creation->bearing_count > 0) {
+    int frc = build_forest(creation, forest, error);
+    if (frc != MH_OK) return frc;
+  }
   const uint32_t n = creation->body_count, c = creation->collider_count;
   if ((n > 0 && (!creation->poses || !creation->inverse_masses || !creation->masses)) ||
       (c > 0 && !creation->colliders) || (creation->suppression_count > 0 && !creation->suppressed_pairs))
@@ -371,6 +601,10 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   }
   UPLOAD(cols, creation->colliders, (size_t)c * sizeof(mh_collider));
   UPLOAD(supp, creation->suppressed_pairs, (size_t)creation->suppression_count * sizeof(mh_pair));
+  if (creation->bearing_count > 0) {
+    rc = upload_forest(p, creation, forest, error);
+    if (rc != MH_OK) { mh_physics_destroy(p); return rc; }
+  }
   hipError_t he = hipHostMalloc(reinterpret_cast<void**>(&p->host_diag), kDiagWords * sizeof(uint32_t),
                                 hipHostMallocDefault);
   if (he != hipSuccess) { mh_physics_destroy(p); return hip_fail(error, he, "hipHostMalloc"); }
@@ -445,22 +679,24 @@ int mh_physics_profile_tick(mh_physics* p, uint64_t tick, mh_phase_timings* out)
   if (rc != MH_OK) return rc;
   encode_tick(p, p->stream, p->events);
   HIP_TRY_NOERR(hipGetLastError());
-  HIP_TRY_NOERR(hipEventSynchronize(p->events[9]));
-  float ms[9];
-  for (int i = 0; i < 9; ++i) HIP_TRY_NOERR(hipEventElapsedTime(&ms[i], p->events[i], p->events[i + 1]));
+  HIP_TRY_NOERR(hipEventSynchronize(p->events[kMarkEnd]));
+  float ms[kEvents - 1];
+  for (int i = 0; i + 1 < kEvents; ++i) HIP_TRY_NOERR(hipEventElapsedTime(&ms[i], p->events[i], p->events[i + 1]));
   mh_phase_timings t{};
-  t.integrate_ms = ms[0];
-  t.morton_ms = ms[1];
-  t.sort_ms = ms[2];
-  t.lbvh_ms = ms[3];
-  t.traverse_ms = ms[4];
-  t.narrowphase_ms = ms[5];
-  t.prepare_ms = ms[6];
-  t.solve_ms = ms[7];
+  t.integrate_ms = ms[kMarkStart];
+  t.mechanism_ms = ms[kMarkIntegrate];
+  t.morton_ms = ms[kMarkMechanism];
+  t.sort_ms = ms[kMarkMorton];
+  t.lbvh_ms = ms[kMarkSort];
+  t.traverse_ms = ms[kMarkLbvh];
+  t.narrowphase_ms = ms[kMarkTraverse];
+  t.prepare_ms = ms[kMarkNarrow];
+  t.solve_ms = ms[kMarkPrepare];
   t.persist_ms = 0.0;  // persist is fused into the last solver pass
-  t.snapshot_ms = 0.0; // publication is fused into integration for unarticulated scenes
+  t.validation_ms = ms[kMarkSolve];
+  t.snapshot_ms = 0.0;  // fused into integration, or into validation for articulated ticks
   float total = 0.0f;
-  HIP_TRY_NOERR(hipEventElapsedTime(&total, p->events[0], p->events[9]));
+  HIP_TRY_NOERR(hipEventElapsedTime(&total, p->events[kMarkStart], p->events[kMarkEnd]));
   t.total_ms = total;
   p->timing.t = t;
   p->timing.valid = true;
@@ -506,9 +742,11 @@ int mh_physics_read_last_tick(mh_physics* p, mh_tick_readback* out) {
     const mh_phase_timings& t = p->timing.t;
     out->has_timings = 1;
     out->integration_ms = t.integrate_ms;
+    out->mechanism_ms = t.mechanism_ms;
     out->broadphase_ms = t.morton_ms + t.sort_ms + t.lbvh_ms + t.traverse_ms;
     out->narrowphase_ms = t.narrowphase_ms;
     out->contact_solver_ms = t.prepare_ms + t.solve_ms + t.persist_ms;
+    out->bearings_ms = t.validation_ms;
     out->snapshot_ms = t.snapshot_ms;
     out->gpu_tick_ms = out->integration_ms + out->mechanism_ms + out->broadphase_ms + out->narrowphase_ms +
                        out->contact_solver_ms + out->bearings_ms + out->snapshot_ms;
@@ -644,12 +882,17 @@ int mh_physics_read_buffer(mh_physics* p, int buffer, void* dst, size_t bytes, s
 int mh_physics_write_drives(mh_physics* p, const mh_mechanism_drive* drives, size_t count, mh_error_detail* error) {
   if (error) std::memset(error, 0, sizeof(*error));
   if (!p) return fail(error, MH_ERR_INVALID_ARGUMENT, "null handle");
-  (void)drives;
-  // This build uploads no tree bearings, so the compiled coordinate count is 0 (device.rs:1622-1644).
-  if (count != 0) {
-    if (error) { error->provided = count; error->required = 0; }
+  // write_mechanism_drives (device.rs:1622-1644): the row count must match the coordinates.
+  const size_t required = p->d.coordinate_count;
+  if (count != required) {
+    if (error) { error->provided = count; error->required = required; }
     return fail(error, MH_ERR_DRIVE_STATE_COUNT, "drive state row count does not match the scene");
   }
+  if (count == 0) return MH_OK;
+  if (!drives) return fail(error, MH_ERR_INVALID_ARGUMENT, "null drive rows");
+  HIP_TRY(hipMemcpyAsync(const_cast<float4*>(p->d.drives), drives, count * sizeof(mh_mechanism_drive),
+                         hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
   return MH_OK;
 }
 
@@ -657,11 +900,27 @@ int mh_physics_init_coordinates(mh_physics* p, const mh_mechanism_coordinate* co
                                 mh_error_detail* error) {
   if (error) std::memset(error, 0, sizeof(*error));
   if (!p) return fail(error, MH_ERR_INVALID_ARGUMENT, "null handle");
-  (void)coordinates;
-  if (count != 0) {
-    if (error) { error->provided = count; error->required = 0; }
+  // initialize_mechanism_coordinates (device.rs:1646-1662).
+  const size_t required = p->d.coordinate_count;
+  if (count != required) {
+    if (error) { error->provided = count; error->required = required; }
     return fail(error, MH_ERR_COORDINATE_STATE_COUNT, "coordinate state row count does not match the scene");
   }
+  if (count == 0) return MH_OK;
+  if (!coordinates) return fail(error, MH_ERR_INVALID_ARGUMENT, "null coordinate rows");
+  HIP_TRY(hipMemcpyAsync(p->d.coords, coordinates, count * sizeof(mh_mechanism_coordinate), hipMemcpyHostToDevice,
+                         p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return MH_OK;
+}
+
+int mh_physics_read_coordinates(mh_physics* p, mh_mechanism_coordinate* out, size_t capacity) {
+  if (!p || (!out && p->d.coordinate_count > 0)) return MH_ERR_INVALID_ARGUMENT;
+  if (capacity < p->d.coordinate_count) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  if (p->d.coordinate_count)
+    HIP_TRY_NOERR(hipMemcpy(out, p->d.coords, p->d.coordinate_count * sizeof(mh_mechanism_coordinate),
+                            hipMemcpyDeviceToHost));
   return MH_OK;
 }
 

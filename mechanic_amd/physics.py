@@ -132,9 +132,10 @@ class GpuPhysics:
             return ctypes.cast(x, ctypes.c_void_p).value
 
         creation = _lib.Creation(
-            scene.body_count, len(scene.colliders), 0, len(scene.suppressed_pairs),
+            scene.body_count, len(scene.colliders), scene.bearing_count, len(scene.suppressed_pairs),
             ptr(scene.poses), ptr(scene.inverse_masses), ptr(scene.masses), ptr(scene.inertias),
-            ptr(scene.colliders), None, ptr(supp), ptr(scene.body_components), ptr(scene.root_flags))
+            ptr(scene.colliders), ptr(scene.bearings), ptr(supp), ptr(scene.body_components), ptr(scene.root_flags),
+            ptr(scene.mechanism_bodies), ptr(scene.preorder), ptr(scene.drives), scene.coordinate_count, 0)
         cfg = config.to_c()
         handle = ctypes.c_void_p()
         err = _lib.ErrorDetail()
@@ -143,7 +144,9 @@ class GpuPhysics:
             raise GpuPhysicsError(_PHYSICS_KINDS[rc], err.required, err.capacity, err.provided, err.message.decode())
         if rc != _lib.MH_OK:
             raise DeviceError(f"mh_physics_create: {err.message.decode()} ({rc})")
-        return cls(handle.value, scene.body_count, config)
+        gpu = cls(handle.value, scene.body_count, config)
+        gpu._coordinate_count = scene.coordinate_count
+        return gpu
 
     @staticmethod
     def check_capacity(body_count: int, collider_count: int, bearing_count: int = 0) -> None:
@@ -319,6 +322,14 @@ class GpuPhysics:
         _check(_lib.load().mh_physics_read_residual(self._h, ctypes.byref(r)), "read_residual")
         return dict(rms=r.rms, mean_penetration=r.mean_penetration, max_penetration=r.max_penetration,
                     active_contacts=r.active_contacts)
+
+
+    def read_coordinates(self) -> np.ndarray:
+        """(Q, 2) float32 array of (angle, angular velocity) per coordinate."""
+        q = int(getattr(self, "_coordinate_count", 0))
+        arr = (_lib.MechanismCoordinate * max(q, 1))()
+        _check(_lib.load().mh_physics_read_coordinates(self._h, arr, q), "read_coordinates")
+        return np.ctypeslib.as_array(arr).view(np.float32).reshape(-1, 2)[:q].copy()
 
 
 def device_count() -> int:

@@ -28,6 +28,13 @@ class Scene:
     suppressed_pairs: list = field(default_factory=list)
     body_components: object = None
     root_flags: object = None
+    # Articulated rows (mechanic_amd.articulated.compile_mechanisms).
+    bearings: object = None
+    bearing_count: int = 0
+    mechanism_bodies: object = None
+    preorder: object = None
+    drives: object = None
+    coordinate_count: int = 0
 
     @property
     def collider_count(self) -> int:

@@ -1,8 +1,8 @@
 // ORACLE — test infrastructure only.
 //
-// A CPU restatement of the reference's per-tick rigid-body step for unarticulated
-// scenes (GpuPhysics::dispatch_tick, crates/mechanic-gpu/src/device.rs:859-961,
-// and the WGSL kernels it encodes). Only tests/, the smoke check and the bench's
+// A CPU restatement of the reference's per-tick rigid-body step
+// (GpuPhysics::dispatch_tick, crates/mechanic-gpu/src/device.rs:859-961, and the
+// WGSL kernels it encodes), articulated scenes included. Only tests/, the smoke check and the bench's
 // CPU-baseline leg load it, always as the checker; the product never does.
 //
 // Scalar type: `Real` is float for the FP32 parity oracle and double for the FP64
@@ -17,7 +17,10 @@
 //    pair. The reference's open-addressing table (collision.wgsl:499-538) can
 //    reclaim a stale slot that precedes a pair's own slot on its probe path, and
 //    then that pair loses its warm start in an order-dependent way; the map never
-//    does. With no stale slot on the path both agree.
+//    does. With no stale slot on the path both agree;
+//  * the closure solve's gradient / diagonal sums and its PCG dot products
+//    accumulate in FP64 (the reference adds FP32 values with compare-and-swap
+//    atomics in scheduling order, closure.wgsl:94-126, and runs its dots in FP32).
 //
 // Pinning: see tests/test_oracle.py (closed forms, brute force, the reference's
 // own SAT known-answer vectors, collision.rs:199-232).
@@ -174,6 +177,20 @@ struct World {
   // persistent manifolds keyed by the ordered collider pair (low << 32 | high)
   std::unordered_map<uint64_t, Manifold<R>> manifolds;
   std::vector<std::vector<std::pair<uint32_t, uint32_t>>> pair_bufs;
+  // articulated rows (device.rs:1684-1808)
+  struct BearingRow { V3<R> anchor_a, anchor_b, axis_a, axis_b; uint32_t a, b, coordinate, closure; };
+  struct MechBody { uint32_t parent, tree_bearing, direction, root; V3<R> bind_position; V4<R> bind_rotation; };
+  struct Drive { uint32_t mode; R max_acceleration, max_speed, target_speed, target_angle, min_angle, max_angle; };
+  struct Link { V4<R> position, rotation; uint32_t ancestor, resolved; };
+  uint32_t bearing_count = 0, coordinate_count = 0, max_depth = 0, jump_rounds = 0;
+  bool mech_active = false, has_dynamic_root = false;
+  std::vector<BearingRow> bearings;
+  std::vector<MechBody> mbodies;
+  std::vector<uint32_t> preorder;
+  std::vector<R> coord_angle, coord_speed;
+  std::vector<Drive> drives;
+  std::vector<Link> links_a, links_b;
+  std::vector<int32_t> mdeltas;
 };
 
 template <typename R> V4<R> load4(const float* f) { return {R(f[0]), R(f[1]), R(f[2]), R(f[3])}; }
@@ -768,6 +785,440 @@ template <typename R> void solve_apply(World<R>& w) {
   }
 }
 
+// ---- articulated path (articulated.wgsl, mechanism.wgsl, closure.wgsl, bearings.wgsl) ----
+constexpr uint32_t NON_CONVERGENCE = 4u;
+
+template <typename R> V3<R> normalize_v(V3<R> v) { return divs(v, length(v)); }
+template <typename R> V4<R> normalize_q(V4<R> q) {
+  R l = std::sqrt(dot4(q, q));
+  return {q.x / l, q.y / l, q.z / l, q.w / l};
+}
+template <typename R> R clampr(R e, R lo, R hi) { return mn(mx(e, lo), hi); }
+template <typename R> bool finite3v(V3<R> v) {
+  const R lim = R(3.402823e+38f);
+  return v.x == v.x && v.y == v.y && v.z == v.z && std::fabs(v.x) < lim && std::fabs(v.y) < lim &&
+         std::fabs(v.z) < lim;
+}
+template <typename R> uint32_t round_u32(R x) {
+  R r = std::nearbyint(x);
+  if (!(r > R(0))) return 0u;
+  if (r >= R(4294967296.0)) return 0xffffffffu;
+  return (uint32_t)r;
+}
+
+// world_inverse_inertia (articulated.wgsl:97-105).
+template <typename R> V3<R> wii(const World<R>& w, uint32_t b, V3<R> v) {
+  V4<R> rotation = w.rot[b];
+  V3<R> local = quat_rotate(conj(rotation), v);
+  const V4<R>* m = &w.masses[4 * (size_t)b];
+  V3<R> r = add(add(mul(xyz(m[1]), local.x), mul(xyz(m[2]), local.y)), mul(xyz(m[3]), local.z));
+  return quat_rotate(rotation, r);
+}
+template <typename R> R mass_inv(const World<R>& w, uint32_t b) { return w.masses[4 * (size_t)b].x; }
+
+template <typename R> void add_mdelta(World<R>& w, uint32_t b, V3<R> linear, V3<R> angular) {
+  add_delta(&w.mdeltas[6 * (size_t)b], linear, angular);
+}
+
+// solve_linear_axis / solve_angular_axis (articulated.wgsl:118-155).
+template <typename R> void bearing_linear(World<R>& w, uint32_t a, uint32_t b, V3<R> arm_a, V3<R> arm_b, V3<R> rel,
+                                          V3<R> dir) {
+  V3<R> angular_a = cross(arm_a, dir), angular_b = cross(arm_b, dir);
+  R den = ((mass_inv(w, a) + mass_inv(w, b)) + dot(angular_a, wii(w, a, angular_a))) + dot(angular_b, wii(w, b, angular_b));
+  if (den <= R(1.0e-12f)) return;
+  V3<R> impulse = mul(dir, (R(0.5f) * dot(rel, dir)) / den);
+  add_mdelta(w, a, mul(impulse, mass_inv(w, a)), wii(w, a, cross(arm_a, impulse)));
+  add_mdelta(w, b, mul(neg(impulse), mass_inv(w, b)), wii(w, b, cross(arm_b, neg(impulse))));
+}
+template <typename R> void bearing_angular(World<R>& w, uint32_t a, uint32_t b, V3<R> rel, V3<R> axis) {
+  V3<R> ia = wii(w, a, axis), ib = wii(w, b, axis);
+  R den = dot(axis, add(ia, ib));
+  if (den <= R(1.0e-12f)) return;
+  R impulse = (R(0.5f) * dot(rel, axis)) / den;
+  add_mdelta(w, a, v3<R>(0, 0, 0), mul(ia, impulse));
+  add_mdelta(w, b, v3<R>(0, 0, 0), mul(neg(ib), impulse));
+}
+
+// project_bearing_velocities + apply_velocity_deltas (articulated.wgsl:281-326).
+template <typename R> void bearing_projection(World<R>& w) {
+  for (uint32_t i = 0; i < w.bearing_count; ++i) {
+    const auto& r = w.bearings[i];
+    V3<R> arm_a = quat_rotate(w.rot[r.a], r.anchor_a), arm_b = quat_rotate(w.rot[r.b], r.anchor_b);
+    V3<R> va = add(xyz(w.lin[r.a]), cross(xyz(w.ang[r.a]), arm_a));
+    V3<R> vb = add(xyz(w.lin[r.b]), cross(xyz(w.ang[r.b]), arm_b));
+    V3<R> rel = sub(vb, va);
+    bearing_linear(w, r.a, r.b, arm_a, arm_b, rel, v3<R>(1, 0, 0));
+    bearing_linear(w, r.a, r.b, arm_a, arm_b, rel, v3<R>(0, 1, 0));
+    bearing_linear(w, r.a, r.b, arm_a, arm_b, rel, v3<R>(0, 0, 1));
+    V3<R> axis_a = normalize_v(quat_rotate(w.rot[r.a], r.axis_a));
+    V3<R> axis_b = normalize_v(quat_rotate(w.rot[r.b], r.axis_b));
+    V3<R> hinge = normalize_v(add(axis_a, axis_b));
+    V3<R> helper = std::fabs(hinge.x) > R(0.8f) ? v3<R>(0, 1, 0) : v3<R>(1, 0, 0);
+    V3<R> ta = normalize_v(cross(hinge, helper));
+    V3<R> tb = cross(hinge, ta);
+    V3<R> rel_ang = sub(xyz(w.ang[r.b]), xyz(w.ang[r.a]));
+    bearing_angular(w, r.a, r.b, rel_ang, ta);
+    bearing_angular(w, r.a, r.b, rel_ang, tb);
+  }
+  const R s = R(1048576.0f);
+  for (uint32_t b = 0; b < w.body_count; ++b) {
+    int32_t* d = &w.mdeltas[6 * (size_t)b];
+    V3<R> ld = divs(v3<R>(R(d[0]), R(d[1]), R(d[2])), s);
+    V3<R> ad = divs(v3<R>(R(d[3]), R(d[4]), R(d[5])), s);
+    for (int k = 0; k < 6; ++k) d[k] = 0;
+    V3<R> l = add(xyz(w.lin[b]), ld), a = add(xyz(w.ang[b]), ad);
+    w.lin[b] = {l.x, l.y, l.z, R(0)};
+    w.ang[b] = {a.x, a.y, a.z, R(0)};
+  }
+}
+
+template <typename R> V3<R> permitted_axis(const World<R>& w, uint32_t body) {
+  const auto& m = w.mbodies[body];
+  const auto& r = w.bearings[m.tree_bearing];
+  return m.direction != 0u ? neg(normalize_v(quat_rotate(w.rot[m.parent], r.axis_b)))
+                           : normalize_v(quat_rotate(w.rot[m.parent], r.axis_a));
+}
+template <typename R> R stabilized_speed(V3<R> axis, R speed) {
+  bool aligned = std::fabs(axis.y) > R(0.999f);
+  return aligned && std::fabs(speed) < R(0.005f) ? R(0) : speed;
+}
+
+// advance_coordinates (articulated.wgsl:357-407).
+template <typename R> void advance_coordinates(World<R>& w) {
+  for (uint32_t body = 0; body < w.body_count; ++body) {
+    const auto& m = w.mbodies[body];
+    if (m.root != 0u) continue;
+    uint32_t c = w.bearings[m.tree_bearing].coordinate;
+    if (c == INVALID) continue;
+    V3<R> axis = permitted_axis(w, body);
+    R measured = dot(sub(xyz(w.ang[body]), xyz(w.ang[m.parent])), axis) * w.ang_damp;
+    const auto& dr = w.drives[c];
+    R speed;
+    if (dr.mode == 0u) {
+      speed = stabilized_speed(axis, measured);
+    } else {
+      R desired = clampr(dr.target_speed, -dr.max_speed, dr.max_speed);
+      if (dr.mode == 2u) {
+        R error = dr.target_angle - w.coord_angle[c];
+        R brake = std::sqrt((R(2) * dr.max_acceleration) * std::fabs(error));
+        desired = sgn(error) * mn(brake, dr.max_speed);
+        if (std::fabs(error) < R(0.0005f)) desired = R(0);
+      }
+      R budget = dr.max_acceleration * w.dt;
+      speed = measured + clampr(desired - measured, -budget, budget);
+    }
+    R angle = w.coord_angle[c] + speed * w.dt;
+    if (angle < dr.min_angle) {
+      angle = dr.min_angle;
+      speed = R(0);
+    } else if (angle > dr.max_angle) {
+      angle = dr.max_angle;
+      speed = R(0);
+    }
+    w.coord_angle[c] = angle;
+    w.coord_speed[c] = speed;
+  }
+}
+
+// capture_coordinates (articulated.wgsl:409-421).
+template <typename R> void capture_coordinates(World<R>& w) {
+  for (uint32_t body = 0; body < w.body_count; ++body) {
+    const auto& m = w.mbodies[body];
+    if (m.root != 0u) continue;
+    uint32_t c = w.bearings[m.tree_bearing].coordinate;
+    if (c == INVALID) continue;
+    V3<R> axis = permitted_axis(w, body);
+    w.coord_speed[c] = stabilized_speed(axis, dot(sub(xyz(w.ang[body]), xyz(w.ang[m.parent])), axis));
+  }
+}
+
+// reconstruct_body_velocities (articulated.wgsl:423-456), serial over preorder.
+template <typename R> void reconstruct(World<R>& w) {
+  for (uint32_t t = 0; t < w.body_count; ++t) {
+    uint32_t body = w.preorder[t];
+    const auto& m = w.mbodies[body];
+    if (m.root != 0u) continue;
+    const auto& r = w.bearings[m.tree_bearing];
+    V4<R> rp = w.rot[m.parent], rb = w.rot[body];
+    V3<R> axis = normalize_v(quat_rotate(rp, r.axis_a));
+    V3<R> parent_anchor = quat_rotate(rp, r.anchor_a);
+    V3<R> child_anchor = quat_rotate(rb, r.anchor_b);
+    if (m.direction != 0u) {
+      axis = neg(normalize_v(quat_rotate(rp, r.axis_b)));
+      parent_anchor = quat_rotate(rp, r.anchor_b);
+      child_anchor = quat_rotate(rb, r.anchor_a);
+    }
+    R speed = w.coord_speed[r.coordinate];
+    V3<R> angular = add(xyz(w.ang[m.parent]), mul(axis, speed));
+    V3<R> anchor_velocity = add(xyz(w.lin[m.parent]), cross(xyz(w.ang[m.parent]), parent_anchor));
+    V3<R> l = sub(anchor_velocity, cross(angular, child_anchor));
+    w.lin[body] = {l.x, l.y, l.z, R(0)};
+    w.ang[body] = {angular.x, angular.y, angular.z, R(0)};
+  }
+}
+
+// prepare_links, jump_a_to_b / jump_b_to_a (mechanism.wgsl:75-147).
+template <typename R> V4<R> axis_rotation(V3<R> axis, R angle) {
+  R half = angle * R(0.5f);
+  V3<R> a = mul(normalize_v(axis), std::sin(half));
+  return {a.x, a.y, a.z, std::cos(half)};
+}
+template <typename R> void forward_kinematics(World<R>& w) {
+  using L = typename World<R>::Link;
+  for (uint32_t body = 0; body < w.body_count; ++body) {
+    const auto& m = w.mbodies[body];
+    if (m.root != 0u) {
+      w.links_a[body] = L{{R(0), R(0), R(0), R(0)}, {R(0), R(0), R(0), R(1)}, body, 1u};
+      continue;
+    }
+    const auto& r = w.bearings[m.tree_bearing];
+    R angle = w.coord_angle[r.coordinate];
+    V4<R> rr;
+    V3<R> rp;
+    if (m.direction == 0u) {
+      rr = quat_multiply(axis_rotation(r.axis_a, angle), m.bind_rotation);
+      rp = sub(r.anchor_a, quat_rotate(rr, r.anchor_b));
+    } else {
+      rr = quat_multiply(axis_rotation(r.axis_b, -angle), m.bind_rotation);
+      rp = sub(r.anchor_b, quat_rotate(rr, r.anchor_a));
+    }
+    w.links_a[body] = L{{rp.x, rp.y, rp.z, R(0)}, rr, m.parent, 0u};
+  }
+  bool in_a = true;
+  for (uint32_t round = 0; round < w.jump_rounds; ++round) {
+    const std::vector<L>& src = in_a ? w.links_a : w.links_b;
+    std::vector<L>& dst = in_a ? w.links_b : w.links_a;
+    for (uint32_t body = 0; body < w.body_count; ++body) {
+      const L child = src[body];
+      if (child.resolved != 0u) {
+        dst[body] = child;
+        continue;
+      }
+      const L parent = src[child.ancestor];
+      V3<R> pos = add(xyz(parent.position), quat_rotate(parent.rotation, xyz(child.position)));
+      dst[body] = L{{pos.x, pos.y, pos.z, R(0)}, normalize_q(quat_multiply(parent.rotation, child.rotation)),
+                    parent.ancestor, parent.resolved};
+    }
+    in_a = !in_a;
+  }
+}
+template <typename R> const std::vector<typename World<R>::Link>& final_links(const World<R>& w) {
+  return (w.jump_rounds % 2u == 0u) ? w.links_a : w.links_b;
+}
+
+// publish_a / publish_b (mechanism.wgsl:149-185).
+template <typename R> void publish(World<R>& w) {
+  const auto& links = final_links(w);
+  for (uint32_t body = 0; body < w.body_count; ++body) {
+    const auto& l = links[body];
+    uint32_t root = l.ancestor;
+    if (body == root) continue;
+    V3<R> p = add(xyz(w.pos[root]), quat_rotate(w.rot[root], xyz(l.position)));
+    w.pos[body] = {p.x, p.y, p.z, R(0)};
+    w.rot[body] = normalize_q(quat_multiply(w.rot[root], l.rotation));
+  }
+}
+
+// Joint axis / anchor of a tree body in its root's frame (closure.wgsl:172-185).
+template <typename R> void joint_of(const World<R>& w, uint32_t body, V3<R>& axis, V3<R>& anchor) {
+  const auto& m = w.mbodies[body];
+  const auto& r = w.bearings[m.tree_bearing];
+  const auto& pp = final_links(w)[m.parent];
+  if (m.direction == 0u) {
+    axis = normalize_v(quat_rotate(pp.rotation, r.axis_a));
+    anchor = add(xyz(pp.position), quat_rotate(pp.rotation, r.anchor_a));
+  } else {
+    axis = neg(normalize_v(quat_rotate(pp.rotation, r.axis_b)));
+    anchor = add(xyz(pp.position), quat_rotate(pp.rotation, r.anchor_b));
+  }
+}
+
+// The closure Newton loop (device.rs:976-1020; closure.wgsl:233-427). Gradient
+// and diagonal sums and the PCG dot products accumulate in FP64; vectors stay in R.
+template <typename R> void closure_newton(World<R>& w, int max_steps = 12) {
+  const uint32_t nq = w.coordinate_count;
+  std::vector<uint32_t> closures;
+  for (uint32_t i = 0; i < w.bearing_count; ++i)
+    if (w.bearings[i].closure != 0u) closures.push_back(i);
+  if (closures.empty()) return;
+  struct End { V3<R> anchor_a, axis_a, anchor_b, axis_b, res_pos, res_axis; bool active; };
+  std::vector<End> ends(closures.size());
+  std::vector<R> solution(nq), residual(nq), pre(nq), direction(nq), op(nq), diagonal(nq);
+  // Coordinates on the walk from `body` to its root.
+  auto walk = [&](uint32_t body, auto&& visit) {
+    while (w.mbodies[body].root == 0u) {
+      visit(body, w.bearings[w.mbodies[body].tree_bearing].coordinate);
+      body = w.mbodies[body].parent;
+    }
+  };
+  for (int step = 0; step < max_steps; ++step) {
+    uint32_t state0 = 0, state1 = 0;
+    const auto& links = final_links(w);
+    for (size_t k = 0; k < closures.size(); ++k) {
+      const auto& r = w.bearings[closures[k]];
+      const auto& pa = links[r.a];
+      const auto& pb = links[r.b];
+      End& e = ends[k];
+      e.anchor_a = add(xyz(pa.position), quat_rotate(pa.rotation, r.anchor_a));
+      e.anchor_b = add(xyz(pb.position), quat_rotate(pb.rotation, r.anchor_b));
+      e.axis_a = normalize_v(quat_rotate(pa.rotation, r.axis_a));
+      e.axis_b = normalize_v(quat_rotate(pb.rotation, r.axis_b));
+      e.res_pos = sub(e.anchor_a, e.anchor_b);
+      e.res_axis = sub(e.axis_a, e.axis_b);
+      e.active = false;
+      if (!finite3v(e.res_pos) || !finite3v(e.res_axis)) {
+        w.diag[0] |= INVALID_NUMERIC;
+        state0 = state1 = 0xffffffffu;
+        continue;
+      }
+      uint32_t um = round_u32(length(e.res_pos) * R(1000000.0f));
+      R degrees = std::acos(clampr(dot(e.axis_a, e.axis_b), R(-1), R(1))) * R(57.295779513f);
+      uint32_t ud = round_u32(degrees * R(1000000.0f));
+      state0 = std::max(state0, um);
+      state1 = std::max(state1, ud);
+      e.active = !(um <= 10u && ud <= 1000u);
+    }
+    if (!(state0 > 10u || state1 > 1000u)) break;
+    std::vector<double> g(nq, 0.0), dg(nq, 0.0);
+    for (size_t k = 0; k < closures.size(); ++k) {
+      const End& e = ends[k];
+      if (!e.active) continue;
+      const auto& r = w.bearings[closures[k]];
+      for (uint32_t side = 0; side < 2; ++side) {
+        V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
+        R sign = side ? R(-1) : R(1);
+        walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+          V3<R> jaxis, janchor;
+          joint_of(w, body, jaxis, janchor);
+          V3<R> pd = mul(cross(jaxis, sub(end_anchor, janchor)), sign);
+          V3<R> ad = mul(cross(jaxis, end_axis), sign);
+          R gradient = dot(e.res_pos, pd) + R(1) * dot(e.res_axis, ad);
+          R diag = dot(pd, pd) + R(1) * dot(ad, ad);
+          if (gradient == gradient && diag == diag) {
+            g[c] += (double)gradient;
+            dg[c] += (double)diag;
+          } else {
+            w.diag[0] |= INVALID_NUMERIC;
+          }
+        });
+      }
+    }
+    double rz = 0.0;
+    for (uint32_t c = 0; c < nq; ++c) {
+      R gf = R(g[c]), df = R(dg[c]);
+      diagonal[c] = df;
+      residual[c] = df > R(1.0e-12f) ? -gf : R(0);
+      pre[c] = df > R(1.0e-12f) ? residual[c] / df : R(0);
+      solution[c] = R(0);
+      direction[c] = pre[c];
+      rz += (double)residual[c] * (double)pre[c];
+    }
+    for (int it = 0; it < 8; ++it) {
+      std::fill(op.begin(), op.end(), R(0));
+      for (size_t k = 0; k < closures.size(); ++k) {
+        const End& e = ends[k];
+        const auto& r = w.bearings[closures[k]];
+        V3<R> prod_pos[2], prod_axis[2];
+        for (uint32_t side = 0; side < 2; ++side) {
+          V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
+          R sign = side ? R(-1) : R(1);
+          V3<R> pp = v3<R>(0, 0, 0), pa = pp;
+          walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+            V3<R> jaxis, janchor;
+            joint_of(w, body, jaxis, janchor);
+            pp = add(pp, mul(mul(cross(jaxis, sub(end_anchor, janchor)), sign), direction[c]));
+            pa = add(pa, mul(mul(cross(jaxis, end_axis), sign), direction[c]));
+          });
+          prod_pos[side] = pp;
+          prod_axis[side] = pa;
+        }
+        V3<R> product_pos = add(prod_pos[0], prod_pos[1]), product_axis = add(prod_axis[0], prod_axis[1]);
+        for (uint32_t side = 0; side < 2; ++side) {
+          V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
+          R sign = side ? R(-1) : R(1);
+          walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+            V3<R> jaxis, janchor;
+            joint_of(w, body, jaxis, janchor);
+            V3<R> pd = mul(cross(jaxis, sub(end_anchor, janchor)), sign);
+            V3<R> ad = mul(cross(jaxis, end_axis), sign);
+            op[c] = op[c] + (dot(pd, product_pos) + R(1) * dot(ad, product_axis));
+          });
+        }
+      }
+      double dad = 0.0;
+      for (uint32_t c = 0; c < nq; ++c) dad += (double)direction[c] * (double)op[c];
+      R alpha = std::fabs(dad) > 1.0e-20 ? R(rz / dad) : R(0);
+      double rz_next = 0.0;
+      for (uint32_t c = 0; c < nq; ++c) {
+        solution[c] = solution[c] + alpha * direction[c];
+        residual[c] = residual[c] - alpha * op[c];
+        pre[c] = diagonal[c] > R(1.0e-12f) ? residual[c] / diagonal[c] : R(0);
+        rz_next += (double)residual[c] * (double)pre[c];
+      }
+      R beta = std::fabs(rz) > 1.0e-20 ? R(rz_next / rz) : R(0);
+      for (uint32_t c = 0; c < nq; ++c) direction[c] = pre[c] + beta * direction[c];
+      rz = rz_next;
+    }
+    for (uint32_t c = 0; c < nq; ++c) {
+      R step_c = clampr(R(0.35f) * solution[c], R(-0.2f), R(0.2f));
+      if (step_c == step_c) w.coord_angle[c] = w.coord_angle[c] + step_c;
+      else w.diag[0] |= INVALID_NUMERIC;
+    }
+    forward_kinematics(w);
+  }
+}
+
+// validate_articulated_state (articulated.wgsl:458-484).
+template <typename R> void validate_articulated(World<R>& w) {
+  for (uint32_t b = 0; b < w.body_count; ++b) {
+    R rl = dot4(w.rot[b], w.rot[b]);
+    bool valid = finite4(w.pos[b]) && finite4(w.rot[b]) && finite4(w.lin[b]) && finite4(w.ang[b]) && rl > R(0.5f) &&
+                 rl < R(1.5f);
+    const auto& m = w.mbodies[b];
+    if (m.root == 0u) {
+      uint32_t c = w.bearings[m.tree_bearing].coordinate;
+      R a = w.coord_angle[c], v = w.coord_speed[c];
+      const R lim = R(3.402823e+38f);
+      valid = valid && a == a && std::fabs(a) < lim && v == v && std::fabs(v) < lim;
+    }
+    if (!valid) w.diag[0] |= INVALID_NUMERIC;
+  }
+}
+
+// validate_bearings / validate_mechanism_bearings (bearings.wgsl:53-102).
+template <typename R> void validate_bearings(World<R>& w) {
+  for (uint32_t i = 0; i < w.bearing_count; ++i) {
+    const auto& r = w.bearings[i];
+    V4<R> pa, ra, pb, rb;
+    if (w.mech_active) {
+      const auto& links = final_links(w);
+      pa = links[r.a].position; ra = links[r.a].rotation; pb = links[r.b].position; rb = links[r.b].rotation;
+    } else {
+      pa = w.pos[r.a]; ra = w.rot[r.a]; pb = w.pos[r.b]; rb = w.rot[r.b];
+    }
+    V3<R> anchor_a = add(xyz(pa), quat_rotate(ra, r.anchor_a)), anchor_b = add(xyz(pb), quat_rotate(rb, r.anchor_b));
+    V3<R> axis_a = normalize_v(quat_rotate(ra, r.axis_a)), axis_b = normalize_v(quat_rotate(rb, r.axis_b));
+    uint32_t um = round_u32(length(sub(anchor_a, anchor_b)) * R(1000000.0f));
+    R degrees = std::atan2(length(cross(axis_a, axis_b)), clampr(dot(axis_a, axis_b), R(-1), R(1))) *
+                R(57.295779513f);
+    uint32_t ud = round_u32(degrees * R(1000000.0f));
+    w.diag[3] = std::max(w.diag[3], um);
+    w.diag[4] = std::max(w.diag[4], ud);
+    if (r.closure != 0u && (um > 10u || ud > 1000u)) w.diag[0] |= NON_CONVERGENCE;
+  }
+}
+
+// encode_mechanism_passes (device.rs:963-1048).
+template <typename R> void mechanism_passes(World<R>& w) {
+  const uint32_t iterations = std::max(1u, w.iterations);
+  for (uint32_t i = 0; i < iterations; ++i) bearing_projection(w);
+  advance_coordinates(w);
+  forward_kinematics(w);
+  closure_newton(w);
+  publish(w);
+  reconstruct(w);
+}
+
 // One tick, pass order of dispatch_tick (device.rs:859-961) and
 // encode_collision_passes (device.rs:1219-1431).
 // `stop` ends the tick early for phase tests: 1 after integration, 2 after the
@@ -776,38 +1227,52 @@ template <typename R> void step(World<R>& w, uint64_t tick64, int stop) {
   const uint32_t tick = (uint32_t)tick64;
   std::memset(w.diag, 0, sizeof(w.diag));
   integrate(w);
-  if (stop == 1 || !(w.collisions && w.collider_count > 0)) return;
-  update_world_masses(w);
-  compute_morton(w);
-  sort_keys(w);
-  build_topology(w);
-  build_bounds(w);
-  traverse(w);
-  if (stop == 2) return;
-  narrowphase(w);
-  generate_ground_contacts(w);
-  if (stop == 3) return;
-  // Collider pair of each contact before prepare overwrites metadata z/w.
-  std::vector<uint64_t> keys(w.contacts.size());
-  for (size_t i = 0; i < w.contacts.size(); ++i)
-    keys[i] = ((uint64_t)std::min(w.contacts[i].m_z, w.contacts[i].m_w) << 32) |
-              std::max(w.contacts[i].m_z, w.contacts[i].m_w);
-  prepare_contacts(w, tick);
-  w.deltas.assign(6 * (size_t)w.body_count, 0);
-  const uint32_t iterations = std::max(1u, w.iterations);
-  for (uint32_t it = 0; it < iterations; ++it) {
-    solve_pass(w, it == 0);
-    if (!w.active.empty()) solve_apply(w);
+  if (w.mech_active) mechanism_passes(w);
+  if (stop == 1) return;
+  if (w.collisions && w.collider_count > 0) {
+    update_world_masses(w);
+    compute_morton(w);
+    sort_keys(w);
+    build_topology(w);
+    build_bounds(w);
+    traverse(w);
+    if (stop == 2) return;
+    narrowphase(w);
+    generate_ground_contacts(w);
+    if (stop == 3) return;
+    // Collider pair of each contact before prepare overwrites metadata z/w.
+    std::vector<uint64_t> keys(w.contacts.size());
+    for (size_t i = 0; i < w.contacts.size(); ++i)
+      keys[i] = ((uint64_t)std::min(w.contacts[i].m_z, w.contacts[i].m_w) << 32) |
+                std::max(w.contacts[i].m_z, w.contacts[i].m_w);
+    prepare_contacts(w, tick);
+    w.deltas.assign(6 * (size_t)w.body_count, 0);
+    const uint32_t iterations = std::max(1u, w.iterations);
+    const bool interleave = w.mech_active && w.has_dynamic_root;
+    for (uint32_t it = 0; it < iterations; ++it) {
+      solve_pass(w, it == 0);
+      if (!w.active.empty()) solve_apply(w);
+      if (interleave) bearing_projection(w);
+    }
+    // persist_contacts (collision.wgsl:1071-1088)
+    for (uint32_t ci : w.active) {
+      const Contact<R>& c = w.contacts[ci];
+      Manifold<R>& m = w.manifolds[keys[ci]];
+      m.tick = tick;
+      m.count = c.m_w;
+      m.normal_penetration = c.normal_penetration;
+      m.point_impulse = c.arm_a_impulse;
+    }
+    if (w.mech_active) {
+      // encode_post_contact_mechanism (device.rs:1128-1155).
+      if (!w.has_dynamic_root)
+        for (uint32_t i = 0; i < iterations; ++i) bearing_projection(w);
+      capture_coordinates(w);
+      if (!w.has_dynamic_root) reconstruct(w);
+    }
   }
-  // persist_contacts (collision.wgsl:1071-1088)
-  for (uint32_t ci : w.active) {
-    const Contact<R>& c = w.contacts[ci];
-    Manifold<R>& m = w.manifolds[keys[ci]];
-    m.tick = tick;
-    m.count = c.m_w;
-    m.normal_penetration = c.normal_penetration;
-    m.point_impulse = c.arm_a_impulse;
-  }
+  if (w.mech_active) validate_articulated(w);
+  if (w.bearing_count > 0) validate_bearings(w);
 }
 
 }  // namespace oracle
@@ -1022,6 +1487,127 @@ size_t oracle_get_world_masses(OracleHandle* h, float* out, size_t cap) {
     size_t n = std::min(cap, w.world_masses.size());
     for (size_t i = 0; i < n; ++i) put4(out + 4 * i, w.world_masses[i]);
     return w.world_masses.size();
+  };
+  return h->fp64 ? get(h->d) : get(h->f);
+}
+
+// Articulated rows: bearings (80-byte mh_bearing rows), mechanism bodies
+// (64-byte rows), preorder, drives (32-byte rows; NULL = passive).
+void oracle_set_mechanisms(OracleHandle* h, uint32_t nb, const float* bearings, const float* mbodies,
+                           const uint32_t* preorder, uint32_t nq, const float* drives) {
+  auto set = [&](auto& w) {
+    using W = typename std::remove_reference<decltype(w)>::type;
+    using R = typename std::remove_reference<decltype(w.dt)>::type;
+    w.bearing_count = nb;
+    w.coordinate_count = nq;
+    w.bearings.resize(nb);
+    for (uint32_t i = 0; i < nb; ++i) {
+      const float* r = bearings + 20 * (size_t)i;
+      uint32_t meta[4];
+      std::memcpy(meta, r + 16, 16);
+      w.bearings[i] = typename W::BearingRow{{R(r[0]), R(r[1]), R(r[2])}, {R(r[4]), R(r[5]), R(r[6])},
+                                             {R(r[8]), R(r[9]), R(r[10])}, {R(r[12]), R(r[13]), R(r[14])},
+                                             meta[0], meta[1], meta[2], meta[3]};
+    }
+    const uint32_t n = w.body_count;
+    w.mbodies.resize(n);
+    w.max_depth = 0;
+    w.has_dynamic_root = false;
+    for (uint32_t b = 0; b < n; ++b) {
+      const float* r = mbodies + 16 * (size_t)b;
+      uint32_t meta[4], trav[4];
+      std::memcpy(meta, r, 16);
+      std::memcpy(trav, r + 4, 16);
+      w.mbodies[b] = typename W::MechBody{meta[0], meta[1], meta[2], meta[3], {R(r[8]), R(r[9]), R(r[10])},
+                                          {R(r[12]), R(r[13]), R(r[14]), R(r[15])}};
+      if (meta[3] == 0u) w.max_depth = std::max(w.max_depth, trav[1]);
+      if (meta[3] != 0u && w.inv_mass[b] > R(0)) w.has_dynamic_root = true;
+    }
+    w.preorder.assign(preorder, preorder + n);
+    w.jump_rounds = 0;
+    uint32_t covered = 1;
+    while (covered < w.max_depth) { covered *= 2; ++w.jump_rounds; }
+    if (w.max_depth > 0) w.jump_rounds = std::max(w.jump_rounds, 1u);
+    w.mech_active = w.max_depth > 0;
+    w.coord_angle.assign(nq, R(0));
+    w.coord_speed.assign(nq, R(0));
+    w.drives.resize(nq);
+    for (uint32_t q = 0; q < nq; ++q) {
+      if (drives) {
+        const float* r = drives + 8 * (size_t)q;
+        uint32_t mode;
+        std::memcpy(&mode, r, 4);
+        w.drives[q] = typename W::Drive{mode, R(r[1]), R(r[2]), R(r[3]), R(r[4]), R(r[5]), R(r[6])};
+      } else {
+        w.drives[q] = typename W::Drive{0u, R(0), R(0), R(0), R(0), -R(INFINITY), R(INFINITY)};
+      }
+    }
+    typename W::Link identity{{R(0), R(0), R(0), R(0)}, {R(0), R(0), R(0), R(1)}, 0u, 1u};
+    w.links_a.assign(n, identity);
+    w.links_b.assign(n, identity);
+    for (uint32_t b = 0; b < n; ++b) w.links_a[b].ancestor = w.links_b[b].ancestor = b;
+    w.mdeltas.assign(6 * (size_t)n, 0);
+  };
+  if (h->fp64) set(h->d);
+  else set(h->f);
+}
+
+void oracle_get_coordinates(OracleHandle* h, float* out) {
+  auto get = [&](auto& w) {
+    for (uint32_t q = 0; q < w.coordinate_count; ++q) {
+      out[2 * q] = (float)w.coord_angle[q];
+      out[2 * q + 1] = (float)w.coord_speed[q];
+    }
+  };
+  if (h->fp64) get(h->d);
+  else get(h->f);
+}
+
+void oracle_set_coordinates(OracleHandle* h, const float* in) {
+  auto set = [&](auto& w) {
+    using R = typename std::remove_reference<decltype(w.dt)>::type;
+    for (uint32_t q = 0; q < w.coordinate_count; ++q) {
+      w.coord_angle[q] = R(in[2 * q]);
+      w.coord_speed[q] = R(in[2 * q + 1]);
+    }
+  };
+  if (h->fp64) set(h->d);
+  else set(h->f);
+}
+
+// Forward kinematics from the current coordinates, then `steps` closure Newton
+// steps (no dynamics), for the closure-solver checks.
+void oracle_fk_newton(OracleHandle* h, int steps) {
+  auto run = [&](auto& w) {
+    forward_kinematics(w);
+    if (steps > 0) closure_newton(w, steps);
+  };
+  if (h->fp64) run(h->d);
+  else run(h->f);
+}
+
+// Per closure: anchor residual (3) and axis residual (3) from the final links.
+size_t oracle_closure_residuals(OracleHandle* h, double* out, size_t cap) {
+  auto get = [&](auto& w) {
+    size_t k = 0;
+    const auto& links = final_links(w);
+    for (uint32_t i = 0; i < w.bearing_count; ++i) {
+      const auto& r = w.bearings[i];
+      if (r.closure == 0u) continue;
+      if (k < cap) {
+        const auto& pa = links[r.a];
+        const auto& pb = links[r.b];
+        auto aa = add(xyz(pa.position), quat_rotate(pa.rotation, r.anchor_a));
+        auto ab = add(xyz(pb.position), quat_rotate(pb.rotation, r.anchor_b));
+        auto xa = normalize_v(quat_rotate(pa.rotation, r.axis_a));
+        auto xb = normalize_v(quat_rotate(pb.rotation, r.axis_b));
+        double* o = out + 6 * k;
+        o[0] = aa.x - ab.x; o[1] = aa.y - ab.y; o[2] = aa.z - ab.z;
+        o[3] = xa.x - xb.x; o[4] = xa.y - xb.y; o[5] = xa.z - xb.z;
+      }
+      ++k;
+    }
+    return k;
   };
   return h->fp64 ? get(h->d) : get(h->f);
 }

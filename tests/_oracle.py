@@ -41,6 +41,12 @@ def load():
     lib.oracle_get_aabbs.argtypes = [P, P, sz, i32]
     lib.oracle_morton_code.restype = u32
     lib.oracle_morton_code.argtypes = [f32, f32, f32]
+    lib.oracle_set_mechanisms.argtypes = [P, u32, P, P, P, u32, P]
+    lib.oracle_get_coordinates.argtypes = [P, P]
+    lib.oracle_set_coordinates.argtypes = [P, P]
+    lib.oracle_fk_newton.argtypes = [P, i32]
+    lib.oracle_closure_residuals.restype = sz
+    lib.oracle_closure_residuals.argtypes = [P, P, sz]
     lib.oracle_obb_sat_f64.restype = i32
     lib.oracle_obb_sat_f64.argtypes = [P] * 7
     _lib = lib
@@ -71,6 +77,13 @@ class Oracle:
                                    _addr(scene.masses), _addr(scene.colliders), _addr(supp) if len(supp) else None,
                                    _addr(comps), _addr(roots), iterations, friction, pair_capacity,
                                    int(collisions), int(self_collisions), int(fp64), threads)
+        self.q = 0
+        if getattr(scene, "bearing_count", 0):
+            pre = np.ascontiguousarray(scene.preorder, np.uint32)
+            self._keep.append(pre)
+            self.q = scene.coordinate_count
+            lib.oracle_set_mechanisms(self.h, scene.bearing_count, _addr(scene.bearings), _addr(scene.mechanism_bodies),
+                                      _addr(pre), self.q, _addr(scene.drives) if self.q else None)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -108,6 +121,25 @@ class Oracle:
         p = np.ascontiguousarray(poses, np.float32)
         v = np.ascontiguousarray(velocities, np.float32)
         load().oracle_set_bodies(self.h, p.ctypes.data, v.ctypes.data)
+
+    def coordinates(self):
+        out = np.zeros((max(self.q, 1), 2), np.float32)
+        load().oracle_get_coordinates(self.h, out.ctypes.data)
+        return out[: self.q]
+
+    def set_coordinates(self, rows):
+        a = np.ascontiguousarray(rows, np.float32).reshape(-1, 2)
+        load().oracle_set_coordinates(self.h, a.ctypes.data)
+
+    def fk_newton(self, steps=0):
+        load().oracle_fk_newton(self.h, steps)
+
+    def closure_residuals(self):
+        lib = load()
+        n = lib.oracle_closure_residuals(self.h, None, 0)
+        out = np.zeros((max(n, 1), 6), np.float64)
+        lib.oracle_closure_residuals(self.h, out.ctypes.data, n)
+        return out[:n]
 
     def diag(self):
         d = np.zeros(8, np.uint32)

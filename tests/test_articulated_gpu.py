@@ -1,0 +1,171 @@
+"""GPU tests of the articulated path against the oracle and closed forms:
+hinge chains, a closed four-bar, drives and limits, a small articulated pile
+with a dynamic root (bearing projection inside the contact solve), the row-count
+errors of write_mechanism_drives / initialize_mechanism_coordinates, and the
+100,000-body articulated scene."""
+import math
+
+import numpy as np
+import pytest
+
+import mechanic_amd as mh
+from mechanic_amd import articulated as A
+from mechanic_amd.physics import GpuPhysics, GpuPhysicsConfig, GpuPhysicsError
+from tests._oracle import Oracle
+from tests.test_articulated_oracle import DT, _check_hinge_composition, axis_rot, quat_rot
+
+pytestmark = pytest.mark.gpu
+
+# FK evaluates sin/cos with the device's math library and the oracle with the
+# host's, so articulated states agree to a tolerance rather than bit for bit.
+POSE_TOL = 2e-5
+
+
+def run_both(scene, ticks, collisions=False, friction=0.6, iterations=8, coordinates=None, self_collisions=True):
+    cfg = GpuPhysicsConfig(collisions_enabled=collisions, friction=friction, solver_iterations=iterations,
+                           mechanism_self_collisions=self_collisions)
+    gpu = GpuPhysics.new_with_config(scene, cfg)
+    ora = Oracle(scene, iterations=iterations, friction=friction, collisions=collisions,
+                 self_collisions=self_collisions, threads=4)
+    if coordinates is not None:
+        gpu.initialize_mechanism_coordinates([mh._lib.MechanismCoordinate(float(a), float(v)) for a, v in coordinates])
+        ora.set_coordinates(np.asarray(coordinates, np.float32))
+    for t in range(1, ticks + 1):
+        gpu.dispatch_tick(t)
+        ora.step(t)
+    gp, gv = gpu.read_bodies()
+    gc = gpu.read_coordinates()
+    flags = gpu.read_last_tick().error_flags
+    gpu.close()
+    op, ov = ora.bodies()
+    return (gp, gv, gc, flags), (op, ov, ora.coordinates(), int(ora.diag()[0]))
+
+
+def test_hinge_chain_one_tick_matches_oracle():
+    scene = A.hinge_chain(links=6, length=0.4)
+    (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 1)
+    assert gf == of == 0
+    np.testing.assert_allclose(gp, op, rtol=0, atol=POSE_TOL)
+    np.testing.assert_allclose(gv, ov, rtol=0, atol=POSE_TOL)
+    np.testing.assert_allclose(gc, oc, rtol=0, atol=POSE_TOL)
+
+
+def test_hinge_chain_swings_like_the_oracle():
+    scene = A.hinge_chain(links=6, length=0.4)
+    (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 60)
+    assert gf == of == 0
+    np.testing.assert_allclose(gc[:, 0], oc[:, 0], atol=2e-4)
+    np.testing.assert_allclose(gp[:, :3], op[:, :3], atol=5e-4)
+    _check_hinge_composition(scene, gp, gc, POSE_TOL)
+
+
+def test_four_bar_matches_oracle_and_stays_closed():
+    scene = A.four_bar()
+    start = [(0.2, 0.0), (-0.2, 0.0), (0.2, 0.0)]
+    (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 60, coordinates=start)
+    np.testing.assert_allclose(gc[:, 0], oc[:, 0], atol=2e-4)
+    np.testing.assert_allclose(gp[:, :3], op[:, :3], atol=5e-4)
+    c = gc[:, 0]
+    assert abs(c[0] - 0.2) > 0.02
+    np.testing.assert_allclose(c, [c[0], -c[0], c[0]], atol=2e-3)
+
+
+def test_driven_vertical_hinge_closed_form():
+    drive = A.Drive(mode=1, max_acceleration=float("inf"), max_speed=1.0, target_speed=1.0)
+    scene = A.hinge_chain(links=1, length=0.6, axis=(0.0, 1.0, 0.0), drive=drive)
+    k = 45
+    with GpuPhysics.new_with_config(scene, GpuPhysicsConfig(collisions_enabled=False)) as gpu:
+        for t in range(1, k + 1):
+            gpu.dispatch_tick(t)
+        p, v = gpu.read_bodies()
+        c = gpu.read_coordinates()
+        assert gpu.read_last_tick().error_flags == 0
+    assert c[0, 0] == pytest.approx(k * DT, rel=1e-5)
+    anchor = np.array([0.0, 3.0, 0.0])
+    expect = anchor + quat_rot(axis_rot((0, 1, 0), k * DT), np.array([0.3, 3.0, 0.0]) - anchor)
+    np.testing.assert_allclose(p[1, :3], expect, atol=2e-6)
+    np.testing.assert_allclose(v[1, 4:7], [0.0, 1.0, 0.0], atol=2e-6)
+
+
+def test_drive_limits_and_angle_mode():
+    drive = A.Drive(mode=1, max_acceleration=100.0, max_speed=2.0, target_speed=2.0, min_angle=-0.5, max_angle=0.5)
+    scene = A.hinge_chain(links=1, length=0.6, axis=(0.0, 1.0, 0.0), drive=drive)
+    with GpuPhysics.new_with_config(scene, GpuPhysicsConfig(collisions_enabled=False)) as gpu:
+        for t in range(1, 40):
+            gpu.dispatch_tick(t)
+        c = gpu.read_coordinates()
+        assert c[0, 0] == np.float32(0.5) and c[0, 1] == 0.0
+        # Reprogram the drive to seek an angle (write_mechanism_drives).
+        gpu.write_mechanism_drives([A.Drive(mode=2, max_acceleration=20.0, max_speed=2.0, target_angle=-0.3).row()])
+        for t in range(40, 280):
+            gpu.dispatch_tick(t)
+        c = gpu.read_coordinates()
+    assert abs(c[0, 0] + 0.3) < 2e-3
+
+
+def test_row_count_errors():
+    scene = A.hinge_chain(links=3)
+    with GpuPhysics.new_with_config(scene, GpuPhysicsConfig(collisions_enabled=False)) as gpu:
+        with pytest.raises(GpuPhysicsError) as e:
+            gpu.write_mechanism_drives([A.PASSIVE.row()] * 2)
+        assert e.value.kind == "DriveStateCount" and e.value.provided == 2 and e.value.required == 3
+        with pytest.raises(GpuPhysicsError) as e:
+            gpu.initialize_mechanism_coordinates([mh._lib.MechanismCoordinate(0.0, 0.0)] * 4)
+        assert e.value.kind == "CoordinateStateCount" and e.value.required == 3
+
+
+def small_articulated_pile():
+    return A.articulated_pile(seed=5, nx=4, ny=2, nz=4, cylinders=150, tree_bearings=80, closures=6)
+
+
+def test_small_articulated_pile_one_tick_matches_oracle():
+    """A dynamic root exists (free cuboids), so a bearing projection follows
+    every contact pass."""
+    scene = small_articulated_pile()
+    assert scene.bearing_count > 64
+    (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 1, collisions=True, self_collisions=False)
+    assert gf == of
+    np.testing.assert_allclose(gp, op, rtol=0, atol=POSE_TOL)
+    np.testing.assert_allclose(gv, ov, rtol=0, atol=1e-4)
+    np.testing.assert_allclose(gc, oc, rtol=0, atol=1e-4)
+
+
+def test_small_articulated_pile_rollout_statistics():
+    scene = small_articulated_pile()
+    (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 60, collisions=True, self_collisions=False)
+    assert gf & mh.INVALID_NUMERIC_FLAG == 0 and of & mh.INVALID_NUMERIC_FLAG == 0
+    # Trajectories drift apart at float rounding; compare distributions.
+    assert abs(gp[:, 1].mean() - op[:, 1].mean()) < 0.02
+    assert abs(np.abs(gv[:, :3]).mean() - np.abs(ov[:, :3]).mean()) < 0.05 * max(np.abs(ov[:, :3]).mean(), 0.1)
+
+
+def test_graph_replay_matches_direct_dispatch():
+    scene = small_articulated_pile()
+    out = []
+    for graph in (True, False):
+        cfg = GpuPhysicsConfig(friction=0.6, use_graph=graph, mechanism_self_collisions=False)
+        with GpuPhysics.new_with_config(scene, cfg) as gpu:
+            for t in range(1, 21):
+                gpu.dispatch_tick(t)
+            p, v = gpu.read_bodies()
+            out.append((p.view(np.uint32).copy(), v.view(np.uint32).copy(), gpu.read_coordinates().view(np.uint32)))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_articulated_scene_at_scale():
+    """The 100,000-body articulated scene: 90,000 cuboids, 10,000 cylinders,
+    5,000 bearings (500 closures), 20% of the tree bearings driven."""
+    scene = A.articulated_pile()
+    assert scene.body_count == 100_000 and scene.bearing_count == 5000
+    cfg = GpuPhysicsConfig(friction=0.6, mechanism_self_collisions=False)
+    with GpuPhysics.new_with_config(scene, cfg) as gpu:
+        for t in range(1, 11):
+            gpu.dispatch_tick(t)
+        rb = gpu.read_last_tick()
+        p, v = gpu.read_bodies()
+        c = gpu.read_coordinates()
+    assert rb.error_flags & (mh.INVALID_NUMERIC_FLAG | mh.PAIR_OVERFLOW_FLAG) == 0
+    assert np.all(np.isfinite(p)) and np.all(np.isfinite(c))
+    assert rb.contact_count > 0
+    _check_hinge_composition(scene, p, c, 5e-5)

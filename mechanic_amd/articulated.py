@@ -19,10 +19,13 @@ compile.rs:504-611) and the mechanism lowering of GpuPhysics
   compiled with identity root rotation, compile.rs:331).
 
 `articulated_pile` builds the 100,000-body articulated scene: 90,000 cuboids in
-the jittered pile lattice plus 10,000 solid or hollow cylinders, 5,000 of them
-hinged into hanging zigzag chains of 2-20 links under static roots, with 500
-loop-closure bearings and 20% of the tree bearings driven at +-2 rad/s within
-+-pi/2. Each cylinder is one OBB collider (half extents r, L/2, r).
+the jittered pile lattice plus 10,000 solid or hollow cylinders, about half of
+them hinged into zigzag chains of 2-20 links hanging from static roots in
+vertical planes, with 500 loop-closure bearings (planar four-joint loops) and 20%
+of the bearings driven at +-2 rad/s within +-pi/2 by gentle motors
+(0.5 rad/s^2), none inside a loop. Chains are laid out so that none can reach
+the ground or another chain; the other cylinders lie free on the ground. Each
+cylinder is one OBB collider (half extents r, L/2, r).
 """
 from __future__ import annotations
 
@@ -250,6 +253,13 @@ def _quat_axis_angle(axis, angle):
     return np.array([axis[0] * s, axis[1] * s, axis[2] * s, math.cos(angle * 0.5)])
 
 
+def _quat_mul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + bw * ax + (ay * bz - az * by), aw * by + bw * ay + (az * bx - ax * bz),
+                     aw * bz + bw * az + (ax * by - ay * bx), aw * bw - (ax * bx + ay * by + az * bz)])
+
+
 def _quat_matrix(q):
     x, y, z, w = q
     return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
@@ -325,7 +335,7 @@ def _concat(a: Scene, b: Scene) -> Scene:
 
 def articulated_pile(seed: int = 2, nx: int = 50, ny: int = 36, nz: int = 50, cylinders: int = 10_000,
                      tree_bearings: int = 4_500, closures: int = 500, driven_fraction: float = 0.2,
-                     chain_lengths=(2, 20)) -> Scene:
+                     chain_lengths=(2, 20), drive_acceleration: float = 0.5) -> Scene:
     """The articulated configuration (defaults: 90,000 cuboids + 10,000 cylinders,
     5,000 bearings of which 500 close loops, 20% of the tree bearings driven)."""
     rng = np.random.default_rng(seed)
@@ -346,48 +356,48 @@ def articulated_pile(seed: int = 2, nx: int = 50, ny: int = 36, nz: int = 50, cy
         used += L
         bearings_left -= L - 1
     free = list(range(used, cylinders))
+    # Links of one chain share one diameter and bore (drawn once per chain, so
+    # every cylinder's marginal distribution is unchanged): the Jacobi bearing
+    # projection does not converge across the ~3000:1 mass ratios of
+    # independently drawn neighbours.
+    for chain in chains:
+        outer_d[chain] = outer_d[chain[0]]
+        bore[chain] = bore[chain[0]]
 
     positions = np.zeros((cylinders, 3))
     rotations = np.zeros((cylinders, 4))
     rotations[:, 3] = 1.0
     static = np.zeros(cylinders, dtype=bool)
     specs: List[BearingSpec] = []
-    # Chains hang as zigzags in vertical x-y planes from static roots, on a grid
-    # around the pile (cells inside the pile's footprint are skipped).
-    spacing = 10.0
-    side = int(math.ceil(math.sqrt(len(chains) + 16))) + 2
-    cells = [(i, j) for i in range(side) for j in range(side)
-             if max(abs(i - side / 2 + 0.5), abs(j - side / 2 + 0.5)) * spacing > 12.0]
+    # Chains hang as zigzags in vertical x-y planes from static roots, hinged
+    # about z, so every link stays in its chain's plane. A chain's reach is its
+    # total length: roots sit that high above the ground, chains sharing a plane
+    # are two reaches apart, and planes are 3 m apart (more than any diameter),
+    # so no chain can touch the ground or another chain whatever its joints do.
     z_axis = np.array([0.0, 0.0, 1.0])
-    drive_rows = []
+    per_row = 5
+    x_cursor = {}
     for c, chain in enumerate(chains):
-        ci, cj = cells[c]
-        base_x = (ci - side / 2 + 0.5) * spacing
-        base_z = (cj - side / 2 + 0.5) * spacing
-        dirs = []
-        for k in range(len(chain)):
-            swing = math.radians(rng.uniform(20.0, 60.0)) * (1 if k % 2 == 0 else -1)
-            dirs.append(-math.pi / 2 + swing)
-        drop = sum(abs(math.sin(a)) * length[b] for a, b in zip(dirs, chain))
-        joint = np.array([base_x, drop + 1.0 + float(outer_d[chain].max()), base_z])
+        row = c // per_row
+        reach = float(length[chain].sum())
+        x0 = x_cursor.get(row, 20.0) + reach + 1.0
+        x_cursor[row] = x0 + reach + 1.0
+        joint = np.array([x0, reach + 2.0, -40.0 - 3.0 * row])
         for k, b in enumerate(chain):
-            a = dirs[k]
+            a = -math.pi / 2 + math.radians(rng.uniform(20.0, 60.0)) * (1 if k % 2 == 0 else -1)
             d = np.array([math.cos(a), math.sin(a), 0.0])
             positions[b] = joint + d * (length[b] * 0.5)
             # Local y (the cylinder axis) along d: rotate about z by a - pi/2.
             rotations[b] = _quat_axis_angle(z_axis, a - math.pi / 2)
             if k > 0:
-                driven = rng.uniform() < driven_fraction
-                drive = None
-                if driven:
-                    drive = Drive(mode=1, max_acceleration=40.0, max_speed=2.0,
-                                  target_speed=2.0 if rng.uniform() < 0.5 else -2.0,
-                                  min_angle=-math.pi / 2, max_angle=math.pi / 2)
-                specs.append(BearingSpec(chain[k - 1], b, joint.copy(), z_axis, drive))
+                specs.append(BearingSpec(chain[k - 1], b, joint.copy(), z_axis, None))
             joint = joint + d * length[b]
         static[chain[0]] = True
     # Loop closures: link a to link a + 3 at the centre of a + 3 (a planar
     # four-joint loop, one degree of freedom), spread over chains of >= 4 links.
+    tree_count = len(specs)
+    bearing_of_child = {s_.b: i for i, s_ in enumerate(specs)}
+    in_loop = set()
     long_chains = [ch for ch in chains if len(ch) >= 4]
     made = 0
     slot = {id(ch): 0 for ch in long_chains}
@@ -401,17 +411,26 @@ def articulated_pile(seed: int = 2, nx: int = 50, ny: int = 36, nz: int = 50, cy
                 continue
             a, b = ch[start], ch[start + 3]
             specs.append(BearingSpec(a, b, positions[b].copy(), z_axis, None))
+            in_loop.update(bearing_of_child[ch[start + k]] for k in (1, 2, 3))
             slot[id(ch)] = start + 4
             made += 1
             progressed = True
         if not progressed:
             break
-    # Free cylinders lie on a grid beyond the chains, axis along x, resting 0.3 m up.
+    # Drives: driven_fraction of all bearings, on tree bearings outside every
+    # loop (two motors in one single-degree-of-freedom loop would fight).
+    eligible = [i for i in range(tree_count) if i not in in_loop]
+    want = min(len(eligible), int(round(driven_fraction * len(specs))))
+    for i in sorted(rng.choice(len(eligible), size=want, replace=False)) if want else []:
+        specs[eligible[i]].drive = Drive(mode=1, max_acceleration=drive_acceleration, max_speed=2.0,
+                                         target_speed=2.0 if rng.uniform() < 0.5 else -2.0,
+                                         min_angle=-math.pi / 2, max_angle=math.pi / 2)
+    # Free cylinders lie on a grid on the other side of the pile, axis along x,
+    # resting 0.3 m up.
     grid = int(math.ceil(math.sqrt(max(len(free), 1))))
-    offset = side * spacing / 2 + 10.0
     for k, b in enumerate(free):
         gi, gj = divmod(k, grid)
-        positions[b] = [offset + gi * 4.5, outer_d[b] * 0.5 + 0.3, (gj - grid / 2) * 4.5]
+        positions[b] = [-20.0 - gi * 4.5, outer_d[b] * 0.5 + 0.3, (gj - grid / 2) * 4.5]
         rotations[b] = _quat_axis_angle(z_axis, -math.pi / 2)
     half = np.stack([outer_d * 0.5, length * 0.5, outer_d * 0.5], axis=1)
     mi = [cylinder_mass(outer_d[i] * 0.5, bore[i] * 0.5, length[i]) for i in range(cylinders)]

@@ -37,6 +37,8 @@ extern "C" {
 #define MH_MAX_COLLIDERS 131072u      /* lib.rs:42 */
 #define MH_MAX_CONTACT_PAIRS 2097152u /* lib.rs:45 */
 #define MH_SNAPSHOT_RING_SIZE 3u      /* lib.rs:51 */
+#define MH_MAX_BATCH_BODIES 1048576u  /* bodies (and colliders) of a scene batch */
+#define MH_MAX_BATCH_SCENES 4096u
 #define MH_INVALID_INDEX 0xffffffffu
 
 #define MH_DRIVE_MODE_PASSIVE 0u /* abi.rs:256 */
@@ -256,6 +258,33 @@ MH_API int mh_physics_init_coordinates(mh_physics* physics, const mh_mechanism_c
                                        size_t count, mh_error_detail* error);
 /* Current coordinate rows (angle, angular velocity), for tests and tools. */
 MH_API int mh_physics_read_coordinates(mh_physics* physics, mh_mechanism_coordinate* out, size_t capacity);
+
+/* ---- scene batches and launch tuning (no counterpart in the reference) ----
+ * A batch handle steps many independent scenes -- each with its own ground
+ * plane and coordinates -- in one tick (one graph launch). Each scene's results
+ * are those of the same scene in a handle of its own. Bodies and colliders are
+ * concatenated scene after scene; dispatch, readback and impulses address the
+ * concatenated rows. Scenes may not hold bearings. */
+MH_API int mh_batch_create(const mh_creation* scenes, uint32_t scene_count, const mh_config* config,
+                           mh_physics** out, mh_error_detail* error);
+MH_API uint32_t mh_batch_scene_count(const mh_physics* physics);
+MH_API int mh_batch_scene_range(const mh_physics* physics, uint32_t scene, uint32_t* body_first,
+                                uint32_t* body_count, uint32_t* collider_first, uint32_t* collider_count);
+
+/* Launch shape of a handle's kernels; 0 keeps a field's default. Changing it
+ * re-captures the tick graph; results do not depend on it. */
+typedef struct {
+  uint32_t sort_tile;       /* bitonic tile width: 256..2048, power of two (default 2048) */
+  uint32_t sort_levels;     /* butterfly levels fused per global sort launch: 1..4 (default 4) */
+  uint32_t narrow_blocks;   /* workgroups of the narrowphase (default 4096) */
+  uint32_t prepare_blocks;  /* workgroups of contact preparation (default 2048) */
+  uint32_t solve_blocks;    /* workgroups of each solver pass (default 2048) */
+  uint32_t scan_blocks;     /* workgroups of the broadphase bucket scan (default 2048) */
+  uint32_t reserved[2];
+} mh_tuning;
+
+MH_API int mh_physics_set_tuning(mh_physics* physics, const mh_tuning* tuning);
+MH_API int mh_physics_get_tuning(const mh_physics* physics, mh_tuning* tuning);
 
 /* ---- learned warm start (no counterpart in the reference) --------------
  * A 16 -> 64 -> 64 -> 3 GELU network (tanh form) predicts, per active contact,

@@ -39,6 +39,8 @@ MAX_BODIES = 131072
 MAX_BEARINGS = 262144
 MAX_COLLIDERS = 131072
 MAX_CONTACT_PAIRS = 2097152
+MAX_BATCH_BODIES = 1048576
+MAX_BATCH_SCENES = 4096
 SNAPSHOT_RING_SIZE = 3
 INVALID_INDEX = 0xFFFFFFFF
 
@@ -150,6 +152,11 @@ class Config(ctypes.Structure):
                 ("warm_start_mode", c_uint32), ("use_graph", c_uint32), ("residual", c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [(n, c_uint32) for n in ("sort_tile", "sort_levels", "narrow_blocks", "prepare_blocks",
+                                        "solve_blocks", "scan_blocks")] + [("reserved", c_uint32 * 2)]
+
+
 class ContactResidual(ctypes.Structure):
     _fields_ = [("rms", c_double), ("mean_penetration", c_double), ("max_penetration", c_double),
                 ("active_contacts", c_uint32), ("reserved", c_uint32)]
@@ -227,6 +234,12 @@ def load() -> ctypes.CDLL:
         "mh_physics_write_drives": (c_int, [H, POINTER(MechanismDrive), c_size_t, POINTER(ErrorDetail)]),
         "mh_physics_init_coordinates": (c_int, [H, POINTER(MechanismCoordinate), c_size_t, POINTER(ErrorDetail)]),
         "mh_physics_read_coordinates": (c_int, [H, POINTER(MechanismCoordinate), c_size_t]),
+        "mh_batch_create": (c_int, [POINTER(Creation), c_uint32, POINTER(Config), POINTER(H), POINTER(ErrorDetail)]),
+        "mh_batch_scene_count": (c_uint32, [H]),
+        "mh_batch_scene_range": (c_int, [H, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
+                                         POINTER(c_uint32)]),
+        "mh_physics_set_tuning": (c_int, [H, POINTER(Tuning)]),
+        "mh_physics_get_tuning": (c_int, [H, POINTER(Tuning)]),
         "mh_mlp_create": (c_int, [c_uint64, POINTER(H)]),
         "mh_mlp_destroy": (None, [H]),
         "mh_mlp_param_count": (c_uint32, []),

@@ -123,6 +123,20 @@ struct Dev {
   double* resid;          // 4 per residual block: sum r^2, active contacts, sum penetration, max penetration
   const float* mlp;       // network parameters (mlp.hip layout), or null
 
+  // ---- scene batches: several independent scenes in one handle ----
+  uint32_t scene_count;       // 1 for a single scene
+  uint32_t key_index_bits;    // collider-index bits of a sort key (32 for a single scene)
+  const uint32_t* cscene;     // scene of each collider (batches only)
+  const uint32_t* scene_first;// scene_count + 1: first sorted leaf of each scene (batches only)
+
+  // ---- launch shape (tunable; 0 = default) ----
+  uint32_t t_sort_tile;       // bitonic tile width: 512, 1024 or 2048 keys
+  uint32_t t_sort_levels;     // butterfly levels per global sort kernel: 1..4
+  uint32_t t_narrow_blocks;   // workgroup cap of the narrowphase
+  uint32_t t_prepare_blocks;  // workgroup cap of contact preparation
+  uint32_t t_solve_blocks;    // workgroup cap of the solver passes
+  uint32_t t_scan_blocks;     // workgroups of the bucket scan
+
   // ---- articulated path (bearings, coordinates, forward kinematics, closures) ----
   uint32_t bearing_count;
   uint32_t coordinate_count;

@@ -47,7 +47,12 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
     v3 we = (vabs(p.axes[0]) * p.half.x + vabs(p.axes[1]) * p.half.y) + vabs(p.axes[2]) * p.half.z;
     d.aabbs[2 * (size_t)i] = f4(p.center - we, 0.0f);
     d.aabbs[2 * (size_t)i + 1] = f4(p.center + we, 0.0f);
-    d.keys[i] = ((uint64_t)morton_code(p.center) << 32) | i;
+    // Sort key: (scene, Morton code, collider); a single scene has no scene field
+    // and a 32-bit collider field, the reference's (code << 32 | collider).
+    const uint64_t code = morton_code(p.center);
+    d.keys[i] = d.scene_count > 1
+                    ? ((uint64_t)d.cscene[i] << (30u + d.key_index_bits)) | (code << d.key_index_bits) | i
+                    : (code << 32) | i;
     // Ground plane y = 0.
     const v3 down = mk(0.0f, -1.0f, 0.0f);
     support = support_point(p, down);
@@ -278,7 +283,7 @@ __device__ __forceinline__ bool build_node_impl(Dev& d, uint32_t node) {
   d.parents[right_ref] = node + 1u;
   if (node == 0) d.parents[0] = 0u;
   d.visits[node] = 0u;
-  const bool bucket = (keys[first] >> 32) == (keys[last] >> 32);
+  const bool bucket = (keys[first] >> d.key_index_bits) == (keys[last] >> d.key_index_bits);
   const uint32_t left_enc = split == first ? ((uint32_t)split | kLeafBit) : (uint32_t)split;
   const uint32_t left_word = bucket ? ((uint32_t)first | kBucketBit) : left_enc;
   d.inodes[2 * (size_t)node].w = __uint_as_float(left_word);
@@ -354,7 +359,7 @@ __global__ __launch_bounds__(256) void k_refit(Dev d) {
   const bool valid = leaf < (uint32_t)c;
   Box box = empty_box();
   if (valid) {
-    const uint32_t collider = (uint32_t)d.keys[leaf];
+    const uint32_t collider = (uint32_t)(d.keys[leaf] & (~0ull >> (64u - d.key_index_bits)));
     float4 mn = d.aabbs[2 * (size_t)collider];
     float4 mx = d.aabbs[2 * (size_t)collider + 1];
     d.leaves[2 * (size_t)leaf] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(collider));
@@ -492,6 +497,7 @@ struct Packet {
   bool valid;
   float4 qmn, qmx;
   uint32_t self, self_body;
+  uint32_t scene, scene_end;  // this lane's scene and the end of its sorted leaf range
   float4 umn, umx;  // union box of the packet's queries (wave-uniform)
 };
 
@@ -515,11 +521,17 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
   k.qmx = k.qmn;
   k.self = 0;
   k.self_body = 0;
+  k.scene = 0;
+  k.scene_end = d.collider_count;
   if (k.valid) {
     k.qmn = d.leaves[2 * (size_t)k.p];
     k.qmx = d.leaves[2 * (size_t)k.p + 1];
     k.self = __float_as_uint(k.qmn.w);
     k.self_body = d.leaf_body[k.p];
+    if (d.scene_count > 1) {
+      k.scene = d.cscene[k.self];
+      k.scene_end = d.scene_first[k.scene + 1];
+    }
   }
   const float inf = __builtin_huge_valf();
   k.umn = make_float4(wave_min(k.valid ? k.qmn.x : inf), wave_min(k.valid ? k.qmn.y : inf),
@@ -536,6 +548,8 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
 __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
                                            const Packet& k, uint32_t start, uint32_t last) {
   const bool check_suppression = d.suppression_count != 0;
+  // A range is one leaf or one bucket, so all of it lies in one scene.
+  const bool same_scene = d.scene_count <= 1 || k.scene_end > start;
   for (uint32_t base = start; base <= last; base += 64u) {
     const uint32_t q = base + lane;
     float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
@@ -553,7 +567,8 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
       const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
       const uint32_t other = bcast(__float_as_uint(cmn.w), j);
       const uint32_t other_body = bcast(cbody, j);
-      bool hit = k.valid && base + j > k.p && overlaps(k.qmn, k.qmx, bmn, bmx) && other_body != k.self_body;
+      bool hit = k.valid && same_scene && base + j > k.p && overlaps(k.qmn, k.qmx, bmn, bmx) &&
+                 other_body != k.self_body;
       if (hit && check_suppression) hit = !suppressed(d, k.self_body, other_body);
       const uint64_t mask = __ballot(hit);
       if (mask != 0ull) {
@@ -625,7 +640,9 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
       const uint32_t left_word = __float_as_uint(mn.w);
       const uint32_t esc = __float_as_uint(mx.w);
       const uint32_t range_last = (esc & kPosMask) - 1u;
-      const bool want = k.valid && range_last > k.p && overlaps(k.qmn, k.qmx, mn, mx);
+      bool want = k.valid && range_last > k.p && overlaps(k.qmn, k.qmx, mn, mx);
+      // In a batch, skip subtrees that start past the end of the query's scene.
+      if (d.scene_count > 1 && want) want = d.ranges[pos].x < k.scene_end;
       if (__ballot(want) == 0ull) {
         cur = esc;
         continue;
@@ -688,13 +705,15 @@ void launch_morton(hipStream_t s, const Dev& d) {
 
 void launch_sort(hipStream_t s, const Dev& d) {
   const uint32_t n = d.sort_count;  // power of two, >= 256
-  const uint32_t tile = n < kTile ? n : kTile;
+  const uint32_t tile_cap = d.t_sort_tile ? d.t_sort_tile : kTile;
+  const uint32_t tile = n < tile_cap ? n : tile_cap;
+  const uint32_t max_levels = d.t_sort_levels ? d.t_sort_levels : 4u;
   k_sort_tile<<<n / tile, tile / kPer, 0, s>>>(d.keys, tile);
   for (uint32_t k = tile * 2; k <= n; k <<= 1) {
     uint32_t j = k >> 1;
     while (j >= tile) {
       uint32_t span = 31u - (uint32_t)__builtin_clz(j) - (31u - (uint32_t)__builtin_clz(tile)) + 1u;
-      uint32_t levels = span < 4u ? span : 4u;
+      uint32_t levels = span < max_levels ? span : max_levels;
       uint32_t threads = n >> levels;
       if (levels == 4) k_sort_global<4><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
       else if (levels == 3) k_sort_global<3><<<blocks_for(threads), kBlock, 0, s>>>(d.keys, n, k, j);
@@ -715,7 +734,7 @@ void launch_lbvh(hipStream_t s, const Dev& d) {
 void launch_traverse(hipStream_t s, const Dev& d) {
   if (d.collider_count < 2) return;
   k_traverse<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
-  k_scan<<<2048, kBlock, 0, s>>>(d);
+  k_scan<<<d.t_scan_blocks ? d.t_scan_blocks : 2048u, kBlock, 0, s>>>(d);
 }
 
 }  // namespace mh

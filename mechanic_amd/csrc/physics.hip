@@ -47,6 +47,7 @@ struct mh_physics {
   Timing timing;
   const mh_mlp* mlp = nullptr;
   bool mech_active = false;  // some body has a parent (device.rs:2232)
+  std::vector<uint32_t> body_first, scene_first;  // batch layout (empty for one scene)
 };
 
 namespace {
@@ -430,22 +431,35 @@ int mh_device_count(int* count) {
   return e == hipSuccess ? MH_OK : MH_ERR_NO_DEVICE;
 }
 
-int mh_physics_create(const mh_creation* creation, const mh_config* config_in, mh_physics** out,
-                      mh_error_detail* error) {
+}  // extern "C"
+
+namespace {
+
+// Scene layout of a batch handle.
+struct BatchInfo {
+  std::vector<uint32_t> cscene;       // scene of each collider
+  std::vector<uint32_t> scene_first;  // first collider (= first sorted leaf) of each scene, + total
+  std::vector<uint32_t> body_first;   // first body of each scene, + total
+};
+
+int create_impl(const mh_creation* creation, const mh_config* config_in, const BatchInfo* batch, mh_physics** out,
+                mh_error_detail* error) {
   if (error) std::memset(error, 0, sizeof(*error));
   if (!creation || !out) return fail(error, MH_ERR_INVALID_ARGUMENT, "null creation or output");
   *out = nullptr;
+  const uint32_t body_cap = batch ? MH_MAX_BATCH_BODIES : MH_MAX_BODIES;
+  const uint32_t collider_cap = batch ? MH_MAX_BATCH_BODIES : MH_MAX_COLLIDERS;
   // Capacity checks before any device work (device.rs:405-422).
-  if (creation->body_count > MH_MAX_BODIES) {
-    if (error) { error->required = creation->body_count; error->capacity = MH_MAX_BODIES; }
+  if (creation->body_count > body_cap) {
+    if (error) { error->required = creation->body_count; error->capacity = body_cap; }
     return fail(error, MH_ERR_BODY_CAPACITY, "scene exceeds the body capacity");
   }
   if (creation->bearing_count > MH_MAX_BEARINGS) {
     if (error) { error->required = creation->bearing_count; error->capacity = MH_MAX_BEARINGS; }
     return fail(error, MH_ERR_BEARING_CAPACITY, "scene exceeds the bearing capacity");
   }
-  if (creation->collider_count > MH_MAX_COLLIDERS) {
-    if (error) { error->required = creation->collider_count; error->capacity = MH_MAX_COLLIDERS; }
+  if (creation->collider_count > collider_cap) {
+    if (error) { error->required = creation->collider_count; error->capacity = collider_cap; }
     return fail(error, MH_ERR_COLLIDER_CAPACITY, "scene exceeds the collider capacity");
   }
   Forest forest;
@@ -605,6 +619,23 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
     rc = upload_forest(p, creation, forest, error);
     if (rc != MH_OK) { mh_physics_destroy(p); return rc; }
   }
+  d.scene_count = 1;
+  d.key_index_bits = 32;
+  if (batch) {
+    uint32_t *cscene, *scene_first;
+    ALLOC(cscene, c);
+    ALLOC(scene_first, batch->scene_first.size());
+    UPLOAD(cscene, batch->cscene.data(), (size_t)c * sizeof(uint32_t));
+    UPLOAD(scene_first, batch->scene_first.data(), batch->scene_first.size() * sizeof(uint32_t));
+    d.cscene = cscene;
+    d.scene_first = scene_first;
+    d.scene_count = (uint32_t)batch->scene_first.size() - 1u;
+    uint32_t bits = 1;
+    while ((1u << bits) < c) ++bits;
+    d.key_index_bits = bits;
+    p->body_first = batch->body_first;
+    p->scene_first = batch->scene_first;
+  }
   hipError_t he = hipHostMalloc(reinterpret_cast<void**>(&p->host_diag), kDiagWords * sizeof(uint32_t),
                                 hipHostMallocDefault);
   if (he != hipSuccess) { mh_physics_destroy(p); return hip_fail(error, he, "hipHostMalloc"); }
@@ -614,6 +645,156 @@ int mh_physics_create(const mh_creation* creation, const mh_config* config_in, m
   return MH_OK;
 #undef ALLOC
 #undef UPLOAD
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_physics_create(const mh_creation* creation, const mh_config* config, mh_physics** out,
+                      mh_error_detail* error) {
+  return create_impl(creation, config, nullptr, out, error);
+}
+
+// Several independent scenes in one handle: bodies, colliders and suppression
+// pairs are concatenated scene after scene; scene ids in the sort keys keep
+// every scene's broadphase to itself, so each scene steps exactly as it would
+// alone.
+int mh_batch_create(const mh_creation* scenes, uint32_t scene_count, const mh_config* config, mh_physics** out,
+                    mh_error_detail* error) {
+  if (error) std::memset(error, 0, sizeof(*error));
+  if (!scenes || !out || scene_count == 0 || scene_count > MH_MAX_BATCH_SCENES)
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "batch needs 1..MH_MAX_BATCH_SCENES scenes");
+  *out = nullptr;
+  BatchInfo info;
+  uint64_t bodies = 0, colliders = 0, supp = 0;
+  for (uint32_t s = 0; s < scene_count; ++s) {
+    const mh_creation& c = scenes[s];
+    if (c.bearing_count > 0) return fail(error, MH_ERR_UNSUPPORTED, "batched scenes cannot hold bearings");
+    if (c.body_count > MH_MAX_BODIES) {
+      if (error) { error->required = c.body_count; error->capacity = MH_MAX_BODIES; error->body_index = s; }
+      return fail(error, MH_ERR_BODY_CAPACITY, "a batched scene exceeds the body capacity");
+    }
+    if (c.collider_count > MH_MAX_COLLIDERS) {
+      if (error) { error->required = c.collider_count; error->capacity = MH_MAX_COLLIDERS; error->body_index = s; }
+      return fail(error, MH_ERR_COLLIDER_CAPACITY, "a batched scene exceeds the collider capacity");
+    }
+    if ((c.body_count > 0 && (!c.poses || !c.inverse_masses || !c.masses)) ||
+        (c.collider_count > 0 && !c.colliders) || (c.suppression_count > 0 && !c.suppressed_pairs))
+      return fail(error, MH_ERR_INVALID_ARGUMENT, "missing row arrays");
+    bodies += c.body_count;
+    colliders += c.collider_count;
+    supp += c.suppression_count;
+  }
+  if (bodies > MH_MAX_BATCH_BODIES || colliders > MH_MAX_BATCH_BODIES) {
+    if (error) { error->required = bodies > colliders ? bodies : colliders; error->capacity = MH_MAX_BATCH_BODIES; }
+    return fail(error, MH_ERR_BODY_CAPACITY, "batch exceeds the batch body capacity");
+  }
+  std::vector<mh_transform> poses;
+  std::vector<float> inv;
+  std::vector<mh_mass> masses;
+  std::vector<mh_collider> cols;
+  std::vector<mh_pair> pairs;
+  std::vector<uint32_t> comps, roots;
+  poses.reserve(bodies); inv.reserve(bodies); masses.reserve(bodies); cols.reserve(colliders); pairs.reserve(supp);
+  comps.reserve(bodies); roots.reserve(bodies);
+  info.cscene.reserve(colliders);
+  for (uint32_t s = 0; s < scene_count; ++s) {
+    const mh_creation& c = scenes[s];
+    const uint32_t b0 = (uint32_t)poses.size();
+    info.body_first.push_back(b0);
+    info.scene_first.push_back((uint32_t)cols.size());
+    for (uint32_t i = 0; i < c.body_count; ++i) {
+      poses.push_back(c.poses[i]);
+      inv.push_back(c.inverse_masses[i]);
+      masses.push_back(c.masses[i]);
+      comps.push_back(c.body_components ? b0 + c.body_components[i] : b0 + i);
+      roots.push_back(c.root_flags ? c.root_flags[i] : 1u);
+    }
+    for (uint32_t i = 0; i < c.collider_count; ++i) {
+      mh_collider row = c.colliders[i];
+      if (row.metadata[0] >= c.body_count)
+        return fail(error, MH_ERR_INVALID_ARGUMENT, "collider refers to a body outside its scene");
+      row.metadata[0] += b0;
+      cols.push_back(row);
+      info.cscene.push_back(s);
+    }
+    for (uint32_t i = 0; i < c.suppression_count; ++i)
+      pairs.push_back(mh_pair{c.suppressed_pairs[i].collider_a + b0, c.suppressed_pairs[i].collider_b + b0});
+  }
+  info.body_first.push_back((uint32_t)poses.size());
+  info.scene_first.push_back((uint32_t)cols.size());
+  mh_creation merged{};
+  merged.body_count = (uint32_t)poses.size();
+  merged.collider_count = (uint32_t)cols.size();
+  merged.suppression_count = (uint32_t)pairs.size();
+  merged.poses = poses.data();
+  merged.inverse_masses = inv.data();
+  merged.masses = masses.data();
+  merged.colliders = cols.data();
+  merged.suppressed_pairs = pairs.data();
+  merged.body_components = comps.data();
+  merged.root_flags = roots.data();
+  return create_impl(&merged, config, &info, out, error);
+}
+
+int mh_batch_scene_range(const mh_physics* p, uint32_t scene, uint32_t* body_first, uint32_t* body_count,
+                         uint32_t* collider_first, uint32_t* collider_count) {
+  if (!p) return MH_ERR_INVALID_ARGUMENT;
+  if (p->body_first.empty()) {
+    if (scene != 0) return MH_ERR_INVALID_ARGUMENT;
+    if (body_first) *body_first = 0;
+    if (body_count) *body_count = p->d.body_count;
+    if (collider_first) *collider_first = 0;
+    if (collider_count) *collider_count = p->d.collider_count;
+    return MH_OK;
+  }
+  if (scene + 1 >= p->body_first.size()) return MH_ERR_INVALID_ARGUMENT;
+  if (body_first) *body_first = p->body_first[scene];
+  if (body_count) *body_count = p->body_first[scene + 1] - p->body_first[scene];
+  if (collider_first) *collider_first = p->scene_first[scene];
+  if (collider_count) *collider_count = p->scene_first[scene + 1] - p->scene_first[scene];
+  return MH_OK;
+}
+
+uint32_t mh_batch_scene_count(const mh_physics* p) {
+  return p ? (p->body_first.empty() ? 1u : (uint32_t)p->body_first.size() - 1u) : 0u;
+}
+
+int mh_physics_set_tuning(mh_physics* p, const mh_tuning* t) {
+  if (!p || !t) return MH_ERR_INVALID_ARGUMENT;
+  if (t->sort_tile && (t->sort_tile < 256 || t->sort_tile > 2048 || (t->sort_tile & (t->sort_tile - 1))))
+    return MH_ERR_INVALID_ARGUMENT;
+  if (t->sort_levels > 4) return MH_ERR_INVALID_ARGUMENT;
+  if (t->narrow_blocks > (1u << 16) || t->prepare_blocks > (1u << 16) || t->solve_blocks > (1u << 16) ||
+      t->scan_blocks > (1u << 16))
+    return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  Dev& d = p->d;
+  d.t_sort_tile = t->sort_tile;
+  d.t_sort_levels = t->sort_levels;
+  d.t_narrow_blocks = t->narrow_blocks;
+  d.t_prepare_blocks = t->prepare_blocks;
+  d.t_solve_blocks = t->solve_blocks;
+  d.t_scan_blocks = t->scan_blocks;
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+  return MH_OK;
+}
+
+int mh_physics_get_tuning(const mh_physics* p, mh_tuning* t) {
+  if (!p || !t) return MH_ERR_INVALID_ARGUMENT;
+  std::memset(t, 0, sizeof(*t));
+  const Dev& d = p->d;
+  t->sort_tile = d.t_sort_tile ? d.t_sort_tile : 2048u;
+  t->sort_levels = d.t_sort_levels ? d.t_sort_levels : 4u;
+  t->narrow_blocks = d.t_narrow_blocks ? d.t_narrow_blocks : 4096u;
+  t->prepare_blocks = d.t_prepare_blocks ? d.t_prepare_blocks : 2048u;
+  t->solve_blocks = d.t_solve_blocks ? d.t_solve_blocks : 2048u;
+  t->scan_blocks = d.t_scan_blocks ? d.t_scan_blocks : 2048u;
+  return MH_OK;
 }
 
 void mh_physics_destroy(mh_physics* p) {

@@ -112,10 +112,8 @@ class GpuPhysics:
     def new(cls, scene: Scene) -> "GpuPhysics":
         return cls.new_with_config(scene, GpuPhysicsConfig())
 
-    @classmethod
-    def new_with_config(cls, scene: Scene, config: GpuPhysicsConfig) -> "GpuPhysics":
-        lib = _lib.load()
-        keep = []
+    @staticmethod
+    def _creation(scene: Scene, keep: list) -> "_lib.Creation":
         supp = None
         if scene.suppressed_pairs:
             pairs = sorted((min(a, b), max(a, b)) for a, b in scene.suppressed_pairs)
@@ -131,11 +129,57 @@ class GpuPhysics:
                 return arr.ctypes.data
             return ctypes.cast(x, ctypes.c_void_p).value
 
-        creation = _lib.Creation(
+        return _lib.Creation(
             scene.body_count, len(scene.colliders), scene.bearing_count, len(scene.suppressed_pairs),
             ptr(scene.poses), ptr(scene.inverse_masses), ptr(scene.masses), ptr(scene.inertias),
             ptr(scene.colliders), ptr(scene.bearings), ptr(supp), ptr(scene.body_components), ptr(scene.root_flags),
             ptr(scene.mechanism_bodies), ptr(scene.preorder), ptr(scene.drives), scene.coordinate_count, 0)
+
+    @classmethod
+    def new_batch(cls, scenes, config: Optional[GpuPhysicsConfig] = None) -> "GpuPhysics":
+        """One handle stepping many independent scenes (mh_batch_create); rows
+        are concatenated scene after scene, see `scene_range`."""
+        lib = _lib.load()
+        config = config or GpuPhysicsConfig()
+        keep: list = []
+        rows = (_lib.Creation * len(scenes))(*[cls._creation(sc, keep) for sc in scenes])
+        cfg = config.to_c()
+        handle = ctypes.c_void_p()
+        err = _lib.ErrorDetail()
+        rc = lib.mh_batch_create(rows, len(scenes), ctypes.byref(cfg), ctypes.byref(handle), ctypes.byref(err))
+        if rc in _PHYSICS_KINDS:
+            raise GpuPhysicsError(_PHYSICS_KINDS[rc], err.required, err.capacity, err.provided, err.message.decode())
+        if rc != _lib.MH_OK:
+            raise DeviceError(f"mh_batch_create: {err.message.decode()} ({rc})")
+        gpu = cls(handle.value, sum(sc.body_count for sc in scenes), config)
+        gpu._coordinate_count = 0
+        return gpu
+
+    def scene_count(self) -> int:
+        return int(_lib.load().mh_batch_scene_count(self._h))
+
+    def scene_range(self, scene: int):
+        """(first body, body count, first collider, collider count) of one scene."""
+        vals = [ctypes.c_uint32() for _ in range(4)]
+        _check(_lib.load().mh_batch_scene_range(self._h, scene, *[ctypes.byref(v) for v in vals]), "scene_range")
+        return tuple(v.value for v in vals)
+
+    def set_tuning(self, **fields) -> None:
+        t = _lib.Tuning()
+        for k, v in fields.items():
+            setattr(t, k, int(v))
+        _check(_lib.load().mh_physics_set_tuning(self._h, ctypes.byref(t)), "set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        _check(_lib.load().mh_physics_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
+        return {n: getattr(t, n) for n, _ in _lib.Tuning._fields_ if n != "reserved"}
+
+    @classmethod
+    def new_with_config(cls, scene: Scene, config: GpuPhysicsConfig) -> "GpuPhysics":
+        lib = _lib.load()
+        keep = []
+        creation = cls._creation(scene, keep)
         cfg = config.to_c()
         handle = ctypes.c_void_p()
         err = _lib.ErrorDetail()

@@ -113,3 +113,30 @@ def boxes(centers, half_extents, rotations=None, inverse_mass=1.0, inverse_inert
         colliders[i].half_extents[:] = [float(half[i, 0]), float(half[i, 1]), float(half[i, 2]), 0.0]
         colliders[i].metadata[:] = [i, i, 0, 0]
     return Scene(n, poses, inv, masses, inertias, colliders)
+
+
+def batch_scenes(count: int = 256, seed_base: int = 1000, smallest: int = 64, largest: int = 4096,
+                 size_seed: int = 3) -> list:
+    """The batched configuration: `count` independent jittered drops like the
+    1,000-cuboid pile, scene s seeded with seed_base + s and holding
+    round(exp(U[ln smallest, ln largest])) cuboids (the bottom layers of a
+    near-cubic lattice)."""
+    rng = np.random.default_rng(size_seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(smallest), np.log(largest), count))).astype(int)
+    out = []
+    for s, n in enumerate(sizes):
+        side = int(np.ceil(np.cbrt(n)))
+        ny = int(np.ceil(n / (side * side)))
+        full = cuboid_pile(side, ny, side, seed=seed_base + s)
+        out.append(truncate(full, int(n)))
+    return out
+
+
+def truncate(scene: Scene, n: int) -> Scene:
+    """The first n bodies of a single-collider-per-body scene."""
+    poses = (_lib.Transform * n)(*list(scene.poses)[:n])
+    inv = (ctypes.c_float * n)(*list(scene.inverse_masses)[:n])
+    masses = (_lib.Mass * n)(*list(scene.masses)[:n])
+    inertias = (_lib.SpatialInertia * n)(*list(scene.inertias)[:n])
+    cols = (_lib.Collider * n)(*list(scene.colliders)[:n])
+    return Scene(n, poses, inv, masses, inertias, cols)

@@ -5,10 +5,13 @@ jittered, yawed lattice with 0.27 m spacing dropped 0.5 m onto the ground
 plane; dt = 1/60 s, gravity -9.81, friction 0.6, restitution 0, 8 solver
 iterations. One step is one whole tick (integration, broadphase, narrowphase,
 contact solve, snapshot) replayed as a captured hipGraph with inputs resident in
-HBM. Also compares the learned warm start with the reference's cache warm start
-at equal contact residual. Prints one JSON line.
+HBM. Also reports, as sub-objects of the same JSON line: the learned warm start
+against the reference's cache warm start at equal contact residual, the
+256-scene batch (autotuned launch shape against the default) and the articulated
+100,000-body scene.
 
     python bench.py [--gpus 1] [--steps 600] [--warmup 60] [--no-cpu] [--no-learned]
+                    [--no-batched] [--no-articulated]
 """
 import argparse
 import json
@@ -60,7 +63,7 @@ def cpu_baseline(scene, friction, iterations, budget_s=15.0, max_ticks=40):
                       f"100k-cuboid pile ({dt:.1f} s)"}
 
 
-def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain):
+def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain, tuning=None):
     """Learned warm start against the reference's cache warm start on the same
     pile: trained on 2M logged contacts of seeds 10-19 (32 solver passes), the
     smallest pass count whose mean contact residual over ticks 1..300 does not
@@ -110,6 +113,8 @@ def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain
     cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=chosen, use_graph=True,
                               warm_start=mh._lib.WARM_START_LEARNED)
     gpu = mh.GpuPhysics.new_with_config(scene, cfg)
+    if tuning:
+        gpu.set_tuning(**tuning)
     gpu.set_mlp(net)
     gpu.dispatch_ticks(1, 1)
     gpu.synchronize()
@@ -125,6 +130,86 @@ def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain
     return out
 
 
+def save_tuning(autotune, bodies, tuned, workload):
+    """Adds the winner to the cache file named by MECHANIC_TUNING_OUT, if set."""
+    out_path = os.environ.get("MECHANIC_TUNING_OUT")
+    if not out_path:
+        return
+    entries = autotune.load_cache(out_path)
+    entries[autotune.cache_key(autotune.device_arch(), bodies)] = {
+        "tuning": tuned["tuning"], "ms": tuned["ms"], "default_ms": tuned["default_ms"], "workload": workload}
+    autotune.save_cache(entries, out_path)
+
+
+def timed_run(make, steps, tuning=None):
+    """Steps/s of a fresh handle under the headline protocol (graph captured on
+    tick 1, ticks 2..steps+1 timed), with its last-tick diagnostics."""
+    gpu = make()
+    if tuning:
+        gpu.set_tuning(**tuning)
+    gpu.dispatch_ticks(1, 1)
+    gpu.synchronize()
+    t0 = time.perf_counter()
+    gpu.dispatch_ticks(2, steps)
+    gpu.synchronize()
+    value = steps / (time.perf_counter() - t0)
+    rb = gpu.read_last_tick()
+    gpu.close()
+    return value, rb
+
+
+def batched(mh, steps):
+    """256 independent scenes of 64..4,096 cuboids (seeds 1000+s) in one handle,
+    one graph launch per step; the launch shape autotuned against the default."""
+    from mechanic_amd import autotune
+    from mechanic_amd.scenes import batch_scenes
+
+    scenes = batch_scenes()
+    cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8)
+    make = lambda: mh.GpuPhysics.new_batch(scenes, cfg)  # noqa: E731
+    t0 = time.perf_counter()
+    tuned = autotune.autotune(make, warm=20, ticks=40)
+    tune_s = time.perf_counter() - t0
+    default_v, _ = timed_run(make, steps)
+    tuned_v, rb = timed_run(make, steps, tuned["tuning"])
+    bodies = sum(s.body_count for s in scenes)
+    save_tuning(autotune, bodies, tuned, "256-scene batch")
+    return {"workload": "256 scenes, sizes round(exp(U[ln 64, ln 4096])) cuboids, seeds 1000+s, friction 0.6, "
+                        f"8 iterations, ticks 2..{steps + 1}",
+            "scenes": len(scenes), "bodies": bodies, "steps": steps,
+            "steps_per_s_default": round(default_v, 2), "steps_per_s_tuned": round(tuned_v, 2),
+            "scene_steps_per_s_tuned": round(tuned_v * len(scenes), 1),
+            "tuning": tuned["tuning"], "tuning_trials": len(tuned["trials"]), "tune_s": round(tune_s, 2),
+            "error_flags": rb.error_flags, "contacts": rb.contact_count}
+
+
+def articulated(mh, steps):
+    """The articulated 100,000-body scene: steps/s and ms per phase."""
+    from mechanic_amd import articulated as A
+
+    scene = A.articulated_pile()
+    cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8, mechanism_self_collisions=False)
+    make = lambda: mh.GpuPhysics.new_with_config(scene, cfg)  # noqa: E731
+    value, rb = timed_run(make, steps)
+    gpu = make()
+    gpu.dispatch_ticks(1, steps)
+    acc = {}
+    n = 10
+    for i in range(n):
+        t = gpu.profile_tick(steps + 1 + i)
+        for k, v in t.items():
+            acc[k] = acc.get(k, 0.0) + v / n
+    prb = gpu.read_last_tick()
+    gpu.close()
+    return {"workload": "90,000 cuboids + 10,000 cylinders (OBB), 4,500 tree bearings in chains of 2-20 links, "
+                        "500 loop closures, 1,000 drives at +-2 rad/s, friction 0.6, 8 iterations, "
+                        f"mechanism self-collisions off, ticks 2..{steps + 1}",
+            "steps": steps, "steps_per_s": round(value, 2),
+            "phase_ms": {k[:-3]: round(v, 4) for k, v in acc.items() if k.endswith("_ms")},
+            "error_flags": rb.error_flags, "anchor_residual_m": round(prb.anchor_residual_meters, 6),
+            "contacts": prb.contact_count}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +218,10 @@ def main():
     ap.add_argument("--profile-ticks", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--tuning", choices=("auto", "cached", "default"), default="auto",
+                    help="launch shape: autotune now, the committed cache, or the defaults")
+    ap.add_argument("--no-articulated", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus})",
@@ -152,7 +241,23 @@ def main():
     warm.synchronize()
     warm.close()
 
+    # Launch shape (results never depend on it): the autotuner's winner for this
+    # scene, the committed cache entry for its size bucket, or the defaults.
+    from mechanic_amd import autotune
+    make = lambda: mh.GpuPhysics.new_with_config(scene, cfg)  # noqa: E731
+    t_tune = time.perf_counter()
+    if args.tuning == "auto":
+        tuned = autotune.autotune(make, warm=20, ticks=40)
+        save_tuning(autotune, scene.body_count, tuned, "100k pile")
+    elif args.tuning == "cached":
+        tuned = {"tuning": autotune.cached_tuning(scene.body_count) or dict(autotune.DEFAULTS)}
+    else:
+        tuned = {"tuning": dict(autotune.DEFAULTS)}
+    tune_s = time.perf_counter() - t_tune
+    default_value = timed_run(make, args.steps)[0] if args.tuning == "auto" else None
+
     gpu = mh.GpuPhysics.new_with_config(scene, cfg)
+    gpu.set_tuning(**tuned["tuning"])
     gpu.dispatch_ticks(1, 1)  # captures the tick graph
     gpu.synchronize()
     t0 = time.perf_counter()
@@ -201,7 +306,10 @@ def main():
     gpu.close()
 
     value = steps / wall
-    learned = None if args.no_learned else learned_warm_start(mh, scene, friction, iterations, steps, value)
+    learned = None if args.no_learned else learned_warm_start(mh, scene, friction, iterations, steps, value,
+                                                                  tuned["tuning"])
+    batch = None if args.no_batched else batched(mh, steps)
+    art = None if args.no_articulated else articulated(mh, min(steps, 300))
     cpu = None if args.no_cpu else cpu_baseline(scene, friction, iterations)
 
     out = {
@@ -227,11 +335,16 @@ def main():
         "counts": {"pairs": round(p), "contacts": round(k), "active_contacts": round(a)},
         "error_flags": readback.error_flags,
         "engine_tps_sync": round(engine_tps, 2),
+        "tuning": tuned["tuning"],
+        "steps_per_s_default_tuning": None if default_value is None else round(default_value, 2),
+        "tune_s": round(tune_s, 2),
         "roofline": {"bound": "hbm", "kernel_group": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None},
         "cpu_baseline": cpu,
         "learned_warm_start": learned,
+        "batched": batch,
+        "articulated": art,
     }
     print(json.dumps(out), flush=True)
 

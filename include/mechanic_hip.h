@@ -211,6 +211,8 @@ typedef struct mh_physics mh_physics;
 MH_API mh_config mh_config_default(void);
 MH_API const char* mh_status_string(int code);
 MH_API int mh_device_count(int* count);
+/* Architecture name of the current device (e.g. "gfx950:sramecc+:xnack-"). */
+MH_API int mh_device_arch(char* out, size_t capacity);
 
 /* GpuPhysics::new_with_config (device.rs:399-811). */
 MH_API int mh_physics_create(const mh_creation* creation, const mh_config* config,

@@ -215,6 +215,7 @@ def load() -> ctypes.CDLL:
         "mh_config_default": (Config, []),
         "mh_status_string": (ctypes.c_char_p, [c_int]),
         "mh_device_count": (c_int, [POINTER(c_int)]),
+        "mh_device_arch": (c_int, [ctypes.c_char_p, c_size_t]),
         "mh_physics_create": (c_int, [POINTER(Creation), POINTER(Config), POINTER(H), POINTER(ErrorDetail)]),
         "mh_physics_destroy": (None, [H]),
         "mh_physics_body_count": (c_uint32, [H]),

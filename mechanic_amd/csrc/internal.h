@@ -17,9 +17,9 @@ constexpr uint32_t kBucketBit = 0x40000000u;  // internal node whose leaves shar
 constexpr uint32_t kBlock = 256;
 // Contact sides a body can gather without atomics per tick; further sides
 // accumulate through the atomic overflow path.
-constexpr uint32_t kMaxSlots = 32;
-constexpr uint32_t kOverflowA = 0x40000000u;  // solver row flag bits in the contact-index word
-constexpr uint32_t kOverflowB = 0x80000000u;
+constexpr uint32_t kMaxSlots = 32;      // contact-side delta slots per body (body-major)
+constexpr uint32_t kSlotOverflow = 31;  // slot code of a side past the body's slots (atomics instead)
+constexpr uint32_t kContactMask = 0x3fffffu;  // contact index bits of a solver row's word (2^22 pairs)
 constexpr uint32_t kFeatures = 16;      // learned warm-start inputs per contact
 constexpr uint32_t kResidBlocks = 512;  // residual kernel grid
 enum LearnMode : uint32_t { kLearnOff = 0, kLearnLog = 1, kLearnInfer = 2 };
@@ -104,11 +104,10 @@ struct Dev {
   uint32_t* active;       // contact index per active contact
   float4* rows;           // 4 per active contact: solver rows
   float* lambda;          // accumulated normal impulse per active contact
-  int4* cdelta;           // 2 int4 per contact side: that side's Q20 deltas of the last pass
+  int4* cdelta;           // body-major: 2 int4 per (body, slot) -- a contact side's Q20 deltas of the last pass
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)
   int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
   uint32_t* slot_count;   // contact sides per body this tick
-  uint32_t* incid;        // kMaxSlots x body_count (slot-major): contact side = 2 * active + side
 
   // ---- persistent manifolds (ping-pong by tick parity) ----
   uint64_t* htab[2];      // open-addressing index: key -> manifold row

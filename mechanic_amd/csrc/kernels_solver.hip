@@ -107,9 +107,12 @@ __device__ __forceinline__ Q20 side_delta(const SolverBody& s, v3 arm, v3 impuls
   return q;
 }
 
-__device__ __forceinline__ void put_side(const Dev& d, uint32_t side, const Q20& q) {
-  d.cdelta[2 * (size_t)side] = make_int4(q.v[0], q.v[1], q.v[2], q.v[3]);
-  d.cdelta[2 * (size_t)side + 1] = make_int4(q.v[4], q.v[5], 0, 0);
+// A contact side's deltas go to its body's slot, so each body's sides lie
+// together for the gather.
+__device__ __forceinline__ void put_side(const Dev& d, uint32_t body, uint32_t slot, const Q20& q) {
+  int4* p = d.cdelta + 2 * ((size_t)body * kMaxSlots + slot);
+  p[0] = make_int4(q.v[0], q.v[1], q.v[2], q.v[3]);
+  p[1] = make_int4(q.v[4], q.v[5], 0, 0);
 }
 
 __device__ __forceinline__ void add_overflow(const Dev& d, uint32_t body, const Q20& q) {
@@ -214,16 +217,16 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
     if (active) {
       const uint32_t ci = __float_as_uint(r3.z);
       const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
-      uint32_t flags = 0;
-      // Register both contact sides with their bodies for the gather.
+      // Give both contact sides a delta slot of their body (kSlotOverflow: add
+      // with atomics instead). Word: contact | slot A << 22 | slot B << 27.
       uint32_t sa = atomicAdd(&d.slot_count[body_a], 1u);
-      if (sa < kMaxSlots) d.incid[(size_t)sa * d.body_count + body_a] = 2u * a;
-      else flags |= kOverflowA;
+      sa = sa < kSlotOverflow ? sa : kSlotOverflow;
+      uint32_t sb = kSlotOverflow;
       if (body_b != kInvalid) {
-        uint32_t sb = atomicAdd(&d.slot_count[body_b], 1u);
-        if (sb < kMaxSlots) d.incid[(size_t)sb * d.body_count + body_b] = 2u * a + 1u;
-        else flags |= kOverflowB;
+        sb = atomicAdd(&d.slot_count[body_b], 1u);
+        sb = sb < kSlotOverflow ? sb : kSlotOverflow;
       }
+      const uint32_t flags = (sa << 22) | (sb << 27);
       d.active[a] = ci;
       d.lambda[a] = cached_lambda;
       float4* row = d.rows + 4 * (size_t)a;
@@ -320,21 +323,22 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
     if (first && !applies && d.learn_mode == kLearnLog) d.fsum[ai] = make_float4(0.f, 0.f, 0.f, 0.f);
     // A contact that applies no impulse adds zero deltas (collision.wgsl:793-804).
     const Q20 qa = applies ? side_delta(sa, arm_a, -impulse) : Q20{};
-    if (word & kOverflowA) {
+    const uint32_t slot_a = (word >> 22) & 31u, slot_b = word >> 27;
+    if (slot_a == kSlotOverflow) {
       if (applies) add_overflow(d, body_a, qa);
     } else {
-      put_side(d, 2u * ai, qa);
+      put_side(d, body_a, slot_a, qa);
     }
     if (has_b) {
       const Q20 qb = applies ? side_delta(sb, arm_b, impulse) : Q20{};
-      if (word & kOverflowB) {
+      if (slot_b == kSlotOverflow) {
         if (applies) add_overflow(d, body_b, qb);
       } else {
-        put_side(d, 2u * ai + 1u, qb);
+        put_side(d, body_b, slot_b, qb);
       }
     }
     if (last) {
-      const uint32_t ci = word & ~(kOverflowA | kOverflowB);
+      const uint32_t ci = word & kContactMask;
       rows_now[3 * (size_t)ci + 2].w = lambda;
     }
   }
@@ -354,11 +358,12 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
     const SolverBody sb = load_body(d, b);
     uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
     const uint32_t count = d.slot_count[b];
-    const uint32_t n = count < kMaxSlots ? count : kMaxSlots;
+    const uint32_t n = count < kSlotOverflow ? count : kSlotOverflow;
+    // The body's sides are adjacent slots: independent loads, no index lookup.
+    const int4* p = d.cdelta + 2 * (size_t)b * kMaxSlots;
     for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t side = d.incid[(size_t)k * d.body_count + b];
-      const int4 x = d.cdelta[2 * (size_t)side];
-      const int4 y = d.cdelta[2 * (size_t)side + 1];
+      const int4 x = p[2 * k];
+      const int4 y = p[2 * k + 1];
       acc[0] += (uint32_t)x.x;
       acc[1] += (uint32_t)x.y;
       acc[2] += (uint32_t)x.z;
@@ -366,7 +371,7 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
       acc[4] += (uint32_t)y.x;
       acc[5] += (uint32_t)y.y;
     }
-    if (count > kMaxSlots) {
+    if (count > kSlotOverflow) {
       int32_t* p = d.deltas + 6 * (size_t)b;
 #pragma unroll
       for (int c = 0; c < 6; ++c) {

@@ -424,6 +424,17 @@ const char* mh_status_string(int code) {
   }
 }
 
+int mh_device_arch(char* out, size_t capacity) {
+  if (!out || capacity == 0) return MH_ERR_INVALID_ARGUMENT;
+  out[0] = 0;
+  int device = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return MH_ERR_NO_DEVICE;
+  std::snprintf(out, capacity, "%s", prop.gcnArchName);
+  return MH_OK;
+}
+
 int mh_device_count(int* count) {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -539,7 +550,7 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   uint4* work;
   uint64_t *keys, *h0, *h1;
   int32_t* deltas;
-  uint32_t *slot_count, *incid;
+  uint32_t* slot_count;
   float4 *m0, *m1;
   TickState* tick;
   ALLOC(pos, n); ALLOC(rot, n); ALLOC(lin, n); ALLOC(ang, n);
@@ -560,11 +571,10 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   ALLOC(active, cfg.pair_capacity);
   ALLOC(rows, 4 * (size_t)cfg.pair_capacity);
   ALLOC(lambda, cfg.pair_capacity);
-  ALLOC(cdelta, 4 * (size_t)cfg.pair_capacity);
+  ALLOC(cdelta, 2 * (size_t)kMaxSlots * n);
   ALLOC(sbody, 4 * (size_t)n);
   ALLOC(deltas, 6 * (size_t)n);
   ALLOC(slot_count, n);
-  ALLOC(incid, (size_t)kMaxSlots * n);
   ALLOC(h0, cfg.pair_capacity); ALLOC(h1, cfg.pair_capacity);
   ALLOC(m0, 3 * (size_t)cfg.pair_capacity); ALLOC(m1, 3 * (size_t)cfg.pair_capacity);
   for (int s = 0; s < 3; ++s) {
@@ -587,7 +597,7 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
   d.ranges = ranges; d.leaf_body = leaf_body; d.top = top; d.block_box = block_box; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
- d.slot_count = slot_count; d.incid = incid; d.lambda = lambda; d.cdelta = cdelta; d.sbody = sbody;
+ d.slot_count = slot_count; d.lambda = lambda; d.cdelta = cdelta; d.sbody = sbody;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;
   d.diag[0] = diag; d.diag[1] = diag + kDiagWords;
   d.tick = tick;

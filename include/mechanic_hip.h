@@ -276,13 +276,14 @@ MH_API int mh_batch_scene_range(const mh_physics* physics, uint32_t scene, uint3
 /* Launch shape of a handle's kernels; 0 keeps a field's default. Changing it
  * re-captures the tick graph; results do not depend on it. */
 typedef struct {
-  uint32_t sort_tile;       /* bitonic tile width: 256..2048, power of two (default 2048) */
-  uint32_t sort_levels;     /* butterfly levels fused per global sort launch: 1..4 (default 4) */
+  uint32_t sort_tile;       /* bitonic: tile width 256..2048, power of two (default 2048) */
+  uint32_t sort_levels;     /* bitonic: butterfly levels fused per global launch, 1..4 (default 4) */
   uint32_t narrow_blocks;   /* workgroups of the narrowphase (default 4096) */
   uint32_t prepare_blocks;  /* workgroups of contact preparation (default 2048) */
   uint32_t solve_blocks;    /* workgroups of each solver pass (default 2048) */
   uint32_t scan_blocks;     /* workgroups of the broadphase bucket scan (default 2048) */
-  uint32_t reserved[2];
+  uint32_t sort_kind;       /* 0: bitonic sort (default), 1: radix sort */
+  uint32_t reserved;
 } mh_tuning;
 
 MH_API int mh_physics_set_tuning(mh_physics* physics, const mh_tuning* tuning);

@@ -20,6 +20,7 @@ constexpr uint32_t kBlock = 256;
 constexpr uint32_t kMaxSlots = 32;      // contact-side delta slots per body (body-major)
 constexpr uint32_t kSlotOverflow = 31;  // slot code of a side past the body's slots (atomics instead)
 constexpr uint32_t kContactMask = 0x3fffffu;  // contact index bits of a solver row's word (2^22 pairs)
+constexpr uint32_t kRadixTile = 1024;  // keys per radix-sort workgroup
 constexpr uint32_t kFeatures = 16;      // learned warm-start inputs per contact
 constexpr uint32_t kResidBlocks = 512;  // residual kernel grid
 enum LearnMode : uint32_t { kLearnOff = 0, kLearnLog = 1, kLearnInfer = 2 };
@@ -128,7 +129,14 @@ struct Dev {
   const uint32_t* cscene;     // scene of each collider (batches only)
   const uint32_t* scene_first;// scene_count + 1: first sorted leaf of each scene (batches only)
 
+  // ---- radix sort of the keys ----
+  uint64_t* keys_alt;         // collider_count: ping-pong buffer
+  uint32_t* rs_counts;        // 256 * rs_blocks: per-pass digit counts, bin-major, then offsets
+  uint32_t rs_blocks;         // key tiles of kRadixTile
+  uint32_t rs_passes;         // 8-bit digit passes over the (scene, code) bits
+
   // ---- launch shape (tunable; 0 = default) ----
+  uint32_t t_sort_kind;       // 0: bitonic sort, 1: radix sort
   uint32_t t_sort_tile;       // bitonic tile width: 512, 1024 or 2048 keys
   uint32_t t_sort_levels;     // butterfly levels per global sort kernel: 1..4
   uint32_t t_narrow_blocks;   // workgroup cap of the narrowphase

@@ -208,6 +208,102 @@ __global__ __launch_bounds__(256) void k_sort_global(uint64_t* keys, uint32_t co
 }
 
 // ---------------------------------------------------------------------------
+// Radix sort of the collider_count real keys (LSD, 8-bit digits) over the bits
+// above the collider field. k_morton writes the keys in collider order and every
+// pass is stable, so equal (scene, code) fields keep collider order: the result
+// is the order of the full keys, and the padding slots past collider_count are
+// already in place. Per pass: digit counts per 1024-key tile, one exclusive scan
+// of the (digit, tile) counts, and a stable scatter that ranks each tile's keys
+// row by row with wavefront ballots.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t radix_digit(const Dev& d, uint64_t key, uint32_t pass) {
+  return (uint32_t)(key >> (d.key_index_bits + 8u * pass)) & 0xffu;
+}
+
+__global__ __launch_bounds__(256) void k_radix_count(Dev d, const uint64_t* src, uint32_t pass) {
+  __shared__ uint32_t hist[256];
+  const uint32_t t = threadIdx.x;
+  hist[t] = 0u;
+  __syncthreads();
+  const uint32_t first = blockIdx.x * kRadixTile;
+#pragma unroll
+  for (uint32_t r = 0; r < kRadixTile / 256; ++r) {
+    const uint32_t i = first + r * 256u + t;
+    if (i < d.collider_count) atomicAdd(&hist[radix_digit(d, src[i], pass)], 1u);
+  }
+  __syncthreads();
+  d.rs_counts[(size_t)t * d.rs_blocks + blockIdx.x] = hist[t];
+}
+
+// Exclusive scan of the 256 * rs_blocks (digit, tile) counts in digit-major order.
+__global__ __launch_bounds__(1024) void k_radix_scan(Dev d) {
+  __shared__ uint32_t part[1024];
+  const uint32_t n = 256u * d.rs_blocks;
+  const uint32_t per = (n + 1023u) / 1024u;
+  const uint32_t begin = threadIdx.x * per;
+  const uint32_t end = begin + per < n ? begin + per : n;
+  uint32_t sum = 0;
+  for (uint32_t i = begin; i < end; ++i) sum += d.rs_counts[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024u; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - sum;
+  for (uint32_t i = begin; i < end; ++i) {
+    const uint32_t c = d.rs_counts[i];
+    d.rs_counts[i] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_radix_scatter(Dev d, const uint64_t* src, uint64_t* dst, uint32_t pass) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wave_count[4][256];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  running[t] = 0u;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) wave_count[w][t] = 0u;
+  __syncthreads();
+  const uint32_t first = blockIdx.x * kRadixTile;
+  const uint64_t lower = lane ? (~0ull >> (64u - lane)) : 0ull;
+  for (uint32_t r = 0; r < kRadixTile / 256; ++r) {
+    const uint32_t i = first + r * 256u + t;
+    const bool valid = i < d.collider_count;
+    const uint64_t key = valid ? src[i] : 0ull;
+    const uint32_t digit = valid ? radix_digit(d, key, pass) : 0u;
+    // Lanes holding the same digit.
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) {
+      const uint64_t ones = __ballot((digit >> b) & 1u);
+      peers &= ((digit >> b) & 1u) ? ones : ~ones;
+    }
+    const uint32_t rank_in_wave = (uint32_t)__popcll(peers & lower);
+    if (valid && rank_in_wave == 0u) wave_count[wave][digit] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t rank = running[digit] + rank_in_wave;
+      for (uint32_t w = 0; w < wave; ++w) rank += wave_count[w][digit];
+      dst[d.rs_counts[(size_t)digit * d.rs_blocks + blockIdx.x] + rank] = key;
+    }
+    __syncthreads();
+    running[t] += wave_count[0][t] + wave_count[1][t] + wave_count[2][t] + wave_count[3][t];
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) wave_count[w][t] = 0u;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_radix_copy(Dev d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.collider_count) d.keys[i] = d.keys_alt[i];
+}
+
+// ---------------------------------------------------------------------------
 // LBVH (Karras 2012) over the sorted keys, lbvh.wgsl:219-319.
 // Internal node n covers a contiguous range of sorted leaves; its record holds
 // (min, left word) and (max, escape). A child or escape is encoded as a sorted
@@ -704,6 +800,19 @@ void launch_morton(hipStream_t s, const Dev& d) {
 }
 
 void launch_sort(hipStream_t s, const Dev& d) {
+  if (d.t_sort_kind == 1u) {
+    if (d.collider_count < 2) return;
+    uint64_t* buf[2] = {d.keys, d.keys_alt};
+    for (uint32_t pass = 0; pass < d.rs_passes; ++pass) {
+      const uint64_t* src = buf[pass & 1u];
+      uint64_t* dst = buf[(pass & 1u) ^ 1u];
+      k_radix_count<<<d.rs_blocks, 256, 0, s>>>(d, src, pass);
+      k_radix_scan<<<1, 1024, 0, s>>>(d);
+      k_radix_scatter<<<d.rs_blocks, 256, 0, s>>>(d, src, dst, pass);
+    }
+    if (d.rs_passes & 1u) k_radix_copy<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
+    return;
+  }
   const uint32_t n = d.sort_count;  // power of two, >= 256
   const uint32_t tile_cap = d.t_sort_tile ? d.t_sort_tile : kTile;
   const uint32_t tile = n < tile_cap ? n : tile_cap;

@@ -629,6 +629,15 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
     rc = upload_forest(p, creation, forest, error);
     if (rc != MH_OK) { mh_physics_destroy(p); return rc; }
   }
+  {
+    uint64_t* keys_alt;
+    ALLOC(keys_alt, c);
+    d.keys_alt = keys_alt;
+    d.rs_blocks = (c + kRadixTile - 1) / kRadixTile;
+    uint32_t* rs_counts;
+    ALLOC(rs_counts, 256 * (size_t)(d.rs_blocks ? d.rs_blocks : 1));
+    d.rs_counts = rs_counts;
+  }
   d.scene_count = 1;
   d.key_index_bits = 32;
   if (batch) {
@@ -645,6 +654,12 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
     d.key_index_bits = bits;
     p->body_first = batch->body_first;
     p->scene_first = batch->scene_first;
+  }
+  {
+    // Digit passes over the (scene, code) bits above the collider field.
+    uint32_t scene_bits = 0;
+    while ((1u << scene_bits) < d.scene_count) ++scene_bits;
+    d.rs_passes = (30u + scene_bits + 7u) / 8u;
   }
   hipError_t he = hipHostMalloc(reinterpret_cast<void**>(&p->host_diag), kDiagWords * sizeof(uint32_t),
                                 hipHostMallocDefault);
@@ -775,12 +790,13 @@ int mh_physics_set_tuning(mh_physics* p, const mh_tuning* t) {
   if (!p || !t) return MH_ERR_INVALID_ARGUMENT;
   if (t->sort_tile && (t->sort_tile < 256 || t->sort_tile > 2048 || (t->sort_tile & (t->sort_tile - 1))))
     return MH_ERR_INVALID_ARGUMENT;
-  if (t->sort_levels > 4) return MH_ERR_INVALID_ARGUMENT;
+  if (t->sort_levels > 4 || t->sort_kind > 1) return MH_ERR_INVALID_ARGUMENT;
   if (t->narrow_blocks > (1u << 16) || t->prepare_blocks > (1u << 16) || t->solve_blocks > (1u << 16) ||
       t->scan_blocks > (1u << 16))
     return MH_ERR_INVALID_ARGUMENT;
   HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
   Dev& d = p->d;
+  d.t_sort_kind = t->sort_kind;
   d.t_sort_tile = t->sort_tile;
   d.t_sort_levels = t->sort_levels;
   d.t_narrow_blocks = t->narrow_blocks;
@@ -798,6 +814,7 @@ int mh_physics_get_tuning(const mh_physics* p, mh_tuning* t) {
   if (!p || !t) return MH_ERR_INVALID_ARGUMENT;
   std::memset(t, 0, sizeof(*t));
   const Dev& d = p->d;
+  t->sort_kind = d.t_sort_kind;
   t->sort_tile = d.t_sort_tile ? d.t_sort_tile : 2048u;
   t->sort_levels = d.t_sort_levels ? d.t_sort_levels : 4u;
   t->narrow_blocks = d.t_narrow_blocks ? d.t_narrow_blocks : 4096u;

@@ -77,6 +77,17 @@ def test_tuning_never_changes_results():
     assert np.array_equal(base[1].view(np.uint32), tuned[1].view(np.uint32))
 
 
+def test_radix_sort_matches_bitonic():
+    """The LSD radix key sort is a drop-in for the bitonic one: same order, same
+    pairs, bit-identical states (scene-keyed batch keys use the widest keys)."""
+    scenes = batch_scenes(count=8, smallest=64, largest=1200, size_seed=9)
+    base = run_batch(scenes, 30)
+    radix = run_batch(scenes, 30, dict(sort_kind=1))
+    assert base[3].pair_count == radix[3].pair_count and base[3].contact_count == radix[3].contact_count
+    assert np.array_equal(base[0].view(np.uint32), radix[0].view(np.uint32))
+    assert np.array_equal(base[1].view(np.uint32), radix[1].view(np.uint32))
+
+
 def test_tuning_round_trip_and_validation():
     scene = cuboid_pile(3, 3, 3, seed=0)
     with GpuPhysics.new_with_config(scene, CFG) as gpu:

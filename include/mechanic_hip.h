@@ -281,9 +281,10 @@ typedef struct {
   uint32_t narrow_blocks;   /* workgroups of the narrowphase (default 4096) */
   uint32_t prepare_blocks;  /* workgroups of contact preparation (default 2048) */
   uint32_t solve_blocks;    /* workgroups of each solver pass (default 2048) */
-  uint32_t scan_blocks;     /* workgroups of the broadphase bucket scan (default 2048) */
-  uint32_t sort_kind;       /* 0: bitonic sort (default), 1: radix sort */
-  uint32_t reserved;
+  uint32_t scan_blocks;     /* workgroups of the broadphase candidate scan (default 4096) */
+  uint32_t sort_kind;       /* 0: radix sort (default), 1: bitonic sort */
+  uint32_t traverse_kind;   /* 0: packet walk + separate scan of its leaf ranges (default), 1: one
+                               walk per leaf, 2: packet walk scanning its ranges itself */
 } mh_tuning;
 
 MH_API int mh_physics_set_tuning(mh_physics* physics, const mh_tuning* tuning);

@@ -154,7 +154,7 @@ class Config(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, c_uint32) for n in ("sort_tile", "sort_levels", "narrow_blocks", "prepare_blocks",
-                                        "solve_blocks", "scan_blocks", "sort_kind", "reserved")]
+                                        "solve_blocks", "scan_blocks", "sort_kind", "traverse_kind")]
 
 
 class ContactResidual(ctypes.Structure):

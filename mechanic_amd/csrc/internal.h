@@ -21,6 +21,9 @@ constexpr uint32_t kMaxSlots = 32;      // contact-side delta slots per body (bo
 constexpr uint32_t kSlotOverflow = 31;  // slot code of a side past the body's slots (atomics instead)
 constexpr uint32_t kContactMask = 0x3fffffu;  // contact index bits of a solver row's word (2^22 pairs)
 constexpr uint32_t kRadixTile = 1024;  // keys per radix-sort workgroup
+constexpr uint32_t kRadixMaxPasses = 8;  // 8-bit digit passes at most (64-bit keys)
+constexpr uint32_t kRadixTicket = kRadixMaxPasses * 256;  // rs_hist word of pass 0's tile ticket
+constexpr uint32_t kRadixHistWords = kRadixTicket + kRadixMaxPasses;
 constexpr uint32_t kFeatures = 16;      // learned warm-start inputs per contact
 constexpr uint32_t kResidBlocks = 512;  // residual kernel grid
 enum LearnMode : uint32_t { kLearnOff = 0, kLearnLog = 1, kLearnInfer = 2 };
@@ -39,7 +42,7 @@ enum DiagWord : uint32_t {
 // Private per-tick counters (cleared by the Morton kernel).
 enum CounterWord : uint32_t {
   kCountFrontier = 0,
-  kCountWork = 1,
+  kCountWork = 1,  // unused
   kCountTop = 2,
   kCounterWords = 16,
 };
@@ -93,8 +96,10 @@ struct Dev {
   uint32_t* leaf_body;    // body of the collider at each sorted leaf
   uint32_t* top;          // internal nodes whose leaves span several refit workgroups
   float4* block_box;      // 2 per refit workgroup: box of its 256 leaves
-  uint4* work;            // traversal work items: (packet first leaf, first, last, 0)
+  uint4* work;            // traversal work items, kItemStage per packet: (packet first leaf, first, last, 0)
   uint32_t work_capacity;
+  uint32_t* work_count;   // items recorded per packet
+  uint32_t* work_first;   // packets + 1: exclusive prefix of work_count (k_scan_prefix)
   uint32_t* counters;     // per-tick private counters, see CounterWord
   uint32_t* parents;      // reference encoding of node_parents (parent + 1)
   uint32_t* visits;       // refit arrival counters
@@ -131,18 +136,20 @@ struct Dev {
 
   // ---- radix sort of the keys ----
   uint64_t* keys_alt;         // collider_count: ping-pong buffer
-  uint32_t* rs_counts;        // 256 * rs_blocks: per-pass digit counts, bin-major, then offsets
+  uint32_t* rs_hist;          // kRadixHistWords: 256 digit counts per pass, then one tile ticket per pass
+  uint32_t* rs_status;        // rs_passes * rs_blocks * 256: per-tile digit counts published for look-back
   uint32_t rs_blocks;         // key tiles of kRadixTile
   uint32_t rs_passes;         // 8-bit digit passes over the (scene, code) bits
 
   // ---- launch shape (tunable; 0 = default) ----
-  uint32_t t_sort_kind;       // 0: bitonic sort, 1: radix sort
+  uint32_t t_sort_kind;       // 0: radix sort, 1: bitonic sort
   uint32_t t_sort_tile;       // bitonic tile width: 512, 1024 or 2048 keys
   uint32_t t_sort_levels;     // butterfly levels per global sort kernel: 1..4
   uint32_t t_narrow_blocks;   // workgroup cap of the narrowphase
   uint32_t t_prepare_blocks;  // workgroup cap of contact preparation
   uint32_t t_solve_blocks;    // workgroup cap of the solver passes
-  uint32_t t_scan_blocks;     // workgroups of the bucket scan
+  uint32_t t_scan_blocks;     // workgroups of k_scan
+  uint32_t t_traverse_kind;   // 0: packet walk + k_scan, 1: one walk per leaf, 2: packet walk scanning inline
 
   // ---- articulated path (bearings, coordinates, forward kinematics, closures) ----
   uint32_t bearing_count;

@@ -36,6 +36,13 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kCounterWords) d.counters[i] = 0u;  // sort_count >= 256 threads always run
+  const bool radix = d.t_sort_kind == 0u;
+  if (radix) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j = i; j < kRadixHistWords; j += stride) d.rs_hist[j] = 0u;
+    const uint32_t status_words = d.rs_passes * d.rs_blocks * 256u;
+    for (uint32_t j = i; j < status_words; j += stride) d.rs_status[j] = 0u;
+  }
   bool emit = false;
   ColliderPose p;
   v3 support;
@@ -50,9 +57,11 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
     // Sort key: (scene, Morton code, collider); a single scene has no scene field
     // and a 32-bit collider field, the reference's (code << 32 | collider).
     const uint64_t code = morton_code(p.center);
-    d.keys[i] = d.scene_count > 1
-                    ? ((uint64_t)d.cscene[i] << (30u + d.key_index_bits)) | (code << d.key_index_bits) | i
-                    : (code << 32) | i;
+    // The radix sort starts from the buffer its last pass does not write.
+    uint64_t* keys = radix && (d.rs_passes & 1u) ? d.keys_alt : d.keys;
+    keys[i] = d.scene_count > 1
+                  ? ((uint64_t)d.cscene[i] << (30u + d.key_index_bits)) | (code << d.key_index_bits) | i
+                  : (code << 32) | i;
     // Ground plane y = 0.
     const v3 down = mk(0.0f, -1.0f, 0.0f);
     support = support_point(p, down);
@@ -212,95 +221,141 @@ __global__ __launch_bounds__(256) void k_sort_global(uint64_t* keys, uint32_t co
 // above the collider field. k_morton writes the keys in collider order and every
 // pass is stable, so equal (scene, code) fields keep collider order: the result
 // is the order of the full keys, and the padding slots past collider_count are
-// already in place. Per pass: digit counts per 1024-key tile, one exclusive scan
-// of the (digit, tile) counts, and a stable scatter that ranks each tile's keys
-// row by row with wavefront ballots.
+// already in place.
+//
+// One launch counts every pass's digits (a permutation leaves them unchanged),
+// then each pass is one launch with decoupled look-back (Merrill & Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016): a
+// workgroup takes the next tile ticket, ranks its 1024 keys stably with
+// wavefront ballots, publishes its per-digit counts, and sums the counts of the
+// tiles before it from their published words. A tile only waits for tiles with
+// smaller tickets, which started before it, so every wait ends.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kRsAggregate = 1u << 30;  // status word: the tile's own count
+constexpr uint32_t kRsPrefix = 2u << 30;     // status word: count over tiles 0..this one
+constexpr uint32_t kRsCount = (1u << 30) - 1u;
+constexpr uint32_t kRsSpinLimit = 1u << 22;  // reads of an unpublished word before giving up
+
 __device__ __forceinline__ uint32_t radix_digit(const Dev& d, uint64_t key, uint32_t pass) {
   return (uint32_t)(key >> (d.key_index_bits + 8u * pass)) & 0xffu;
 }
 
-__global__ __launch_bounds__(256) void k_radix_count(Dev d, const uint64_t* src, uint32_t pass) {
-  __shared__ uint32_t hist[256];
-  const uint32_t t = threadIdx.x;
-  hist[t] = 0u;
+__device__ __forceinline__ uint32_t load_status(const uint32_t* w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_status(uint32_t* w, uint32_t v) {
+  __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Digit counts of every pass over all keys (rs_hist zeroed by k_morton).
+__global__ __launch_bounds__(1024) void k_radix_hist(Dev d, const uint64_t* src) {
+  __shared__ uint32_t hist[kRadixMaxPasses * 256];
+  const uint32_t passes = d.rs_passes;
+  for (uint32_t i = threadIdx.x; i < passes * 256u; i += blockDim.x) hist[i] = 0u;
   __syncthreads();
-  const uint32_t first = blockIdx.x * kRadixTile;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.collider_count; i += gridDim.x * blockDim.x) {
+    const uint64_t key = src[i];
+    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&hist[p * 256u + radix_digit(d, key, p)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < passes * 256u; i += blockDim.x)
+    if (hist[i]) atomicAdd(&d.rs_hist[i], hist[i]);
+}
+
+// Exclusive prefix sum over a 256-thread workgroup.
+__device__ __forceinline__ uint32_t exclusive_scan256(uint32_t v, uint32_t* wave_sums) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
 #pragma unroll
-  for (uint32_t r = 0; r < kRadixTile / 256; ++r) {
-    const uint32_t i = first + r * 256u + t;
-    if (i < d.collider_count) atomicAdd(&hist[radix_digit(d, src[i], pass)], 1u);
+  for (uint32_t off = 1; off < 64u; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
   }
+  if (lane == 63u) wave_sums[wave] = x;
   __syncthreads();
-  d.rs_counts[(size_t)t * d.rs_blocks + blockIdx.x] = hist[t];
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wave; ++w) before += wave_sums[w];
+  return before + x - v;
 }
 
-// Exclusive scan of the 256 * rs_blocks (digit, tile) counts in digit-major order.
-__global__ __launch_bounds__(1024) void k_radix_scan(Dev d) {
-  __shared__ uint32_t part[1024];
-  const uint32_t n = 256u * d.rs_blocks;
-  const uint32_t per = (n + 1023u) / 1024u;
-  const uint32_t begin = threadIdx.x * per;
-  const uint32_t end = begin + per < n ? begin + per : n;
-  uint32_t sum = 0;
-  for (uint32_t i = begin; i < end; ++i) sum += d.rs_counts[i];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  for (uint32_t off = 1; off < 1024u; off <<= 1) {
-    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[threadIdx.x] - sum;
-  for (uint32_t i = begin; i < end; ++i) {
-    const uint32_t c = d.rs_counts[i];
-    d.rs_counts[i] = run;
-    run += c;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_radix_scatter(Dev d, const uint64_t* src, uint64_t* dst, uint32_t pass) {
-  __shared__ uint32_t running[256];
-  __shared__ uint32_t wave_count[4][256];
+__global__ __launch_bounds__(256) void k_radix_pass(Dev d, const uint64_t* src, uint64_t* dst, uint32_t pass) {
+  constexpr uint32_t kRows = kRadixTile / 256;
+  __shared__ uint32_t offset[kRows][4][256];  // keys of digit t before (row, wave) within the tile
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wave_sums[4];
+  __shared__ uint32_t ticket;
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-  running[t] = 0u;
+  if (t == 0) ticket = atomicAdd(&d.rs_hist[kRadixTicket + pass], 1u);
 #pragma unroll
-  for (uint32_t w = 0; w < 4; ++w) wave_count[w][t] = 0u;
+  for (uint32_t r = 0; r < kRows; ++r)
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) offset[r][w][t] = 0u;
   __syncthreads();
-  const uint32_t first = blockIdx.x * kRadixTile;
+  const uint32_t tile = ticket;
+  const uint32_t first = tile * kRadixTile;
   const uint64_t lower = lane ? (~0ull >> (64u - lane)) : 0ull;
-  for (uint32_t r = 0; r < kRadixTile / 256; ++r) {
+  uint64_t key[kRows];
+  uint32_t digit[kRows], rank[kRows];
+#pragma unroll
+  for (uint32_t r = 0; r < kRows; ++r) {
     const uint32_t i = first + r * 256u + t;
     const bool valid = i < d.collider_count;
-    const uint64_t key = valid ? src[i] : 0ull;
-    const uint32_t digit = valid ? radix_digit(d, key, pass) : 0u;
+    key[r] = valid ? src[i] : 0ull;
+    digit[r] = valid ? radix_digit(d, key[r], pass) : 256u;
     // Lanes holding the same digit.
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (uint32_t b = 0; b < 8; ++b) {
-      const uint64_t ones = __ballot((digit >> b) & 1u);
-      peers &= ((digit >> b) & 1u) ? ones : ~ones;
+      const uint64_t ones = __ballot((digit[r] >> b) & 1u);
+      peers &= ((digit[r] >> b) & 1u) ? ones : ~ones;
     }
-    const uint32_t rank_in_wave = (uint32_t)__popcll(peers & lower);
-    if (valid && rank_in_wave == 0u) wave_count[wave][digit] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t rank = running[digit] + rank_in_wave;
-      for (uint32_t w = 0; w < wave; ++w) rank += wave_count[w][digit];
-      dst[d.rs_counts[(size_t)digit * d.rs_blocks + blockIdx.x] + rank] = key;
-    }
-    __syncthreads();
-    running[t] += wave_count[0][t] + wave_count[1][t] + wave_count[2][t] + wave_count[3][t];
-#pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) wave_count[w][t] = 0u;
-    __syncthreads();
+    rank[r] = (uint32_t)__popcll(peers & lower);
+    if (valid && rank[r] == 0u) offset[r][wave][digit[r]] = (uint32_t)__popcll(peers);
   }
-}
-
-__global__ __launch_bounds__(256) void k_radix_copy(Dev d) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < d.collider_count) d.keys[i] = d.keys_alt[i];
+  __syncthreads();
+  // Thread t owns digit t: counts become offsets in tile order (row, then wave).
+  uint32_t run = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kRows; ++r)
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+      const uint32_t c = offset[r][w][t];
+      offset[r][w][t] = run;
+      run += c;
+    }
+  uint32_t* status = d.rs_status + (size_t)pass * d.rs_blocks * 256u;
+  uint32_t before = 0;
+  if (tile == 0) {
+    store_status(&status[t], kRsPrefix | run);
+  } else {
+    store_status(&status[(size_t)tile * 256u + t], kRsAggregate | run);
+    uint32_t j = tile - 1u, spins = 0;
+    for (;;) {
+      const uint32_t v = load_status(&status[(size_t)j * 256u + t]);
+      if ((v & ~kRsCount) == 0u) {
+        if (++spins >= kRsSpinLimit) {
+          atomicOr(&d.diag[tick_now(d) & 1][kDiagFlags], 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      before += v & kRsCount;
+      if (v & kRsPrefix) break;
+      --j;  // tile 0 always publishes a prefix, so j stays >= 0
+    }
+    store_status(&status[(size_t)tile * 256u + t], kRsPrefix | (before + run));
+  }
+  // Keys of smaller digits over all tiles, plus keys of digit t in earlier tiles.
+  const uint32_t smaller = exclusive_scan256(d.rs_hist[pass * 256u + t], wave_sums);
+  base[t] = smaller + before;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kRows; ++r) {
+    if (digit[r] >= 256u) continue;
+    const uint32_t to = base[digit[r]] + offset[r][wave][digit[r]] + rank[r];
+    if (to < d.collider_count) dst[to] = key[r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -644,8 +699,6 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
 __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
                                            const Packet& k, uint32_t start, uint32_t last) {
   const bool check_suppression = d.suppression_count != 0;
-  // A range is one leaf or one bucket, so all of it lies in one scene.
-  const bool same_scene = d.scene_count <= 1 || k.scene_end > start;
   for (uint32_t base = start; base <= last; base += 64u) {
     const uint32_t q = base + lane;
     float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
@@ -663,7 +716,9 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
       const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
       const uint32_t other = bcast(__float_as_uint(cmn.w), j);
       const uint32_t other_body = bcast(cbody, j);
-      bool hit = k.valid && same_scene && base + j > k.p && overlaps(k.qmn, k.qmx, bmn, bmx) &&
+      // A merged range may run into the next scene: candidates past the end of
+      // the query's scene are no pairs.
+      bool hit = k.valid && base + j > k.p && base + j < k.scene_end && overlaps(k.qmn, k.qmx, bmn, bmx) &&
                  other_body != k.self_body;
       if (hit && check_suppression) hit = !suppressed(d, k.self_body, other_body);
       const uint64_t mask = __ballot(hit);
@@ -680,44 +735,83 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
   }
 }
 
-constexpr uint32_t kItemStage = 64;  // staged work items per wavefront
+constexpr uint32_t kItemStage = 64;  // work items per packet
 
-// Phase 1: the packet walks the tree and records the leaf ranges (buckets and
-// single leaves) it must scan as work items. If the work list is full, the
-// range is scanned here instead.
+// Block index remapped so the workgroups of one XCD (dispatched round-robin,
+// blockIdx % 8) take one contiguous share of the sorted leaves: neighbouring
+// packets share tree nodes, which then stay in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
+
+// Writes the staged pairs of every wavefront of the workgroup with one atomic
+// reservation for the whole workgroup (all threads must call it). Appends to one
+// counter serialise across the device, so they are made as few as possible.
+__device__ __forceinline__ void flush_pairs_block(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
+                                                  uint32_t* lds) {
+  const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  if (lane == 0) lds[wave] = st.count;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < waves; ++w) {
+      const uint32_t n = lds[w];
+      lds[w] = total;
+      total += n;
+    }
+    const uint32_t base = total ? atomicAdd(&diag[kDiagPairs], total) : 0u;
+    if (total && base + total > d.pair_capacity) atomicOr(&diag[kDiagFlags], 1u);
+    lds[waves] = base;
+  }
+  __syncthreads();
+  const uint32_t base = lds[waves] + lds[wave];
+  for (uint32_t i = lane; i < st.count; i += 64u) {
+    const uint32_t row = base + i;
+    if (row < d.pair_capacity) d.pairs[row] = st.buf[i];
+  }
+  st.count = 0;
+}
+
+// The packet walks the tree and collects the leaf ranges (buckets and single
+// leaves) it must scan, in depth-first order. Ranges that continue the previous
+// one are merged, so most of a packet's leaves are scanned 64 at a time. With
+// kInline the wavefront scans each merged range itself; otherwise it records
+// them, cut into items of at most 64 leaves, in the packet's kItemStage slots
+// for k_scan to spread over many wavefronts (ranges past the slots are scanned
+// here).
+template <bool kInline>
 __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
-  __shared__ uint4 item_mem[4][kItemStage];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t c = d.collider_count;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wave_first = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  const uint32_t wave_first = (xcd_block() * blockDim.x + threadIdx.x) & ~63u;
   if (wave_first >= c) return;  // whole wavefront past the last leaf
   const Packet k = load_packet(d, wave_first, lane);
   PairStage st{stage_mem[threadIdx.x >> 6], 0u};
-  uint4* items = item_mem[threadIdx.x >> 6];
+  uint4* items = d.work + (size_t)(wave_first / 64u) * kItemStage;
   uint32_t n_items = 0;  // wave-uniform
 
-  auto flush_items = [&]() {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_wave_barrier();
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&d.counters[kCountWork], n_items);
-    base = bcast(base, 0u);
-    if (lane < n_items && base + lane < d.work_capacity) d.work[base + lane] = items[lane];
-    // Items past the list's capacity are scanned by this wavefront.
-    for (uint32_t i = 0; i < n_items; ++i) {
-      if (base + i >= d.work_capacity) {
-        const uint4 it = items[i];
-        scan_range(d, diag, st, lane, k, it.y, it.z);
+  auto emit_range = [&](uint32_t first, uint32_t last) {
+    if constexpr (kInline) {
+      scan_range(d, diag, st, lane, k, first, last);
+    } else {
+      for (uint32_t a = first; a <= last; a += 64u) {
+        const uint32_t b = last - a > 63u ? a + 63u : last;
+        if (n_items < kItemStage) {
+          if (lane == 0) items[n_items] = make_uint4(wave_first, a, b, 0u);
+          n_items += 1;
+        } else {
+          scan_range(d, diag, st, lane, k, a, b);
+        }
       }
     }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_wave_barrier();
-    n_items = 0;
   };
 
+  uint32_t run_first = 1u, run_last = 0u;  // pending merged range (empty while first > last)
   uint32_t cur = 0u;  // the root (c >= 2)
   uint32_t steps_left = 2u * c + 1u;
   while (cur != c) {
@@ -753,41 +847,182 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
     }
     const uint32_t start = first > wave_first ? first : wave_first + 1u;
     if (start <= last) {
-      if (lane == 0) items[n_items] = make_uint4(wave_first, start, last, 0u);
-      n_items += 1;
-      if (n_items == kItemStage) flush_items();
+      if (run_first <= run_last && run_last + 1u == start) {
+        run_last = last;
+      } else {
+        if (run_first <= run_last) emit_range(run_first, run_last);
+        run_first = start;
+        run_last = last;
+      }
     }
     cur = next;
   }
-  if (n_items) flush_items();
+  if (run_first <= run_last) emit_range(run_first, run_last);
+  if (!kInline && lane == 0) d.work_count[wave_first / 64u] = n_items;
   if (st.count) flush_pairs(d, diag, st, lane);
 }
 
-// Phase 2: every wavefront takes a contiguous share of the work items and scans
-// each range against the item's packet.
+// Exclusive prefix of the per-packet item counts (one workgroup).
+__global__ __launch_bounds__(1024) void k_scan_prefix(Dev d) {
+  __shared__ uint32_t wave_sums[16];
+  const uint32_t packets = (d.collider_count + 63u) / 64u;
+  const uint32_t per = (packets + 1023u) / 1024u;
+  const uint32_t begin = threadIdx.x * per;
+  const uint32_t end = begin + per < packets ? begin + per : packets;
+  uint32_t sum = 0;
+  for (uint32_t i = begin; i < end; ++i) sum += d.work_count[i];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = sum;
+#pragma unroll
+  for (uint32_t off = 1; off < 64u; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63u) wave_sums[wave] = x;
+  __syncthreads();
+  uint32_t run = x - sum;
+  for (uint32_t w = 0; w < wave; ++w) run += wave_sums[w];
+  for (uint32_t i = begin; i < end; ++i) {
+    d.work_first[i] = run;
+    run += d.work_count[i];
+  }
+  if (threadIdx.x == 1023u) d.work_first[packets] = run;
+}
+
+// Every wavefront takes a contiguous share of the work items (in packet order,
+// through the prefix of the per-packet counts) and scans each item's leaves
+// against its packet.
 __global__ __launch_bounds__(256) void k_scan(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
+  __shared__ uint32_t flush_lds[5];
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t requested = *reinterpret_cast<volatile uint32_t*>(&d.counters[kCountWork]);
-  const uint32_t count = requested < d.work_capacity ? requested : d.work_capacity;
+  const uint32_t packets = (d.collider_count + 63u) / 64u;
+  const uint32_t total = d.work_first[packets];
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint32_t share = (count + waves - 1u) / waves;
+  const uint32_t share = (total + waves - 1u) / waves;
   const uint32_t begin = wave * share;
-  const uint32_t end = begin + share < count ? begin + share : count;
-  if (begin >= end) return;
+  const uint32_t end = begin + share < total ? begin + share : total;
   PairStage st{stage_mem[threadIdx.x >> 6], 0u};
-  uint32_t packet_first = kInvalid;
-  Packet k{};
-  for (uint32_t i = begin; i < end; ++i) {
-    const uint4 it = d.work[i];
-    if (it.x != packet_first) {
-      packet_first = it.x;
-      k = load_packet(d, packet_first, lane);
+  if (begin < end) {
+    // Packet holding item `begin`: work_first[lo] <= begin < work_first[lo + 1].
+    uint32_t lo = 0, hi = packets;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) / 2u;
+      if (d.work_first[mid] <= begin) lo = mid;
+      else hi = mid;
     }
-    scan_range(d, diag, st, lane, k, it.y, it.z);
+    uint32_t pk = lo, pk_first = d.work_first[pk], pk_next = d.work_first[pk + 1];
+    Packet k = load_packet(d, pk * 64u, lane);
+    for (uint32_t i = begin; i < end; ++i) {
+      if (i >= pk_next) {
+        do {
+          ++pk;
+          pk_first = pk_next;
+          pk_next = d.work_first[pk + 1];
+        } while (i >= pk_next);
+        k = load_packet(d, pk * 64u, lane);
+      }
+      const uint4 it = d.work[(size_t)pk * kItemStage + (i - pk_first)];
+      scan_range(d, diag, st, lane, k, it.y, it.z);
+    }
+  }
+  flush_pairs_block(d, diag, st, lane, flush_lds);
+}
+
+// One walk per leaf (the reference's per-leaf traversal, lbvh.wgsl:321-398,
+// without its stack): lane p follows escapes from the root, descending into a
+// node when its own query overlaps it and the node holds leaves right of p, and
+// tests the leaves it reaches (a bucket's leaves one per step). Each step is one
+// node or leaf per lane; hits are compacted by ballot into the wavefront's LDS
+// stage as in the packet walk.
+__global__ __launch_bounds__(256) void k_traverse_lane(Dev d) {
+  __shared__ uint2 stage_mem[4][kStage];
+  const uint64_t tick = tick_now(d);
+  uint32_t* diag = d.diag[tick & 1];
+  const uint32_t c = d.collider_count;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t p = xcd_block() * blockDim.x + threadIdx.x;
+  if ((p & ~63u) >= c) return;  // whole wavefront past the last leaf
+  PairStage st{stage_mem[threadIdx.x >> 6], 0u};
+  const bool check_suppression = d.suppression_count != 0;
+  bool active = p < c;
+  float4 qmn = make_float4(0.f, 0.f, 0.f, 0.f), qmx = qmn;
+  uint32_t self = 0, self_body = 0, scene_end = c;
+  if (active) {
+    qmn = d.leaves[2 * (size_t)p];
+    qmx = d.leaves[2 * (size_t)p + 1];
+    self = __float_as_uint(qmn.w);
+    self_body = d.leaf_body[p];
+    if (d.scene_count > 1) scene_end = d.scene_first[d.cscene[self] + 1];
+  }
+  uint32_t cur = 0u;                   // the root (c >= 2)
+  uint32_t scan = 0u, scan_last = 0u;  // bucket leaves still to test, when scan <= scan_last
+  uint32_t after = 0u;                 // escape of the bucket being scanned
+  bool scanning = false;
+  uint32_t steps_left = 4u * c + 4u;
+  while (__ballot(active) != 0ull) {
+    bool hit = false;
+    uint32_t other = 0u;
+    if (active) {
+      uint32_t leaf = kInvalid;
+      if (scanning) {
+        leaf = scan++;
+        if (scan > scan_last) {
+          scanning = false;
+          cur = after;
+        }
+      } else if (cur & kLeafBit) {
+        leaf = cur & kPosMask;
+        cur = __float_as_uint(d.leaves[2 * (size_t)leaf + 1].w);
+      } else {
+        const uint32_t pos = cur & kPosMask;
+        const float4 mn = d.inodes[2 * (size_t)pos];
+        const float4 mx = d.inodes[2 * (size_t)pos + 1];
+        const uint32_t left_word = __float_as_uint(mn.w);
+        const uint32_t esc = __float_as_uint(mx.w);
+        const uint32_t range_last = (esc & kPosMask) - 1u;
+        bool want = range_last > p && overlaps(qmn, qmx, mn, mx);
+        if (d.scene_count > 1 && want) want = d.ranges[pos].x < scene_end;
+        if (!want) {
+          cur = esc;
+        } else if (!(left_word & kBucketBit)) {
+          cur = left_word;
+        } else {
+          const uint32_t first = left_word & (kBucketBit - 1u);
+          scan = first > p ? first : p + 1u;
+          scan_last = range_last;
+          after = esc;
+          scanning = scan <= scan_last;
+          if (!scanning) cur = esc;
+        }
+      }
+      if (leaf != kInvalid && leaf > p && leaf < scene_end) {
+        const float4 bmn = d.leaves[2 * (size_t)leaf];
+        const float4 bmx = d.leaves[2 * (size_t)leaf + 1];
+        const uint32_t other_body = d.leaf_body[leaf];
+        other = __float_as_uint(bmn.w);
+        hit = overlaps(qmn, qmx, bmn, bmx) && other_body != self_body;
+        if (hit && check_suppression) hit = !suppressed(d, self_body, other_body);
+      }
+      if (!scanning && cur == c) active = false;
+      if (active && steps_left-- == 0u) {
+        atomicOr(&diag[kDiagFlags], 1u);
+        active = false;
+      }
+    }
+    const uint64_t mask = __ballot(hit);
+    if (mask != 0ull) {
+      const uint32_t cnt = (uint32_t)__popcll(mask);
+      if (st.count + cnt > kStage) flush_pairs(d, diag, st, lane);
+      if (hit) {
+        const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        st.buf[st.count + rank] = make_uint2(self < other ? self : other, self < other ? other : self);
+      }
+      st.count += cnt;
+    }
   }
   if (st.count) flush_pairs(d, diag, st, lane);
 }
@@ -800,17 +1035,17 @@ void launch_morton(hipStream_t s, const Dev& d) {
 }
 
 void launch_sort(hipStream_t s, const Dev& d) {
-  if (d.t_sort_kind == 1u) {
-    if (d.collider_count < 2) return;
+  if (d.t_sort_kind == 0u) {
+    // k_morton wrote the keys where the last pass lands in d.keys.
+    if (d.rs_blocks == 0) return;
     uint64_t* buf[2] = {d.keys, d.keys_alt};
+    const uint32_t odd = d.rs_passes & 1u;
+    const uint32_t hist_blocks = d.rs_blocks < 64u ? d.rs_blocks : 64u;
+    k_radix_hist<<<hist_blocks, 1024, 0, s>>>(d, buf[odd]);
     for (uint32_t pass = 0; pass < d.rs_passes; ++pass) {
-      const uint64_t* src = buf[pass & 1u];
-      uint64_t* dst = buf[(pass & 1u) ^ 1u];
-      k_radix_count<<<d.rs_blocks, 256, 0, s>>>(d, src, pass);
-      k_radix_scan<<<1, 1024, 0, s>>>(d);
-      k_radix_scatter<<<d.rs_blocks, 256, 0, s>>>(d, src, dst, pass);
+      const uint32_t from = (pass + odd) & 1u;
+      k_radix_pass<<<d.rs_blocks, 256, 0, s>>>(d, buf[from], buf[from ^ 1u], pass);
     }
-    if (d.rs_passes & 1u) k_radix_copy<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
     return;
   }
   const uint32_t n = d.sort_count;  // power of two, >= 256
@@ -842,8 +1077,16 @@ void launch_lbvh(hipStream_t s, const Dev& d) {
 
 void launch_traverse(hipStream_t s, const Dev& d) {
   if (d.collider_count < 2) return;
-  k_traverse<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
-  k_scan<<<d.t_scan_blocks ? d.t_scan_blocks : 2048u, kBlock, 0, s>>>(d);
+  const uint32_t blocks = blocks_for(d.collider_count);
+  if (d.t_traverse_kind == 1u) {
+    k_traverse_lane<<<blocks, kBlock, 0, s>>>(d);
+  } else if (d.t_traverse_kind == 2u) {
+    k_traverse<true><<<blocks, kBlock, 0, s>>>(d);
+  } else {
+    k_traverse<false><<<blocks, kBlock, 0, s>>>(d);
+    k_scan_prefix<<<1, 1024, 0, s>>>(d);
+    k_scan<<<d.t_scan_blocks ? d.t_scan_blocks : 4096u, kBlock, 0, s>>>(d);
+  }
 }
 
 }  // namespace mh

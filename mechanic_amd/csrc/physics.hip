@@ -565,6 +565,11 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   const uint32_t packets = (c + 63) / 64;
   d.work_capacity = 64 * packets < 4096 ? 4096 : 64 * packets;
   ALLOC(work, d.work_capacity);
+  uint32_t *work_count, *work_first;
+  ALLOC(work_count, packets);
+  ALLOC(work_first, (size_t)packets + 1);
+  d.work_count = work_count;
+  d.work_first = work_first;
   ALLOC(visits, c > 1 ? c - 1 : 1);
   ALLOC(pairs, cfg.pair_capacity);
   ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
@@ -634,9 +639,9 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
     ALLOC(keys_alt, c);
     d.keys_alt = keys_alt;
     d.rs_blocks = (c + kRadixTile - 1) / kRadixTile;
-    uint32_t* rs_counts;
-    ALLOC(rs_counts, 256 * (size_t)(d.rs_blocks ? d.rs_blocks : 1));
-    d.rs_counts = rs_counts;
+    uint32_t* rs_hist;
+    ALLOC(rs_hist, kRadixHistWords);
+    d.rs_hist = rs_hist;
   }
   d.scene_count = 1;
   d.key_index_bits = 32;
@@ -660,6 +665,9 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
     uint32_t scene_bits = 0;
     while ((1u << scene_bits) < d.scene_count) ++scene_bits;
     d.rs_passes = (30u + scene_bits + 7u) / 8u;
+    uint32_t* rs_status;
+    ALLOC(rs_status, (size_t)d.rs_passes * 256u * (d.rs_blocks ? d.rs_blocks : 1u));
+    d.rs_status = rs_status;
   }
   hipError_t he = hipHostMalloc(reinterpret_cast<void**>(&p->host_diag), kDiagWords * sizeof(uint32_t),
                                 hipHostMallocDefault);
@@ -790,7 +798,7 @@ int mh_physics_set_tuning(mh_physics* p, const mh_tuning* t) {
   if (!p || !t) return MH_ERR_INVALID_ARGUMENT;
   if (t->sort_tile && (t->sort_tile < 256 || t->sort_tile > 2048 || (t->sort_tile & (t->sort_tile - 1))))
     return MH_ERR_INVALID_ARGUMENT;
-  if (t->sort_levels > 4 || t->sort_kind > 1) return MH_ERR_INVALID_ARGUMENT;
+  if (t->sort_levels > 4 || t->sort_kind > 1 || t->traverse_kind > 2) return MH_ERR_INVALID_ARGUMENT;
   if (t->narrow_blocks > (1u << 16) || t->prepare_blocks > (1u << 16) || t->solve_blocks > (1u << 16) ||
       t->scan_blocks > (1u << 16))
     return MH_ERR_INVALID_ARGUMENT;
@@ -803,6 +811,7 @@ int mh_physics_set_tuning(mh_physics* p, const mh_tuning* t) {
   d.t_prepare_blocks = t->prepare_blocks;
   d.t_solve_blocks = t->solve_blocks;
   d.t_scan_blocks = t->scan_blocks;
+  d.t_traverse_kind = t->traverse_kind;
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->graph_exec = nullptr;
@@ -820,7 +829,8 @@ int mh_physics_get_tuning(const mh_physics* p, mh_tuning* t) {
   t->narrow_blocks = d.t_narrow_blocks ? d.t_narrow_blocks : 4096u;
   t->prepare_blocks = d.t_prepare_blocks ? d.t_prepare_blocks : 2048u;
   t->solve_blocks = d.t_solve_blocks ? d.t_solve_blocks : 2048u;
-  t->scan_blocks = d.t_scan_blocks ? d.t_scan_blocks : 2048u;
+  t->scan_blocks = d.t_scan_blocks ? d.t_scan_blocks : 4096u;
+  t->traverse_kind = d.t_traverse_kind;
   return MH_OK;
 }
 

@@ -173,7 +173,7 @@ class GpuPhysics:
     def get_tuning(self) -> dict:
         t = _lib.Tuning()
         _check(_lib.load().mh_physics_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
-        return {n: getattr(t, n) for n, _ in _lib.Tuning._fields_ if n != "reserved"}
+        return {n: getattr(t, n) for n, _ in _lib.Tuning._fields_}
 
     @classmethod
     def new_with_config(cls, scene: Scene, config: GpuPhysicsConfig) -> "GpuPhysics":

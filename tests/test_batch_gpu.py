@@ -77,21 +77,26 @@ def test_tuning_never_changes_results():
     assert np.array_equal(base[1].view(np.uint32), tuned[1].view(np.uint32))
 
 
-def test_radix_sort_matches_bitonic():
-    """The LSD radix key sort is a drop-in for the bitonic one: same order, same
-    pairs, bit-identical states (scene-keyed batch keys use the widest keys)."""
+@pytest.mark.parametrize("tuning", [dict(sort_kind=1), dict(traverse_kind=1), dict(traverse_kind=2),
+                                    dict(sort_kind=1, traverse_kind=1)])
+def test_broadphase_variants_match_default(tuning):
+    """The key sorts and pair walks are drop-ins for each other: same pairs,
+    bit-identical states (scene-keyed batch keys are the widest keys, and every
+    walk must stay inside each scene)."""
     scenes = batch_scenes(count=8, smallest=64, largest=1200, size_seed=9)
     base = run_batch(scenes, 30)
-    radix = run_batch(scenes, 30, dict(sort_kind=1))
-    assert base[3].pair_count == radix[3].pair_count and base[3].contact_count == radix[3].contact_count
-    assert np.array_equal(base[0].view(np.uint32), radix[0].view(np.uint32))
-    assert np.array_equal(base[1].view(np.uint32), radix[1].view(np.uint32))
+    other = run_batch(scenes, 30, tuning)
+    assert base[3].pair_count == other[3].pair_count and base[3].contact_count == other[3].contact_count
+    assert other[3].error_flags == 0
+    assert np.array_equal(base[0].view(np.uint32), other[0].view(np.uint32))
+    assert np.array_equal(base[1].view(np.uint32), other[1].view(np.uint32))
 
 
 def test_tuning_round_trip_and_validation():
     scene = cuboid_pile(3, 3, 3, seed=0)
     with GpuPhysics.new_with_config(scene, CFG) as gpu:
-        assert gpu.get_tuning()["sort_tile"] == 2048
+        t = gpu.get_tuning()
+        assert t["sort_tile"] == 2048 and t["sort_kind"] == 0 and t["traverse_kind"] == 0 and t["scan_blocks"] == 4096
         gpu.set_tuning(sort_tile=1024, scan_blocks=64)
         t = gpu.get_tuning()
         assert t["sort_tile"] == 1024 and t["scan_blocks"] == 64 and t["solve_blocks"] == 2048

@@ -18,6 +18,13 @@ pytestmark = pytest.mark.gpu
 
 INVALID = 0xFFFFFFFF
 
+# Broadphase variants selected by the launch tuning: the key sort (radix or
+# bitonic) and the pair walk (packet walk handing its merged leaf ranges to a
+# separate scan, one walk per leaf, or the packet walk scanning them itself).
+# Every variant must give the same keys, tree and pair set.
+BROADPHASE = {"radix-scan": {}, "bitonic": {"sort_kind": 1}, "lane": {"traverse_kind": 1},
+              "bitonic-lane": {"sort_kind": 1, "traverse_kind": 1}, "inline": {"traverse_kind": 2}}
+
 
 def make(scene, **cfg):
     iterations = cfg.pop("solver_iterations", 8)
@@ -25,10 +32,13 @@ def make(scene, **cfg):
     collisions = cfg.pop("collisions_enabled", True)
     pair_capacity = cfg.pop("pair_capacity", _lib.MAX_CONTACT_PAIRS)
     use_graph = cfg.pop("use_graph", True)
+    tuning = cfg.pop("tuning", None)
     assert not cfg
     gpu = mh.GpuPhysics.new_with_config(scene, mh.GpuPhysicsConfig(
         collisions_enabled=collisions, solver_iterations=iterations, friction=friction,
         pair_capacity=pair_capacity, use_graph=use_graph))
+    if tuning:
+        gpu.set_tuning(**tuning)
     ora = Oracle(scene, iterations=iterations, friction=friction, pair_capacity=pair_capacity,
                  collisions=collisions)
     return gpu, ora
@@ -95,10 +105,11 @@ def compare_broadphase(gpu, ora, scene):
     return int(diag[1])
 
 
+@pytest.mark.parametrize("variant", list(BROADPHASE))
 @pytest.mark.parametrize("name,ticks", [("pile_1k", 1), ("pile_1k", 90)])
-def test_broadphase_bit_exact(name, ticks):
+def test_broadphase_bit_exact(name, ticks, variant):
     scene = mh.named_pile(name)
-    gpu, ora = make(scene, use_graph=False)
+    gpu, ora = make(scene, use_graph=False, tuning=BROADPHASE[variant])
     for tick in range(1, ticks):
         gpu.dispatch_tick(tick)
         ora.step(tick)
@@ -239,9 +250,10 @@ def test_external_impulse_matches_oracle():
         gpu.apply_impulse(0, [0, np.inf, 0], [1, 0, 0])
 
 
-def test_pile_100k_first_tick_broadphase_and_contacts():
+@pytest.mark.parametrize("variant", list(BROADPHASE))
+def test_pile_100k_first_tick_broadphase_and_contacts(variant):
     scene = mh.named_pile("pile_100k")
-    gpu, ora = make(scene, use_graph=False)
+    gpu, ora = make(scene, use_graph=False, tuning=BROADPHASE[variant])
     ora.set_threads(8)
     gpu.debug_tick(1, 3)
     ora.step_partial(1, 3)
@@ -249,12 +261,13 @@ def test_pile_100k_first_tick_broadphase_and_contacts():
     compare_contacts(gpu, ora)
 
 
+@pytest.mark.parametrize("variant", ["radix-scan", "bitonic-lane"])
 @pytest.mark.parametrize("count", [2, 3, 5, 16, 17, 33])
-def test_small_stacks_many_ticks(count):
+def test_small_stacks_many_ticks(count, variant):
     # Fewer colliders than the per-tick counters: every counter must still reset.
     centers = [[0.01 * (i % 3), 0.125 + 0.26 * i, -0.01 * (i % 2)] for i in range(count)]
     scene = mh.boxes(centers, [0.125] * 3, inverse_mass=0.064, inverse_inertia=6.144)
-    gpu, ora = make(scene, friction=0.6)
+    gpu, ora = make(scene, friction=0.6, tuning=BROADPHASE[variant])
     for tick in range(1, 241):
         gpu.dispatch_tick(tick)
         ora.step(tick)

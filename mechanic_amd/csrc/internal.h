@@ -108,9 +108,9 @@ struct Dev {
   // ---- contacts and solver ----
   float4* contacts;       // 4 per contact, mh_contact layout
   uint32_t* active;       // contact index per active contact
-  float4* rows;           // 4 per active contact: solver rows
+  float4* rows;           // 4 planes of pair_capacity: plane k holds solver row word k of each active contact
   float* lambda;          // accumulated normal impulse per active contact
-  int4* cdelta;           // body-major: 2 int4 per (body, slot) -- a contact side's Q20 deltas of the last pass
+  int4* cdelta;           // slot-major: 2 int4 per (slot, body) -- a contact side's Q20 deltas of the last pass
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)
   int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
   uint32_t* slot_count;   // contact sides per body this tick

@@ -109,8 +109,10 @@ __device__ __forceinline__ Q20 side_delta(const SolverBody& s, v3 arm, v3 impuls
 
 // A contact side's deltas go to its body's slot, so each body's sides lie
 // together for the gather.
+// Slot-major: for one slot, consecutive bodies are adjacent, so the bodies'
+// gather reads whole cache lines per wavefront; a side's 32 bytes stay in one line.
 __device__ __forceinline__ void put_side(const Dev& d, uint32_t body, uint32_t slot, const Q20& q) {
-  int4* p = d.cdelta + 2 * ((size_t)body * kMaxSlots + slot);
+  int4* p = d.cdelta + 2 * ((size_t)slot * d.body_count + body);
   p[0] = make_int4(q.v[0], q.v[1], q.v[2], q.v[3]);
   p[1] = make_int4(q.v[4], q.v[5], 0, 0);
 }
@@ -119,6 +121,15 @@ __device__ __forceinline__ void add_overflow(const Dev& d, uint32_t body, const 
   int32_t* p = d.deltas + 6 * (size_t)body;
 #pragma unroll
   for (int k = 0; k < 6; ++k) atomicAdd(&p[k], q.v[k]);
+}
+
+// Solver rows as four planes (structure of arrays): a wavefront's load of one
+// word is 1 KiB of consecutive rows.
+__device__ __forceinline__ float4 row_word(const Dev& d, uint32_t k, uint32_t ai) {
+  return d.rows[(size_t)k * d.pair_capacity + ai];
+}
+__device__ __forceinline__ void set_row_word(const Dev& d, uint32_t k, uint32_t ai, float4 v) {
+  d.rows[(size_t)k * d.pair_capacity + ai] = v;
 }
 
 // prepare_contacts (collision.wgsl:674-736) with the deterministic manifold index.
@@ -229,11 +240,10 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
       const uint32_t flags = (sa << 22) | (sb << 27);
       d.active[a] = ci;
       d.lambda[a] = cached_lambda;
-      float4* row = d.rows + 4 * (size_t)a;
-      row[0] = r0;
-      row[1] = r1;
-      row[2] = r2;
-      row[3] = make_float4(r3.x, r3.y, __uint_as_float(ci | flags), r3.w);
+      set_row_word(d, 0, a, r0);
+      set_row_word(d, 1, a, r1);
+      set_row_word(d, 2, a, r2);
+      set_row_word(d, 3, a, make_float4(r3.x, r3.y, __uint_as_float(ci | flags), r3.w));
     }
   }
 }
@@ -250,8 +260,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
   const float friction = d.friction;
   const XcdSplit xs = xcd_split(count);
   for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
-    const float4* row = d.rows + 4 * (size_t)ai;
-    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+    const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
     const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
     const uint32_t word = __float_as_uint(r3.z);
     const bool has_b = body_b != kInvalid;
@@ -349,28 +358,55 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
 // its contact sides stored (wrapping 32-bit sums: the arithmetic of atomicAdd on
 // i32, in any order) and adds them to its velocities. The last pass also writes
 // the body velocities back to the state arrays.
+//
+// kApplyLanes adjacent lanes share a body: each loads every kApplyLanes-th of
+// its slots, all loads in flight at once, and shuffles combine the partial
+// sums. The slot loads of a body cost one memory latency instead of one per slot.
+constexpr uint32_t kApplyLanes = 4;
+constexpr uint32_t kApplyPerLane = (kSlotOverflow + kApplyLanes - 1) / kApplyLanes;
+
 __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
   const XcdSplit xs = xcd_split(d.body_count);
-  for (uint32_t b = xs.first + threadIdx.x; b < xs.end; b += xs.step) {
-    const SolverBody sb = load_body(d, b);
+  const uint32_t per_block = blockDim.x / kApplyLanes;
+  const uint32_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+  const uint32_t blocks = (gridDim.x - xcd + kXcds - 1) / kXcds;
+  const uint32_t part = threadIdx.x % kApplyLanes;
+  for (uint32_t b0 = xs.begin + local * per_block; b0 < xs.end; b0 += blocks * per_block) {
+    const uint32_t b = b0 + threadIdx.x / kApplyLanes;
+    const bool valid = b < xs.end;
     uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
-    const uint32_t count = d.slot_count[b];
+    const uint32_t count = valid ? d.slot_count[b] : 0u;
     const uint32_t n = count < kSlotOverflow ? count : kSlotOverflow;
-    // The body's sides are adjacent slots: independent loads, no index lookup.
-    const int4* p = d.cdelta + 2 * (size_t)b * kMaxSlots;
-    for (uint32_t k = 0; k < n; ++k) {
-      const int4 x = p[2 * k];
-      const int4 y = p[2 * k + 1];
-      acc[0] += (uint32_t)x.x;
-      acc[1] += (uint32_t)x.y;
-      acc[2] += (uint32_t)x.z;
-      acc[3] += (uint32_t)x.w;
-      acc[4] += (uint32_t)y.x;
-      acc[5] += (uint32_t)y.y;
+    int4 x[kApplyPerLane];
+    int2 y[kApplyPerLane];
+#pragma unroll
+    for (uint32_t j = 0; j < kApplyPerLane; ++j) {
+      const uint32_t k = part + j * kApplyLanes;
+      x[j] = make_int4(0, 0, 0, 0);
+      y[j] = make_int2(0, 0);
+      if (k < n) {
+        const int4* e = d.cdelta + 2 * ((size_t)k * d.body_count + b);
+        x[j] = e[0];
+        y[j] = *reinterpret_cast<const int2*>(e + 1);
+      }
     }
+#pragma unroll
+    for (uint32_t j = 0; j < kApplyPerLane; ++j) {
+      acc[0] += (uint32_t)x[j].x;
+      acc[1] += (uint32_t)x[j].y;
+      acc[2] += (uint32_t)x[j].z;
+      acc[3] += (uint32_t)x[j].w;
+      acc[4] += (uint32_t)y[j].x;
+      acc[5] += (uint32_t)y[j].y;
+    }
+#pragma unroll
+    for (uint32_t m = 1; m < kApplyLanes; m <<= 1)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[c] += (uint32_t)__shfl_xor((int)acc[c], (int)m);
+    if (!valid || part != 0u) continue;
     if (count > kSlotOverflow) {
       int32_t* p = d.deltas + 6 * (size_t)b;
 #pragma unroll
@@ -379,6 +415,7 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
         p[c] = 0;
       }
     }
+    const SolverBody sb = load_body(d, b);
     const v3 lin_delta = mk((float)(int32_t)acc[0], (float)(int32_t)acc[1], (float)(int32_t)acc[2]) / kFixedScale;
     const v3 ang_delta = mk((float)(int32_t)acc[3], (float)(int32_t)acc[4], (float)(int32_t)acc[5]) / kFixedScale;
     const v3 lin = sb.lin + lin_delta;
@@ -400,8 +437,7 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
 // matched, and whether the contact is against the ground. Also returns the
 // tangent basis.
 __device__ __forceinline__ void contact_features(const Dev& d, uint32_t ai, float f[kFeatures], v3& e1, v3& e2) {
-  const float4* row = d.rows + 4 * (size_t)ai;
-  const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+  const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
   const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
   const bool has_b = body_b != kInvalid;
   const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
@@ -460,7 +496,7 @@ __global__ __launch_bounds__(256) void k_targets(Dev d) {
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
   const XcdSplit xs = xcd_split(count);
   for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
-    const float4 n = d.rows[4 * (size_t)ai];
+    const float4 n = row_word(d, 0, ai);
     float t1[3], t2[3];
     tangent_basis(n.x, n.y, n.z, t1, t2);
     const v3 f = xyz(d.fsum[ai]);
@@ -532,8 +568,7 @@ __global__ __launch_bounds__(256) void k_residual(Dev d) {
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
   double sum_r2 = 0.0, n = 0.0, sum_pen = 0.0, max_pen = 0.0;
   for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < count; ai += gridDim.x * blockDim.x) {
-    const float4* row = d.rows + 4 * (size_t)ai;
-    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+    const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
     const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
     const bool has_b = body_b != kInvalid;
     const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
@@ -603,7 +638,7 @@ void launch_prepare(hipStream_t s, const Dev& d) {
 void launch_solve(hipStream_t s, const Dev& d, int bearings) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t solve_grid = grid_for(d.pair_capacity, d.t_solve_blocks ? d.t_solve_blocks : 2048u);
-  const uint32_t body_grid = d.body_count ? (d.body_count + kBlock - 1) / kBlock + kXcds : 0;
+  const uint32_t body_grid = d.body_count ? (d.body_count * kApplyLanes + kBlock - 1) / kBlock + kXcds : 0;
   for (uint32_t it = 0; it < iterations; ++it) {
     const int last = it + 1 == iterations ? 1 : 0;
     k_solve<<<solve_grid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last);

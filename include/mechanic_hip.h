@@ -278,9 +278,9 @@ MH_API int mh_batch_scene_range(const mh_physics* physics, uint32_t scene, uint3
 typedef struct {
   uint32_t sort_tile;       /* bitonic: tile width 256..2048, power of two (default 2048) */
   uint32_t sort_levels;     /* bitonic: butterfly levels fused per global launch, 1..4 (default 4) */
-  uint32_t narrow_blocks;   /* workgroups of the narrowphase (default 4096) */
-  uint32_t prepare_blocks;  /* workgroups of contact preparation (default 2048) */
-  uint32_t solve_blocks;    /* workgroups of each solver pass (default 2048) */
+  uint32_t narrow_blocks;   /* workgroups of the narrowphase (default 1024) */
+  uint32_t prepare_blocks;  /* workgroups of contact preparation (default 1024) */
+  uint32_t solve_blocks;    /* workgroups of each solver pass (default 4096) */
   uint32_t scan_blocks;     /* workgroups of the broadphase candidate scan (default 4096) */
   uint32_t sort_kind;       /* 0: radix sort (default), 1: bitonic sort */
   uint32_t traverse_kind;   /* 0: packet walk + separate scan of its leaf ranges (default), 1: one

@@ -27,13 +27,13 @@ SPACE = {
     "traverse_kind": [0, 1, 2],
     "sort_tile": [512, 1024, 2048],
     "sort_levels": [2, 3, 4],
-    "narrow_blocks": [1024, 2048, 4096, 8192],
+    "narrow_blocks": [512, 1024, 2048, 4096],
     "prepare_blocks": [512, 1024, 2048, 4096],
-    "solve_blocks": [512, 1024, 2048, 4096],
+    "solve_blocks": [1024, 2048, 4096, 8192],
     "scan_blocks": [1024, 2048, 4096, 8192],
 }
-DEFAULTS = {"sort_kind": 0, "traverse_kind": 0, "sort_tile": 2048, "sort_levels": 4, "narrow_blocks": 4096, "prepare_blocks": 2048,
-            "solve_blocks": 2048, "scan_blocks": 4096}
+DEFAULTS = {"sort_kind": 0, "traverse_kind": 0, "sort_tile": 2048, "sort_levels": 4, "narrow_blocks": 1024,
+            "prepare_blocks": 1024, "solve_blocks": 4096, "scan_blocks": 4096}
 CACHE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 
 

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_narrowphase(Dev d) {
 
 void launch_narrowphase(hipStream_t s, const Dev& d) {
   uint32_t blocks = (d.pair_capacity + kBlock - 1) / kBlock;
-  const uint32_t cap = d.t_narrow_blocks ? d.t_narrow_blocks : 4096u;
+  const uint32_t cap = d.t_narrow_blocks ? d.t_narrow_blocks : 1024u;
   if (blocks > cap) blocks = cap;
   k_narrowphase<<<blocks, kBlock, 0, s>>>(d);
 }

@@ -629,7 +629,7 @@ static inline uint32_t grid_for(uint32_t n, uint32_t cap) {
 }
 
 void launch_prepare(hipStream_t s, const Dev& d) {
-  k_prepare<<<grid_for(d.pair_capacity, d.t_prepare_blocks ? d.t_prepare_blocks : 2048u), kBlock, 0, s>>>(d);
+  k_prepare<<<grid_for(d.pair_capacity, d.t_prepare_blocks ? d.t_prepare_blocks : 1024u), kBlock, 0, s>>>(d);
 }
 
 // The contact passes of encode_collision_passes (device.rs:1351-1424); with
@@ -637,7 +637,7 @@ void launch_prepare(hipStream_t s, const Dev& d) {
 // dynamic root, device.rs:1373-1416).
 void launch_solve(hipStream_t s, const Dev& d, int bearings) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
-  const uint32_t solve_grid = grid_for(d.pair_capacity, d.t_solve_blocks ? d.t_solve_blocks : 2048u);
+  const uint32_t solve_grid = grid_for(d.pair_capacity, d.t_solve_blocks ? d.t_solve_blocks : 4096u);
   const uint32_t body_grid = d.body_count ? (d.body_count * kApplyLanes + kBlock - 1) / kBlock + kXcds : 0;
   for (uint32_t it = 0; it < iterations; ++it) {
     const int last = it + 1 == iterations ? 1 : 0;

@@ -826,9 +826,9 @@ int mh_physics_get_tuning(const mh_physics* p, mh_tuning* t) {
   t->sort_kind = d.t_sort_kind;
   t->sort_tile = d.t_sort_tile ? d.t_sort_tile : 2048u;
   t->sort_levels = d.t_sort_levels ? d.t_sort_levels : 4u;
-  t->narrow_blocks = d.t_narrow_blocks ? d.t_narrow_blocks : 4096u;
-  t->prepare_blocks = d.t_prepare_blocks ? d.t_prepare_blocks : 2048u;
-  t->solve_blocks = d.t_solve_blocks ? d.t_solve_blocks : 2048u;
+  t->narrow_blocks = d.t_narrow_blocks ? d.t_narrow_blocks : 1024u;
+  t->prepare_blocks = d.t_prepare_blocks ? d.t_prepare_blocks : 1024u;
+  t->solve_blocks = d.t_solve_blocks ? d.t_solve_blocks : 4096u;
   t->scan_blocks = d.t_scan_blocks ? d.t_scan_blocks : 4096u;
   t->traverse_kind = d.t_traverse_kind;
   return MH_OK;

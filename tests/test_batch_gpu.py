@@ -99,7 +99,7 @@ def test_tuning_round_trip_and_validation():
         assert t["sort_tile"] == 2048 and t["sort_kind"] == 0 and t["traverse_kind"] == 0 and t["scan_blocks"] == 4096
         gpu.set_tuning(sort_tile=1024, scan_blocks=64)
         t = gpu.get_tuning()
-        assert t["sort_tile"] == 1024 and t["scan_blocks"] == 64 and t["solve_blocks"] == 2048
+        assert t["sort_tile"] == 1024 and t["scan_blocks"] == 64 and t["solve_blocks"] == 4096
         with pytest.raises(mh.DeviceError):
             gpu.set_tuning(sort_tile=3000)
 

@@ -43,6 +43,80 @@ def algorithmic_bytes(n, c, s, p, k, a, iterations):
 LAUNCHES = {"integrate": 1, "morton": 2, "lbvh": 2, "traverse": 1, "narrowphase": 1, "prepare": 1}
 
 
+def solve_kernel_bytes(n, a):
+    """Compulsory HBM bytes of one contact-solve pass kernel (k_solve): per active
+    contact its 64-byte solver row, its accumulated impulse read and written
+    (4 + 4) and two contact sides' Q20 velocity deltas written (2 x 24); per body
+    its 64-byte solver state read once."""
+    return 120 * a + 64 * n
+
+
+# What FETCH_SIZE and WRITE_SIZE (KiB per dispatch) read for a plain device copy
+# of a known 256 MiB: FETCH_SIZE counts half the bytes read, WRITE_SIZE all the
+# bytes written (rocprofv3 on MI355X; the copy's summary is kept with the profiles).
+FETCH_KIB_SCALE = 2.0
+WRITE_KIB_SCALE = 1.0
+
+
+TRAFFIC_SETTLE = 5  # ticks that rebuild the contact cache after the state is restored
+
+
+def traffic_probe(args):
+    """Child of measure_traffic, under rocprofv3 --pmc restricted to k_solve: the
+    headline run's final body state in a fresh handle, stepped TRAFFIC_SETTLE +
+    profile_ticks ticks (long runs under counter collection crash the profiler)."""
+    import numpy as np
+
+    import mechanic_amd as mh
+
+    scene = mh.named_pile("pile_100k")
+    gpu = mh.GpuPhysics.new_with_config(scene, mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8))
+    gpu.set_tuning(**json.loads(args.tuning_json))
+    state = np.load(args.state)
+    gpu.write_bodies(state["poses"], state["velocities"])
+    gpu.dispatch_ticks(1, TRAFFIC_SETTLE + args.profile_ticks)
+    gpu.synchronize()
+    gpu.close()
+
+
+def measure_traffic(args, tuning, iterations, state):
+    """HBM bytes per k_solve launch: FETCH_SIZE and WRITE_SIZE from two
+    rocprofv3 --pmc runs (child processes) continuing the headline run's final
+    state, averaged over the launches of the last profile_ticks ticks and scaled
+    by the copy calibration."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    from mechanic_amd.rocprof_summary import counter_tail_average
+
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    kib = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+            cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "k_solve", "-d", tmp, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-probe", "--state", state,
+                   "--profile-ticks", str(args.profile_ticks), "--tuning-json", json.dumps(tuning)]
+            env = dict(os.environ, TMPDIR="/tmp")
+            try:
+                # rocprofv3 may end with a signal of its own after the program and
+                # its results are complete: the results database decides.
+                res = subprocess.run(cmd, cwd="/tmp", env=env, timeout=300, stdout=subprocess.DEVNULL,
+                                     stderr=subprocess.PIPE, text=True)
+            except (subprocess.SubprocessError, OSError) as e:
+                return None, f"{counter} run failed: {type(e).__name__}"
+            dbs = [os.path.join(r, f) for r, _, fs in os.walk(tmp) for f in fs if f.endswith("results.db")]
+            value = counter_tail_average(dbs[0], "k_solve", counter, iterations * args.profile_ticks) if dbs else None
+            if value is None:
+                tail = " | ".join(l for l in res.stderr.strip().splitlines()[-3:] if l)[-300:]
+                return None, f"{counter}: no k_solve samples (exit {res.returncode}: {tail})"
+            kib[counter] = value
+    fetch = kib["FETCH_SIZE"] * FETCH_KIB_SCALE * 1024.0
+    write = kib["WRITE_SIZE"] * WRITE_KIB_SCALE * 1024.0
+    return {"bytes": fetch + write, "read": fetch, "written": write}, None
+
+
 def cpu_baseline(scene, friction, iterations, budget_s=15.0, max_ticks=40):
     from tests._oracle import Oracle
 
@@ -222,7 +296,14 @@ def main():
     ap.add_argument("--tuning", choices=("auto", "cached", "default"), default="auto",
                     help="launch shape: autotune now, the committed cache, or the defaults")
     ap.add_argument("--no-articulated", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tuning-json", default="{}", help=argparse.SUPPRESS)
+    ap.add_argument("--state", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus})",
               file=sys.stderr)
@@ -247,7 +328,15 @@ def main():
     make = lambda: mh.GpuPhysics.new_with_config(scene, cfg)  # noqa: E731
     t_tune = time.perf_counter()
     if args.tuning == "auto":
-        tuned = autotune.autotune(make, warm=20, ticks=40)
+        # Tune on the pile as it is while it is timed (after ~300 ticks), not
+        # while it still falls: every trial restores that state.
+        pre = make()
+        pre.dispatch_ticks(1, 300)
+        pre.synchronize()
+        state = pre.read_bodies()
+        pre.close()
+        cached = autotune.cached_tuning(scene.body_count)
+        tuned = autotune.autotune(make, warm=10, ticks=40, state=state, candidates=[cached] if cached else [])
         save_tuning(autotune, scene.body_count, tuned, "100k pile")
     elif args.tuning == "cached":
         tuned = {"tuning": autotune.cached_tuning(scene.body_count) or dict(autotune.DEFAULTS)}
@@ -271,6 +360,7 @@ def main():
     groups = ["integrate_ms", "morton_ms", "sort_ms", "lbvh_ms", "traverse_ms", "narrowphase_ms", "prepare_ms",
               "solve_ms"]
     acc = {g: 0.0 for g in groups}
+    kernel_ms = {"solve_kernel_ms": 0.0, "apply_kernel_ms": 0.0}
     total = 0.0
     counts = []
     first = steps + 2
@@ -280,6 +370,8 @@ def main():
         counts.append((r.pair_count, r.contact_count, r.active_contact_count))
         for g in groups:
             acc[g] += t[g]
+        for g in kernel_ms:
+            kernel_ms[g] += t[g]
         total += t["total_ms"]
     npt = max(args.profile_ticks, 1)
     phase_ms = {g[:-3]: acc[g] / npt for g in groups}
@@ -290,9 +382,15 @@ def main():
     c = len(scene.colliders)
     s = 1 << (c - 1).bit_length()
     algo = algorithmic_bytes(n, c, s, p, k, a, iterations)
-    dominant = max(phase_ms, key=phase_ms.get)
-    dom_ms = phase_ms[dominant]
-    achieved = algo[dominant] / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    # Roofline of the dominant kernel, k_solve (8 launches a tick): compulsory
+    # bytes per launch over its mean launch span in the profiled ticks, stamped
+    # on the device wall clock by the kernel's own workgroups (first start to
+    # last end). HIP events bracket the kernel groups; around a single kernel
+    # they add several microseconds of their own on this runtime.
+    solve_ms = kernel_ms["solve_kernel_ms"] / npt
+    apply_ms = kernel_ms["apply_kernel_ms"] / npt
+    solve_bytes = solve_kernel_bytes(scene.body_count, a)
+    achieved = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     per_phase_gbs = {g: (algo[g] / (phase_ms[g] * 1e-3) / 1e9 if phase_ms[g] > 0 else 0.0) for g in phase_ms}
 
     # Reference-style synchronous engine TPS: dispatch + diagnostics readback per tick.
@@ -303,6 +401,14 @@ def main():
         gpu.dispatch_tick(base + i)
         gpu.read_last_tick()
     engine_tps = sync_ticks / (time.perf_counter() - t1)
+    import tempfile
+
+    import numpy as np
+
+    state_fd, state_path = tempfile.mkstemp(suffix=".npz", dir="/tmp")
+    os.close(state_fd)
+    final_poses, final_velocities = gpu.read_bodies()
+    np.savez(state_path, poses=final_poses, velocities=final_velocities)
     gpu.close()
 
     value = steps / wall
@@ -311,6 +417,9 @@ def main():
     batch = None if args.no_batched else batched(mh, steps)
     art = None if args.no_articulated else articulated(mh, min(steps, 300))
     cpu = None if args.no_cpu else cpu_baseline(scene, friction, iterations)
+    traffic, traffic_note = (None, "skipped (--no-traffic)") if args.no_traffic else \
+        measure_traffic(args, tuned["tuning"], iterations, state_path)
+    os.remove(state_path)
 
     out = {
         "metric": "steps/sec at 100k bodies + ms/phase",
@@ -338,9 +447,16 @@ def main():
         "tuning": tuned["tuning"],
         "steps_per_s_default_tuning": None if default_value is None else round(default_value, 2),
         "tune_s": round(tune_s, 2),
-        "roofline": {"bound": "hbm", "kernel_group": dominant, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "k_solve", "launches_per_tick": iterations,
+                     "algorithmic_bytes_per_launch": round(solve_bytes), "launch_us": round(solve_ms * 1e3, 2),
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None if traffic is None else round(traffic["bytes"]),
+                     "traffic_read": None if traffic is None else round(traffic["read"]),
+                     "traffic_written": None if traffic is None else round(traffic["written"]),
+                     "traffic_note": traffic_note or "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_solve, continuing "
+                                                     "the final state, x2.0 / x1.0 KiB (copy calibration)"},
+        "kernel_us": {"k_solve": round(solve_ms * 1e3, 2), "k_apply": round(apply_ms * 1e3, 2)},
         "cpu_baseline": cpu,
         "learned_warm_start": learned,
         "batched": batch,

@@ -197,6 +197,8 @@ typedef struct {
   double narrowphase_ms, prepare_ms, solve_ms, persist_ms, snapshot_ms, total_ms;
   double mechanism_ms;   /* bearing projection, coordinates, forward kinematics, closures */
   double validation_ms;  /* articulated-state and bearing validation (with the snapshot) */
+  double solve_kernel_ms;  /* mean span of one contact-solve pass kernel (k_solve), device clock */
+  double apply_kernel_ms;  /* mean span of one velocity-gather kernel (k_apply), device clock */
 } mh_phase_timings;
 
 typedef struct {
@@ -229,7 +231,9 @@ MH_API int mh_physics_apply_impulse(mh_physics* physics, uint32_t body, const fl
 MH_API int mh_physics_dispatch_tick(mh_physics* physics, uint64_t tick_index, uint8_t* snapshot_slot);
 /* `count` consecutive dispatch_tick calls starting at `first_tick`. */
 MH_API int mh_physics_dispatch_ticks(mh_physics* physics, uint64_t first_tick, uint32_t count);
-/* Uncaptured tick with HIP events between kernel groups; fills `timings`. */
+/* Uncaptured tick with HIP events between kernel groups (these include host
+ * launch gaps); the solver-pass kernels also stamp the device wall clock at the
+ * start and end of every workgroup, giving their own spans. Fills `timings`. */
 MH_API int mh_physics_profile_tick(mh_physics* physics, uint64_t tick_index, mh_phase_timings* timings);
 /* GpuPhysics::read_last_tick (device.rs:1439-1524): waits for the stream. */
 MH_API int mh_physics_read_last_tick(mh_physics* physics, mh_tick_readback* out);

@@ -71,11 +71,14 @@ def cached_tuning(bodies: int, path: Optional[str] = None) -> Optional[dict]:
     return None if entry is None else dict(entry["tuning"])
 
 
-def measure(make_handle: Callable, tuning: dict, warm: int = 20, ticks: int = 40) -> float:
-    """Milliseconds per tick of a fresh handle with `tuning`."""
+def measure(make_handle: Callable, tuning: dict, warm: int = 20, ticks: int = 40, state=None) -> float:
+    """Milliseconds per tick of a fresh handle with `tuning`, started from
+    `state` = (poses, velocities) when given (contacts re-form in `warm` ticks)."""
     gpu = make_handle()
     try:
         gpu.set_tuning(**tuning)
+        if state is not None:
+            gpu.write_bodies(state[0], state[1])
         gpu.dispatch_ticks(1, warm)
         gpu.synchronize()
         best = float("inf")
@@ -91,13 +94,16 @@ def measure(make_handle: Callable, tuning: dict, warm: int = 20, ticks: int = 40
         gpu.close()
 
 
-def autotune(make_handle: Callable, warm: int = 20, ticks: int = 40, space: Optional[dict] = None) -> dict:
-    """Coordinate descent from the defaults; returns the winner, its time, the
-    default's time and every trial."""
+def autotune(make_handle: Callable, warm: int = 20, ticks: int = 40, space: Optional[dict] = None, state=None,
+             candidates=()) -> dict:
+    """Coordinate descent from the defaults, every trial from `state` when given
+    (the scene as it is when it is measured); then the winner, the defaults and
+    any extra `candidates` are measured again over twice the ticks and the
+    fastest is kept. Returns it, its time, the default's time and every trial."""
     space = space or SPACE
     current = dict(DEFAULTS)
     trials = []
-    base_ms = measure(make_handle, current, warm, ticks)
+    base_ms = measure(make_handle, current, warm, ticks, state)
     trials.append((dict(current), base_ms))
     best_ms = base_ms
     for field, values in space.items():
@@ -105,8 +111,16 @@ def autotune(make_handle: Callable, warm: int = 20, ticks: int = 40, space: Opti
             if value == current[field]:
                 continue
             cand = dict(current, **{field: value})
-            ms = measure(make_handle, cand, warm, ticks)
+            ms = measure(make_handle, cand, warm, ticks, state)
             trials.append((cand, ms))
             if ms < best_ms:
                 best_ms, current = ms, cand
-    return {"tuning": current, "ms": best_ms, "default_ms": base_ms, "trials": trials}
+    finalists = [current, dict(DEFAULTS)] + [dict(DEFAULTS, **c) for c in candidates if c]
+    confirmed = []
+    for cand in finalists:
+        if any(cand == c for c, _ in confirmed):
+            continue
+        confirmed.append((cand, measure(make_handle, cand, warm, 2 * ticks, state)))
+    current, best_ms = min(confirmed, key=lambda x: x[1])
+    default_ms = next(ms for c, ms in confirmed if c == dict(DEFAULTS))
+    return {"tuning": current, "ms": best_ms, "default_ms": default_ms, "trials": trials + confirmed}

@@ -224,7 +224,12 @@ void launch_lbvh(hipStream_t s, const Dev& d);
 void launch_traverse(hipStream_t s, const Dev& d);
 void launch_narrowphase(hipStream_t s, const Dev& d);
 void launch_prepare(hipStream_t s, const Dev& d);
-void launch_solve(hipStream_t s, const Dev& d, int bearings);
+// With `clock`, the pass kernels stamp the device wall clock per workgroup:
+// pass i of k_solve at clock + 2 * i * solve_grid(d) (start, end per block),
+// pass i of k_apply at clock + 2 * (passes * solve_grid(d) + i * apply_grid(d)).
+void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock = nullptr);
+uint32_t solve_grid(const Dev& d);
+uint32_t apply_grid(const Dev& d);
 void launch_features(hipStream_t s, const Dev& d);
 void launch_targets(hipStream_t s, const Dev& d);
 // Articulated path (kernels_mechanism.hip). `in_contact`: velocities live in the

@@ -252,7 +252,8 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
 // (collision.wgsl:841-906) otherwise; the last pass also runs persist_contacts
 // (collision.wgsl:1071-1088). Each contact side stores its six Q20 integers
 // for the bodies' gather; sides past kMaxSlots add theirs with atomics.
-__global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
+__global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint64_t* clock) {
+  if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   float4* rows_now = d.mrows[tick64 & 1];
@@ -351,6 +352,10 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
       rows_now[3 * (size_t)ci + 2].w = lambda;
     }
   }
+  if (clock) {
+    __syncthreads();
+    if (threadIdx.x == 0) clock[2 * blockIdx.x + 1] = wall_clock64();
+  }
 }
 
 // solve_apply (collision.wgsl:1050-1069); skipped when no contact is active
@@ -365,7 +370,8 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last) {
 constexpr uint32_t kApplyLanes = 4;
 constexpr uint32_t kApplyPerLane = (kSlotOverflow + kApplyLanes - 1) / kApplyLanes;
 
-__global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
+__global__ __launch_bounds__(256) void k_apply(Dev d, int last, uint64_t* clock) {
+  if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
@@ -427,6 +433,10 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last) {
       d.lin[b] = f4(lin, 0.0f);
       d.ang[b] = f4(ang, 0.0f);
     }
+  }
+  if (clock) {
+    __syncthreads();
+    if (threadIdx.x == 0) clock[2 * blockIdx.x + 1] = wall_clock64();
   }
 }
 
@@ -635,14 +645,20 @@ void launch_prepare(hipStream_t s, const Dev& d) {
 // The contact passes of encode_collision_passes (device.rs:1351-1424); with
 // `bearings`, one bearing projection follows every pass (a mechanism with a
 // dynamic root, device.rs:1373-1416).
-void launch_solve(hipStream_t s, const Dev& d, int bearings) {
+uint32_t solve_grid(const Dev& d) { return grid_for(d.pair_capacity, d.t_solve_blocks ? d.t_solve_blocks : 4096u); }
+uint32_t apply_grid(const Dev& d) {
+  return d.body_count ? (d.body_count * kApplyLanes + kBlock - 1) / kBlock + kXcds : 0;
+}
+
+void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
-  const uint32_t solve_grid = grid_for(d.pair_capacity, d.t_solve_blocks ? d.t_solve_blocks : 4096u);
-  const uint32_t body_grid = d.body_count ? (d.body_count * kApplyLanes + kBlock - 1) / kBlock + kXcds : 0;
+  const uint32_t sgrid = solve_grid(d), bgrid = apply_grid(d);
   for (uint32_t it = 0; it < iterations; ++it) {
     const int last = it + 1 == iterations ? 1 : 0;
-    k_solve<<<solve_grid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last);
-    if (body_grid) k_apply<<<body_grid, kBlock, 0, s>>>(d, last);
+    k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, clock ? clock + 2 * (size_t)it * sgrid : nullptr);
+    if (bgrid)
+      k_apply<<<bgrid, kBlock, 0, s>>>(
+          d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr);
     if (bearings) launch_bearing_projection(s, d, 1, last);
   }
 }

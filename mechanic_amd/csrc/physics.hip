@@ -44,6 +44,8 @@ struct mh_physics {
   bool has_last_tick = false;
   uint32_t* host_diag = nullptr;  // pinned
   hipEvent_t events[kEvents] = {};
+  uint64_t* kclock = nullptr;  // device: per-workgroup clock stamps of profiled solver passes
+  size_t kclock_words = 0;
   Timing timing;
   const mh_mlp* mlp = nullptr;
   bool mech_active = false;  // some body has a parent (device.rs:2232)
@@ -106,7 +108,7 @@ uint32_t next_pow2(uint32_t v) {
 // Enqueues one tick in the pass order of dispatch_tick (device.rs:886-951).
 // `stop` ends it early for phase tests (1: after integration and the mechanism
 // passes, 2: after the broadphase, 3: after contact generation).
-void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0) {
+void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uint64_t* clock = nullptr) {
   const Dev& d = p->d;
   const bool mech = p->mech_active;
   auto mark = [&](int i) {
@@ -144,7 +146,7 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0) {
     if (d.learn_mode == kLearnLog) launch_features(s, d);
     if (d.learn_mode == kLearnInfer) launch_mlp_infer(s, d);
     mark(kMarkPrepare);
-    launch_solve(s, d, mech && d.has_dynamic_root);
+    launch_solve(s, d, mech && d.has_dynamic_root, clock);
     if (d.learn_mode == kLearnLog) launch_targets(s, d);
     if (d.residual_on) launch_residual(s, d);
     if (mech) {
@@ -842,6 +844,7 @@ void mh_physics_destroy(mh_physics* p) {
   for (auto& e : p->events)
     if (e) (void)hipEventDestroy(e);
   for (void* a : p->allocations) (void)hipFree(a);
+  if (p->kclock) (void)hipFree(p->kclock);
   if (p->host_diag) (void)hipHostFree(p->host_diag);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -893,9 +896,21 @@ int mh_physics_dispatch_ticks(mh_physics* p, uint64_t first_tick, uint32_t count
 
 int mh_physics_profile_tick(mh_physics* p, uint64_t tick, mh_phase_timings* out) {
   if (!p) return MH_ERR_INVALID_ARGUMENT;
+  const uint32_t passes = p->d.solver_iterations ? p->d.solver_iterations : 1u;
+  const size_t sgrid = solve_grid(p->d), bgrid = apply_grid(p->d);
+  const size_t words = 2 * (size_t)passes * (sgrid + bgrid);
+  if (p->collisions && p->kclock_words < words) {
+    if (p->kclock) (void)hipFree(p->kclock);
+    p->kclock = nullptr;
+    p->kclock_words = 0;
+    HIP_TRY_NOERR(hipMalloc(&p->kclock, words * sizeof(uint64_t)));
+    p->kclock_words = words;
+  }
   int rc = begin_tick(p, tick);
   if (rc != MH_OK) return rc;
-  encode_tick(p, p->stream, p->events);
+  uint64_t* clock = p->collisions ? p->kclock : nullptr;
+  if (clock) HIP_TRY_NOERR(hipMemsetAsync(clock, 0, words * sizeof(uint64_t), p->stream));
+  encode_tick(p, p->stream, p->events, 0, clock);
   HIP_TRY_NOERR(hipGetLastError());
   HIP_TRY_NOERR(hipEventSynchronize(p->events[kMarkEnd]));
   float ms[kEvents - 1];
@@ -916,6 +931,34 @@ int mh_physics_profile_tick(mh_physics* p, uint64_t tick, mh_phase_timings* out)
   float total = 0.0f;
   HIP_TRY_NOERR(hipEventElapsedTime(&total, p->events[kMarkStart], p->events[kMarkEnd]));
   t.total_ms = total;
+  if (clock) {
+    // Span of each pass kernel on the device wall clock: earliest workgroup start
+    // to latest workgroup end (workgroups that returned early stamp no end).
+    std::vector<uint64_t> h(words);
+    HIP_TRY_NOERR(hipMemcpy(h.data(), clock, words * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIP_TRY_NOERR(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, p->device));
+    auto mean_span = [&](size_t base, size_t grid) {
+      double sum = 0.0;
+      uint32_t n = 0;
+      for (uint32_t i = 0; i < passes; ++i) {
+        uint64_t lo = ~0ull, hi = 0;
+        for (size_t b = 0; b < grid; ++b) {
+          const uint64_t s0 = h[base + 2 * (i * grid + b)], s1 = h[base + 2 * (i * grid + b) + 1];
+          if (s1 == 0) continue;
+          lo = s0 < lo ? s0 : lo;
+          hi = s1 > hi ? s1 : hi;
+        }
+        if (hi > lo && lo != ~0ull) {
+          sum += (double)(hi - lo);
+          ++n;
+        }
+      }
+      return n && khz > 0 ? sum / n / (double)khz : 0.0;
+    };
+    t.solve_kernel_ms = mean_span(0, sgrid);
+    t.apply_kernel_ms = mean_span(2 * (size_t)passes * sgrid, bgrid);
+  }
   p->timing.t = t;
   p->timing.valid = true;
   if (out) *out = t;

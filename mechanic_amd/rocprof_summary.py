@@ -34,6 +34,25 @@ def counter_stats(cur):
     return out
 
 
+def counter_tail_average(db: str, kernel_part: str, counter: str, last: int):
+    """Mean per-dispatch value of `counter` over the last `last` dispatches of the
+    kernels whose name contains `kernel_part` (dispatch order), or None."""
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    try:
+        cols = [c[1] for c in cur.execute("pragma table_info(counters_collection)").fetchall()]
+        order = next((c for c in ("dispatch_id", "correlation_id", "start") if c in cols), "rowid")
+        rows = cur.execute(
+            f"select value from counters_collection where counter_name = ? and kernel_name like ? "
+            f"order by {order}", (counter, f"%{kernel_part}%")).fetchall()
+    except sqlite3.Error:
+        return None
+    finally:
+        con.close()
+    vals = [r[0] for r in rows][-last:]
+    return sum(vals) / len(vals) if vals else None
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("db")

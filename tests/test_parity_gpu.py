@@ -294,3 +294,22 @@ def test_driver_publishes_and_stops_on_flags():
     gpu.write_bodies(poses, vel)
     driver.update(100_000_000)
     assert runtime.failure() == rt.InvalidNumericState()
+
+
+def test_profiled_ticks_match_dispatched_ticks():
+    """A profiled tick (events between kernel groups, clock stamps in the solver
+    passes) runs the same tick as a dispatched one."""
+    scene = mh.named_pile("pile_1k")
+    out = []
+    for profiled in (False, True):
+        gpu, _ = make(scene, friction=0.6)
+        for tick in range(1, 31):
+            if profiled and 10 <= tick < 20:
+                t = gpu.profile_tick(tick)
+                assert t["total_ms"] > 0 and t["solve_kernel_ms"] > 0 and t["apply_kernel_ms"] > 0
+                assert t["solve_ms"] >= 8 * t["solve_kernel_ms"]
+            else:
+                gpu.dispatch_tick(tick)
+        p, v = gpu.read_bodies()
+        out.append((p.view(np.uint32).copy(), v.view(np.uint32).copy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

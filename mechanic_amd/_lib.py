@@ -11,7 +11,9 @@ import os
 from ctypes import POINTER, c_char, c_double, c_float, c_int, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmechanic_hip.so")
+# The built library; MECHANIC_HIP_LIB names another build of it (an experiment,
+# a different install). There is no fallback: a missing library is an error.
+LIB_PATH = os.environ.get("MECHANIC_HIP_LIB") or os.path.join(HERE, "libmechanic_hip.so")
 
 F4 = c_float * 4
 U4 = c_uint32 * 4

@@ -96,13 +96,13 @@ struct Dev {
   uint32_t* leaf_body;    // body of the collider at each sorted leaf
   uint32_t* top;          // internal nodes whose leaves span several refit workgroups
   float4* block_box;      // 2 per refit workgroup: box of its 256 leaves
+  float4* leaf_scan;      // 4 per leaf: box of its workgroup's leaves up to it (min, max), from it on (min, max)
   uint4* work;            // traversal work items, kItemStage per packet: (packet first leaf, first, last, 0)
   uint32_t work_capacity;
   uint32_t* work_count;   // items recorded per packet
   uint32_t* work_first;   // packets + 1: exclusive prefix of work_count (k_scan_prefix)
   uint32_t* counters;     // per-tick private counters, see CounterWord
   uint32_t* parents;      // reference encoding of node_parents (parent + 1)
-  uint32_t* visits;       // refit arrival counters
   uint2* pairs;
 
   // ---- contacts and solver ----

@@ -368,81 +368,66 @@ __global__ __launch_bounds__(256) void k_radix_pass(Dev d, const uint64_t* src, 
 // A node whose leaves all share one Morton code is a bucket: its left word is
 // its first leaf with kBucketBit, and traversal scans its leaves directly
 // instead of descending the subtree the reference splits by collider index.
+//
+// k_build: one workgroup per kBlock sorted positions builds the internal nodes
+// and the leaves at those positions, and the boxes of the nodes whose leaves
+// all lie in its own kBlock leaves, climbing its subtrees in LDS. The sorted
+// keys within kBlock of its positions are staged in LDS, which holds every
+// probe of a node whose range is shorter than kBlock; the few longer nodes are
+// searched by a whole wavefront, 64 probes a step. k_refit_top then gives the
+// nodes whose leaves span several workgroups their boxes.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int prefix(const uint64_t* keys, int c, int a, int b) {
-  if (b < 0 || b >= c) return -1;
-  return __clzll((long long)(keys[a] ^ keys[b]));
-}
+constexpr int kKeyWindow = 3 * (int)kBlock;  // sorted keys [first - kBlock, first + 2 kBlock)
+constexpr uint16_t kNoParent = 0xffffu;
+constexpr uint16_t kLocalLeaf = 0x8000u;  // local child word: a leaf, else an internal node
 
-// Escape of a node whose range ends at sorted position b (C = leaf count).
-__device__ __forceinline__ uint32_t escape_after(const uint64_t* keys, int c, int b) {
-  if (b >= c - 1) return (uint32_t)c;
-  int n = b + 1;
-  if (n <= c - 2) {
-    // Internal node n spans [n, ...] iff its direction is +1.
-    if (prefix(keys, c, n, n + 1) > prefix(keys, c, n, n - 1)) return (uint32_t)n;
+// Sorted keys, read from the LDS window when the position lies in it.
+struct KeyView {
+  const uint64_t* lds;
+  const uint64_t* keys;
+  int base, c;
+  __device__ __forceinline__ uint64_t at(int j) const {
+    const unsigned w = (unsigned)(j - base);
+    return w < (unsigned)kKeyWindow ? lds[w] : keys[j];
   }
-  return (uint32_t)n | kLeafBit;
-}
-
-// build_topology (lbvh.wgsl:233-278).
-__device__ __forceinline__ bool build_node_impl(Dev& d, uint32_t node);
-__device__ __forceinline__ bool build_node(Dev& d, uint32_t node) { return build_node_impl(d, node); }
-
-__global__ __launch_bounds__(256) void k_topology(Dev d) {
-  __shared__ uint32_t lds_append[kBlock / 64 + 1];
-  const int c = (int)d.collider_count;
-  const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
-  // The grid is exactly ceil((C-1)/256) blocks; every thread reaches the append.
-  bool top = false;
-  if (c > 1 && node < (uint32_t)(c - 1)) top = build_node(d, node);
-  const uint32_t slot = block_append(&d.counters[kCountTop], top, lds_append);
-  if (top) d.top[slot] = node;
-}
-
-// Builds internal node `node`; returns whether its leaves span several refit
-// workgroups (kBlock leaves each).
-__device__ __forceinline__ bool build_node_impl(Dev& d, uint32_t node) {
-  const int c = (int)d.collider_count;
-  const uint64_t* keys = d.keys;
-  const int i = (int)node;
-  const int dir = prefix(keys, c, i, i + 1) > prefix(keys, c, i, i - 1) ? 1 : -1;
-  const int min_prefix = prefix(keys, c, i, i - dir);
-  int max_len = 2;
-  while (prefix(keys, c, i, i + max_len * dir) > min_prefix) max_len *= 2;
-  int len = 0;
-  for (int step = max_len / 2; step >= 1; step /= 2)
-    if (prefix(keys, c, i, i + (len + step) * dir) > min_prefix) len += step;
-  const int other = i + len * dir;
-  const int node_prefix = prefix(keys, c, i, other);
-  int split_off = 0;
-  int step = (len + 1) / 2;
-  while (step >= 1) {
-    if (prefix(keys, c, i, i + (split_off + step) * dir) > node_prefix) split_off += step;
-    if (step == 1) break;
-    step = (step + 1) / 2;
+  // Common prefix length of key `ka` (at some position a != b) and key b; -1 off the ends.
+  __device__ __forceinline__ int prefix(uint64_t ka, int b) const {
+    if (b < 0 || b >= c) return -1;
+    return __clzll((long long)(ka ^ at(b)));
   }
-  const int split = i + split_off * dir + (dir < 0 ? dir : 0);
-  const int first = i < other ? i : other;
-  const int last = i < other ? other : i;
-  const uint32_t leaf_base = (uint32_t)c - 1u;
-  const uint32_t left_ref = split == first ? leaf_base + (uint32_t)split : (uint32_t)split;
-  const uint32_t right_ref = split + 1 == last ? leaf_base + (uint32_t)(split + 1) : (uint32_t)(split + 1);
-  d.children[node] = make_uint2(left_ref, right_ref);
-  d.ranges[node] = make_uint2((uint32_t)first, (uint32_t)last);
-  d.parents[left_ref] = node + 1u;
-  d.parents[right_ref] = node + 1u;
-  if (node == 0) d.parents[0] = 0u;
-  d.visits[node] = 0u;
-  const bool bucket = (keys[first] >> d.key_index_bits) == (keys[last] >> d.key_index_bits);
-  const uint32_t left_enc = split == first ? ((uint32_t)split | kLeafBit) : (uint32_t)split;
-  const uint32_t left_word = bucket ? ((uint32_t)first | kBucketBit) : left_enc;
-  d.inodes[2 * (size_t)node].w = __uint_as_float(left_word);
-  d.inodes[2 * (size_t)node + 1].w = __uint_as_float(escape_after(keys, c, last));
-  return ((uint32_t)first / kBlock) != ((uint32_t)last / kBlock);
+  // Escape of a node whose range ends at sorted position b (C = leaf count).
+  __device__ __forceinline__ uint32_t escape_after(int b) const {
+    if (b >= c - 1) return (uint32_t)c;
+    const int n = b + 1;
+    if (n <= c - 2) {
+      const uint64_t kn = at(n);
+      // Internal node n spans [n, ...] iff its direction is +1.
+      if (prefix(kn, n + 1) > prefix(kn, n - 1)) return (uint32_t)n;
+    }
+    return (uint32_t)n | kLeafBit;
+  }
+};
+
+// Largest x in [lo, hi) with prefix(key i, i + x dir) > threshold, given that it
+// holds at lo: the common prefix of sorted keys with key i only shrinks away from
+// i, so the x where it holds form a run from lo. Each step probes 64 evenly
+// spaced x, one per lane, and keeps the last that holds; this is the value the
+// sequential doubling and halving searches of build_topology find.
+__device__ __forceinline__ int wave_search(const KeyView& kv, uint64_t ki, int i, int dir, int threshold, int lo,
+                                           int hi) {
+  const int lane = (int)__lane_id();
+  while (hi - lo > 1) {
+    const int step = (hi - lo + 63) / 64;
+    const int x = lo + step * (lane + 1);
+    const bool holds = x < hi && kv.prefix(ki, i + x * dir) > threshold;
+    const uint64_t mask = __ballot(holds);
+    const int best = mask ? lo + step * (64 - __clzll((long long)mask)) : lo;
+    hi = best + step < hi ? best + step : hi;
+    lo = best;
+  }
+  return lo;
 }
 
-// prepare_leaves + build_bounds (lbvh.wgsl:280-319), in two launches.
 // Box unions use min/max that order -0 below +0, so a union gives the same bits
 // in any grouping; the tree's pairwise merges and the range unions below agree.
 __device__ __forceinline__ float umin(float a, float b) {
@@ -450,24 +435,6 @@ __device__ __forceinline__ float umin(float a, float b) {
 }
 __device__ __forceinline__ float umax(float a, float b) {
   return a > b ? a : (b > a ? b : (__builtin_signbit(a) ? b : a));
-}
-
-__device__ __forceinline__ void merge_children(const Dev& d, uint32_t parent, uint32_t leaf_base) {
-  uint2 ch = d.children[parent];
-  const float4* a = ch.x >= leaf_base ? d.leaves + 2 * (size_t)(ch.x - leaf_base) : d.inodes + 2 * (size_t)ch.x;
-  const float4* b = ch.y >= leaf_base ? d.leaves + 2 * (size_t)(ch.y - leaf_base) : d.inodes + 2 * (size_t)ch.y;
-  float4 amn = a[0], amx = a[1], bmn = b[0], bmx = b[1];
-  float4* p = d.inodes + 2 * (size_t)parent;
-  float lw = p[0].w, ew = p[1].w;
-  p[0] = make_float4(umin(amn.x, bmn.x), umin(amn.y, bmn.y), umin(amn.z, bmn.z), lw);
-  p[1] = make_float4(umax(amx.x, bmx.x), umax(amx.y, bmx.y), umax(amx.z, bmx.z), ew);
-}
-
-__device__ __forceinline__ uint32_t arrive_workgroup(uint32_t* counter) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  uint32_t arrival = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  return arrival;
 }
 
 struct Box {
@@ -485,6 +452,14 @@ __device__ __forceinline__ void grow(Box& b, float4 mn, float4 mx) {
   b.y1 = umax(b.y1, mx.y);
   b.z1 = umax(b.z1, mx.z);
 }
+__device__ __forceinline__ Box unite(const Box& a, const Box& b) {
+  return Box{umin(a.x0, b.x0), umin(a.y0, b.y0), umin(a.z0, b.z0),
+             umax(a.x1, b.x1), umax(a.y1, b.y1), umax(a.z1, b.z1)};
+}
+__device__ __forceinline__ Box shfl_box(const Box& b, int lane) {
+  return Box{__shfl(b.x0, lane), __shfl(b.y0, lane), __shfl(b.z0, lane),
+             __shfl(b.x1, lane), __shfl(b.y1, lane), __shfl(b.z1, lane)};
+}
 __device__ __forceinline__ Box wave_union(Box b) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
@@ -498,66 +473,202 @@ __device__ __forceinline__ Box wave_union(Box b) {
   return b;
 }
 
-// Leaves, the box of each workgroup's kBlock leaves, and the nodes whose leaves
-// lie inside one workgroup: the second thread to reach such a node merges its
-// children with workgroup-scope ordering only.
-__global__ __launch_bounds__(256) void k_refit(Dev d) {
-  __shared__ Box lds_box[kBlock / 64];
+struct BuildLds {
+  uint64_t keys[kKeyWindow];
+  Box leaf_box[kBlock];
+  Box node_box[kBlock];
+  uint16_t child[kBlock][2];   // in-workgroup nodes: local child words
+  uint16_t leaf_parent[kBlock];
+  uint16_t node_parent[kBlock];
+  uint32_t visit[kBlock];
+  int2 searched[kBlock];       // long nodes: (length, split offset)
+  uint32_t queue[kBlock];
+  uint32_t queued;
+  Box wave_total[kBlock / 64];
+  uint32_t append[kBlock / 64 + 1];
+};
+
+// build_topology + prepare_leaves + build_bounds (lbvh.wgsl:233-319) for the
+// nodes and leaves at sorted positions [blockIdx.x kBlock, +kBlock).
+__global__ __launch_bounds__(256) void k_build(Dev d) {
+  __shared__ BuildLds s;
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const int c = (int)d.collider_count;
-  const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = leaf < (uint32_t)c;
+  const uint32_t t = threadIdx.x;
+  const int lane = (int)(t & 63u), wave = (int)(t >> 6);
+  const int first_pos = (int)(blockIdx.x * kBlock);
+  const int i = first_pos + (int)t;  // this thread's leaf and internal node
+  const int block_last = min(first_pos + (int)kBlock, c) - 1;
+  const KeyView kv{s.keys, d.keys, first_pos - (int)kBlock, c};
+  for (int w = (int)t; w < kKeyWindow; w += (int)kBlock) {
+    const int j = kv.base + w;
+    if (j >= 0 && j < c) s.keys[w] = d.keys[j];
+  }
+  s.leaf_parent[t] = kNoParent;
+  s.node_parent[t] = kNoParent;
+  s.visit[t] = 0u;
+  if (t == 0) s.queued = 0u;
+  const bool leaf_valid = i < c;
+  uint32_t collider = 0;
+  float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn;
+  if (leaf_valid) {
+    collider = (uint32_t)(d.keys[i] & (~0ull >> (64u - d.key_index_bits)));
+    mn = d.aabbs[2 * (size_t)collider];
+    mx = d.aabbs[2 * (size_t)collider + 1];
+  }
+  __syncthreads();
+
+  // Topology. The doubling search stays in the window up to a length of kBlock.
+  const bool node_valid = i < c - 1;
+  int dir = 1, len = 0, split_off = 0;
+  bool searched = false;
+  if (node_valid) {
+    const uint64_t ki = s.keys[t + kBlock];
+    dir = kv.prefix(ki, i + 1) > kv.prefix(ki, i - 1) ? 1 : -1;
+    const int min_prefix = kv.prefix(ki, i - dir);
+    int max_len = 2;
+    while (max_len <= (int)kBlock && kv.prefix(ki, i + max_len * dir) > min_prefix) max_len *= 2;
+    if (max_len > (int)kBlock) {
+      searched = true;
+      s.queue[atomicAdd(&s.queued, 1u)] = t;
+    } else {
+      for (int step = max_len / 2; step >= 1; step /= 2)
+        if (kv.prefix(ki, i + (len + step) * dir) > min_prefix) len += step;
+      const int node_prefix = kv.prefix(ki, i + len * dir);
+      int step = (len + 1) / 2;
+      while (step >= 1) {
+        if (kv.prefix(ki, i + (split_off + step) * dir) > node_prefix) split_off += step;
+        if (step == 1) break;
+        step = (step + 1) / 2;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t queued = s.queued;
+  for (uint32_t q = (uint32_t)wave; q < queued; q += kBlock / 64) {
+    const int tl = (int)s.queue[q];
+    const int n = first_pos + tl;
+    const uint64_t kn = s.keys[tl + (int)kBlock];
+    const int ndir = kv.prefix(kn, n + 1) > kv.prefix(kn, n - 1) ? 1 : -1;
+    const int min_prefix = kv.prefix(kn, n - ndir);
+    const int bound = ndir > 0 ? c - 1 - n : n;
+    const int nlen = wave_search(kv, kn, n, ndir, min_prefix, 0, bound + 1);
+    const int nsplit = wave_search(kv, kn, n, ndir, kv.prefix(kn, n + nlen * ndir), 0, nlen);
+    if (lane == 0) s.searched[tl] = make_int2(nlen, nsplit);
+  }
+  __syncthreads();
+
+  uint32_t left_word = 0, escape = 0;
+  bool in_block = false;
+  if (node_valid) {
+    if (searched) {
+      len = s.searched[t].x;
+      split_off = s.searched[t].y;
+    }
+    const int other = i + len * dir;
+    const int split = i + split_off * dir + (dir < 0 ? dir : 0);
+    const int first = i < other ? i : other;
+    const int last = i < other ? other : i;
+    const uint32_t leaf_base = (uint32_t)c - 1u;
+    const uint32_t left_ref = split == first ? leaf_base + (uint32_t)split : (uint32_t)split;
+    const uint32_t right_ref = split + 1 == last ? leaf_base + (uint32_t)(split + 1) : (uint32_t)(split + 1);
+    d.children[i] = make_uint2(left_ref, right_ref);
+    d.ranges[i] = make_uint2((uint32_t)first, (uint32_t)last);
+    d.parents[left_ref] = (uint32_t)i + 1u;
+    d.parents[right_ref] = (uint32_t)i + 1u;
+    if (i == 0) d.parents[0] = 0u;
+    const bool bucket = (kv.at(first) >> d.key_index_bits) == (kv.at(last) >> d.key_index_bits);
+    const uint32_t left_enc = split == first ? ((uint32_t)split | kLeafBit) : (uint32_t)split;
+    left_word = bucket ? ((uint32_t)first | kBucketBit) : left_enc;
+    escape = kv.escape_after(last);
+    in_block = first >= first_pos && last <= block_last;
+    if (in_block) {
+      const int ls = split - first_pos, rs = split + 1 - first_pos;
+      s.child[t][0] = (uint16_t)(split == first ? (kLocalLeaf | ls) : ls);
+      s.child[t][1] = (uint16_t)(split + 1 == last ? (kLocalLeaf | rs) : rs);
+      if (split == first) s.leaf_parent[ls] = (uint16_t)t;
+      else s.node_parent[ls] = (uint16_t)t;
+      if (split + 1 == last) s.leaf_parent[rs] = (uint16_t)t;
+      else s.node_parent[rs] = (uint16_t)t;
+    }
+  }
   Box box = empty_box();
-  if (valid) {
-    const uint32_t collider = (uint32_t)(d.keys[leaf] & (~0ull >> (64u - d.key_index_bits)));
-    float4 mn = d.aabbs[2 * (size_t)collider];
-    float4 mx = d.aabbs[2 * (size_t)collider + 1];
-    d.leaves[2 * (size_t)leaf] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(collider));
-    d.leaves[2 * (size_t)leaf + 1] =
-        make_float4(mx.x, mx.y, mx.z, __uint_as_float(escape_after(d.keys, c, (int)leaf)));
-    d.leaf_body[leaf] = __float_as_uint(d.colliders[4 * (size_t)collider + 3].x);
+  if (leaf_valid) {
+    d.leaves[2 * (size_t)i] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(collider));
+    d.leaves[2 * (size_t)i + 1] = make_float4(mx.x, mx.y, mx.z, __uint_as_float(kv.escape_after(i)));
+    d.leaf_body[i] = __float_as_uint(d.colliders[4 * (size_t)collider + 3].x);
     grow(box, mn, mx);
   }
-  box = wave_union(box);
-  if ((threadIdx.x & 63u) == 0) lds_box[threadIdx.x >> 6] = box;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    Box b = lds_box[0];
-    for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
-      grow(b, make_float4(lds_box[w].x0, lds_box[w].y0, lds_box[w].z0, 0.0f),
-           make_float4(lds_box[w].x1, lds_box[w].y1, lds_box[w].z1, 0.0f));
+  s.leaf_box[t] = box;
+  // Nodes whose leaves span several workgroups, for k_refit_top. The append's
+  // barriers also publish the LDS links and leaf boxes above.
+  const bool top = node_valid && !in_block;
+  const uint32_t slot = block_append(&d.counters[kCountTop], top, s.append);
+  if (top) d.top[slot] = (uint32_t)i;
+
+  // Refit inside the workgroup (build_bounds): the second thread to reach a
+  // node merges its children and climbs on.
+  if (leaf_valid && c > 1) {
+    uint32_t p = s.leaf_parent[t];
+    for (int level = 0; level < (int)kBlock && p != kNoParent; ++level) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      const uint32_t arrival = __hip_atomic_fetch_add(&s.visit[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      if (arrival == 0u) break;
+      if (arrival > 1u) {
+        atomicOr(&diag[kDiagFlags], 1u);
+        break;
+      }
+      const uint32_t a = s.child[p][0], b = s.child[p][1];
+      const Box ba = (a & kLocalLeaf) ? s.leaf_box[a & ~kLocalLeaf] : s.node_box[a];
+      const Box bb = (b & kLocalLeaf) ? s.leaf_box[b & ~kLocalLeaf] : s.node_box[b];
+      s.node_box[p] = unite(ba, bb);
+      p = s.node_parent[p];
     }
-    d.block_box[2 * (size_t)blockIdx.x] = make_float4(b.x0, b.y0, b.z0, 0.0f);
-    d.block_box[2 * (size_t)blockIdx.x + 1] = make_float4(b.x1, b.y1, b.z1, 0.0f);
   }
-  if (!valid || c <= 1) return;
-  const uint32_t block_first = blockIdx.x * blockDim.x;
-  const uint32_t block_last = min(block_first + blockDim.x, (uint32_t)c) - 1u;
-  const uint32_t leaf_base = (uint32_t)c - 1u;
-  uint32_t current = leaf_base + leaf;
-  // A Karras tree over unique 64-bit keys is at most 64 levels deep.
-  for (int level = 0; level < 128; ++level) {
-    const uint32_t encoded = d.parents[current];
-    if (encoded == 0u) return;
-    const uint32_t parent = encoded - 1u;
-    const uint2 range = d.ranges[parent];
-    if (!(range.x >= block_first && range.y <= block_last)) return;  // k_refit_top's node
-    const uint32_t arrival = arrive_workgroup(&d.visits[parent]);
-    if (arrival == 0u) return;
-    if (arrival > 1u) {
-      atomicOr(&diag[kDiagFlags], 1u);
-      return;
-    }
-    merge_children(d, parent, leaf_base);
-    current = parent;
+
+  // Union of the workgroup's leaves up to and from each leaf, and of all of them.
+  Box pre = box, suf = box;
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const Box up = shfl_box(pre, lane - m >= 0 ? lane - m : lane);
+    const Box down = shfl_box(suf, lane + m < 64 ? lane + m : lane);
+    if (lane >= m) pre = unite(pre, up);
+    if (lane + m < 64) suf = unite(suf, down);
+  }
+  if (lane == 63) s.wave_total[wave] = pre;
+  __syncthreads();
+  Box all = empty_box();
+#pragma unroll
+  for (int w = 0; w < (int)(kBlock / 64); ++w) {
+    const Box tw = s.wave_total[w];
+    if (w < wave) pre = unite(pre, tw);
+    if (w > wave) suf = unite(suf, tw);
+    all = unite(all, tw);
+  }
+  if (leaf_valid) {
+    float4* e = d.leaf_scan + 4 * (size_t)i;
+    e[0] = make_float4(pre.x0, pre.y0, pre.z0, 0.0f);
+    e[1] = make_float4(pre.x1, pre.y1, pre.z1, 0.0f);
+    e[2] = make_float4(suf.x0, suf.y0, suf.z0, 0.0f);
+    e[3] = make_float4(suf.x1, suf.y1, suf.z1, 0.0f);
+  }
+  if (t == 0) {
+    d.block_box[2 * (size_t)blockIdx.x] = make_float4(all.x0, all.y0, all.z0, 0.0f);
+    d.block_box[2 * (size_t)blockIdx.x + 1] = make_float4(all.x1, all.y1, all.z1, 0.0f);
+  }
+  if (node_valid) {
+    const Box nb = in_block ? s.node_box[t] : empty_box();  // k_refit_top fills the others
+    d.inodes[2 * (size_t)i] = make_float4(nb.x0, nb.y0, nb.z0, __uint_as_float(left_word));
+    d.inodes[2 * (size_t)i + 1] = make_float4(nb.x1, nb.y1, nb.z1, __uint_as_float(escape));
   }
 }
 
-// Nodes whose leaves span several workgroups (listed by k_topology): one
-// wavefront per node unions the leaf boxes of its partial first and last
-// workgroups and the boxes of the whole workgroups between them. No node
-// depends on another, so there is no climb and no cross-workgroup handoff.
+// Nodes whose leaves span several workgroups (listed by k_build): one wavefront
+// per node unites the box of its leaves in its first workgroup (a suffix box),
+// in its last (a prefix box) and the boxes of the whole workgroups between.
+// No node depends on another, so there is no climb and no cross-workgroup handoff.
 __global__ __launch_bounds__(256) void k_refit_top(Dev d) {
   const uint32_t n = *reinterpret_cast<volatile uint32_t*>(&d.counters[kCountTop]);
   const uint32_t lane = threadIdx.x & 63u;
@@ -567,11 +678,11 @@ __global__ __launch_bounds__(256) void k_refit_top(Dev d) {
     const uint2 range = d.ranges[node];
     const uint32_t b0 = range.x / kBlock, b1 = range.y / kBlock;
     Box box = empty_box();
-    const uint32_t head_end = (b0 + 1) * kBlock - 1u < range.y ? (b0 + 1) * kBlock - 1u : range.y;
-    for (uint32_t l = range.x + lane; l <= head_end; l += 64u) grow(box, d.leaves[2 * (size_t)l], d.leaves[2 * (size_t)l + 1]);
+    if (lane == 0) {
+      grow(box, d.leaf_scan[4 * (size_t)range.x + 2], d.leaf_scan[4 * (size_t)range.x + 3]);
+      grow(box, d.leaf_scan[4 * (size_t)range.y], d.leaf_scan[4 * (size_t)range.y + 1]);
+    }
     for (uint32_t b = b0 + 1 + lane; b < b1; b += 64u) grow(box, d.block_box[2 * (size_t)b], d.block_box[2 * (size_t)b + 1]);
-    if (b1 > b0)
-      for (uint32_t l = b1 * kBlock + lane; l <= range.y; l += 64u) grow(box, d.leaves[2 * (size_t)l], d.leaves[2 * (size_t)l + 1]);
     box = wave_union(box);
     if (lane == 0) {
       float4* p = d.inodes + 2 * (size_t)node;
@@ -1070,8 +1181,8 @@ void launch_sort(hipStream_t s, const Dev& d) {
 }
 
 void launch_lbvh(hipStream_t s, const Dev& d) {
-  if (d.collider_count > 1) k_topology<<<blocks_for(d.collider_count - 1), kBlock, 0, s>>>(d);
-  k_refit<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
+  if (d.collider_count == 0) return;
+  k_build<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
   if (d.collider_count > 1) k_refit_top<<<256, kBlock, 0, s>>>(d);
 }
 

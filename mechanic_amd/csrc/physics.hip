@@ -545,10 +545,10 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   float* lambda;
   float4* sbody;
   int4* cdelta;
-  uint32_t *roots, *comps, *parents, *visits, *active, *diag;
+  uint32_t *roots, *comps, *parents, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
   uint32_t *leaf_body, *top, *counters;
-  float4* block_box;
+  float4 *block_box, *leaf_scan;
   uint4* work;
   uint64_t *keys, *h0, *h1;
   int32_t* deltas;
@@ -563,7 +563,7 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   ALLOC(inodes, 2 * (size_t)(c > 1 ? c - 1 : 1)); ALLOC(leaves, 2 * (size_t)c);
   ALLOC(children, c > 1 ? c - 1 : 1); ALLOC(parents, c > 0 ? 2 * (size_t)c - 1 : 1);
   ALLOC(ranges, c > 1 ? c - 1 : 1); ALLOC(leaf_body, c);
-  ALLOC(top, c); ALLOC(block_box, 2 * (size_t)((c + kBlock - 1) / kBlock)); ALLOC(counters, kCounterWords);
+  ALLOC(top, c); ALLOC(block_box, 2 * (size_t)((c + kBlock - 1) / kBlock)); ALLOC(leaf_scan, 4 * (size_t)(c > 0 ? c : 1)); ALLOC(counters, kCounterWords);
   const uint32_t packets = (c + 63) / 64;
   d.work_capacity = 64 * packets < 4096 ? 4096 : 64 * packets;
   ALLOC(work, d.work_capacity);
@@ -572,7 +572,6 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   ALLOC(work_first, (size_t)packets + 1);
   d.work_count = work_count;
   d.work_first = work_first;
-  ALLOC(visits, c > 1 ? c - 1 : 1);
   ALLOC(pairs, cfg.pair_capacity);
   ALLOC(contacts, 4 * (size_t)cfg.pair_capacity);
   ALLOC(active, cfg.pair_capacity);
@@ -601,8 +600,8 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   d.root_flags = roots; d.components = comps;
   d.colliders = cols; d.suppressed = supp;
   d.aabbs = aabbs; d.cposes = cposes; d.keys = keys; d.inodes = inodes; d.leaves = leaves;
-  d.children = children; d.parents = parents; d.visits = visits; d.pairs = pairs;
-  d.ranges = ranges; d.leaf_body = leaf_body; d.top = top; d.block_box = block_box; d.work = work; d.counters = counters;
+  d.children = children; d.parents = parents; d.pairs = pairs;
+  d.ranges = ranges; d.leaf_body = leaf_body; d.top = top; d.block_box = block_box; d.leaf_scan = leaf_scan; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
  d.slot_count = slot_count; d.lambda = lambda; d.cdelta = cdelta; d.sbody = sbody;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;

@@ -289,6 +289,8 @@ typedef struct {
   uint32_t sort_kind;       /* 0: radix sort (default), 1: bitonic sort */
   uint32_t traverse_kind;   /* 0: packet walk + separate scan of its leaf ranges (default), 1: one
                                walk per leaf, 2: packet walk scanning its ranges itself */
+  uint32_t solve_kind;      /* 0: each solver pass runs per contact, then a per-body gather of
+                               the stored deltas (default); 1: per body over its contact sides */
 } mh_tuning;
 
 MH_API int mh_physics_set_tuning(mh_physics* physics, const mh_tuning* tuning);

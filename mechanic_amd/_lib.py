@@ -156,7 +156,8 @@ class Config(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, c_uint32) for n in ("sort_tile", "sort_levels", "narrow_blocks", "prepare_blocks",
-                                        "solve_blocks", "scan_blocks", "sort_kind", "traverse_kind")]
+                                        "solve_blocks", "scan_blocks", "sort_kind", "traverse_kind",
+                                        "solve_kind")]
 
 
 class ContactResidual(ctypes.Structure):

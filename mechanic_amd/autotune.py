@@ -25,6 +25,7 @@ from . import _lib
 SPACE = {
     "sort_kind": [0, 1],
     "traverse_kind": [0, 1, 2],
+    "solve_kind": [0, 1],
     "sort_tile": [512, 1024, 2048],
     "sort_levels": [2, 3, 4],
     "narrow_blocks": [512, 1024, 2048, 4096],
@@ -32,8 +33,8 @@ SPACE = {
     "solve_blocks": [1024, 2048, 4096, 8192],
     "scan_blocks": [1024, 2048, 4096, 8192],
 }
-DEFAULTS = {"sort_kind": 0, "traverse_kind": 0, "sort_tile": 2048, "sort_levels": 4, "narrow_blocks": 1024,
-            "prepare_blocks": 1024, "solve_blocks": 4096, "scan_blocks": 4096}
+DEFAULTS = {"sort_kind": 0, "traverse_kind": 0, "solve_kind": 0, "sort_tile": 2048, "sort_levels": 4,
+            "narrow_blocks": 1024, "prepare_blocks": 1024, "solve_blocks": 4096, "scan_blocks": 4096}
 CACHE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 
 

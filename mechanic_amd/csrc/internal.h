@@ -15,6 +15,7 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kPosMask = 0x7fffffffu;
 constexpr uint32_t kBucketBit = 0x40000000u;  // internal node whose leaves share one Morton code
 constexpr uint32_t kBlock = 256;
+constexpr uint32_t kSideTile = 2048;  // bodies per tile of the contact-side offset scan
 // Contact sides a body can gather without atomics per tick; further sides
 // accumulate through the atomic overflow path.
 constexpr uint32_t kMaxSlots = 32;      // contact-side delta slots per body (body-major)
@@ -44,6 +45,7 @@ enum CounterWord : uint32_t {
   kCountFrontier = 0,
   kCountWork = 1,  // unused
   kCountTop = 2,
+  kCountSideTicket = 3,  // tile tickets of the contact-side offset scan
   kCounterWords = 16,
 };
 
@@ -114,6 +116,13 @@ struct Dev {
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)
   int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
   uint32_t* slot_count;   // contact sides per body this tick
+  float4* sbody_alt;      // second solver record buffer: per-body passes read one and write the other
+  uint2* side_slot;       // per active contact: its sides' slots in their bodies (uncapped)
+  uint32_t* side_off;     // per body: first of its contact-side rows (exclusive scan of slot_count)
+  uint32_t* side_status;  // per kSideTile bodies: look-back status of the offset scan
+  uint32_t side_tiles;
+  float4* srow;           // 4 per contact side (2 * pair_capacity sides): side rows, each body's sides contiguous
+  float* slam;            // accumulated normal impulse per contact side (both sides of a contact agree)
 
   // ---- persistent manifolds (ping-pong by tick parity) ----
   uint64_t* htab[2];      // open-addressing index: key -> manifold row
@@ -150,6 +159,7 @@ struct Dev {
   uint32_t t_solve_blocks;    // workgroup cap of the solver passes
   uint32_t t_scan_blocks;     // workgroups of k_scan
   uint32_t t_traverse_kind;   // 0: packet walk + k_scan, 1: one walk per leaf, 2: packet walk scanning inline
+  uint32_t t_solve_kind;      // 0: per-contact passes + per-body delta gather, 1: per-body passes over side rows
 
   // ---- articulated path (bearings, coordinates, forward kinematics, closures) ----
   uint32_t bearing_count;

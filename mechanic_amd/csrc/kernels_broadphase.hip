@@ -240,13 +240,6 @@ __device__ __forceinline__ uint32_t radix_digit(const Dev& d, uint64_t key, uint
   return (uint32_t)(key >> (d.key_index_bits + 8u * pass)) & 0xffu;
 }
 
-__device__ __forceinline__ uint32_t load_status(const uint32_t* w) {
-  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_status(uint32_t* w, uint32_t v) {
-  __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Digit counts of every pass over all keys (rs_hist zeroed by k_morton).
 __global__ __launch_bounds__(1024) void k_radix_hist(Dev d, const uint64_t* src) {
   __shared__ uint32_t hist[kRadixMaxPasses * 256];

@@ -58,9 +58,8 @@ struct SolverBody {
   float inv_mass;
 };
 
-__device__ __forceinline__ SolverBody load_body(const Dev& d, uint32_t b) {
-  const float4* r = d.sbody + 4 * (size_t)b;
-  const float4 a = r[0], c = r[1], e = r[2], f = r[3];
+__device__ __forceinline__ SolverBody unpack_body(const float4 (&w)[4]) {
+  const float4 a = w[0], c = w[1], e = w[2], f = w[3];
   SolverBody s;
   s.lin = xyz(a);
   s.inv_mass = a.w;
@@ -70,6 +69,14 @@ __device__ __forceinline__ SolverBody load_body(const Dev& d, uint32_t b) {
   s.cz = mk(f.y, f.z, f.w);
   return s;
 }
+
+__device__ __forceinline__ SolverBody load_body(const float4* records, uint32_t b) {
+  const float4* r = records + 4 * (size_t)b;
+  const float4 w[4] = {r[0], r[1], r[2], r[3]};
+  return unpack_body(w);
+}
+
+__device__ __forceinline__ SolverBody load_body(const Dev& d, uint32_t b) { return load_body(d.sbody, b); }
 
 // inverse_inertia (collision.wgsl:109-114).
 __device__ __forceinline__ v3 inv_inertia(const SolverBody& s, v3 v) {
@@ -143,6 +150,10 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   float4* rows_now = d.mrows[tick64 & 1];
   const float4* rows_prev = d.mrows[(tick64 + 1) & 1];
   const uint32_t count = read_count(&diag[kDiagContacts], d.pair_capacity);
+  const bool per_body = d.t_solve_kind == 1u;
+  if (per_body)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.side_tiles; i += gridDim.x * blockDim.x)
+      d.side_status[i] = 0u;
   const XcdSplit xs = xcd_split(count);
   for (uint32_t base = xs.first; base < xs.end; base += xs.step) {
     const uint32_t ci = base + threadIdx.x;
@@ -228,15 +239,14 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
     if (active) {
       const uint32_t ci = __float_as_uint(r3.z);
       const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
-      // Give both contact sides a delta slot of their body (kSlotOverflow: add
-      // with atomics instead). Word: contact | slot A << 22 | slot B << 27.
-      uint32_t sa = atomicAdd(&d.slot_count[body_a], 1u);
-      sa = sa < kSlotOverflow ? sa : kSlotOverflow;
-      uint32_t sb = kSlotOverflow;
-      if (body_b != kInvalid) {
-        sb = atomicAdd(&d.slot_count[body_b], 1u);
-        sb = sb < kSlotOverflow ? sb : kSlotOverflow;
-      }
+      // Give both contact sides a slot of their body: their row in the per-body
+      // passes, and in the per-contact passes their delta slot (kSlotOverflow:
+      // add with atomics instead). Word: contact | slot A << 22 | slot B << 27.
+      const uint32_t slot_a = atomicAdd(&d.slot_count[body_a], 1u);
+      const uint32_t slot_b = body_b != kInvalid ? atomicAdd(&d.slot_count[body_b], 1u) : 0u;
+      if (per_body) d.side_slot[a] = make_uint2(slot_a, slot_b);
+      const uint32_t sa = slot_a < kSlotOverflow ? slot_a : kSlotOverflow;
+      const uint32_t sb = body_b != kInvalid && slot_b < kSlotOverflow ? slot_b : kSlotOverflow;
       const uint32_t flags = (sa << 22) | (sb << 27);
       d.active[a] = ci;
       d.lambda[a] = cached_lambda;
@@ -248,91 +258,102 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   }
 }
 
-// warm_start (collision.wgsl:774-839) when first, solve_accumulate
-// (collision.wgsl:841-906) otherwise; the last pass also runs persist_contacts
-// (collision.wgsl:1071-1088). Each contact side stores its six Q20 integers
-// for the bodies' gather; sides past kMaxSlots add theirs with atomics.
+// One contact's impulse for one pass: warm_start (collision.wgsl:774-839) when
+// `first`, solve_accumulate (collision.wgsl:841-906) otherwise, from the bodies'
+// velocities before the pass. Updates the accumulated normal impulse `lambda`,
+// sets the impulse on side B (side A gets its negation) and returns whether one
+// applies. The `owner` of a contact does its learned-warm-start logging.
+struct ContactRow {
+  v3 normal, arm_a, arm_b;
+  float penetration, denominator;
+  bool analytic;
+};
+
+__device__ __forceinline__ bool contact_pass(const Dev& d, const SolverBody& sa, const SolverBody& sb, bool has_b,
+                                             const ContactRow& r, int first, uint32_t ai, bool owner, float& lambda,
+                                             v3& impulse) {
+  const v3 velocity_b = has_b ? sb.lin + cross(sb.ang, r.arm_b) : mk(0.0f, 0.0f, 0.0f);
+  impulse = mk(0.0f, 0.0f, 0.0f);
+  bool applies = false;
+  if (r.denominator > 0.0f) {
+    const v3 relative = velocity_b - (sa.lin + cross(sa.ang, r.arm_a));
+    const float normal_speed = dot(relative, r.normal);
+    if (r.penetration <= 1.0e-5f && normal_speed >= -1.0e-5f && lambda <= 1.0e-6f) {
+      lambda = 0.0f;
+    } else {
+      const float recovery = r.analytic ? 1.0f : 0.2f;
+      const float bias = fmn((fmx(r.penetration - kSlop, 0.0f) * recovery) / d.delta_seconds, kMaxCorrection);
+      const float step = (kRelaxation * (-normal_speed + bias)) / r.denominator;
+      float bound;
+      if (first) {
+        // A learned prediction replaces the halved cached impulse in full.
+        const float scale = (r.analytic || d.learn_mode == kLearnInfer) ? 1.0f : kWarmStartScale;
+        const float warmed = lambda * scale;
+        const float accumulated = fmx(warmed + step, 0.0f);
+        lambda = accumulated;
+        impulse = r.normal * accumulated;
+        bound = accumulated * d.friction;
+      } else {
+        const float previous = fmx(lambda, 0.0f);
+        const float accumulated = fmx(previous + step, 0.0f);
+        const float delta = accumulated - previous;
+        lambda = accumulated;
+        impulse = r.normal * delta;
+        bound = fabsf(delta) * d.friction;
+      }
+      const v3 tangent_velocity = relative - r.normal * normal_speed;
+      const float tangent_speed = length(tangent_velocity);
+      v3 friction_vec = mk(0.0f, 0.0f, 0.0f);
+      if (first && d.learn_mode == kLearnInfer) {
+        // Learned warm start: the predicted friction impulse, inside the cone.
+        const float4 f = d.finit[ai];
+        const float mag = length(xyz(f));
+        const float cap = bound;
+        friction_vec = mag > cap && mag > 0.0f ? xyz(f) * (cap / mag) : xyz(f);
+        impulse = impulse + friction_vec;
+      } else if (tangent_speed > 1.0e-6f) {
+        const v3 tangent = tangent_velocity / tangent_speed;
+        const float tangent_denominator = denom_of(sa, sb, has_b, r.arm_a, r.arm_b, tangent);
+        const float friction_impulse = fmn(tangent_speed / tangent_denominator, bound);
+        friction_vec = -(tangent * friction_impulse);
+        impulse = impulse - tangent * friction_impulse;
+      }
+      if (owner && d.learn_mode == kLearnLog) {
+        const float4 acc = first ? make_float4(0.f, 0.f, 0.f, 0.f) : d.fsum[ai];
+        d.fsum[ai] = make_float4(acc.x + friction_vec.x, acc.y + friction_vec.y, acc.z + friction_vec.z, 0.0f);
+      }
+      applies = true;
+    }
+  }
+  if (owner && first && !applies && d.learn_mode == kLearnLog) d.fsum[ai] = make_float4(0.f, 0.f, 0.f, 0.f);
+  return applies;
+}
+
+// Per-contact pass (tuning solve_kind 0, the default): the contact's row and both bodies; each
+// contact side stores its six Q20 integers for the bodies' gather (k_apply);
+// sides past kSlotOverflow add theirs with atomics. The last pass also runs
+// persist_contacts (collision.wgsl:1071-1088).
 __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint64_t* clock) {
   if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   float4* rows_now = d.mrows[tick64 & 1];
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
-  const float friction = d.friction;
   const XcdSplit xs = xcd_split(count);
   for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
     const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
     const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
     const uint32_t word = __float_as_uint(r3.z);
     const bool has_b = body_b != kInvalid;
-    const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
-    const float penetration = r0.w;
-    const bool analytic = r2.w < 0.0f;
-    const float denominator = r3.w;
+    const ContactRow row{xyz(r0), xyz(r1), xyz(r2), r0.w, r3.w, r2.w < 0.0f};
     float lambda = d.lambda[ai];
-    const SolverBody sa = load_body(d, body_a);
-    SolverBody sb = sa;
-    v3 velocity_b = mk(0.0f, 0.0f, 0.0f);
-    if (has_b) {
-      sb = load_body(d, body_b);
-      velocity_b = sb.lin + cross(sb.ang, arm_b);
-    }
-    v3 impulse = mk(0.0f, 0.0f, 0.0f);
-    bool applies = false;
-    if (denominator > 0.0f) {
-      const v3 relative = velocity_b - (sa.lin + cross(sa.ang, arm_a));
-      const float normal_speed = dot(relative, normal);
-      if (penetration <= 1.0e-5f && normal_speed >= -1.0e-5f && lambda <= 1.0e-6f) {
-        lambda = 0.0f;
-      } else {
-        const float recovery = analytic ? 1.0f : 0.2f;
-        const float bias = fmn((fmx(penetration - kSlop, 0.0f) * recovery) / d.delta_seconds, kMaxCorrection);
-        const float step = (kRelaxation * (-normal_speed + bias)) / denominator;
-        float bound;
-        if (first) {
-          // A learned prediction replaces the halved cached impulse in full.
-          const float scale = (analytic || d.learn_mode == kLearnInfer) ? 1.0f : kWarmStartScale;
-          const float warmed = lambda * scale;
-          const float accumulated = fmx(warmed + step, 0.0f);
-          lambda = accumulated;
-          impulse = normal * accumulated;
-          bound = accumulated * friction;
-        } else {
-          const float previous = fmx(lambda, 0.0f);
-          const float accumulated = fmx(previous + step, 0.0f);
-          const float delta = accumulated - previous;
-          lambda = accumulated;
-          impulse = normal * delta;
-          bound = fabsf(delta) * friction;
-        }
-        const v3 tangent_velocity = relative - normal * normal_speed;
-        const float tangent_speed = length(tangent_velocity);
-        v3 friction_vec = mk(0.0f, 0.0f, 0.0f);
-        if (first && d.learn_mode == kLearnInfer) {
-          // Learned warm start: the predicted friction impulse, inside the cone.
-          const float4 f = d.finit[ai];
-          const float mag = length(xyz(f));
-          const float cap = bound;
-          friction_vec = mag > cap && mag > 0.0f ? xyz(f) * (cap / mag) : xyz(f);
-          impulse = impulse + friction_vec;
-        } else if (tangent_speed > 1.0e-6f) {
-          const v3 tangent = tangent_velocity / tangent_speed;
-          const float tangent_denominator = denom_of(sa, sb, has_b, arm_a, arm_b, tangent);
-          const float friction_impulse = fmn(tangent_speed / tangent_denominator, bound);
-          friction_vec = -(tangent * friction_impulse);
-          impulse = impulse - tangent * friction_impulse;
-        }
-        if (d.learn_mode == kLearnLog) {
-          const float4 acc = first ? make_float4(0.f, 0.f, 0.f, 0.f) : d.fsum[ai];
-          d.fsum[ai] = make_float4(acc.x + friction_vec.x, acc.y + friction_vec.y, acc.z + friction_vec.z, 0.0f);
-        }
-        applies = true;
-      }
-    }
+    const SolverBody sa = load_body(d.sbody, body_a);
+    const SolverBody sb = has_b ? load_body(d.sbody, body_b) : sa;
+    v3 impulse;
+    const bool applies = contact_pass(d, sa, sb, has_b, row, first, ai, true, lambda, impulse);
     d.lambda[ai] = lambda;
-    if (first && !applies && d.learn_mode == kLearnLog) d.fsum[ai] = make_float4(0.f, 0.f, 0.f, 0.f);
     // A contact that applies no impulse adds zero deltas (collision.wgsl:793-804).
-    const Q20 qa = applies ? side_delta(sa, arm_a, -impulse) : Q20{};
+    const Q20 qa = applies ? side_delta(sa, row.arm_a, -impulse) : Q20{};
     const uint32_t slot_a = (word >> 22) & 31u, slot_b = word >> 27;
     if (slot_a == kSlotOverflow) {
       if (applies) add_overflow(d, body_a, qa);
@@ -340,7 +361,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint6
       put_side(d, body_a, slot_a, qa);
     }
     if (has_b) {
-      const Q20 qb = applies ? side_delta(sb, arm_b, impulse) : Q20{};
+      const Q20 qb = applies ? side_delta(sb, row.arm_b, impulse) : Q20{};
       if (slot_b == kSlotOverflow) {
         if (applies) add_overflow(d, body_b, qb);
       } else {
@@ -356,6 +377,240 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint6
     __syncthreads();
     if (threadIdx.x == 0) clock[2 * blockIdx.x + 1] = wall_clock64();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Per-body passes (tuning solve_kind 1). Once per tick each active contact's two
+// sides are laid out as rows grouped by body (k_side_offsets scans the bodies'
+// side counts, k_scatter_sides writes the rows); each pass then runs per body:
+// its lanes take its sides, compute each contact's impulse exactly as the
+// per-contact pass does (both sides of a contact see the same inputs, so they
+// agree on it and on the accumulated impulse bit for bit), sum the body's own
+// Q20 deltas (wrapping 32-bit sums: the arithmetic of atomicAdd on i32, in any
+// order) and write the body's new velocities into the other record buffer.
+// Nothing is stored per side but the accumulated impulse.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSidePer = kSideTile / kBlock;  // bodies per thread of the offset scan
+constexpr uint32_t kSideAggregate = 1u << 30;      // status word: the tile's own side count
+constexpr uint32_t kSidePrefix = 2u << 30;         // status word: side count over tiles 0..this one
+constexpr uint32_t kSideCount = (1u << 30) - 1u;
+constexpr uint32_t kSideSpinLimit = 1u << 22;      // reads of an unpublished word before giving up
+
+// Exclusive scan of slot_count into side_off (decoupled look-back over tiles of
+// kSideTile bodies, taken in ticket order so a tile only waits for tiles that
+// started before it).
+__global__ __launch_bounds__(256) void k_side_offsets(Dev d) {
+  __shared__ uint32_t lds_tile, lds_before, lds_wave[kBlock / 64];
+  if (threadIdx.x == 0) lds_tile = atomicAdd(&d.counters[kCountSideTicket], 1u);
+  __syncthreads();
+  const uint32_t tile = lds_tile;
+  const uint32_t n = d.body_count;
+  const uint32_t base = tile * kSideTile + threadIdx.x * kSidePer;
+  uint32_t c[kSidePer];
+  uint32_t run = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kSidePer; ++j) {
+    c[j] = base + j < n ? d.slot_count[base + j] : 0u;
+    run += c[j];
+  }
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t incl = run;
+#pragma unroll
+  for (uint32_t m = 1; m < 64; m <<= 1) {
+    const uint32_t up = (uint32_t)__shfl_up((int)incl, m);
+    if (lane >= m) incl += up;
+  }
+  if (lane == 63) lds_wave[wave] = incl;
+  __syncthreads();
+  uint32_t wave_before = 0, total = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kBlock / 64; ++w) {
+    wave_before += w < wave ? lds_wave[w] : 0u;
+    total += lds_wave[w];
+  }
+  if (threadIdx.x == 0) {
+    uint32_t* status = d.side_status;
+    uint32_t before = 0;
+    if (tile == 0) {
+      store_status(&status[0], kSidePrefix | total);
+    } else {
+      store_status(&status[tile], kSideAggregate | total);
+      uint32_t j = tile - 1u, spins = 0;
+      for (;;) {
+        const uint32_t v = load_status(&status[j]);
+        if ((v & ~kSideCount) == 0u) {
+          if (++spins >= kSideSpinLimit) {
+            atomicOr(&d.diag[tick_sv(d) & 1][kDiagFlags], 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        before += v & kSideCount;
+        if (v & kSidePrefix) break;
+        --j;  // tile 0 always publishes a prefix, so j stays >= 0
+      }
+      store_status(&status[tile], kSidePrefix | (before + total));
+    }
+    lds_before = before;
+  }
+  __syncthreads();
+  uint32_t off = lds_before + wave_before + incl - run;
+#pragma unroll
+  for (uint32_t j = 0; j < kSidePer; ++j) {
+    if (base + j < n) d.side_off[base + j] = off;
+    off += c[j];
+  }
+}
+
+
+// Both sides of each active contact as rows of its bodies: (normal, penetration),
+// (arm A, denominator), (arm B, analytic flag), (other body, active index,
+// contact index, side), and the accumulated impulse the first pass starts from.
+__global__ __launch_bounds__(256) void k_scatter_sides(Dev d) {
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
+  const XcdSplit xs = xcd_split(count);
+  for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
+    const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
+    const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+    const uint32_t ci = __float_as_uint(r3.z) & kContactMask;
+    const uint2 slots = d.side_slot[ai];
+    const float lambda = d.lambda[ai];
+    const float4 w1 = make_float4(r1.x, r1.y, r1.z, r3.w);
+    float4* ra = d.srow + 4 * (size_t)(d.side_off[body_a] + slots.x);
+    ra[0] = r0;
+    ra[1] = w1;
+    ra[2] = r2;
+    ra[3] = make_float4(r3.y, __uint_as_float(ai), __uint_as_float(ci), __uint_as_float(0u));
+    d.slam[d.side_off[body_a] + slots.x] = lambda;
+    if (body_b != kInvalid) {
+      float4* rb = d.srow + 4 * (size_t)(d.side_off[body_b] + slots.y);
+      rb[0] = r0;
+      rb[1] = w1;
+      rb[2] = r2;
+      rb[3] = make_float4(r3.x, __uint_as_float(ai), __uint_as_float(ci), __uint_as_float(1u));
+      d.slam[d.side_off[body_b] + slots.y] = lambda;
+    }
+  }
+}
+
+// One solver pass over groups of kGroupBodies consecutive bodies, one workgroup
+// per group: solve_accumulate / warm_start for every contact side of the group
+// and solve_apply (collision.wgsl:1050-1069) for its bodies, reading the records
+// `src` and writing `dst`. The group's sides are contiguous, so the threads take
+// them in turn, one each, with every load of a side independent of the others;
+// a side finds its body by bisecting the group's offsets and adds its Q20
+// integers to the body's sums in LDS. Side A owns the contact's logging and, in
+// the last pass, persist_contacts (collision.wgsl:1071-1088) and the accumulated
+// impulse per active contact. With no active contact the pass copies the records
+// (solve_apply is skipped, finalize_active_contacts, collision.wgsl:744).
+constexpr uint32_t kGroupBodies = 16;
+
+__global__ __launch_bounds__(256) void k_solve_bodies(Dev d, const float4* __restrict__ src, float4* __restrict__ dst,
+                                                      int first, int last, uint64_t* clock) {
+  __shared__ float4 lds_own[kGroupBodies][4];
+  __shared__ uint32_t lds_off[kGroupBodies + 1];
+  __shared__ int32_t lds_acc[6][kGroupBodies];
+  if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  float4* rows_now = d.mrows[tick64 & 1];
+  const bool any = *reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) != 0u;
+  const uint32_t n = d.body_count;
+  const uint32_t groups = (n + kGroupBodies - 1) / kGroupBodies;
+  // Each XCD takes a contiguous run of groups, so the bodies its sides gather
+  // stay in its L2.
+  const uint32_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+  const uint32_t blocks = (gridDim.x - xcd + kXcds - 1) / kXcds;
+  const uint32_t g_end = (uint32_t)(((uint64_t)groups * (xcd + 1)) / kXcds);
+  const uint32_t t = threadIdx.x;
+  for (uint32_t g = (uint32_t)(((uint64_t)groups * xcd) / kXcds) + local; g < g_end; g += blocks) {
+    const uint32_t b0 = g * kGroupBodies;
+    const uint32_t nb = n - b0 < kGroupBodies ? n - b0 : kGroupBodies;
+    if (t < kGroupBodies * 4) {
+      const uint32_t j = t >> 2, w = t & 3u;
+      if (j < nb) lds_own[j][w] = src[4 * (size_t)(b0 + j) + w];
+    }
+    if (t < kGroupBodies) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) lds_acc[c][t] = 0;
+    }
+    if (t < nb) lds_off[t] = d.side_off[b0 + t];
+    if (t == 0) lds_off[nb] = d.side_off[b0 + nb - 1] + d.slot_count[b0 + nb - 1];  // nb >= 1
+    __syncthreads();
+    if (any) {
+      const uint32_t side_first = lds_off[0], side_end = lds_off[nb];
+      for (uint32_t s = side_first + t; s < side_end; s += blockDim.x) {
+        // The body whose sides hold s: the last j with lds_off[j] <= s.
+        uint32_t lo = 0, hi = nb;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (lds_off[mid] <= s) lo = mid;
+          else hi = mid;
+        }
+        const float4* row_s = d.srow + 4 * (size_t)s;
+        const float4 w0 = row_s[0], w1 = row_s[1], w2 = row_s[2], w3 = row_s[3];
+        float lambda = d.slam[s];
+        const uint32_t other = __float_as_uint(w3.x), ai = __float_as_uint(w3.y);
+        const bool side_b = __float_as_uint(w3.w) != 0u;
+        const bool has_b = side_b || other != kInvalid;
+        const float4 ow[4] = {lds_own[lo][0], lds_own[lo][1], lds_own[lo][2], lds_own[lo][3]};
+        const SolverBody so = unpack_body(ow);
+        const SolverBody ob = other != kInvalid ? load_body(src, other) : so;
+        const SolverBody& sa = side_b ? ob : so;
+        const SolverBody& sb = side_b ? so : ob;
+        const ContactRow row{xyz(w0), xyz(w1), xyz(w2), w0.w, w1.w, w2.w < 0.0f};
+        v3 impulse;
+        const bool applies = contact_pass(d, sa, sb, has_b, row, first, ai, !side_b, lambda, impulse);
+        d.slam[s] = lambda;
+        if (!side_b && last) {
+          d.lambda[ai] = lambda;
+          rows_now[3 * (size_t)__float_as_uint(w3.z) + 2].w = lambda;
+        }
+        if (applies) {
+          const Q20 q = side_b ? side_delta(sb, row.arm_b, impulse) : side_delta(sa, row.arm_a, -impulse);
+#pragma unroll
+          for (int c = 0; c < 6; ++c) atomicAdd(&lds_acc[c][lo], q.v[c]);
+        }
+      }
+    }
+    __syncthreads();
+    if (t < nb) {
+      float4* r = dst + 4 * (size_t)(b0 + t);
+      const float4 w0 = lds_own[t][0], w1 = lds_own[t][1];
+      if (!any) {
+        r[0] = w0;
+        r[1] = w1;
+      } else {
+        const v3 lin_delta = mk((float)lds_acc[0][t], (float)lds_acc[1][t], (float)lds_acc[2][t]) / kFixedScale;
+        const v3 ang_delta = mk((float)lds_acc[3][t], (float)lds_acc[4][t], (float)lds_acc[5][t]) / kFixedScale;
+        const v3 lin = xyz(w0) + lin_delta;
+        const v3 ang = xyz(w1) + ang_delta;
+        r[0] = f4(lin, w0.w);
+        r[1] = f4(ang, w1.w);
+        if (last) {
+          d.lin[b0 + t] = f4(lin, 0.0f);
+          d.ang[b0 + t] = f4(ang, 0.0f);
+        }
+      }
+      r[2] = lds_own[t][2];
+      r[3] = lds_own[t][3];
+    }
+    __syncthreads();  // the next group reuses the LDS
+  }
+  if (clock) {
+    __syncthreads();
+    if (threadIdx.x == 0) clock[2 * blockIdx.x + 1] = wall_clock64();
+  }
+}
+
+// The records after an odd number of per-body passes, back into sbody.
+__global__ __launch_bounds__(256) void k_copy_records(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                      uint32_t words) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) dst[i] = src[i];
 }
 
 // solve_apply (collision.wgsl:1050-1069); skipped when no contact is active
@@ -645,21 +900,52 @@ void launch_prepare(hipStream_t s, const Dev& d) {
 // The contact passes of encode_collision_passes (device.rs:1351-1424); with
 // `bearings`, one bearing projection follows every pass (a mechanism with a
 // dynamic root, device.rs:1373-1416).
-uint32_t solve_grid(const Dev& d) { return grid_for(d.pair_capacity, d.t_solve_blocks ? d.t_solve_blocks : 4096u); }
+uint32_t solve_grid(const Dev& d) {
+  const uint32_t cap = d.t_solve_blocks ? d.t_solve_blocks : 4096u;
+  if (d.t_solve_kind != 1u) return grid_for(d.pair_capacity, cap);
+  if (d.body_count == 0) return 0;
+  const uint32_t b = (d.body_count + kGroupBodies - 1) / kGroupBodies + kXcds;
+  return b < cap ? b : cap;
+}
 uint32_t apply_grid(const Dev& d) {
+  if (d.t_solve_kind == 1u) return 0;
   return d.body_count ? (d.body_count * kApplyLanes + kBlock - 1) / kBlock + kXcds : 0;
 }
 
 void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t sgrid = solve_grid(d), bgrid = apply_grid(d);
+  if (d.t_solve_kind != 1u) {
+    for (uint32_t it = 0; it < iterations; ++it) {
+      const int last = it + 1 == iterations ? 1 : 0;
+      k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, clock ? clock + 2 * (size_t)it * sgrid : nullptr);
+      if (bgrid)
+        k_apply<<<bgrid, kBlock, 0, s>>>(
+            d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr);
+      if (bearings) launch_bearing_projection(s, d, 1, last);
+    }
+    return;
+  }
+  if (d.body_count == 0) return;
+  k_side_offsets<<<d.side_tiles, kBlock, 0, s>>>(d);
+  k_scatter_sides<<<grid_for(d.pair_capacity, d.t_prepare_blocks ? d.t_prepare_blocks : 1024u), kBlock, 0, s>>>(d);
+  float4* records[2] = {d.sbody, d.sbody_alt};
   for (uint32_t it = 0; it < iterations; ++it) {
     const int last = it + 1 == iterations ? 1 : 0;
-    k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, clock ? clock + 2 * (size_t)it * sgrid : nullptr);
-    if (bgrid)
-      k_apply<<<bgrid, kBlock, 0, s>>>(
-          d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr);
-    if (bearings) launch_bearing_projection(s, d, 1, last);
+    float4* src = records[it & 1u];
+    float4* dst = records[(it + 1u) & 1u];
+    k_solve_bodies<<<sgrid, kBlock, 0, s>>>(d, src, dst, it == 0 ? 1 : 0, last,
+                                            clock ? clock + 2 * (size_t)it * sgrid : nullptr);
+    if (bearings) {
+      // The bearing projection works on the records the pass just wrote.
+      Dev dk = d;
+      dk.sbody = dst;
+      launch_bearing_projection(s, dk, 1, last);
+    }
+  }
+  if (iterations & 1u) {
+    const uint32_t words = 4 * d.body_count;
+    k_copy_records<<<(words + kBlock - 1) / kBlock, kBlock, 0, s>>>(d.sbody_alt, d.sbody, words);
   }
 }
 

@@ -134,6 +134,14 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
 // thread of the block must call it (block-uniform control flow); `lds` holds
 // kBlock / 64 + 1 words. Returns the row index (may be >= capacity), or
 // kInvalid without `pred`.
+// Look-back status words of the single-pass scans, read and written at agent scope.
+__device__ __forceinline__ uint32_t load_status(const uint32_t* w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_status(uint32_t* w, uint32_t v) {
+  __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, uint32_t* lds) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = threadIdx.x >> 6;

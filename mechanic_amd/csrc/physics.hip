@@ -543,7 +543,10 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
   float* inv_mass;
   float* lambda;
-  float4* sbody;
+  float4 *sbody, *sbody_alt, *srow;
+  float* slam;
+  uint2* side_slot;
+  uint32_t *side_off, *side_status;
   int4* cdelta;
   uint32_t *roots, *comps, *parents, *active, *diag;
   uint2 *supp, *children, *pairs, *ranges;
@@ -578,7 +581,11 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   ALLOC(rows, 4 * (size_t)cfg.pair_capacity);
   ALLOC(lambda, cfg.pair_capacity);
   ALLOC(cdelta, 2 * (size_t)kMaxSlots * n);
-  ALLOC(sbody, 4 * (size_t)n);
+  ALLOC(sbody, 4 * (size_t)n); ALLOC(sbody_alt, 4 * (size_t)n);
+  d.side_tiles = (n + kSideTile - 1) / kSideTile;
+  ALLOC(side_off, n); ALLOC(side_status, d.side_tiles ? d.side_tiles : 1u);
+  ALLOC(side_slot, cfg.pair_capacity);
+  ALLOC(srow, 8 * (size_t)cfg.pair_capacity); ALLOC(slam, 2 * (size_t)cfg.pair_capacity);
   ALLOC(deltas, 6 * (size_t)n);
   ALLOC(slot_count, n);
   ALLOC(h0, cfg.pair_capacity); ALLOC(h1, cfg.pair_capacity);
@@ -604,6 +611,8 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   d.ranges = ranges; d.leaf_body = leaf_body; d.top = top; d.block_box = block_box; d.leaf_scan = leaf_scan; d.work = work; d.counters = counters;
   d.contacts = contacts; d.active = active; d.rows = rows; d.deltas = deltas;
  d.slot_count = slot_count; d.lambda = lambda; d.cdelta = cdelta; d.sbody = sbody;
+  d.sbody_alt = sbody_alt; d.side_off = side_off; d.side_status = side_status; d.side_slot = side_slot;
+  d.srow = srow; d.slam = slam;
   d.htab[0] = h0; d.htab[1] = h1; d.mrows[0] = m0; d.mrows[1] = m1;
   d.diag[0] = diag; d.diag[1] = diag + kDiagWords;
   d.tick = tick;
@@ -799,7 +808,8 @@ int mh_physics_set_tuning(mh_physics* p, const mh_tuning* t) {
   if (!p || !t) return MH_ERR_INVALID_ARGUMENT;
   if (t->sort_tile && (t->sort_tile < 256 || t->sort_tile > 2048 || (t->sort_tile & (t->sort_tile - 1))))
     return MH_ERR_INVALID_ARGUMENT;
-  if (t->sort_levels > 4 || t->sort_kind > 1 || t->traverse_kind > 2) return MH_ERR_INVALID_ARGUMENT;
+  if (t->sort_levels > 4 || t->sort_kind > 1 || t->traverse_kind > 2 || t->solve_kind > 1)
+    return MH_ERR_INVALID_ARGUMENT;
   if (t->narrow_blocks > (1u << 16) || t->prepare_blocks > (1u << 16) || t->solve_blocks > (1u << 16) ||
       t->scan_blocks > (1u << 16))
     return MH_ERR_INVALID_ARGUMENT;
@@ -813,6 +823,7 @@ int mh_physics_set_tuning(mh_physics* p, const mh_tuning* t) {
   d.t_solve_blocks = t->solve_blocks;
   d.t_scan_blocks = t->scan_blocks;
   d.t_traverse_kind = t->traverse_kind;
+  d.t_solve_kind = t->solve_kind;
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->graph_exec = nullptr;
@@ -832,6 +843,7 @@ int mh_physics_get_tuning(const mh_physics* p, mh_tuning* t) {
   t->solve_blocks = d.t_solve_blocks ? d.t_solve_blocks : 4096u;
   t->scan_blocks = d.t_scan_blocks ? d.t_scan_blocks : 4096u;
   t->traverse_kind = d.t_traverse_kind;
+  t->solve_kind = d.t_solve_kind;
   return MH_OK;
 }
 

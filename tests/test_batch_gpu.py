@@ -78,11 +78,11 @@ def test_tuning_never_changes_results():
 
 
 @pytest.mark.parametrize("tuning", [dict(sort_kind=1), dict(traverse_kind=1), dict(traverse_kind=2),
-                                    dict(sort_kind=1, traverse_kind=1)])
+                                    dict(sort_kind=1, traverse_kind=1), dict(solve_kind=1)])
 def test_broadphase_variants_match_default(tuning):
-    """The key sorts and pair walks are drop-ins for each other: same pairs,
-    bit-identical states (scene-keyed batch keys are the widest keys, and every
-    walk must stay inside each scene)."""
+    """The key sorts, pair walks and solver pass organisations are drop-ins for
+    each other: same pairs, bit-identical states (scene-keyed batch keys are the
+    widest keys, and every walk must stay inside each scene)."""
     scenes = batch_scenes(count=8, smallest=64, largest=1200, size_seed=9)
     base = run_batch(scenes, 30)
     other = run_batch(scenes, 30, tuning)
@@ -97,6 +97,7 @@ def test_tuning_round_trip_and_validation():
     with GpuPhysics.new_with_config(scene, CFG) as gpu:
         t = gpu.get_tuning()
         assert t["sort_tile"] == 2048 and t["sort_kind"] == 0 and t["traverse_kind"] == 0 and t["scan_blocks"] == 4096
+        assert t["solve_kind"] == 0
         gpu.set_tuning(sort_tile=1024, scan_blocks=64)
         t = gpu.get_tuning()
         assert t["sort_tile"] == 1024 and t["scan_blocks"] == 64 and t["solve_blocks"] == 4096

@@ -24,6 +24,10 @@ INVALID = 0xFFFFFFFF
 # Every variant must give the same keys, tree and pair set.
 BROADPHASE = {"radix-scan": {}, "bitonic": {"sort_kind": 1}, "lane": {"traverse_kind": 1},
               "bitonic-lane": {"sort_kind": 1, "traverse_kind": 1}, "inline": {"traverse_kind": 2}}
+# Solver pass organisations: per contact with a per-body gather of the stored
+# deltas (default), or per body over its contact-side rows. Both must give the
+# oracle's bits.
+SOLVER = {"per-contact": {}, "per-body": {"solve_kind": 1}}
 
 
 def make(scene, **cfg):
@@ -152,10 +156,11 @@ def test_contacts_bit_exact(ticks):
     compare_contacts(gpu, ora)
 
 
+@pytest.mark.parametrize("solver", list(SOLVER))
 @pytest.mark.parametrize("friction,iterations", [(0.05, 8), (0.6, 8), (0.6, 3), (0.05, 1)])
-def test_full_step_pile_1k_bit_exact(friction, iterations):
+def test_full_step_pile_1k_bit_exact(friction, iterations, solver):
     scene = mh.named_pile("pile_1k")
-    gpu, ora = make(scene, friction=friction, solver_iterations=iterations)
+    gpu, ora = make(scene, friction=friction, solver_iterations=iterations, tuning=SOLVER[solver])
     for tick in range(1, 301):
         gpu.dispatch_tick(tick)
         ora.step(tick)
@@ -261,18 +266,40 @@ def test_pile_100k_first_tick_broadphase_and_contacts(variant):
     compare_contacts(gpu, ora)
 
 
-@pytest.mark.parametrize("variant", ["radix-scan", "bitonic-lane"])
+@pytest.mark.parametrize("variant", ["radix-scan", "bitonic-lane", "per-body"])
 @pytest.mark.parametrize("count", [2, 3, 5, 16, 17, 33])
 def test_small_stacks_many_ticks(count, variant):
     # Fewer colliders than the per-tick counters: every counter must still reset.
     centers = [[0.01 * (i % 3), 0.125 + 0.26 * i, -0.01 * (i % 2)] for i in range(count)]
     scene = mh.boxes(centers, [0.125] * 3, inverse_mass=0.064, inverse_inertia=6.144)
-    gpu, ora = make(scene, friction=0.6, tuning=BROADPHASE[variant])
+    gpu, ora = make(scene, friction=0.6, tuning={**BROADPHASE, **SOLVER}[variant])
     for tick in range(1, 241):
         gpu.dispatch_tick(tick)
         ora.step(tick)
         if tick % 60 == 0:
             assert_bodies_equal(gpu, ora, f"{count} bodies, tick {tick}")
+
+
+@pytest.mark.parametrize("solver", list(SOLVER))
+def test_body_with_many_contact_sides(solver):
+    """A heavy slab carrying 64 cubes: the slab has 64 cube contacts and one
+    ground contact, past the per-contact passes' 31 delta slots (their atomic
+    overflow path) and the same rows for the per-body passes."""
+    grid = [(-1.75 + 0.5 * i, -1.75 + 0.5 * j) for i in range(8) for j in range(8)]
+    centers = [[0.0, 0.1, 0.0]] + [[x, 0.305, z] for x, z in grid]
+    half = [[2.0, 0.1, 2.0]] + [[0.1, 0.1, 0.1]] * 64
+    inv_m = [1.0 / 400.0] + [1.0] * 64
+    inv_i = [0.02] + [150.0] * 64
+    scene = mh.boxes(centers, half, inverse_mass=inv_m, inverse_inertia=inv_i)
+    gpu, ora = make(scene, friction=0.6, tuning=SOLVER[solver])
+    most = 0
+    for tick in range(1, 121):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+        most = max(most, gpu.read_last_tick().active_contact_count)
+        if tick % 30 == 0:
+            assert_bodies_equal(gpu, ora, f"slab, tick {tick}")
+    assert most >= 65 and gpu.read_last_tick().error_flags == 0
 
 
 def test_driver_publishes_and_stops_on_flags():
@@ -296,17 +323,20 @@ def test_driver_publishes_and_stops_on_flags():
     assert runtime.failure() == rt.InvalidNumericState()
 
 
-def test_profiled_ticks_match_dispatched_ticks():
+@pytest.mark.parametrize("solver", list(SOLVER))
+def test_profiled_ticks_match_dispatched_ticks(solver):
     """A profiled tick (events between kernel groups, clock stamps in the solver
-    passes) runs the same tick as a dispatched one."""
+    passes) runs the same tick as a dispatched one. Per-body passes have no
+    separate gather kernel."""
     scene = mh.named_pile("pile_1k")
     out = []
     for profiled in (False, True):
-        gpu, _ = make(scene, friction=0.6)
+        gpu, _ = make(scene, friction=0.6, tuning=SOLVER[solver])
         for tick in range(1, 31):
             if profiled and 10 <= tick < 20:
                 t = gpu.profile_tick(tick)
-                assert t["total_ms"] > 0 and t["solve_kernel_ms"] > 0 and t["apply_kernel_ms"] > 0
+                assert t["total_ms"] > 0 and t["solve_kernel_ms"] > 0
+                assert (t["apply_kernel_ms"] > 0) == (solver == "per-contact")
                 assert t["solve_ms"] >= 8 * t["solve_kernel_ms"]
             else:
                 gpu.dispatch_tick(tick)

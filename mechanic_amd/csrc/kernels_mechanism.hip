@@ -303,29 +303,54 @@ __device__ __forceinline__ void prepare_link(const Dev& d, uint32_t body) {
   store_link(d.links[0], body, Link{f4(rp, 0.0f), rr, m.parent, 0u});
 }
 
-// jump_a_to_b / jump_b_to_a (mechanism.wgsl:121-147) with compose (109-119).
-__device__ __forceinline__ void jump_link(const float4* src, float4* dst, uint32_t body) {
-  const Link child = load_link(src, body);
-  if (child.resolved != 0u) {
-    store_link(dst, body, child);
-    return;
-  }
-  const Link parent = load_link(src, child.ancestor);
+// compose (mechanism.wgsl:109-119): the child's link carried through its
+// ancestor's, as jump_a_to_b / jump_b_to_a (mechanism.wgsl:121-147) apply it.
+__device__ __forceinline__ Link compose_link(const Link& parent, const Link& child) {
   Link out;
   out.position = f4(xyz(parent.position) + quat_rotate(parent.rotation, xyz(child.position)), 0.0f);
   out.rotation = normalize_q(quat_multiply(parent.rotation, child.rotation));
   out.ancestor = parent.ancestor;
   out.resolved = parent.resolved;
-  store_link(dst, body, out);
+  return out;
+}
+
+// One pointer-jumping round over kJumpBatch bodies per thread at a time: every
+// child link, then every parent link, is loaded before any is composed, so a
+// thread waits two memory latencies per batch instead of two per body.
+constexpr int kJumpBatch = 4;
+
+__device__ __forceinline__ void jump_round(const Dev& d, const float4* src, float4* dst) {
+  for (uint32_t i0 = threadIdx.x; i0 < d.mech_count; i0 += kJumpBatch * blockDim.x) {
+    uint32_t body[kJumpBatch];
+    Link child[kJumpBatch], parent[kJumpBatch];
+#pragma unroll
+    for (int k = 0; k < kJumpBatch; ++k) {
+      const uint32_t i = i0 + k * blockDim.x;
+      body[k] = i < d.mech_count ? d.mlist[i] : kInvalid;
+    }
+#pragma unroll
+    for (int k = 0; k < kJumpBatch; ++k)
+      if (body[k] != kInvalid) child[k] = load_link(src, body[k]);
+#pragma unroll
+    for (int k = 0; k < kJumpBatch; ++k)
+      if (body[k] != kInvalid && child[k].resolved == 0u) parent[k] = load_link(src, child[k].ancestor);
+#pragma unroll
+    for (int k = 0; k < kJumpBatch; ++k) {
+      if (body[k] == kInvalid) continue;
+      if (child[k].resolved != 0u) {
+        store_link(dst, body[k], child[k]);
+        continue;
+      }
+      store_link(dst, body[k], compose_link(parent[k], child[k]));
+    }
+  }
 }
 
 __device__ void forward_kinematics(const Dev& d) {
   for (uint32_t i = threadIdx.x; i < d.mech_count; i += blockDim.x) prepare_link(d, d.mlist[i]);
   __syncthreads();
   for (uint32_t round = 0; round < d.jump_rounds; ++round) {
-    const float4* src = d.links[round & 1u];
-    float4* dst = d.links[(round & 1u) ^ 1u];
-    for (uint32_t i = threadIdx.x; i < d.mech_count; i += blockDim.x) jump_link(src, dst, d.mlist[i]);
+    jump_round(d, d.links[round & 1u], d.links[(round & 1u) ^ 1u]);
     __syncthreads();
   }
 }
@@ -375,16 +400,16 @@ __device__ void publish(const Dev& d, const float4* links) {
   __syncthreads();
 }
 
-// Deterministic FP64 sum over the workgroup: per-thread partials, then a fixed
-// halving tree in LDS.
+// Deterministic FP64 sum over the workgroup: a fixed butterfly within each
+// wavefront, then every thread adds the wavefronts' sums in wavefront order, so
+// all threads hold the same bits. Two barriers in all.
 __device__ double block_sum(double v, double* scratch) {
-  scratch[threadIdx.x] = v;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  if ((threadIdx.x & 63u) == 0) scratch[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (uint32_t half = blockDim.x >> 1; half > 0; half >>= 1) {
-    if (threadIdx.x < half) scratch[threadIdx.x] += scratch[threadIdx.x + half];
-    __syncthreads();
-  }
-  const double total = scratch[0];
+  double total = 0.0;
+  for (uint32_t w = 0; w < blockDim.x / 64; ++w) total += scratch[w];
   __syncthreads();
   return total;
 }

@@ -118,6 +118,7 @@ struct Dev {
   uint32_t* slot_count;   // contact sides per body this tick
   float4* sbody_alt;      // second solver record buffer: per-body passes read one and write the other
   uint2* side_slot;       // per active contact: its sides' slots in their bodies (uncapped)
+  uint32_t* side_group;   // per active contact: group representatives and delta slots of its sides
   uint32_t* side_off;     // per body: first of its contact-side rows (exclusive scan of slot_count)
   uint32_t* side_status;  // per kSideTile bodies: look-back status of the offset scan
   uint32_t side_tiles;

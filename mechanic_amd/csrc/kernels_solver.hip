@@ -239,21 +239,19 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
     if (active) {
       const uint32_t ci = __float_as_uint(r3.z);
       const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
-      // Give both contact sides a slot of their body: their row in the per-body
-      // passes, and in the per-contact passes their delta slot (kSlotOverflow:
-      // add with atomics instead). Word: contact | slot A << 22 | slot B << 27.
-      const uint32_t slot_a = atomicAdd(&d.slot_count[body_a], 1u);
-      const uint32_t slot_b = body_b != kInvalid ? atomicAdd(&d.slot_count[body_b], 1u) : 0u;
-      if (per_body) d.side_slot[a] = make_uint2(slot_a, slot_b);
-      const uint32_t sa = slot_a < kSlotOverflow ? slot_a : kSlotOverflow;
-      const uint32_t sb = body_b != kInvalid && slot_b < kSlotOverflow ? slot_b : kSlotOverflow;
-      const uint32_t flags = (sa << 22) | (sb << 27);
+      // Per-body passes: both contact sides take a slot of their body, their row.
+      // (The per-contact passes take slots per group of sides, k_group_sides.)
+      if (per_body) {
+        const uint32_t slot_a = atomicAdd(&d.slot_count[body_a], 1u);
+        const uint32_t slot_b = body_b != kInvalid ? atomicAdd(&d.slot_count[body_b], 1u) : 0u;
+        d.side_slot[a] = make_uint2(slot_a, slot_b);
+      }
       d.active[a] = ci;
       d.lambda[a] = cached_lambda;
       set_row_word(d, 0, a, r0);
       set_row_word(d, 1, a, r1);
       set_row_word(d, 2, a, r2);
-      set_row_word(d, 3, a, make_float4(r3.x, r3.y, __uint_as_float(ci | flags), r3.w));
+      set_row_word(d, 3, a, make_float4(r3.x, r3.y, __uint_as_float(ci), r3.w));
     }
   }
 }
@@ -329,49 +327,141 @@ __device__ __forceinline__ bool contact_pass(const Dev& d, const SolverBody& sa,
   return applies;
 }
 
-// Per-contact pass (tuning solve_kind 0, the default): the contact's row and both bodies; each
-// contact side stores its six Q20 integers for the bodies' gather (k_apply);
-// sides past kSlotOverflow add theirs with atomics. The last pass also runs
-// persist_contacts (collision.wgsl:1071-1088).
+// Contact sides grouped per wavefront (tuning solve_kind 0, the default). The
+// per-contact passes run one wavefront per aligned group of 64 active contacts;
+// the group's 128 sides touch far fewer bodies (about two in five on a settled
+// pile), so each body of a group gets one delta slot, and its sides' Q20
+// integers are summed in LDS before one store. k_group_sides finds, for every
+// side, the group's first side of the same body (its representative) and gives
+// each representative a slot of its body. Packed per active contact:
+// representative of side A | of side B << 7 | slot A << 14 | slot B << 19
+// (kSlotOverflow: add with atomics instead).
+constexpr uint32_t kGroupSlotShift = 14;
+
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+__global__ __launch_bounds__(256) void k_group_sides(Dev d) {
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
+  const uint32_t groups = (count + 63u) / 64u;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < groups; g += waves) {
+    const uint32_t ai = g * 64u + lane;
+    const bool valid = ai < count;
+    uint32_t ba = kInvalid, bb = kInvalid;
+    if (valid) {
+      const float4 r3 = row_word(d, 3, ai);
+      ba = __float_as_uint(r3.x);
+      bb = __float_as_uint(r3.y);
+    }
+    // Side positions: 2 * lane for A, 2 * lane + 1 for B. First position of the
+    // same body, over the group's sides in order.
+    uint32_t rep_a = 2u * lane, rep_b = 2u * lane + 1u;
+    for (uint32_t q = 0; q < 64u; ++q) {
+      const uint32_t qa = lane_value(ba, q), qb = lane_value(bb, q);
+      if (qa == ba && 2u * q < rep_a) rep_a = 2u * q;
+      if (qb == ba && 2u * q + 1u < rep_a) rep_a = 2u * q + 1u;
+      if (qa == bb && 2u * q < rep_b) rep_b = 2u * q;
+      if (qb == bb && 2u * q + 1u < rep_b) rep_b = 2u * q + 1u;
+    }
+    uint32_t own_a = 0u, own_b = 0u;
+    if (valid && rep_a == 2u * lane) own_a = atomicAdd(&d.slot_count[ba], 1u);
+    if (valid && bb != kInvalid && rep_b == 2u * lane + 1u) own_b = atomicAdd(&d.slot_count[bb], 1u);
+    const uint32_t a_from_a = (uint32_t)__shfl((int)own_a, (int)(rep_a >> 1));
+    const uint32_t a_from_b = (uint32_t)__shfl((int)own_b, (int)(rep_a >> 1));
+    const uint32_t b_from_a = (uint32_t)__shfl((int)own_a, (int)(rep_b >> 1));
+    const uint32_t b_from_b = (uint32_t)__shfl((int)own_b, (int)(rep_b >> 1));
+    uint32_t slot_a = (rep_a & 1u) ? a_from_b : a_from_a;
+    uint32_t slot_b = (rep_b & 1u) ? b_from_b : b_from_a;
+    slot_a = slot_a < kSlotOverflow ? slot_a : kSlotOverflow;
+    slot_b = bb != kInvalid && slot_b < kSlotOverflow ? slot_b : kSlotOverflow;
+    if (valid) d.side_group[ai] = rep_a | (rep_b << 7) | (slot_a << kGroupSlotShift) | (slot_b << (kGroupSlotShift + 5u));
+  }
+}
+
+// warm_start (collision.wgsl:774-839) when first, solve_accumulate
+// (collision.wgsl:841-906) otherwise, per contact, with the group's sides
+// summed per body in LDS; the last pass also runs persist_contacts
+// (collision.wgsl:1071-1088). Each XCD takes a contiguous run of whole groups.
 __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint64_t* clock) {
+  __shared__ int32_t lds_q[kBlock / 64][128][6];
   if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   float4* rows_now = d.mrows[tick64 & 1];
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
-  const XcdSplit xs = xcd_split(count);
-  for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
-    const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
-    const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
-    const uint32_t word = __float_as_uint(r3.z);
-    const bool has_b = body_b != kInvalid;
-    const ContactRow row{xyz(r0), xyz(r1), xyz(r2), r0.w, r3.w, r2.w < 0.0f};
-    float lambda = d.lambda[ai];
-    const SolverBody sa = load_body(d.sbody, body_a);
-    const SolverBody sb = has_b ? load_body(d.sbody, body_b) : sa;
-    v3 impulse;
-    const bool applies = contact_pass(d, sa, sb, has_b, row, first, ai, true, lambda, impulse);
-    d.lambda[ai] = lambda;
-    // A contact that applies no impulse adds zero deltas (collision.wgsl:793-804).
-    const Q20 qa = applies ? side_delta(sa, row.arm_a, -impulse) : Q20{};
-    const uint32_t slot_a = (word >> 22) & 31u, slot_b = word >> 27;
-    if (slot_a == kSlotOverflow) {
-      if (applies) add_overflow(d, body_a, qa);
-    } else {
-      put_side(d, body_a, slot_a, qa);
+  const uint32_t groups = (count + 63u) / 64u;
+  const uint32_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+  const uint32_t blocks = (gridDim.x - xcd + kXcds - 1) / kXcds;
+  const uint32_t g_end = (uint32_t)(((uint64_t)groups * (xcd + 1)) / kXcds);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  int32_t(*q)[6] = lds_q[wave];
+  for (uint32_t g = (uint32_t)(((uint64_t)groups * xcd) / kXcds) + local * (kBlock / 64) + wave; g < g_end;
+       g += blocks * (kBlock / 64)) {
+    const uint32_t ai = g * 64u + lane;
+    const bool valid = ai < count;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      q[2 * lane][c] = 0;
+      q[2 * lane + 1][c] = 0;
     }
-    if (has_b) {
-      const Q20 qb = applies ? side_delta(sb, row.arm_b, impulse) : Q20{};
-      if (slot_b == kSlotOverflow) {
-        if (applies) add_overflow(d, body_b, qb);
-      } else {
-        put_side(d, body_b, slot_b, qb);
+    uint32_t body_a = kInvalid, body_b = kInvalid, grp = 0u;
+    if (valid) {
+      const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
+      body_a = __float_as_uint(r3.x);
+      body_b = __float_as_uint(r3.y);
+      grp = d.side_group[ai];
+      const uint32_t word = __float_as_uint(r3.z);
+      const bool has_b = body_b != kInvalid;
+      const ContactRow row{xyz(r0), xyz(r1), xyz(r2), r0.w, r3.w, r2.w < 0.0f};
+      float lambda = d.lambda[ai];
+      const SolverBody sa = load_body(d.sbody, body_a);
+      const SolverBody sb = has_b ? load_body(d.sbody, body_b) : sa;
+      v3 impulse;
+      const bool applies = contact_pass(d, sa, sb, has_b, row, first, ai, true, lambda, impulse);
+      d.lambda[ai] = lambda;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      // A contact that applies no impulse adds zero deltas (collision.wgsl:793-804).
+      if (applies) {
+        const Q20 qa = side_delta(sa, row.arm_a, -impulse);
+        const uint32_t ra = grp & 127u;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) atomicAdd(&q[ra][c], qa.v[c]);
+        if (has_b) {
+          const Q20 qb = side_delta(sb, row.arm_b, impulse);
+          const uint32_t rb = (grp >> 7) & 127u;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) atomicAdd(&q[rb][c], qb.v[c]);
+        }
+      }
+      if (last) rows_now[3 * (size_t)(word & kContactMask) + 2].w = lambda;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // Each body's representative side stores the group's sum for that body.
+    if (valid) {
+      const uint32_t slot_a = (grp >> kGroupSlotShift) & 31u, slot_b = (grp >> (kGroupSlotShift + 5u)) & 31u;
+      if ((grp & 127u) == 2u * lane) {
+        Q20 sum;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sum.v[c] = q[2 * lane][c];
+        if (slot_a == kSlotOverflow) add_overflow(d, body_a, sum);
+        else put_side(d, body_a, slot_a, sum);
+      }
+      if (body_b != kInvalid && ((grp >> 7) & 127u) == 2u * lane + 1u) {
+        Q20 sum;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sum.v[c] = q[2 * lane + 1][c];
+        if (slot_b == kSlotOverflow) add_overflow(d, body_b, sum);
+        else put_side(d, body_b, slot_b, sum);
       }
     }
-    if (last) {
-      const uint32_t ci = word & kContactMask;
-      rows_now[3 * (size_t)ci + 2].w = lambda;
-    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the next group clears the same LDS
   }
   if (clock) {
     __syncthreads();
@@ -916,6 +1006,7 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t sgrid = solve_grid(d), bgrid = apply_grid(d);
   if (d.t_solve_kind != 1u) {
+    k_group_sides<<<grid_for(d.pair_capacity, 1024u), kBlock, 0, s>>>(d);
     for (uint32_t it = 0; it < iterations; ++it) {
       const int last = it + 1 == iterations ? 1 : 0;
       k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, clock ? clock + 2 * (size_t)it * sgrid : nullptr);

@@ -585,6 +585,7 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   d.side_tiles = (n + kSideTile - 1) / kSideTile;
   ALLOC(side_off, n); ALLOC(side_status, d.side_tiles ? d.side_tiles : 1u);
   ALLOC(side_slot, cfg.pair_capacity);
+  ALLOC(d.side_group, cfg.pair_capacity);
   ALLOC(srow, 8 * (size_t)cfg.pair_capacity); ALLOC(slam, 2 * (size_t)cfg.pair_capacity);
   ALLOC(deltas, 6 * (size_t)n);
   ALLOC(slot_count, n);

@@ -19,36 +19,40 @@ import os
 import sys
 import time
 
+import numpy
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def algorithmic_bytes(n, c, s, p, k, a, iterations):
+def algorithmic_bytes(n, c, s, p, k, a, r, iterations):
     """Compulsory HBM bytes per tick and kernel group: each input read once and
-    each output written once (per-unit figures in DESIGN.md)."""
+    each output written once (per-unit figures in DESIGN.md). r is the number of
+    delta slots the tick's contact sides took (one per body per 64-contact group)."""
     return {
         "integrate": 280 * n,
         "morton": 136 * c + 8 * (s - c),
         "sort": 16 * s,
-        "lbvh": 24 * (c - 1) + 72 * c + 112 * (c - 1),
+        "lbvh": 24 * (c - 1) + 136 * c,
         "traverse": 40 * c + 72 * (2 * c - 1) + 8 * p,
         "narrowphase": 200 * p + 64 * k,
         "prepare": 340 * k,
-        "solve": iterations * (330 * a + 112 * n),
+        "solve": 12 * a + 4 * r + iterations * (solve_kernel_bytes(n, a, r) + 24 * r + 112 * n),
     }
 
 
 LAUNCHES = {"integrate": 1, "morton": 2, "lbvh": 2, "traverse": 1, "narrowphase": 1, "prepare": 1}
 
 
-def solve_kernel_bytes(n, a):
+def solve_kernel_bytes(n, a, r):
     """Compulsory HBM bytes of one contact-solve pass kernel (k_solve): per active
     contact its 64-byte solver row, its accumulated impulse read and written
-    (4 + 4) and two contact sides' Q20 velocity deltas written (2 x 24); per body
-    its 64-byte solver state read once."""
-    return 120 * a + 64 * n
+    (4 + 4) and its sides' group word (4); per delta slot the Q20 velocity deltas
+    of one body summed over a 64-contact group (24); per body its 64-byte solver
+    state read once."""
+    return 76 * a + 24 * r + 64 * n
 
 
 # What FETCH_SIZE and WRITE_SIZE (KiB per dispatch) read for a plain device copy
@@ -367,7 +371,8 @@ def main():
     for i in range(args.profile_ticks):
         t = gpu.profile_tick(first + i)
         r = gpu.read_last_tick()
-        counts.append((r.pair_count, r.contact_count, r.active_contact_count))
+        slots = numpy.frombuffer(gpu.read_buffer(mh._lib.BUFFER_BODY_SLOTS, 4 * scene.body_count), numpy.uint32)
+        counts.append((r.pair_count, r.contact_count, r.active_contact_count, int(slots.sum(dtype=numpy.int64))))
         for g in groups:
             acc[g] += t[g]
         for g in kernel_ms:
@@ -378,10 +383,11 @@ def main():
     p = sum(c[0] for c in counts) / npt
     k = sum(c[1] for c in counts) / npt
     a = sum(c[2] for c in counts) / npt
+    slots = sum(c[3] for c in counts) / npt
     n = scene.body_count
     c = len(scene.colliders)
     s = 1 << (c - 1).bit_length()
-    algo = algorithmic_bytes(n, c, s, p, k, a, iterations)
+    algo = algorithmic_bytes(n, c, s, p, k, a, slots, iterations)
     # Roofline of the dominant kernel, k_solve (8 launches a tick): compulsory
     # bytes per launch over its mean launch span in the profiled ticks, stamped
     # on the device wall clock by the kernel's own workgroups (first start to
@@ -389,7 +395,7 @@ def main():
     # they add several microseconds of their own on this runtime.
     solve_ms = kernel_ms["solve_kernel_ms"] / npt
     apply_ms = kernel_ms["apply_kernel_ms"] / npt
-    solve_bytes = solve_kernel_bytes(scene.body_count, a)
+    solve_bytes = solve_kernel_bytes(scene.body_count, a, slots)
     achieved = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     per_phase_gbs = {g: (algo[g] / (phase_ms[g] * 1e-3) / 1e9 if phase_ms[g] > 0 else 0.0) for g in phase_ms}
 
@@ -441,7 +447,7 @@ def main():
         "phase_ms": {g: round(v, 4) for g, v in phase_ms.items()},
         "phase_gbs": {g: round(v, 1) for g, v in per_phase_gbs.items()},
         "profiled_tick_ms": round(total / npt, 4),
-        "counts": {"pairs": round(p), "contacts": round(k), "active_contacts": round(a)},
+        "counts": {"pairs": round(p), "contacts": round(k), "active_contacts": round(a), "delta_slots": round(slots)},
         "error_flags": readback.error_flags,
         "engine_tps_sync": round(engine_tps, 2),
         "tuning": tuned["tuning"],

@@ -357,7 +357,10 @@ enum {
   MH_BUFFER_DIAGNOSTICS = 7,     /* mh_diagnostics of the last tick */
   MH_BUFFER_WORLD_MASSES = 8,    /* 3 x float4 per body */
   MH_BUFFER_MANIFOLDS = 9,       /* mh_persistent_manifold per contact of the last tick */
-  MH_BUFFER_NODE_PARENTS = 10    /* u32 parent+1 per node, reference encoding */
+  MH_BUFFER_NODE_PARENTS = 10,   /* u32 parent+1 per node, reference encoding */
+  MH_BUFFER_BODY_SLOTS = 11      /* u32 per body: delta slots its contact sides took in the last tick (one
+                                    per 64-contact group with a side of the body; per side for the
+                                    per-body passes). No counterpart in the reference. */
 };
 
 /* ---- synthetic scenes (seeded; SplitMix64, u = (x >> 40) * 2^-24) ------- */

@@ -56,6 +56,7 @@ BUFFER_DIAGNOSTICS = 7
 BUFFER_WORLD_MASSES = 8
 BUFFER_MANIFOLDS = 9
 BUFFER_NODE_PARENTS = 10
+BUFFER_BODY_SLOTS = 11
 
 WARM_START_CACHE = 0
 WARM_START_LOG = 1

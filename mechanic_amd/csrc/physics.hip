@@ -1126,6 +1126,7 @@ int mh_physics_read_buffer(mh_physics* p, int buffer, void* dst, size_t bytes, s
     case MH_BUFFER_CONTACTS: src = d.contacts; size = capped(diag_host[kDiagContacts]) * 64; break;
     case MH_BUFFER_DIAGNOSTICS: src = d.diag[tick & 1]; size = 32; break;
     case MH_BUFFER_WORLD_MASSES: src = d.world_masses; size = (size_t)d.body_count * 48; break;
+    case MH_BUFFER_BODY_SLOTS: src = d.slot_count; size = (size_t)d.body_count * 4; break;
     case MH_BUFFER_MANIFOLDS: src = d.mrows[tick & 1]; size = capped(diag_host[kDiagContacts]) * 48; break;
     case MH_BUFFER_NODE_AABBS: {
       // Reference layout (lbvh.wgsl:19, 287): internal nodes then leaves, min/max with w = 0.

@@ -179,6 +179,10 @@ struct Dev {
   int32_t* mdeltas;           // 6 Q20 integers per body
   const uint32_t* mlist;      // non-root mechanism bodies sorted by (depth, preorder)
   const uint32_t* level_offset;  // level_count + 1 offsets into mlist
+  const uint32_t* fk_list;    // non-root mechanism bodies grouped by tree
+  const uint32_t* fk_offset;  // fk_groups + 1 offsets into fk_list: whole trees per forward-kinematics workgroup
+  uint32_t fk_groups;
+  uint32_t* mech_state;       // [0]: this tick's closure Newton loop has converged
   const uint32_t* blist;      // bodies referenced by bearings
   const uint32_t* closure_rows;  // bearing row of each closure
   const uint32_t* cpath_offset;  // 2 * closure_count + 1 offsets: branch a, branch b of each closure

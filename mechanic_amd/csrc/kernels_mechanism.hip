@@ -319,14 +319,15 @@ __device__ __forceinline__ Link compose_link(const Link& parent, const Link& chi
 // thread waits two memory latencies per batch instead of two per body.
 constexpr int kJumpBatch = 4;
 
-__device__ __forceinline__ void jump_round(const Dev& d, const float4* src, float4* dst) {
-  for (uint32_t i0 = threadIdx.x; i0 < d.mech_count; i0 += kJumpBatch * blockDim.x) {
+__device__ __forceinline__ void jump_round(const uint32_t* list, uint32_t begin, uint32_t end, const float4* src,
+                                           float4* dst) {
+  for (uint32_t i0 = begin + threadIdx.x; i0 < end; i0 += kJumpBatch * blockDim.x) {
     uint32_t body[kJumpBatch];
     Link child[kJumpBatch], parent[kJumpBatch];
 #pragma unroll
     for (int k = 0; k < kJumpBatch; ++k) {
       const uint32_t i = i0 + k * blockDim.x;
-      body[k] = i < d.mech_count ? d.mlist[i] : kInvalid;
+      body[k] = i < end ? list[i] : kInvalid;
     }
 #pragma unroll
     for (int k = 0; k < kJumpBatch; ++k)
@@ -346,11 +347,20 @@ __device__ __forceinline__ void jump_round(const Dev& d, const float4* src, floa
   }
 }
 
-__device__ void forward_kinematics(const Dev& d) {
-  for (uint32_t i = threadIdx.x; i < d.mech_count; i += blockDim.x) prepare_link(d, d.mlist[i]);
+// Forward kinematics (prepare_links + pointer jumping, mechanism.wgsl:75-147)
+// over whole trees, one workgroup per group of trees (fk_offset): a link only
+// ever points at its own tree, so the jumping rounds need only the workgroup's
+// barriers, and the groups run on as many CUs as there are groups. `step` >= 0
+// is the pass after Newton step `step`, skipped once the loop has converged;
+// the tick's first pass (step < 0) clears that flag.
+__global__ __launch_bounds__(256) void k_forward_kinematics(Dev d, int step) {
+  if (step >= 0 && *reinterpret_cast<volatile uint32_t*>(&d.mech_state[0]) != 0u) return;
+  if (step < 0 && blockIdx.x == 0 && threadIdx.x == 0) d.mech_state[0] = 0u;
+  const uint32_t begin = d.fk_offset[blockIdx.x], end = d.fk_offset[blockIdx.x + 1];
+  for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) prepare_link(d, d.fk_list[i]);
   __syncthreads();
   for (uint32_t round = 0; round < d.jump_rounds; ++round) {
-    jump_round(d, d.links[round & 1u], d.links[(round & 1u) ^ 1u]);
+    jump_round(d.fk_list, begin, end, d.links[round & 1u], d.links[(round & 1u) ^ 1u]);
     __syncthreads();
   }
 }
@@ -433,12 +443,20 @@ __device__ __forceinline__ void joint_of(const Dev& d, const float4* links, uint
   }
 }
 
-// The closure Newton loop (device.rs:976-1020, closure.wgsl:233-427).
-__device__ void closure_newton(const Dev& d, const float4* links, uint32_t* diag) {
+// One step of the closure Newton loop (device.rs:976-1020, closure.wgsl:233-427)
+// in one workgroup: evaluate the closures, and unless all are inside tolerance
+// (then the loop is over: mech_state[0] is set and every later step and
+// forward-kinematics pass of this tick returns at once), the gradient, the PCG
+// solve and the clamped step of the coordinates. k_forward_kinematics follows
+// each step.
+__global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
   __shared__ uint32_t state[2];
   __shared__ double scratch[kPoseThreads];
+  if (*reinterpret_cast<volatile uint32_t*>(&d.mech_state[0]) != 0u) return;
+  uint32_t* diag = diag_now(d);
+  const float4* links = d.links[d.jump_rounds & 1u];
   const uint32_t nc = d.closure_count, nq = d.coordinate_count;
-  for (int step = 0; step < kNewtonSteps; ++step) {
+  {
     if (threadIdx.x == 0) {
       state[0] = 0u;
       state[1] = 0u;
@@ -477,7 +495,10 @@ __device__ void closure_newton(const Dev& d, const float4* links, uint32_t* diag
     __syncthreads();
     // finalize_closures: the rest of this step, and every later step, only
     // while some closure is outside tolerance.
-    if (!(state[0] > kAnchorTolerance || state[1] > kAxisTolerance)) break;
+    if (!(state[0] > kAnchorTolerance || state[1] > kAxisTolerance)) {
+      if (threadIdx.x == 0) d.mech_state[0] = 1u;
+      return;
+    }
     // Joint axes and anchors of every coordinate.
     for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
       v3 axis, anchor;
@@ -589,20 +610,13 @@ __device__ void closure_newton(const Dev& d, const float4* links, uint32_t* diag
       if (step_c == step_c) d.coords[c].x = d.coords[c].x + step_c;
       else atomicOr(&diag[kDiagFlags], kFlagInvalid);
     }
-    __syncthreads();
-    forward_kinematics(d);
   }
 }
 
-// Forward kinematics, the closure Newton loop, pose publication and the
-// parent-to-child velocity reconstruction of encode_mechanism_passes
-// (device.rs:963-1048), in one workgroup.
-__global__ __launch_bounds__(kPoseThreads) void k_mechanism_pose(Dev d, int closures) {
-  uint32_t* diag = diag_now(d);
-  forward_kinematics(d);
-  const float4* links = d.links[d.jump_rounds & 1u];
-  if (closures) closure_newton(d, links, diag);
-  publish(d, links);
+// Pose publication and the parent-to-child velocity reconstruction of
+// encode_mechanism_passes (device.rs:963-1048), in one workgroup.
+__global__ __launch_bounds__(kPoseThreads) void k_publish(Dev d) {
+  publish(d, d.links[d.jump_rounds & 1u]);
   reconstruct_levels(d);
 }
 
@@ -710,8 +724,18 @@ void launch_advance_coordinates(hipStream_t s, const Dev& d) {
   if (d.mech_count) k_advance<<<blocks_for(d.mech_count), 256, 0, s>>>(d);
 }
 
+// Forward kinematics, the closure Newton loop (each step followed by forward
+// kinematics), publication and reconstruction (device.rs:963-1048).
 void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures) {
-  k_mechanism_pose<<<1, kPoseThreads, 0, s>>>(d, closures);
+  if (d.fk_groups == 0) return;
+  k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, -1);
+  if (closures) {
+    for (int step = 0; step < kNewtonSteps; ++step) {
+      k_newton_step<<<1, kPoseThreads, 0, s>>>(d);
+      k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, step);
+    }
+  }
+  k_publish<<<1, kPoseThreads, 0, s>>>(d);
 }
 
 void launch_capture_coordinates(hipStream_t s, const Dev& d) {

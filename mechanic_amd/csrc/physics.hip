@@ -200,7 +200,7 @@ bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1])
 // (what create_mechanism_resources derives, device.rs:1684-2232).
 struct Forest {
   std::vector<uint32_t> mlist, level_offset, blist, closure_rows, cpath_offset, cpath, coord_offset, coord_entries,
-      coord_body;
+      coord_body, fk_list, fk_offset;
   uint32_t max_depth = 0, jump_rounds = 0;
   bool has_dynamic_root = false;
 };
@@ -244,6 +244,22 @@ int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
     while (end < f.mlist.size() && depth[f.mlist[end]] == level) ++end;
     f.level_offset.push_back(end);
   }
+  // Trees for the forward-kinematics workgroups: each non-root body's tree root
+  // (parents come first in mlist's depth order), bodies of one tree together,
+  // whole trees packed into groups of about kFkGroupBodies bodies.
+  constexpr uint32_t kFkGroupBodies = 256;
+  std::vector<uint32_t> tree(n, kInvalid);
+  for (uint32_t b : f.mlist) tree[b] = root[parent[b]] ? parent[b] : tree[parent[b]];
+  f.fk_list = f.mlist;
+  std::stable_sort(f.fk_list.begin(), f.fk_list.end(), [&](uint32_t x, uint32_t y) { return tree[x] < tree[y]; });
+  f.fk_offset.push_back(0);
+  for (uint32_t i = 0; i < f.fk_list.size();) {
+    uint32_t j = i;
+    while (j < f.fk_list.size() && tree[f.fk_list[j]] == tree[f.fk_list[i]]) ++j;  // one tree: [i, j)
+    if (f.fk_offset.back() < i && j - f.fk_offset.back() > kFkGroupBodies) f.fk_offset.push_back(i);
+    i = j;
+  }
+  if (!f.fk_list.empty()) f.fk_offset.push_back((uint32_t)f.fk_list.size());
   // pointer_jump_rounds (device.rs:2103-2111).
   uint32_t covered = 1;
   while (covered < f.max_depth) {
@@ -355,6 +371,14 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
   uint32_t *mlist, *level_offset, *blist, *closure_rows, *cpath_offset, *cpath, *coord_offset, *coord_entries,
       *coord_body;
   TRY(alloc_upload(p, &mlist, f.mlist, error));
+  uint32_t *fk_list, *fk_offset, *mech_state;
+  TRY(alloc_upload(p, &fk_list, f.fk_list.empty() ? std::vector<uint32_t>{0u} : f.fk_list, error));
+  TRY(alloc_upload(p, &fk_offset, f.fk_offset.empty() ? std::vector<uint32_t>{0u} : f.fk_offset, error));
+  TRY(alloc_upload(p, &mech_state, std::vector<uint32_t>(4, 0u), error));
+  d.fk_list = fk_list;
+  d.fk_offset = fk_offset;
+  d.fk_groups = f.fk_offset.empty() ? 0u : (uint32_t)f.fk_offset.size() - 1u;
+  d.mech_state = mech_state;
   TRY(alloc_upload(p, &level_offset, f.level_offset, error));
   TRY(alloc_upload(p, &blist, f.blist, error));
   TRY(alloc_upload(p, &closure_rows, f.closure_rows, error));

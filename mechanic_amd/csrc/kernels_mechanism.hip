@@ -543,7 +543,16 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       rz_part += (double)residual * (double)pre;
     }
     double rz = block_sum(rz_part, scratch);
+    // The search direction of iteration `it` is p = z + beta p_prev (the
+    // preconditioned start at it = 0); each phase forms it from the previous
+    // iteration's z and p, which the update phase then replaces, so no pass
+    // is spent storing it.
+    float beta = 0.0f;
     for (int it = 0; it < kPcgIterations; ++it) {
+      auto direction_of = [&](uint32_t c) {
+        const float* q = d.pcg + 8 * (size_t)c;
+        return it == 0 ? q[3] : q[2] + beta * q[3];
+      };
       // branch_product for every closure (closure.wgsl:188-220).
       for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
         const float4* s = d.cl_state + 6 * (size_t)k;
@@ -555,7 +564,7 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
           for (uint32_t e = d.cpath_offset[2 * k + side]; e < d.cpath_offset[2 * k + side + 1]; ++e) {
             const uint32_t c = d.cpath[e];
             const v3 jaxis = xyz(d.joints[2 * (size_t)c]), janchor = xyz(d.joints[2 * (size_t)c + 1]);
-            const float direction = d.pcg[8 * (size_t)c + 3];
+            const float direction = direction_of(c);
             pp = pp + (cross(jaxis, end_anchor - janchor) * sign) * direction;
             pa = pa + (cross(jaxis, end_axis) * sign) * direction;
           }
@@ -582,27 +591,24 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
         }
         float* q = d.pcg + 8 * (size_t)c;
         q[4] = op;
-        dad_part += (double)q[3] * (double)op;
+        dad_part += (double)direction_of(c) * (double)op;
       }
       const double dad = block_sum(dad_part, scratch);
       const float alpha = fabs(dad) > 1.0e-20 ? (float)(rz / dad) : 0.0f;
       double rz_next_part = 0.0;
       for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
+        const float p = direction_of(c);
         float* q = d.pcg + 8 * (size_t)c;
-        q[0] = q[0] + alpha * q[3];
+        q[3] = p;
+        q[0] = q[0] + alpha * p;
         q[1] = q[1] - alpha * q[4];
         const float df = q[6];
         q[2] = df > 1.0e-12f ? q[1] / df : 0.0f;
         rz_next_part += (double)q[1] * (double)q[2];
       }
-      const double rz_next = block_sum(rz_next_part, scratch);
-      const float beta = fabs(rz) > 1.0e-20 ? (float)(rz_next / rz) : 0.0f;
-      for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
-        float* q = d.pcg + 8 * (size_t)c;
-        q[3] = q[2] + beta * q[3];
-      }
+      const double rz_next = block_sum(rz_next_part, scratch);  // its barriers publish the update
+      beta = fabs(rz) > 1.0e-20 ? (float)(rz_next / rz) : 0.0f;
       rz = rz_next;
-      __syncthreads();
     }
     // Apply the clamped Newton step (closure.wgsl:416-426).
     for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {

@@ -731,6 +731,12 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last, uint64_t* clock)
     uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
     const uint32_t count = valid ? d.slot_count[b] : 0u;
     const uint32_t n = count < kSlotOverflow ? count : kSlotOverflow;
+    // The body's velocity words, loaded alongside its slots.
+    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+    if (valid && part == 0u) {
+      v0 = d.sbody[4 * (size_t)b];
+      v1 = d.sbody[4 * (size_t)b + 1];
+    }
     int4 x[kApplyPerLane];
     int2 y[kApplyPerLane];
 #pragma unroll
@@ -766,14 +772,13 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last, uint64_t* clock)
         p[c] = 0;
       }
     }
-    const SolverBody sb = load_body(d, b);
     const v3 lin_delta = mk((float)(int32_t)acc[0], (float)(int32_t)acc[1], (float)(int32_t)acc[2]) / kFixedScale;
     const v3 ang_delta = mk((float)(int32_t)acc[3], (float)(int32_t)acc[4], (float)(int32_t)acc[5]) / kFixedScale;
-    const v3 lin = sb.lin + lin_delta;
-    const v3 ang = sb.ang + ang_delta;
+    const v3 lin = xyz(v0) + lin_delta;
+    const v3 ang = xyz(v1) + ang_delta;
     float4* r = d.sbody + 4 * (size_t)b;
-    r[0] = f4(lin, sb.inv_mass);
-    r[1] = f4(ang, sb.cx.x);
+    r[0] = f4(lin, v0.w);
+    r[1] = f4(ang, v1.w);
     if (last) {
       d.lin[b] = f4(lin, 0.0f);
       d.ang[b] = f4(ang, 0.0f);

@@ -281,6 +281,31 @@ def test_small_stacks_many_ticks(count, variant):
 
 
 @pytest.mark.parametrize("solver", list(SOLVER))
+def test_delta_slots_per_body(solver):
+    """MH_BUFFER_BODY_SLOTS after a tick whose contacts are all active and fit
+    one 64-contact group: the per-contact passes give every body with a contact
+    side one delta slot (its group's sides summed); the per-body passes one
+    slot per side."""
+    count = 33
+    centers = [[0.01 * (i % 3), 0.125 + 0.26 * i, -0.01 * (i % 2)] for i in range(count)]
+    scene = mh.boxes(centers, [0.125] * 3, inverse_mass=0.064, inverse_inertia=6.144)
+    gpu, _ = make(scene, friction=0.6, tuning=SOLVER[solver])
+    for tick in range(1, 121):
+        gpu.dispatch_tick(tick)
+    r = gpu.read_last_tick()
+    assert 0 < r.active_contact_count == r.contact_count <= 64
+    rows = np.frombuffer(gpu.read_buffer(_lib.BUFFER_CONTACTS, r.contact_count * 64), np.uint32).reshape(-1, 16)
+    sides = np.zeros(count, np.int64)
+    for a, b in rows[:, :2]:
+        sides[a] += 1
+        if b != 0xFFFFFFFF:
+            sides[b] += 1
+    slots = np.frombuffer(gpu.read_buffer(_lib.BUFFER_BODY_SLOTS, 4 * count), np.uint32)
+    expected = (sides > 0).astype(np.int64) if solver == "per-contact" else sides
+    assert slots.tolist() == expected.tolist()
+
+
+@pytest.mark.parametrize("solver", list(SOLVER))
 def test_body_with_many_contact_sides(solver):
     """A heavy slab carrying 64 cubes: the slab has 64 cube contacts and one
     ground contact, past the per-contact passes' 31 delta slots (their atomic

@@ -327,6 +327,30 @@ def test_body_with_many_contact_sides(solver):
     assert most >= 65 and gpu.read_last_tick().error_flags == 0
 
 
+@pytest.mark.parametrize("solver", list(SOLVER))
+def test_body_in_more_groups_than_slots(solver):
+    """A heavy slab carrying 2,116 cubes: its contact sides fall in about 33
+    groups of 64 contacts, so the per-contact passes run out of its 31 delta
+    slots and add the rest with atomics (the overflow path); the per-body passes
+    lay out all its sides. Both against the oracle, bit for bit."""
+    grid = [(-11.25 + 0.5 * i, -11.25 + 0.5 * j) for i in range(46) for j in range(46)]
+    centers = [[0.0, 0.1, 0.0]] + [[x, 0.305, z] for x, z in grid]
+    half = [[12.0, 0.1, 12.0]] + [[0.1, 0.1, 0.1]] * len(grid)
+    inv_m = [1.0 / 4000.0] + [1.0] * len(grid)
+    inv_i = [0.0005] + [150.0] * len(grid)
+    scene = mh.boxes(centers, half, inverse_mass=inv_m, inverse_inertia=inv_i)
+    gpu, ora = make(scene, friction=0.6, tuning=SOLVER[solver])
+    for tick in range(1, 41):
+        gpu.dispatch_tick(tick)
+        ora.step(tick)
+    assert_bodies_equal(gpu, ora, "slab with 2,116 cubes, tick 40")
+    r = gpu.read_last_tick()
+    assert r.active_contact_count > 31 * 64 and r.error_flags == 0
+    if solver == "per-contact":
+        slots = np.frombuffer(gpu.read_buffer(_lib.BUFFER_BODY_SLOTS, 4 * scene.body_count), np.uint32)
+        assert slots[0] > 31  # the slab took a slot in every group, past its 31
+
+
 def test_driver_publishes_and_stops_on_flags():
     from mechanic_amd import runtime as rt
     scene = mh.named_pile("pile_1k")

@@ -1176,7 +1176,7 @@ void launch_sort(hipStream_t s, const Dev& d) {
 void launch_lbvh(hipStream_t s, const Dev& d) {
   if (d.collider_count == 0) return;
   k_build<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
-  if (d.collider_count > 1) k_refit_top<<<256, kBlock, 0, s>>>(d);
+  if (d.collider_count > 1) k_refit_top<<<1024, kBlock, 0, s>>>(d);
 }
 
 void launch_traverse(hipStream_t s, const Dev& d) {

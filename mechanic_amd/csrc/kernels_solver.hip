@@ -1011,7 +1011,7 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t sgrid = solve_grid(d), bgrid = apply_grid(d);
   if (d.t_solve_kind != 1u) {
-    k_group_sides<<<grid_for(d.pair_capacity, 1024u), kBlock, 0, s>>>(d);
+    k_group_sides<<<grid_for(d.pair_capacity / 64u * kBlock, 4096u), kBlock, 0, s>>>(d);
     for (uint32_t it = 0; it < iterations; ++it) {
       const int last = it + 1 == iterations ? 1 : 0;
       k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, clock ? clock + 2 * (size_t)it * sgrid : nullptr);

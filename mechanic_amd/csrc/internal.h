@@ -43,7 +43,7 @@ enum DiagWord : uint32_t {
 // Private per-tick counters (cleared by the Morton kernel).
 enum CounterWord : uint32_t {
   kCountFrontier = 0,
-  kCountWork = 1,  // unused
+  kCountReserved = 1,
   kCountTop = 2,
   kCountSideTicket = 3,  // tile tickets of the contact-side offset scan
   kCounterWords = 16,
@@ -112,10 +112,10 @@ struct Dev {
   uint32_t* active;       // contact index per active contact
   float4* rows;           // 4 planes of pair_capacity: plane k holds solver row word k of each active contact
   float* lambda;          // accumulated normal impulse per active contact
-  int4* cdelta;           // slot-major: 2 int4 per (slot, body) -- a contact side's Q20 deltas of the last pass
+  int4* cdelta;           // slot-major: 2 int4 per (slot, body) -- one body's Q20 deltas of a group of sides, last pass
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)
   int32_t* deltas;        // 6 per body, Q20 fixed-point velocity deltas (overflow path)
-  uint32_t* slot_count;   // contact sides per body this tick
+  uint32_t* slot_count;   // delta slots per body this tick (per group of sides; per side for the per-body passes)
   float4* sbody_alt;      // second solver record buffer: per-body passes read one and write the other
   uint2* side_slot;       // per active contact: its sides' slots in their bodies (uncapped)
   uint32_t* side_group;   // per active contact: group representatives and delta slots of its sides

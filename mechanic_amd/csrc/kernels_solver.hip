@@ -114,10 +114,10 @@ __device__ __forceinline__ Q20 side_delta(const SolverBody& s, v3 arm, v3 impuls
   return q;
 }
 
-// A contact side's deltas go to its body's slot, so each body's sides lie
-// together for the gather.
-// Slot-major: for one slot, consecutive bodies are adjacent, so the bodies'
-// gather reads whole cache lines per wavefront; a side's 32 bytes stay in one line.
+// One delta slot of a body: the Q20 deltas of the body's sides in one group of
+// 64 contacts, summed. Slot-major: for one slot, consecutive bodies are
+// adjacent, so the bodies' gather reads whole cache lines per wavefront; a
+// slot's 32 bytes stay in one line.
 __device__ __forceinline__ void put_side(const Dev& d, uint32_t body, uint32_t slot, const Q20& q) {
   int4* p = d.cdelta + 2 * ((size_t)slot * d.body_count + body);
   p[0] = make_int4(q.v[0], q.v[1], q.v[2], q.v[3]);
@@ -705,8 +705,8 @@ __global__ __launch_bounds__(256) void k_copy_records(const float4* __restrict__
 
 // solve_apply (collision.wgsl:1050-1069); skipped when no contact is active
 // (finalize_active_contacts, collision.wgsl:744). Each body sums the Q20 deltas
-// its contact sides stored (wrapping 32-bit sums: the arithmetic of atomicAdd on
-// i32, in any order) and adds them to its velocities. The last pass also writes
+// in its slots (wrapping 32-bit sums: the arithmetic of atomicAdd on i32, in any
+// order) and adds them to its velocities. The last pass also writes
 // the body velocities back to the state arrays.
 //
 // kApplyLanes adjacent lanes share a body: each loads every kApplyLanes-th of

@@ -916,50 +916,107 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   };
 
   uint32_t run_first = 1u, run_last = 0u;  // pending merged range (empty while first > last)
-  uint32_t cur = 0u;  // the root (c >= 2)
-  uint32_t steps_left = 2u * c + 1u;
-  while (cur != c) {
+  auto visit_range = [&](uint32_t first, uint32_t last) {
+    const uint32_t start = first > wave_first ? first : wave_first + 1u;
+    if (start > last) return;
+    if (run_first <= run_last && run_last + 1u == start) {
+      run_last = last;
+    } else {
+      if (run_first <= run_last) emit_range(run_first, run_last);
+      run_first = start;
+      run_last = last;
+    }
+  };
+  // A node or leaf with sorted leaves [first, last] is wanted when one of the
+  // packet's queries overlaps its box and lies left of its last leaf (and, in a
+  // batch, the node starts before the end of that query's scene).
+  auto wanted = [&](float4 mn, float4 mx, uint32_t first, uint32_t last) {
+    bool want = k.valid && last > k.p && overlaps(k.qmn, k.qmx, mn, mx);
+    if (d.scene_count > 1 && want) want = first < k.scene_end;
+    return __ballot(want) != 0ull;
+  };
+
+  // Depth-first walk over the internal nodes the packet enters (frames: left
+  // word and sorted leaf range). Each step loads both children of the frame's
+  // node at once -- in Karras order they sit at the split and right after it --
+  // and tests them, so a rejected child costs no round trip of its own. Wanted
+  // leaves and buckets become leaf ranges; a wanted right sibling waits on a
+  // wave-uniform stack held one entry per lane (left word kInvalid marks a leaf
+  // range), so the ranges come out in depth-first order as before.
+  uint32_t stack_left = 0, stack_first = 0, stack_last = 0;
+  uint32_t depth = 0;
+  uint32_t fr_left, fr_first = 0u, fr_last = c - 1u;
+  bool walking = false;
+  {
+    const float4 mn = d.inodes[0], mx = d.inodes[1];
+    if (wanted(mn, mx, 0u, c - 1u)) {
+      fr_left = __float_as_uint(mn.w);
+      if (fr_left & kBucketBit) visit_range(0u, c - 1u);
+      else walking = true;
+    }
+  }
+  uint32_t steps_left = c;
+  while (walking) {
     if (steps_left-- == 0u) {
       if (lane == 0) atomicOr(&diag[kDiagFlags], 1u);
       break;
     }
-    const uint32_t pos = cur & kPosMask;
-    uint32_t first, last, next;
-    if (cur & kLeafBit) {
-      first = last = pos;
-      next = __float_as_uint(d.leaves[2 * (size_t)pos + 1].w);
-    } else {
-      const float4 mn = d.inodes[2 * (size_t)pos];
-      const float4 mx = d.inodes[2 * (size_t)pos + 1];
-      const uint32_t left_word = __float_as_uint(mn.w);
-      const uint32_t esc = __float_as_uint(mx.w);
-      const uint32_t range_last = (esc & kPosMask) - 1u;
-      bool want = k.valid && range_last > k.p && overlaps(k.qmn, k.qmx, mn, mx);
-      // In a batch, skip subtrees that start past the end of the query's scene.
-      if (d.scene_count > 1 && want) want = d.ranges[pos].x < k.scene_end;
-      if (__ballot(want) == 0ull) {
-        cur = esc;
-        continue;
-      }
-      if (!(left_word & kBucketBit)) {
-        cur = left_word;
-        continue;
-      }
-      first = left_word & (kBucketBit - 1u);
-      last = range_last;
-      next = esc;
-    }
-    const uint32_t start = first > wave_first ? first : wave_first + 1u;
-    if (start <= last) {
-      if (run_first <= run_last && run_last + 1u == start) {
-        run_last = last;
+    const uint32_t s = fr_left & kPosMask;  // split: left child s, right child s + 1
+    const bool l_leaf = s == fr_first, r_leaf = s + 1u == fr_last;
+    const float4* lrec = (l_leaf ? d.leaves : d.inodes) + 2 * (size_t)s;
+    const float4* rrec = (r_leaf ? d.leaves : d.inodes) + 2 * (size_t)(s + 1u);
+    const float4 lmn = lrec[0], lmx = lrec[1], rmn = rrec[0], rmx = rrec[1];
+    const bool wl = wanted(lmn, lmx, fr_first, s);
+    const bool wr = wanted(rmn, rmx, s + 1u, fr_last);
+    const uint32_t l_word = __float_as_uint(lmn.w), r_word = __float_as_uint(rmn.w);
+    const bool l_range = l_leaf || (l_word & kBucketBit);
+    const bool r_range = r_leaf || (r_word & kBucketBit);
+    const uint32_t r_first = s + 1u, r_last = fr_last;
+    bool next = false;
+    if (wl) {
+      if (l_range) {
+        visit_range(fr_first, s);
       } else {
-        if (run_first <= run_last) emit_range(run_first, run_last);
-        run_first = start;
-        run_last = last;
+        next = true;
+        fr_left = l_word;
+        fr_last = s;
       }
     }
-    cur = next;
+    if (wr) {
+      if (next) {
+        if (depth == 64u) {  // deeper than the key bits allow
+          if (lane == 0) atomicOr(&diag[kDiagFlags], 1u);
+          break;
+        }
+        if (lane == depth) {
+          stack_left = r_range ? kInvalid : r_word;
+          stack_first = r_first;
+          stack_last = r_last;
+        }
+        ++depth;
+      } else if (r_range) {
+        visit_range(r_first, r_last);
+      } else {
+        next = true;
+        fr_left = r_word;
+        fr_first = r_first;
+        fr_last = r_last;
+      }
+    }
+    while (!next && depth != 0u) {
+      --depth;
+      const uint32_t left = bcast(stack_left, depth);
+      const uint32_t first = bcast(stack_first, depth), last = bcast(stack_last, depth);
+      if (left == kInvalid) {
+        visit_range(first, last);
+      } else {
+        next = true;
+        fr_left = left;
+        fr_first = first;
+        fr_last = last;
+      }
+    }
+    walking = next;
   }
   if (run_first <= run_last) emit_range(run_first, run_last);
   if (!kInline && lane == 0) d.work_count[wave_first / 64u] = n_items;

@@ -99,8 +99,8 @@ def autotune(make_handle: Callable, warm: int = 20, ticks: int = 40, space: Opti
              candidates=()) -> dict:
     """Coordinate descent from the defaults, every trial from `state` when given
     (the scene as it is when it is measured); then the winner, the defaults and
-    any extra `candidates` are measured again over twice the ticks and the
-    fastest is kept. Returns it, its time, the default's time and every trial."""
+    any extra `candidates` are measured again over twice the ticks, in two
+    alternating rounds, and the fastest is kept. Returns it, its time, the default's time and every trial."""
     space = space or SPACE
     current = dict(DEFAULTS)
     trials = []
@@ -116,12 +116,17 @@ def autotune(make_handle: Callable, warm: int = 20, ticks: int = 40, space: Opti
             trials.append((cand, ms))
             if ms < best_ms:
                 best_ms, current = ms, cand
-    finalists = [current, dict(DEFAULTS)] + [dict(DEFAULTS, **c) for c in candidates if c]
-    confirmed = []
-    for cand in finalists:
-        if any(cand == c for c, _ in confirmed):
-            continue
-        confirmed.append((cand, measure(make_handle, cand, warm, 2 * ticks, state)))
+    finalists = []
+    for cand in [current, dict(DEFAULTS)] + [dict(DEFAULTS, **c) for c in candidates if c]:
+        if cand not in finalists:
+            finalists.append(cand)
+    # Two alternating rounds over fresh handles, so that drift between handles
+    # and over time does not favour whichever finalist happens to run first.
+    best = [float("inf")] * len(finalists)
+    for _ in range(2):
+        for i, cand in enumerate(finalists):
+            best[i] = min(best[i], measure(make_handle, cand, warm, 2 * ticks, state))
+    confirmed = list(zip(finalists, best))
     current, best_ms = min(confirmed, key=lambda x: x[1])
     default_ms = next(ms for c, ms in confirmed if c == dict(DEFAULTS))
     return {"tuning": current, "ms": best_ms, "default_ms": default_ms, "trials": trials + confirmed}

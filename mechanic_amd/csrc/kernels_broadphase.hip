@@ -800,9 +800,14 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
 // lane loads one candidate, candidates that miss the packet's union box are
 // dropped by one ballot, and all lanes test each remaining candidate after a
 // v_readlane broadcast.
-__device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
-                                           const Packet& k, uint32_t start, uint32_t last) {
-  const bool check_suppression = d.suppression_count != 0;
+// The candidate test is branch-free: the lanes whose leaf lies left of candidate
+// position x (p < x, the lanes below x - first) are a prefix of the wavefront,
+// taken as a mask rather than compared per lane; invalid lanes (p >= C) never
+// lie left of a candidate. The scene bound (batches) and the suppression list
+// are separate instantiations, chosen once per range.
+template <bool kBatch, bool kSuppress>
+__device__ __forceinline__ void scan_range_as(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
+                                              const Packet& k, uint32_t start, uint32_t last) {
   for (uint32_t base = start; base <= last; base += 64u) {
     const uint32_t q = base + lane;
     float4 cmn = make_float4(0.f, 0.f, 0.f, 0.f), cmx = cmn;
@@ -816,17 +821,27 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
     while (todo != 0ull) {
       const uint32_t j = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
       todo &= todo - 1ull;
-      const float4 bmn = make_float4(bcast(cmn.x, j), bcast(cmn.y, j), bcast(cmn.z, j), 0.0f);
-      const float4 bmx = make_float4(bcast(cmx.x, j), bcast(cmx.y, j), bcast(cmx.z, j), 0.0f);
-      const uint32_t other = bcast(__float_as_uint(cmn.w), j);
+      const float bx0 = bcast(cmn.x, j), by0 = bcast(cmn.y, j), bz0 = bcast(cmn.z, j);
+      const float bx1 = bcast(cmx.x, j), by1 = bcast(cmx.y, j), bz1 = bcast(cmx.z, j);
       const uint32_t other_body = bcast(cbody, j);
+      const uint32_t x = base + j;  // > k.first: ranges start right of the packet's first leaf
+      const uint32_t left = (uint32_t)__builtin_amdgcn_readfirstlane((int)(x - k.first));  // wave-uniform
+      const uint64_t lanes_left = left >= 64u ? ~0ull : (1ull << left) - 1ull;
+      bool hit = (k.qmn.x <= bx1) & (k.qmn.y <= by1) & (k.qmn.z <= bz1) & (bx0 <= k.qmx.x) & (by0 <= k.qmx.y) &
+                 (bz0 <= k.qmx.z) & (other_body != k.self_body);
       // A merged range may run into the next scene: candidates past the end of
       // the query's scene are no pairs.
-      bool hit = k.valid && base + j > k.p && base + j < k.scene_end && overlaps(k.qmn, k.qmx, bmn, bmx) &&
-                 other_body != k.self_body;
-      if (hit && check_suppression) hit = !suppressed(d, k.self_body, other_body);
-      const uint64_t mask = __ballot(hit);
+      if constexpr (kBatch) hit &= x < k.scene_end;
+      uint64_t mask = __builtin_amdgcn_ballot_w64(hit) & lanes_left;
+      if constexpr (kSuppress) {
+        if (mask != 0ull) {
+          hit = ((mask >> lane) & 1ull) && !suppressed(d, k.self_body, other_body);
+          mask = __builtin_amdgcn_ballot_w64(hit);
+        }
+      }
       if (mask != 0ull) {
+        hit = (mask >> lane) & 1ull;
+        const uint32_t other = bcast(__float_as_uint(cmn.w), j);
         const uint32_t cnt = (uint32_t)__popcll(mask);
         if (st.count + cnt > kStage) flush_pairs(d, diag, st, lane);
         if (hit) {
@@ -836,6 +851,18 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
         st.count += cnt;
       }
     }
+  }
+}
+
+__device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
+                                           const Packet& k, uint32_t start, uint32_t last) {
+  const bool batch = d.scene_count > 1, suppress = d.suppression_count != 0;
+  if (batch) {
+    if (suppress) scan_range_as<true, true>(d, diag, st, lane, k, start, last);
+    else scan_range_as<true, false>(d, diag, st, lane, k, start, last);
+  } else {
+    if (suppress) scan_range_as<false, true>(d, diag, st, lane, k, start, last);
+    else scan_range_as<false, false>(d, diag, st, lane, k, start, last);
   }
 }
 

@@ -722,6 +722,16 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ float4 uniform4(float4 v) {
+  return make_float4(uniform(v.x), uniform(v.y), uniform(v.z), uniform(v.w));
+}
+
+// Lanes [0, n) of a wavefront (n wave-uniform).
+__device__ __forceinline__ uint64_t lanes_below(uint32_t n) { return n >= 64u ? ~0ull : (1ull << n) - 1ull; }
+
 constexpr uint32_t kStage = 512;  // staged pairs per wavefront
 
 struct PairStage {
@@ -825,8 +835,7 @@ __device__ __forceinline__ void scan_range_as(const Dev& d, uint32_t* diag, Pair
       const float bx1 = bcast(cmx.x, j), by1 = bcast(cmx.y, j), bz1 = bcast(cmx.z, j);
       const uint32_t other_body = bcast(cbody, j);
       const uint32_t x = base + j;  // > k.first: ranges start right of the packet's first leaf
-      const uint32_t left = (uint32_t)__builtin_amdgcn_readfirstlane((int)(x - k.first));  // wave-uniform
-      const uint64_t lanes_left = left >= 64u ? ~0ull : (1ull << left) - 1ull;
+      const uint64_t lanes_left = lanes_below((uint32_t)__builtin_amdgcn_readfirstlane((int)(x - k.first)));
       bool hit = (k.qmn.x <= bx1) & (k.qmn.y <= by1) & (k.qmn.z <= bz1) & (bx0 <= k.qmx.x) & (by0 <= k.qmx.y) &
                  (bz0 <= k.qmx.z) & (other_body != k.self_body);
       // A merged range may run into the next scene: candidates past the end of
@@ -912,19 +921,31 @@ __device__ __forceinline__ void flush_pairs_block(const Dev& d, uint32_t* diag, 
 // them, cut into items of at most 64 leaves, in the packet's kItemStage slots
 // for k_scan to spread over many wavefronts (ranges past the slots are scanned
 // here).
+//
+// One workgroup per packet: its wavefronts (walkers) descend together and split
+// at the first nodes where both children must be entered, each half of the group
+// taking one child, until every walker walks alone. A packet whose union box
+// covers a large part of the tree thus shares its walk -- the longest walks set
+// the kernel's time -- and each range is recorded by one walker only (the first
+// of its group), so the packet's items are the same set as one walker's.
 template <bool kInline>
 __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
+  __shared__ uint32_t slots;  // the packet's item slots taken so far
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t c = d.collider_count;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wave_first = (xcd_block() * blockDim.x + threadIdx.x) & ~63u;
-  if (wave_first >= c) return;  // whole wavefront past the last leaf
+  const uint32_t packet = xcd_block();
+  const uint32_t wave_first = packet * 64u;
+  if (wave_first >= c) return;  // whole workgroup past the last leaf
+  const uint32_t walker = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t walkers = blockDim.x >> 6;
+  if (threadIdx.x == 0) slots = 0u;
+  __syncthreads();
   const Packet k = load_packet(d, wave_first, lane);
-  PairStage st{stage_mem[threadIdx.x >> 6], 0u};
-  uint4* items = d.work + (size_t)(wave_first / 64u) * kItemStage;
-  uint32_t n_items = 0;  // wave-uniform
+  PairStage st{stage_mem[walker], 0u};
+  uint4* items = d.work + (size_t)packet * kItemStage;
 
   auto emit_range = [&](uint32_t first, uint32_t last) {
     if constexpr (kInline) {
@@ -932,9 +953,11 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
     } else {
       for (uint32_t a = first; a <= last; a += 64u) {
         const uint32_t b = last - a > 63u ? a + 63u : last;
-        if (n_items < kItemStage) {
-          if (lane == 0) items[n_items] = make_uint4(wave_first, a, b, 0u);
-          n_items += 1;
+        uint32_t slot = 0u;
+        if (lane == 0) slot = atomicAdd(&slots, 1u);
+        slot = bcast(slot, 0u);
+        if (slot < kItemStage) {
+          if (lane == 0) items[slot] = make_uint4(wave_first, a, b, 0u);
         } else {
           scan_range(d, diag, st, lane, k, a, b);
         }
@@ -957,29 +980,35 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   // A node or leaf with sorted leaves [first, last] is wanted when one of the
   // packet's queries overlaps its box and lies left of its last leaf (and, in a
   // batch, the node starts before the end of that query's scene).
+  // Branch-free: the lanes left of `last` (p < last) are a prefix of the
+  // packet, and invalid lanes (p >= C) are never left of a node.
+  const bool batch = d.scene_count > 1;
   auto wanted = [&](float4 mn, float4 mx, uint32_t first, uint32_t last) {
-    bool want = k.valid && last > k.p && overlaps(k.qmn, k.qmx, mn, mx);
-    if (d.scene_count > 1 && want) want = first < k.scene_end;
-    return __ballot(want) != 0ull;
+    bool want = (k.qmn.x <= mx.x) & (k.qmn.y <= mx.y) & (k.qmn.z <= mx.z) & (mn.x <= k.qmx.x) &
+                (mn.y <= k.qmx.y) & (mn.z <= k.qmx.z);
+    if (batch) want &= first < k.scene_end;
+    return (__builtin_amdgcn_ballot_w64(want) & lanes_below(last > wave_first ? last - wave_first : 0u)) != 0ull;
   };
 
   // Depth-first walk over the internal nodes the packet enters (frames: left
   // word and sorted leaf range). Each step loads both children of the frame's
   // node at once -- in Karras order they sit at the split and right after it --
   // and tests them, so a rejected child costs no round trip of its own. Wanted
-  // leaves and buckets become leaf ranges; a wanted right sibling waits on a
-  // wave-uniform stack held one entry per lane (left word kInvalid marks a leaf
-  // range), so the ranges come out in depth-first order as before.
+  // leaves and buckets become leaf ranges. A walker alone keeps a wanted right
+  // sibling on a wave-uniform stack held one entry per lane (left word kInvalid
+  // marks a leaf range), so its ranges come out in depth-first order; a group
+  // splits instead, and its first walker records a range beside a frame at once.
+  uint32_t group_lo = 0u, group_n = walkers;  // walkers [group_lo, group_lo + group_n) walk this path
   uint32_t stack_left = 0, stack_first = 0, stack_last = 0;
   uint32_t depth = 0;
   uint32_t fr_left, fr_first = 0u, fr_last = c - 1u;
   bool walking = false;
   {
-    const float4 mn = d.inodes[0], mx = d.inodes[1];
+    const float4 mn = uniform4(d.inodes[0]), mx = uniform4(d.inodes[1]);
     if (wanted(mn, mx, 0u, c - 1u)) {
       fr_left = __float_as_uint(mn.w);
-      if (fr_left & kBucketBit) visit_range(0u, c - 1u);
-      else walking = true;
+      if (!(fr_left & kBucketBit)) walking = true;
+      else if (walker == 0u) visit_range(0u, c - 1u);
     }
   }
   uint32_t steps_left = c;
@@ -988,48 +1017,75 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
       if (lane == 0) atomicOr(&diag[kDiagFlags], 1u);
       break;
     }
+    // Wave-uniform by construction (ballots, broadcasts, scalar records); said
+    // so, the compiler keeps the walk on the scalar unit.
+    fr_left = (uint32_t)__builtin_amdgcn_readfirstlane((int)fr_left);
+    fr_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)fr_first);
+    fr_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)fr_last);
     const uint32_t s = fr_left & kPosMask;  // split: left child s, right child s + 1
     const bool l_leaf = s == fr_first, r_leaf = s + 1u == fr_last;
     const float4* lrec = (l_leaf ? d.leaves : d.inodes) + 2 * (size_t)s;
     const float4* rrec = (r_leaf ? d.leaves : d.inodes) + 2 * (size_t)(s + 1u);
-    const float4 lmn = lrec[0], lmx = lrec[1], rmn = rrec[0], rmx = rrec[1];
+    // The records are the same for every lane: held in scalar registers, the
+    // walk's bookkeeping below runs on the scalar unit.
+    const float4 lmn = uniform4(lrec[0]), lmx = uniform4(lrec[1]), rmn = uniform4(rrec[0]), rmx = uniform4(rrec[1]);
     const bool wl = wanted(lmn, lmx, fr_first, s);
     const bool wr = wanted(rmn, rmx, s + 1u, fr_last);
     const uint32_t l_word = __float_as_uint(lmn.w), r_word = __float_as_uint(rmn.w);
     const bool l_range = l_leaf || (l_word & kBucketBit);
     const bool r_range = r_leaf || (r_word & kBucketBit);
     const uint32_t r_first = s + 1u, r_last = fr_last;
+    const bool lead = walker == group_lo;
     bool next = false;
-    if (wl) {
-      if (l_range) {
-        visit_range(fr_first, s);
-      } else {
-        next = true;
+    if (group_n > 1u && wl && wr && !l_range && !r_range) {
+      // Two frames: the lower half of the group takes the left, the upper the right.
+      const uint32_t half = group_n >> 1;
+      next = true;
+      if (walker < group_lo + half) {
         fr_left = l_word;
         fr_last = s;
-      }
-    }
-    if (wr) {
-      if (next) {
-        if (depth == 64u) {  // deeper than the key bits allow
-          if (lane == 0) atomicOr(&diag[kDiagFlags], 1u);
-          break;
-        }
-        if (lane == depth) {
-          stack_left = r_range ? kInvalid : r_word;
-          stack_first = r_first;
-          stack_last = r_last;
-        }
-        ++depth;
-      } else if (r_range) {
-        visit_range(r_first, r_last);
       } else {
-        next = true;
         fr_left = r_word;
         fr_first = r_first;
         fr_last = r_last;
+        group_lo += half;
+      }
+      group_n = half;
+    } else {
+      if (wl) {
+        if (l_range) {
+          if (lead) visit_range(fr_first, s);
+        } else {
+          next = true;
+          fr_left = l_word;
+          fr_last = s;
+        }
+      }
+      if (wr) {
+        if (next && group_n > 1u) {
+          if (lead) visit_range(r_first, r_last);  // a range: two frames would have split
+        } else if (next) {
+          if (depth == 64u) {  // deeper than the key bits allow
+            if (lane == 0) atomicOr(&diag[kDiagFlags], 1u);
+            break;
+          }
+          if (lane == depth) {
+            stack_left = r_range ? kInvalid : r_word;
+            stack_first = r_first;
+            stack_last = r_last;
+          }
+          ++depth;
+        } else if (r_range) {
+          if (lead) visit_range(r_first, r_last);
+        } else {
+          next = true;
+          fr_left = r_word;
+          fr_first = r_first;
+          fr_last = r_last;
+        }
       }
     }
+    // Only a walker alone pushes, so it alone pops.
     while (!next && depth != 0u) {
       --depth;
       const uint32_t left = bcast(stack_left, depth);
@@ -1046,8 +1102,11 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
     walking = next;
   }
   if (run_first <= run_last) emit_range(run_first, run_last);
-  if (!kInline && lane == 0) d.work_count[wave_first / 64u] = n_items;
   if (st.count) flush_pairs(d, diag, st, lane);
+  if constexpr (!kInline) {
+    __syncthreads();
+    if (threadIdx.x == 0) d.work_count[packet] = slots < kItemStage ? slots : kItemStage;
+  }
 }
 
 // Exclusive prefix of the per-packet item counts (one workgroup).
@@ -1265,13 +1324,13 @@ void launch_lbvh(hipStream_t s, const Dev& d) {
 
 void launch_traverse(hipStream_t s, const Dev& d) {
   if (d.collider_count < 2) return;
-  const uint32_t blocks = blocks_for(d.collider_count);
+  const uint32_t packets = (d.collider_count + 63u) / 64u;  // one workgroup of walkers each
   if (d.t_traverse_kind == 1u) {
-    k_traverse_lane<<<blocks, kBlock, 0, s>>>(d);
+    k_traverse_lane<<<blocks_for(d.collider_count), kBlock, 0, s>>>(d);
   } else if (d.t_traverse_kind == 2u) {
-    k_traverse<true><<<blocks, kBlock, 0, s>>>(d);
+    k_traverse<true><<<packets, kBlock, 0, s>>>(d);
   } else {
-    k_traverse<false><<<blocks, kBlock, 0, s>>>(d);
+    k_traverse<false><<<packets, kBlock, 0, s>>>(d);
     k_scan_prefix<<<1, 1024, 0, s>>>(d);
     k_scan<<<d.t_scan_blocks ? d.t_scan_blocks : 4096u, kBlock, 0, s>>>(d);
   }

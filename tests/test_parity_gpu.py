@@ -122,6 +122,19 @@ def test_broadphase_bit_exact(name, ticks, variant):
     compare_broadphase(gpu, ora, scene)
 
 
+@pytest.mark.parametrize("variant", list(BROADPHASE))
+@pytest.mark.parametrize("shape", [(5, 5, 5), (9, 2, 9), (2, 33, 2)])
+def test_broadphase_partial_packets_bit_exact(shape, variant):
+    # 125, 162 and 132 overlapping cuboids: the last packet of 64 leaves is
+    # partial, and the walkers of a packet split at nodes whose children both
+    # overlap it. The keys, tree and pair set must still be the oracle's.
+    scene = mh.cuboid_pile(*shape, seed=11, spacing=0.22)
+    gpu, ora = make(scene, use_graph=False, tuning=BROADPHASE[variant])
+    gpu.debug_tick(1, 2)
+    ora.step_partial(1, 2)
+    assert compare_broadphase(gpu, ora, scene) > 0
+
+
 def contact_key(rows):
     u = rows.view(np.uint32)
     return u

@@ -1,6 +1,8 @@
 // Broadphase: Morton keys and AABBs (with analytic ground contacts fused in),
-// an LDS-staged bitonic sort, Karras LBVH topology, an atomic bottom-up refit
-// and a stackless pair traversal with wavefront-ballot-compacted output.
+// an LSD radix sort (or an LDS-staged bitonic sort), Karras LBVH topology with
+// the refit of each workgroup's subtrees in LDS and of the nodes spanning
+// workgroups from prefix/suffix boxes, and a packet pair traversal with
+// wavefront-ballot-compacted output.
 #include "internal.h"
 #include "mh_geometry.h"
 #include "mh_math.h"
@@ -687,12 +689,12 @@ __global__ __launch_bounds__(256) void k_refit_top(Dev d) {
 }
 
 // ---------------------------------------------------------------------------
-// Pair traversal (lbvh.wgsl:321-398). A wavefront walks the tree as one packet
-// of 64 consecutive sorted leaves (queries), without a stack, following escapes:
-// it descends into a node when any of its queries overlaps the node and the node
-// holds leaves right of that query. At a bucket it scans the bucket's leaves,
-// 64 at a time, each lane testing its query against every candidate broadcast
-// from the lane that loaded it. A lane only pairs with leaves right of its own,
+// Pair traversal (lbvh.wgsl:321-398). A packet of 64 consecutive sorted leaves
+// (queries, one per lane) walks the tree together: it descends into a node when
+// any of its queries overlaps the node and the node holds leaves right of that
+// query. At a bucket it scans the bucket's leaves, 64 at a time, each lane
+// testing its query against every candidate broadcast from the lane that loaded
+// it. A lane only pairs with leaves right of its own,
 // so each unordered overlapping pair is found once and written as
 // (lower collider, higher collider): the set the reference's per-leaf stack
 // traversal emits. Pairs are staged in LDS per wavefront and flushed with one

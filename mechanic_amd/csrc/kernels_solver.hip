@@ -338,18 +338,33 @@ __device__ __forceinline__ bool contact_pass(const Dev& d, const SolverBody& sa,
 // (kSlotOverflow: add with atomics instead).
 constexpr uint32_t kGroupSlotShift = 14;
 
-__device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t lane) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+// Per-wavefront LDS table of the group's bodies: open addressing over kSideTable
+// entries (at most 128 bodies, so the table never fills), each holding a body
+// and the first side position seen for it.
+constexpr uint32_t kSideTable = 256;
+
+__device__ __forceinline__ uint32_t side_slot_of(uint32_t* bodies, uint32_t body) {
+  uint32_t h = (body * 0x9e3779b1u) >> 24;  // top 8 bits: kSideTable entries
+  for (uint32_t probe = 0; probe < kSideTable; ++probe) {
+    const uint32_t prev = atomicCAS(&bodies[h], kInvalid, body);
+    if (prev == kInvalid || prev == body) return h;
+    h = (h + 1u) & (kSideTable - 1u);
+  }
+  return kInvalid;  // unreachable: at most 128 bodies in 256 entries
 }
 
 __global__ __launch_bounds__(256) void k_group_sides(Dev d) {
+  __shared__ uint32_t lds_body[kBlock / 64][kSideTable];
+  __shared__ uint32_t lds_first[kBlock / 64][kSideTable];
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
   const uint32_t groups = (count + 63u) / 64u;
-  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  for (uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < groups; g += waves) {
+  uint32_t* bodies = lds_body[wave];
+  uint32_t* firsts = lds_first[wave];
+  for (uint32_t g = blockIdx.x * (blockDim.x >> 6) + wave; g < groups; g += waves) {
     const uint32_t ai = g * 64u + lane;
     const bool valid = ai < count;
     uint32_t ba = kInvalid, bb = kInvalid;
@@ -359,15 +374,27 @@ __global__ __launch_bounds__(256) void k_group_sides(Dev d) {
       bb = __float_as_uint(r3.y);
     }
     // Side positions: 2 * lane for A, 2 * lane + 1 for B. First position of the
-    // same body, over the group's sides in order.
-    uint32_t rep_a = 2u * lane, rep_b = 2u * lane + 1u;
-    for (uint32_t q = 0; q < 64u; ++q) {
-      const uint32_t qa = lane_value(ba, q), qb = lane_value(bb, q);
-      if (qa == ba && 2u * q < rep_a) rep_a = 2u * q;
-      if (qb == ba && 2u * q + 1u < rep_a) rep_a = 2u * q + 1u;
-      if (qa == bb && 2u * q < rep_b) rep_b = 2u * q;
-      if (qb == bb && 2u * q + 1u < rep_b) rep_b = 2u * q + 1u;
+    // same body, over the group's sides in order: the minimum position any side
+    // of the body records in the body's table entry.
+    for (uint32_t e = lane; e < kSideTable; e += 64u) {
+      bodies[e] = kInvalid;
+      firsts[e] = kInvalid;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t ha = ba != kInvalid ? side_slot_of(bodies, ba) : kInvalid;
+    const uint32_t hb = bb != kInvalid ? side_slot_of(bodies, bb) : kInvalid;
+    if (ha != kInvalid) atomicMin(&firsts[ha], 2u * lane);
+    if (hb != kInvalid) atomicMin(&firsts[hb], 2u * lane + 1u);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t rep_a = ha != kInvalid ? firsts[ha] : 2u * lane;
+    const uint32_t rep_b = hb != kInvalid ? firsts[hb] : 2u * lane + 1u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the next group clears the same table
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     uint32_t own_a = 0u, own_b = 0u;
     if (valid && rep_a == 2u * lane) own_a = atomicAdd(&d.slot_count[ba], 1u);
     if (valid && bb != kInvalid && rep_b == 2u * lane + 1u) own_b = atomicAdd(&d.slot_count[bb], 1u);

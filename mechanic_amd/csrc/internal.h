@@ -232,7 +232,6 @@ void launch_end_tick(hipStream_t s, const Dev& d);
 void launch_integrate(hipStream_t s, const Dev& d, bool collisions, bool snapshot_now);
 void launch_snapshot(hipStream_t s, const Dev& d);
 void launch_external_impulse(hipStream_t s, const Dev& d, float4 point, float4 impulse, uint32_t body);
-void launch_clear_table(hipStream_t s, const Dev& d);
 void launch_morton(hipStream_t s, const Dev& d);
 void launch_sort(hipStream_t s, const Dev& d);
 void launch_lbvh(hipStream_t s, const Dev& d);

@@ -140,17 +140,6 @@ __global__ void k_external_impulse(Dev d, float4 world_point, float4 impulse4, u
   d.ang[body] = f4(ang, 0.0f);
 }
 
-// Zero-fills the manifold index that this tick inserts into.
-__global__ __launch_bounds__(256) void k_clear_table(Dev d) {
-  const uint64_t tick = tick_u64(d);
-  uint64_t* table = d.htab[tick & 1];
-  const uint32_t n = d.hash_mask + 1;
-  ulonglong2* t2 = reinterpret_cast<ulonglong2*>(table);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n / 2; i += gridDim.x * blockDim.x) {
-    t2[i] = make_ulonglong2(0ull, 0ull);
-  }
-}
-
 static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 void launch_set_tick(hipStream_t s, const Dev& d, uint64_t tick) {
@@ -168,12 +157,6 @@ void launch_snapshot(hipStream_t s, const Dev& d) {
 }
 void launch_external_impulse(hipStream_t s, const Dev& d, float4 point, float4 impulse, uint32_t body) {
   k_external_impulse<<<1, 64, 0, s>>>(d, point, impulse, body);
-}
-void launch_clear_table(hipStream_t s, const Dev& d) {
-  uint32_t n2 = (d.hash_mask + 1) / 2;
-  uint32_t blocks = blocks_for(n2);
-  if (blocks > 2048) blocks = 2048;
-  k_clear_table<<<blocks, kBlock, 0, s>>>(d);
 }
 
 }  // namespace mh

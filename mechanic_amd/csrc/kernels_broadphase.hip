@@ -30,7 +30,7 @@ __device__ __forceinline__ uint32_t morton_code(v3 c) {
 }
 
 // compute_morton (lbvh.wgsl:108-130) + generate_ground_contacts
-// (collision.wgsl:622-664). One thread per sort slot; slots past the collider
+// (collision.wgsl:622-664), and the clear of this tick's manifold index. One thread per sort slot; slots past the collider
 // count get the padding key (0xffffffff, slot).
 __global__ __launch_bounds__(256) void k_morton(Dev d) {
   __shared__ uint32_t lds_append[kBlock / 64 + 1];
@@ -38,6 +38,13 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kCounterWords) d.counters[i] = 0u;  // sort_count >= 256 threads always run
+  {
+    // This tick's manifold index starts empty (prepare_contacts inserts into it;
+    // the previous tick's index, read by the lookups, is the other one).
+    ulonglong2* table = reinterpret_cast<ulonglong2*>(d.htab[tick & 1]);
+    const uint32_t words = (d.hash_mask + 1u) / 2u, stride = gridDim.x * blockDim.x;
+    for (uint32_t j = i; j < words; j += stride) table[j] = make_ulonglong2(0ull, 0ull);
+  }
   const bool radix = d.t_sort_kind == 0u;
   if (radix) {
     const uint32_t stride = gridDim.x * blockDim.x;

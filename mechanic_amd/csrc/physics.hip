@@ -129,7 +129,6 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uin
   mark(kMarkMechanism);
   if (p->collisions && stop != 1) {
     if (mech) launch_world_masses(s, d);
-    launch_clear_table(s, d);
     launch_morton(s, d);
     mark(kMarkMorton);
     launch_sort(s, d);

@@ -40,6 +40,10 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d, int collisions, int sn
   const uint64_t tick = tick_u64(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  // The radix sort's digit histogram and tile tickets start the tick at zero
+  // (k_morton adds its workgroups' digit counts into it).
+  if (d.t_sort_kind == 0u)
+    for (uint32_t j = i; j < kRadixHistWords; j += gridDim.x * blockDim.x) d.rs_hist[j] = 0u;
   if (i >= d.body_count) return;
 
   float4 position = d.pos[i];

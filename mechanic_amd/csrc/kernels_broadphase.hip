@@ -29,11 +29,18 @@ __device__ __forceinline__ uint32_t morton_code(v3 c) {
   return expand_bits(quantize(c.x)) | (expand_bits(quantize(c.y)) << 1) | (expand_bits(quantize(c.z)) << 2);
 }
 
+// Digit of a sort key in one LSD radix pass (8 bits above the collider index).
+__device__ __forceinline__ uint32_t radix_digit(const Dev& d, uint64_t key, uint32_t pass) {
+  return (uint32_t)(key >> (d.key_index_bits + 8u * pass)) & 0xffu;
+}
+
 // compute_morton (lbvh.wgsl:108-130) + generate_ground_contacts
-// (collision.wgsl:622-664), and the clear of this tick's manifold index. One thread per sort slot; slots past the collider
+// (collision.wgsl:622-664), the clear of this tick's manifold index and the
+// radix sort's digit counts. One thread per sort slot; slots past the collider
 // count get the padding key (0xffffffff, slot).
 __global__ __launch_bounds__(256) void k_morton(Dev d) {
   __shared__ uint32_t lds_append[kBlock / 64 + 1];
+  __shared__ uint32_t lds_hist[kRadixMaxPasses * 256];  // this workgroup's digit counts per radix pass
   const uint64_t tick = tick_now(d);
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,10 +55,11 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
   const bool radix = d.t_sort_kind == 0u;
   if (radix) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t j = i; j < kRadixHistWords; j += stride) d.rs_hist[j] = 0u;
     const uint32_t status_words = d.rs_passes * d.rs_blocks * 256u;
     for (uint32_t j = i; j < status_words; j += stride) d.rs_status[j] = 0u;
+    for (uint32_t j = threadIdx.x; j < d.rs_passes * 256u; j += blockDim.x) lds_hist[j] = 0u;
   }
+  __syncthreads();
   bool emit = false;
   ColliderPose p;
   v3 support;
@@ -68,9 +76,13 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
     const uint64_t code = morton_code(p.center);
     // The radix sort starts from the buffer its last pass does not write.
     uint64_t* keys = radix && (d.rs_passes & 1u) ? d.keys_alt : d.keys;
-    keys[i] = d.scene_count > 1
-                  ? ((uint64_t)d.cscene[i] << (30u + d.key_index_bits)) | (code << d.key_index_bits) | i
-                  : (code << 32) | i;
+    const uint64_t key = d.scene_count > 1
+                             ? ((uint64_t)d.cscene[i] << (30u + d.key_index_bits)) | (code << d.key_index_bits) | i
+                             : (code << 32) | i;
+    keys[i] = key;
+    if (radix)
+      for (uint32_t pass = 0; pass < d.rs_passes; ++pass)
+        atomicAdd(&lds_hist[pass * 256u + radix_digit(d, key, pass)], 1u);
     // Ground plane y = 0.
     const v3 down = mk(0.0f, -1.0f, 0.0f);
     support = support_point(p, down);
@@ -86,6 +98,13 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
   // The grid covers sort_count (a multiple of 256) exactly, so every thread of
   // every block reaches the block-wide append.
   uint32_t row = block_append(&diag[kDiagContacts], emit, lds_append);
+  // The append's barriers also complete the digit counts: add them to the
+  // radix sort's histogram (zeroed by k_integrate at the start of the tick).
+  if (radix)
+    for (uint32_t j = threadIdx.x; j < d.rs_passes * 256u; j += blockDim.x) {
+      const uint32_t n = lds_hist[j];
+      if (n) atomicAdd(&d.rs_hist[j], n);
+    }
   if (!emit) return;
   if (row < d.pair_capacity) {
     float4 bp = d.pos[p.body];
@@ -244,25 +263,6 @@ constexpr uint32_t kRsAggregate = 1u << 30;  // status word: the tile's own coun
 constexpr uint32_t kRsPrefix = 2u << 30;     // status word: count over tiles 0..this one
 constexpr uint32_t kRsCount = (1u << 30) - 1u;
 constexpr uint32_t kRsSpinLimit = 1u << 22;  // reads of an unpublished word before giving up
-
-__device__ __forceinline__ uint32_t radix_digit(const Dev& d, uint64_t key, uint32_t pass) {
-  return (uint32_t)(key >> (d.key_index_bits + 8u * pass)) & 0xffu;
-}
-
-// Digit counts of every pass over all keys (rs_hist zeroed by k_morton).
-__global__ __launch_bounds__(1024) void k_radix_hist(Dev d, const uint64_t* src) {
-  __shared__ uint32_t hist[kRadixMaxPasses * 256];
-  const uint32_t passes = d.rs_passes;
-  for (uint32_t i = threadIdx.x; i < passes * 256u; i += blockDim.x) hist[i] = 0u;
-  __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.collider_count; i += gridDim.x * blockDim.x) {
-    const uint64_t key = src[i];
-    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&hist[p * 256u + radix_digit(d, key, p)], 1u);
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < passes * 256u; i += blockDim.x)
-    if (hist[i]) atomicAdd(&d.rs_hist[i], hist[i]);
-}
 
 // Exclusive prefix sum over a 256-thread workgroup.
 __device__ __forceinline__ uint32_t exclusive_scan256(uint32_t v, uint32_t* wave_sums) {
@@ -1296,8 +1296,6 @@ void launch_sort(hipStream_t s, const Dev& d) {
     if (d.rs_blocks == 0) return;
     uint64_t* buf[2] = {d.keys, d.keys_alt};
     const uint32_t odd = d.rs_passes & 1u;
-    const uint32_t hist_blocks = d.rs_blocks < 64u ? d.rs_blocks : 64u;
-    k_radix_hist<<<hist_blocks, 1024, 0, s>>>(d, buf[odd]);
     for (uint32_t pass = 0; pass < d.rs_passes; ++pass) {
       const uint32_t from = (pass + odd) & 1u;
       k_radix_pass<<<d.rs_blocks, 256, 0, s>>>(d, buf[from], buf[from ^ 1u], pass);

@@ -410,6 +410,8 @@ __device__ void publish(const Dev& d, const float4* links) {
   __syncthreads();
 }
 
+constexpr uint32_t kPathBatch = 4;  // path entries of a closure side loaded per step
+
 // Deterministic FP64 sum over the workgroup: a fixed butterfly within each
 // wavefront, then every thread adds the wavefronts' sums in wavefront order, so
 // all threads hold the same bits. Two barriers in all.
@@ -553,26 +555,45 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
         const float* q = d.pcg + 8 * (size_t)c;
         return it == 0 ? q[3] : q[2] + beta * q[3];
       };
-      // branch_product for every closure (closure.wgsl:188-220).
-      for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
-        const float4* s = d.cl_state + 6 * (size_t)k;
-        v3 pos_sum[2], axis_sum[2];
-        for (uint32_t side = 0; side < 2; ++side) {
+      // branch_product for every closure (closure.wgsl:188-220): adjacent lanes
+      // take a closure's two sides, each walking its path kPathBatch entries a
+      // step with all their loads in flight, summing in path order as before.
+      for (uint32_t slot = threadIdx.x; slot < 2u * ((nc + 31u) / 32u) * 32u; slot += blockDim.x) {
+        const uint32_t k = slot >> 1, side = slot & 1u;
+        v3 pp = mk(0.0f, 0.0f, 0.0f), pa = pp;
+        if (k < nc) {
+          const float4* s = d.cl_state + 6 * (size_t)k;
           const v3 end_anchor = xyz(s[side ? 2 : 0]), end_axis = xyz(s[side ? 3 : 1]);
           const float sign = side ? -1.0f : 1.0f;
-          v3 pp = mk(0.0f, 0.0f, 0.0f), pa = pp;
-          for (uint32_t e = d.cpath_offset[2 * k + side]; e < d.cpath_offset[2 * k + side + 1]; ++e) {
-            const uint32_t c = d.cpath[e];
-            const v3 jaxis = xyz(d.joints[2 * (size_t)c]), janchor = xyz(d.joints[2 * (size_t)c + 1]);
-            const float direction = direction_of(c);
-            pp = pp + (cross(jaxis, end_anchor - janchor) * sign) * direction;
-            pa = pa + (cross(jaxis, end_axis) * sign) * direction;
+          const uint32_t e0 = d.cpath_offset[2 * k + side], e1 = d.cpath_offset[2 * k + side + 1];
+          for (uint32_t e = e0; e < e1; e += kPathBatch) {
+            uint32_t cs[kPathBatch];
+            float4 ja[kPathBatch], jb[kPathBatch];
+            float dir[kPathBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kPathBatch; ++j) cs[j] = d.cpath[e + j < e1 ? e + j : e];
+#pragma unroll
+            for (uint32_t j = 0; j < kPathBatch; ++j) {
+              ja[j] = d.joints[2 * (size_t)cs[j]];
+              jb[j] = d.joints[2 * (size_t)cs[j] + 1];
+              dir[j] = direction_of(cs[j]);
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kPathBatch; ++j) {
+              if (e + j >= e1) break;
+              const v3 jaxis = xyz(ja[j]), janchor = xyz(jb[j]);
+              pp = pp + (cross(jaxis, end_anchor - janchor) * sign) * dir[j];
+              pa = pa + (cross(jaxis, end_axis) * sign) * dir[j];
+            }
           }
-          pos_sum[side] = pp;
-          axis_sum[side] = pa;
         }
-        d.cl_product[2 * (size_t)k] = f4(pos_sum[0] + pos_sum[1], 0.0f);
-        d.cl_product[2 * (size_t)k + 1] = f4(axis_sum[0] + axis_sum[1], 0.0f);
+        // Side 0 (even lane) adds side 1's sums from its neighbour.
+        const v3 pp1 = mk(__shfl_xor(pp.x, 1), __shfl_xor(pp.y, 1), __shfl_xor(pp.z, 1));
+        const v3 pa1 = mk(__shfl_xor(pa.x, 1), __shfl_xor(pa.y, 1), __shfl_xor(pa.z, 1));
+        if (k < nc && side == 0u) {
+          d.cl_product[2 * (size_t)k] = f4(pp + pp1, 0.0f);
+          d.cl_product[2 * (size_t)k + 1] = f4(pa + pa1, 0.0f);
+        }
       }
       __syncthreads();
       // branch_transpose gathered per coordinate (closure.wgsl:222-252, 365-381).

@@ -410,7 +410,8 @@ __device__ void publish(const Dev& d, const float4* links) {
   __syncthreads();
 }
 
-constexpr uint32_t kPathBatch = 4;  // path entries of a closure side loaded per step
+constexpr uint32_t kPathBatch = 4;
+constexpr uint32_t kLdsCoordinates = 4608;  // coordinates whose PCG data fits in LDS  // path entries of a closure side loaded per step
 
 // Deterministic FP64 sum over the workgroup: a fixed butterfly within each
 // wavefront, then every thread adds the wavefronts' sums in wavefront order, so
@@ -454,10 +455,34 @@ __device__ __forceinline__ void joint_of(const Dev& d, const float4* links, uint
 __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
   __shared__ uint32_t state[2];
   __shared__ double scratch[kPoseThreads];
+  // With at most kLdsCoordinates coordinates, the step's joint frames and the
+  // PCG's z and p are also held in LDS (147 KiB), where every iteration's
+  // gathers read them; d.joints and d.pcg stay the full record.
+  __shared__ float lds_joint[kLdsCoordinates][6];
+  __shared__ float lds_zp[kLdsCoordinates][2];
   if (*reinterpret_cast<volatile uint32_t*>(&d.mech_state[0]) != 0u) return;
   uint32_t* diag = diag_now(d);
   const float4* links = d.links[d.jump_rounds & 1u];
   const uint32_t nc = d.closure_count, nq = d.coordinate_count;
+  const bool in_lds = nq <= kLdsCoordinates;
+  auto joint = [&](uint32_t c, v3& axis, v3& anchor) {
+    if (in_lds) {
+      const float* j = lds_joint[c];
+      axis = mk(j[0], j[1], j[2]);
+      anchor = mk(j[3], j[4], j[5]);
+    } else {
+      axis = xyz(d.joints[2 * (size_t)c]);
+      anchor = xyz(d.joints[2 * (size_t)c + 1]);
+    }
+  };
+  auto set_zp = [&](uint32_t c, float* q, float z, float p) {
+    q[2] = z;
+    q[3] = p;
+    if (in_lds) {
+      lds_zp[c][0] = z;
+      lds_zp[c][1] = p;
+    }
+  };
   {
     if (threadIdx.x == 0) {
       state[0] = 0u;
@@ -507,12 +532,22 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       joint_of(d, links, d.coord_body[c], axis, anchor);
       d.joints[2 * (size_t)c] = f4(axis, 0.0f);
       d.joints[2 * (size_t)c + 1] = f4(anchor, 0.0f);
+      if (in_lds) {
+        float* j = lds_joint[c];
+        j[0] = axis.x;
+        j[1] = axis.y;
+        j[2] = axis.z;
+        j[3] = anchor.x;
+        j[4] = anchor.y;
+        j[5] = anchor.z;
+      }
     }
     __syncthreads();
     // Gradient and Jacobi diagonal (accumulate_branch), then the PCG start.
     double rz_part = 0.0;
     for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
-      const v3 jaxis = xyz(d.joints[2 * (size_t)c]), janchor = xyz(d.joints[2 * (size_t)c + 1]);
+      v3 jaxis, janchor;
+      joint(c, jaxis, janchor);
       double g = 0.0, dg = 0.0;
       for (uint32_t e = d.coord_offset[c]; e < d.coord_offset[c + 1]; ++e) {
         const uint32_t entry = d.coord_entries[e], k = entry >> 1, side = entry & 1u;
@@ -537,8 +572,7 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       float* q = d.pcg + 8 * (size_t)c;
       q[0] = 0.0f;
       q[1] = residual;
-      q[2] = pre;
-      q[3] = pre;
+      set_zp(c, q, pre, pre);
       q[4] = 0.0f;
       q[5] = gf;
       q[6] = df;
@@ -552,8 +586,16 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
     float beta = 0.0f;
     for (int it = 0; it < kPcgIterations; ++it) {
       auto direction_of = [&](uint32_t c) {
-        const float* q = d.pcg + 8 * (size_t)c;
-        return it == 0 ? q[3] : q[2] + beta * q[3];
+        float z, p;
+        if (in_lds) {
+          z = lds_zp[c][0];
+          p = lds_zp[c][1];
+        } else {
+          const float* q = d.pcg + 8 * (size_t)c;
+          z = q[2];
+          p = q[3];
+        }
+        return it == 0 ? p : z + beta * p;
       };
       // branch_product for every closure (closure.wgsl:188-220): adjacent lanes
       // take a closure's two sides, each walking its path kPathBatch entries a
@@ -574,8 +616,10 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
             for (uint32_t j = 0; j < kPathBatch; ++j) cs[j] = d.cpath[e + j < e1 ? e + j : e];
 #pragma unroll
             for (uint32_t j = 0; j < kPathBatch; ++j) {
-              ja[j] = d.joints[2 * (size_t)cs[j]];
-              jb[j] = d.joints[2 * (size_t)cs[j] + 1];
+              v3 jx, jn;
+              joint(cs[j], jx, jn);
+              ja[j] = f4(jx, 0.0f);
+              jb[j] = f4(jn, 0.0f);
               dir[j] = direction_of(cs[j]);
             }
 #pragma unroll
@@ -599,7 +643,8 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       // branch_transpose gathered per coordinate (closure.wgsl:222-252, 365-381).
       double dad_part = 0.0;
       for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
-        const v3 jaxis = xyz(d.joints[2 * (size_t)c]), janchor = xyz(d.joints[2 * (size_t)c + 1]);
+        v3 jaxis, janchor;
+        joint(c, jaxis, janchor);
         float op = 0.0f;
         for (uint32_t e = d.coord_offset[c]; e < d.coord_offset[c + 1]; ++e) {
           const uint32_t entry = d.coord_entries[e], k = entry >> 1, side = entry & 1u;
@@ -620,12 +665,12 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       for (uint32_t c = threadIdx.x; c < nq; c += blockDim.x) {
         const float p = direction_of(c);
         float* q = d.pcg + 8 * (size_t)c;
-        q[3] = p;
         q[0] = q[0] + alpha * p;
         q[1] = q[1] - alpha * q[4];
         const float df = q[6];
-        q[2] = df > 1.0e-12f ? q[1] / df : 0.0f;
-        rz_next_part += (double)q[1] * (double)q[2];
+        const float z = df > 1.0e-12f ? q[1] / df : 0.0f;
+        set_zp(c, q, z, p);
+        rz_next_part += (double)q[1] * (double)z;
       }
       const double rz_next = block_sum(rz_next_part, scratch);  // its barriers publish the update
       beta = fabs(rz) > 1.0e-20 ? (float)(rz_next / rz) : 0.0f;

@@ -410,8 +410,8 @@ __device__ void publish(const Dev& d, const float4* links) {
   __syncthreads();
 }
 
-constexpr uint32_t kPathBatch = 4;
-constexpr uint32_t kLdsCoordinates = 4608;  // coordinates whose PCG data fits in LDS  // path entries of a closure side loaded per step
+constexpr uint32_t kPathIndices = 16;  // path indices of a closure side loaded per step
+constexpr uint32_t kLdsCoordinates = 4608;  // coordinates whose PCG data fits in LDS
 
 // Deterministic FP64 sum over the workgroup: a fixed butterfly within each
 // wavefront, then every thread adds the wavefronts' sums in wavefront order, so
@@ -584,6 +584,19 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
     // iteration's z and p, which the update phase then replaces, so no pass
     // is spent storing it.
     float beta = 0.0f;
+    // Closure sides of the product phase, two per closure on adjacent lanes; a
+    // multiple of 64 so that lane pairs stay whole.
+    const uint32_t slots = 2u * ((nc + 31u) / 32u) * 32u;
+    v3 my_anchor = mk(0.0f, 0.0f, 0.0f), my_axis = my_anchor;
+    uint32_t my_e0 = 0u, my_e1 = 0u;
+    if ((threadIdx.x >> 1) < nc) {
+      const uint32_t k = threadIdx.x >> 1, side = threadIdx.x & 1u;
+      const float4* s = d.cl_state + 6 * (size_t)k;
+      my_anchor = xyz(s[side ? 2 : 0]);
+      my_axis = xyz(s[side ? 3 : 1]);
+      my_e0 = d.cpath_offset[2 * k + side];
+      my_e1 = d.cpath_offset[2 * k + side + 1];
+    }
     for (int it = 0; it < kPcgIterations; ++it) {
       auto direction_of = [&](uint32_t c) {
         float z, p;
@@ -598,36 +611,33 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
         return it == 0 ? p : z + beta * p;
       };
       // branch_product for every closure (closure.wgsl:188-220): adjacent lanes
-      // take a closure's two sides, each walking its path kPathBatch entries a
-      // step with all their loads in flight, summing in path order as before.
-      for (uint32_t slot = threadIdx.x; slot < 2u * ((nc + 31u) / 32u) * 32u; slot += blockDim.x) {
+      // take a closure's two sides. A side's end point and path range do not
+      // change during the solve (the first slot of each thread keeps them in
+      // registers); its path indices are loaded kPathIndices at a time and the
+      // joints and directions read from LDS, summing in path order as before.
+      for (uint32_t slot = threadIdx.x; slot < slots; slot += blockDim.x) {
         const uint32_t k = slot >> 1, side = slot & 1u;
         v3 pp = mk(0.0f, 0.0f, 0.0f), pa = pp;
         if (k < nc) {
+          const bool own = slot == threadIdx.x;
           const float4* s = d.cl_state + 6 * (size_t)k;
-          const v3 end_anchor = xyz(s[side ? 2 : 0]), end_axis = xyz(s[side ? 3 : 1]);
+          const v3 end_anchor = own ? my_anchor : xyz(s[side ? 2 : 0]);
+          const v3 end_axis = own ? my_axis : xyz(s[side ? 3 : 1]);
           const float sign = side ? -1.0f : 1.0f;
-          const uint32_t e0 = d.cpath_offset[2 * k + side], e1 = d.cpath_offset[2 * k + side + 1];
-          for (uint32_t e = e0; e < e1; e += kPathBatch) {
-            uint32_t cs[kPathBatch];
-            float4 ja[kPathBatch], jb[kPathBatch];
-            float dir[kPathBatch];
+          const uint32_t e0 = own ? my_e0 : d.cpath_offset[2 * k + side];
+          const uint32_t e1 = own ? my_e1 : d.cpath_offset[2 * k + side + 1];
+          for (uint32_t e = e0; e < e1; e += kPathIndices) {
+            uint32_t cs[kPathIndices];
 #pragma unroll
-            for (uint32_t j = 0; j < kPathBatch; ++j) cs[j] = d.cpath[e + j < e1 ? e + j : e];
+            for (uint32_t j = 0; j < kPathIndices; ++j) cs[j] = d.cpath[e + j < e1 ? e + j : e];
 #pragma unroll
-            for (uint32_t j = 0; j < kPathBatch; ++j) {
-              v3 jx, jn;
-              joint(cs[j], jx, jn);
-              ja[j] = f4(jx, 0.0f);
-              jb[j] = f4(jn, 0.0f);
-              dir[j] = direction_of(cs[j]);
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < kPathBatch; ++j) {
+            for (uint32_t j = 0; j < kPathIndices; ++j) {
               if (e + j >= e1) break;
-              const v3 jaxis = xyz(ja[j]), janchor = xyz(jb[j]);
-              pp = pp + (cross(jaxis, end_anchor - janchor) * sign) * dir[j];
-              pa = pa + (cross(jaxis, end_axis) * sign) * dir[j];
+              v3 jaxis, janchor;
+              joint(cs[j], jaxis, janchor);
+              const float dir = direction_of(cs[j]);
+              pp = pp + (cross(jaxis, end_anchor - janchor) * sign) * dir;
+              pa = pa + (cross(jaxis, end_axis) * sign) * dir;
             }
           }
         }

@@ -325,6 +325,19 @@ MH_API int mh_mlp_train(mh_mlp* mlp, uint32_t epochs, uint32_t batch, float lear
 MH_API int mh_mlp_gradient(mh_mlp* mlp, float* gradient, float* loss);
 /* MFMA inference of host feature rows; out receives 3 floats per row. */
 MH_API int mh_mlp_predict(mh_mlp* mlp, const float* features, size_t rows, float* out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (online fine-tuning inside a serving or stepping loop): the loss and
+ * gradient of mh_mlp_gradient, with the same bits, over `rows` caller-owned
+ * DEVICE rows (features: 16 floats per row, targets: 3 floats per row), written
+ * to caller-owned DEVICE outputs (gradient: mh_mlp_param_count() floats, loss:
+ * 1 float). Allocates nothing: the per-block partial gradients pass through a
+ * fixed 2.8 MB window in the library's static device memory, held by every
+ * GPU the library runs on until the process ends. rows must be 1..2^28
+ * (MH_ERR_INVALID_ARGUMENT otherwise). Runs on the network's stream and
+ * returns when the outputs are written. Calls that overlap on one GPU (several
+ * threads or networks at once) share that window and race. */
+MH_API int mh_mlp_gradient_noalloc(mh_mlp* mlp, const float* features, const float* targets, size_t rows,
+                                   float* gradient, float* loss);
 
 /* Attaches a network to a handle (NULL detaches). The network must outlive its
  * use; weights changed by training are seen by the next tick. */

@@ -256,6 +256,7 @@ def load() -> ctypes.CDLL:
         "mh_mlp_train": (c_int, [H, c_uint32, c_uint32, c_float, POINTER(c_float)]),
         "mh_mlp_gradient": (c_int, [H, POINTER(c_float), POINTER(c_float)]),
         "mh_mlp_predict": (c_int, [H, POINTER(c_float), c_size_t, POINTER(c_float)]),
+        "mh_mlp_gradient_noalloc": (c_int, [H, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
         "mh_physics_set_mlp": (c_int, [H, H]),
         "mh_physics_read_training_rows": (c_int, [H, POINTER(c_float), POINTER(c_float), c_size_t,
                                                   POINTER(c_size_t)]),

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mechanic_hip.h"
@@ -49,6 +50,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float* net, const flo
 // ---------------------------------------------------------------------------
 constexpr int kChunk = 32;
 
+// `order` maps batch rows to dataset rows; null takes the rows in dataset order.
 __global__ __launch_bounds__(256) void k_mlp_grad(const float* net, const float* feat, const float* targ,
                                                   const uint32_t* order, uint32_t first, uint32_t rows,
                                                   float inv_batch, float* partial) {
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* net, const float*
     const uint32_t row = base + c;
     float v = 0.0f;
     if (row < first + rows) {
-      const uint32_t src = order[row];
+      const uint32_t src = order ? order[row] : row;
       v = (feat[(size_t)src * kFeatures + k] - net[kOffInMean + k]) * net[kOffInScale + k];
     }
     X[c][k] = v;
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* net, const float*
       for (int k = 0; k < kH; ++k) y += H2[t][k] * net[kOffW3 + k * kOut + o];
       float dy = 0.0f;
       if (row < first + rows) {
-        const uint32_t src = order[row];
+        const uint32_t src = order ? order[row] : row;
         const float target = (targ[3 * (size_t)src + o] - net[kOffOutMean + o]) / net[kOffOutScale + o];
         const float err = y - target;
         loss += err * err;
@@ -173,6 +175,29 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* net, float* m1, float* 
   m2[p] = v;
   net[p] -= lr * (a / correction1) / (sqrtf(v / correction2) + eps);
 }
+
+// Running sums over windows of block partials: sums[p] (kParams gradient
+// entries, then the loss) continues from its previous value -- zero for the
+// first window -- and adds the window's partials in block order. Window after
+// window this is the same left-to-right sum k_mlp_adam forms over all blocks
+// at once, so the result has the same bits.
+__global__ __launch_bounds__(256) void k_mlp_fold(const float* partial, uint32_t blocks, int first_window,
+                                                  float* grad, float* loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kParams) return;
+  float* out = p == kParams ? loss : grad + p;
+  float g = first_window ? 0.0f : *out;
+  for (uint32_t b = 0; b < blocks; ++b) g += partial[(size_t)b * (kParams + 1) + p];
+  *out = g;
+}
+
+// Block partials of the allocation-free gradient (mh_mlp_gradient_noalloc):
+// one window of kFoldBlocks blocks of kChunk rows (4,096 rows, 2.8 MB). Every
+// device the library runs on holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns.
+constexpr uint32_t kFoldBlocks = 128;
+constexpr size_t kMaxNoallocRows = (size_t)1 << 28;  // the dataset cap of mh_mlp_set_dataset
+static __device__ __align__(256) float scratch[kFoldBlocks * (kParams + 1)];
 
 }  // namespace mh
 
@@ -392,6 +417,56 @@ int mh_mlp_gradient(mh_mlp* m, float* grad, float* loss) {
   MLP_TRY(hipStreamSynchronize(m->stream));
   MLP_TRY(hipMemcpy(grad, m->scratch, kParams * sizeof(float), hipMemcpyDeviceToHost));
   if (loss) MLP_TRY(hipMemcpy(loss, m->scratch + kParams, sizeof(float), hipMemcpyDeviceToHost));
+  return MH_OK;
+}
+
+namespace {
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current.
+constexpr int kMaxDevices = 64;
+std::mutex scratch_lock;
+float* scratch_ptr[kMaxDevices] = {};
+
+int device_scratch(float** out) {
+  int device = 0;
+  MLP_TRY(hipGetDevice(&device));
+  if (device < 0 || device >= kMaxDevices) return MH_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> guard(scratch_lock);
+  if (!scratch_ptr[device]) {
+    void* p = nullptr;
+    MLP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(mh::scratch)));
+    scratch_ptr[device] = static_cast<float*>(p);
+  }
+  *out = scratch_ptr[device];
+  return MH_OK;
+}
+
+}  // namespace
+
+// mh_mlp_gradient over caller-owned device rows, with no allocation: the block
+// partials go through the file's static device window, kFoldBlocks blocks at a
+// time, folded in block order into the caller's gradient and loss.
+int mh_mlp_gradient_noalloc(mh_mlp* m, const float* features, const float* targets, size_t rows, float* gradient,
+                            float* loss) {
+  if (!m || !features || !targets || !gradient || !loss) return MH_ERR_INVALID_ARGUMENT;
+  if (rows == 0 || rows > kMaxNoallocRows) return MH_ERR_INVALID_ARGUMENT;
+  float* partial = nullptr;
+  const int rc = device_scratch(&partial);
+  if (rc != MH_OK) return rc;
+  const uint32_t n = (uint32_t)rows;
+  const uint32_t blocks = (n + kChunk - 1) / kChunk;
+  const float inv_batch = 1.0f / (float)n;
+  for (uint32_t b0 = 0; b0 < blocks; b0 += kFoldBlocks) {
+    const uint32_t wb = blocks - b0 < kFoldBlocks ? blocks - b0 : kFoldBlocks;
+    const uint32_t first = b0 * kChunk;
+    // Rows of this window: the block at b0 starts at row `first`; the kernel
+    // keeps the whole batch's row bound so the last block masks the same rows.
+    k_mlp_grad<<<wb, 256, 0, m->stream>>>(m->net, features, targets, nullptr, first, n - first, inv_batch, partial);
+    k_mlp_fold<<<(kParams + 1 + 255) / 256, 256, 0, m->stream>>>(partial, wb, b0 == 0 ? 1 : 0, gradient, loss);
+  }
+  MLP_TRY(hipGetLastError());
+  MLP_TRY(hipStreamSynchronize(m->stream));
   return MH_OK;
 }
 

@@ -94,6 +94,13 @@ class WarmStartNet:
         _check(_lib.load().mh_mlp_gradient(self._h, _fp(g), ctypes.byref(loss)), "mh_mlp_gradient")
         return g, float(loss.value)
 
+    def gradient_noalloc(self, features_ptr: int, targets_ptr: int, rows: int, gradient_ptr: int,
+                         loss_ptr: int) -> None:
+        """mh_mlp_gradient_noalloc: the gradient and loss of `rows` device rows
+        into device outputs (raw device addresses), allocating nothing."""
+        _check(_lib.load().mh_mlp_gradient_noalloc(self._h, features_ptr, targets_ptr, int(rows), gradient_ptr,
+                                                   loss_ptr), "mh_mlp_gradient_noalloc")
+
     def predict(self, features: np.ndarray) -> np.ndarray:
         f = np.ascontiguousarray(features, np.float32)
         out = np.empty((f.shape[0], _lib.MLP_OUTPUTS), np.float32)

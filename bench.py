@@ -5,21 +5,24 @@ jittered, yawed lattice with 0.27 m spacing dropped 0.5 m onto the ground
 plane; dt = 1/60 s, gravity -9.81, friction 0.6, restitution 0, 8 solver
 iterations. One step is one whole tick (integration, broadphase, narrowphase,
 contact solve, snapshot) replayed as a captured hipGraph with inputs resident in
-HBM. Also reports, as sub-objects of the same JSON line: the learned warm start
-against the reference's cache warm start at equal contact residual, the
-256-scene batch (autotuned launch shape against the default) and the articulated
-100,000-body scene.
+HBM. The timed window is the settled pile: the K steps after the first
+SETTLE_TICKS (300) ticks from the drop, where config 1's 600 steps spend most of
+their time; the falling window (ticks 2..K+1) and the whole 600-step run are
+reported beside it. Also reports, as sub-objects of the same JSON line: the
+learned warm start against the reference's cache warm start at equal contact
+residual and penetration, the 256-scene batch (autotuned launch shape against
+the default), the articulated 100,000-body scene and the CPU oracle at one and
+at all host threads.
 
     python bench.py [--gpus 1] [--steps 600] [--warmup 60] [--no-cpu] [--no-learned]
-                    [--no-batched] [--no-articulated]
+                    [--no-batched] [--no-articulated] [--no-traffic] [--no-graph]
 """
 import argparse
 import json
 import os
 import sys
+import tempfile
 import time
-
-import numpy
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -63,12 +66,15 @@ WRITE_KIB_SCALE = 1.0
 
 
 TRAFFIC_SETTLE = 5  # ticks that rebuild the contact cache after the state is restored
+SETTLE_TICKS = 300  # ticks from the drop before the timed window
 
 
 def traffic_probe(args):
     """Child of measure_traffic, under rocprofv3 --pmc restricted to k_solve: the
-    headline run's final body state in a fresh handle, stepped TRAFFIC_SETTLE +
-    profile_ticks ticks (long runs under counter collection crash the profiler)."""
+    settled state in a fresh handle, stepped TRAFFIC_SETTLE + profile_ticks
+    ticks (long runs under counter collection crash the profiler). Writes the
+    active-contact and delta-slot counts of the profiled ticks, so the parent
+    forms the algorithmic bytes of exactly the launches the counters cover."""
     import numpy as np
 
     import mechanic_amd as mh
@@ -78,16 +84,25 @@ def traffic_probe(args):
     gpu.set_tuning(**json.loads(args.tuning_json))
     state = np.load(args.state)
     gpu.write_bodies(state["poses"], state["velocities"])
-    gpu.dispatch_ticks(1, TRAFFIC_SETTLE + args.profile_ticks)
-    gpu.synchronize()
+    first = int(state["tick"]) + 1
+    gpu.dispatch_ticks(first, TRAFFIC_SETTLE)
+    counts = []
+    for i in range(args.profile_ticks):
+        gpu.dispatch_tick(first + TRAFFIC_SETTLE + i)
+        r = gpu.read_last_tick()
+        slots = np.frombuffer(gpu.read_buffer(mh._lib.BUFFER_BODY_SLOTS, 4 * scene.body_count), np.uint32)
+        counts.append([r.active_contact_count, int(slots.sum(dtype=np.int64))])
     gpu.close()
+    with open(args.counts_out, "w") as f:
+        json.dump({"bodies": scene.body_count, "ticks": counts}, f)
 
 
 def measure_traffic(args, tuning, iterations, state):
     """HBM bytes per k_solve launch: FETCH_SIZE and WRITE_SIZE from two
-    rocprofv3 --pmc runs (child processes) continuing the headline run's final
-    state, averaged over the launches of the last profile_ticks ticks and scaled
-    by the copy calibration."""
+    rocprofv3 --pmc runs (child processes) continuing the settled state,
+    averaged over the launches of the child's profile_ticks ticks and scaled by
+    the copy calibration, with the algorithmic bytes of the same launches.
+    A child that ends with a non-zero status is a failed measurement."""
     import shutil
     import subprocess
     import tempfile
@@ -97,55 +112,83 @@ def measure_traffic(args, tuning, iterations, state):
     if shutil.which("rocprofv3") is None:
         return None, "rocprofv3 not found"
     kib = {}
+    algorithmic = []
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+            counts_path = os.path.join(tmp, "counts.json")
             cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "k_solve", "-d", tmp, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--traffic-probe", "--state", state,
-                   "--profile-ticks", str(args.profile_ticks), "--tuning-json", json.dumps(tuning)]
+                   "--profile-ticks", str(args.profile_ticks), "--tuning-json", json.dumps(tuning),
+                   "--counts-out", counts_path]
             env = dict(os.environ, TMPDIR="/tmp")
             try:
-                # rocprofv3 may end with a signal of its own after the program and
-                # its results are complete: the results database decides.
                 res = subprocess.run(cmd, cwd="/tmp", env=env, timeout=300, stdout=subprocess.DEVNULL,
                                      stderr=subprocess.PIPE, text=True)
             except (subprocess.SubprocessError, OSError) as e:
                 return None, f"{counter} run failed: {type(e).__name__}"
+            tail = " | ".join(l for l in res.stderr.strip().splitlines()[-3:] if l)[-300:]
+            if res.returncode != 0 or not os.path.exists(counts_path):
+                return None, f"{counter} run exited with status {res.returncode}: {tail}"
             dbs = [os.path.join(r, f) for r, _, fs in os.walk(tmp) for f in fs if f.endswith("results.db")]
             value = counter_tail_average(dbs[0], "k_solve", counter, iterations * args.profile_ticks) if dbs else None
             if value is None:
-                tail = " | ".join(l for l in res.stderr.strip().splitlines()[-3:] if l)[-300:]
-                return None, f"{counter}: no k_solve samples (exit {res.returncode}: {tail})"
+                return None, f"{counter}: no k_solve samples ({tail})"
             kib[counter] = value
+            with open(counts_path) as f:
+                c = json.load(f)
+            algorithmic += [solve_kernel_bytes(c["bodies"], a, r) for a, r in c["ticks"]]
     fetch = kib["FETCH_SIZE"] * FETCH_KIB_SCALE * 1024.0
     write = kib["WRITE_SIZE"] * WRITE_KIB_SCALE * 1024.0
-    return {"bytes": fetch + write, "read": fetch, "written": write}, None
+    algo = sum(algorithmic) / len(algorithmic)
+    return {"bytes": fetch + write, "read": fetch, "written": write, "algorithmic": algo}, None
 
 
-def cpu_baseline(scene, friction, iterations, budget_s=15.0, max_ticks=40):
+def cpu_baseline(scene, friction, iterations, state):
+    """The FP32 oracle (the C++ restatement of the reference kernels) on the
+    host cores, from the same settled state as the timed window: all host
+    threads (capped at the 16 a GPU's share of the host holds) for up to 8 s,
+    then one thread for up to 12 s. Reports steps/s and ms per phase."""
+    import numpy as np
+
     from tests._oracle import Oracle
 
     try:
-        threads = len(os.sched_getaffinity(0))
+        available = len(os.sched_getaffinity(0))
     except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
-    ora = Oracle(scene, iterations=iterations, friction=friction, threads=threads)
-    t0 = time.perf_counter()
-    ticks = 0
-    while ticks < max_ticks and time.perf_counter() - t0 < budget_s:
-        ticks += 1
-        ora.step(ticks)
-    dt = time.perf_counter() - t0
-    return {"value": ticks / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP32 C++ restatement, OpenMP {threads} threads, ticks 1..{ticks} of the same "
-                      f"100k-cuboid pile ({dt:.1f} s)"}
+        available = os.cpu_count() or 1
+    threads = max(1, min(available, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
+    data = np.load(state)
+    first = int(data["tick"]) + 1
+    out = {}
+    for label, t, budget, cap in (("all", threads, 8.0, 30), ("one", 1, 12.0, 12)):
+        ora = Oracle(scene, iterations=iterations, friction=friction, threads=t)
+        ora.set_bodies(data["poses"], data["velocities"])
+        ora.step(first)  # rebuilds the manifold cache, untimed
+        ora.phase_ms(reset=True)
+        t0 = time.perf_counter()
+        ticks = 0
+        while ticks < cap and time.perf_counter() - t0 < budget:
+            ora.step(first + 1 + ticks)
+            ticks += 1
+        dt = time.perf_counter() - t0
+        phases = {k: round(v / ticks, 2) for k, v in ora.phase_ms().items() if v > 0.0}
+        out[label] = {"threads": t, "ticks": ticks, "steps_per_s": round(ticks / dt, 3), "phase_ms": phases}
+        del ora
+    return {"value": out["all"]["steps_per_s"], "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 C++ restatement (-O3, no contraction), settled 100k pile restored at tick "
+                      f"{first - 1}, one untimed tick, then {out['all']['ticks']} ticks on {threads} threads and "
+                      f"{out['one']['ticks']} on one thread",
+            "single_thread_steps_per_s": out["one"]["steps_per_s"],
+            "phase_ms_per_tick": {"threads_" + str(threads): out["all"]["phase_ms"],
+                                  "threads_1": out["one"]["phase_ms"]}}
 
 
-def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain, tuning=None):
+def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain, tuning=None, use_graph=True):
     """Learned warm start against the reference's cache warm start on the same
     pile: trained on 2M logged contacts of seeds 10-19 (32 solver passes), the
     smallest pass count whose mean contact residual over ticks 1..300 does not
-    exceed the plain 8-pass one, and its steps/s under the headline protocol."""
+    exceed the plain 8-pass one, with the mean and maximum penetration of both
+    runs, and its steps/s under the headline protocol (settled window)."""
     import numpy as np
     from mechanic_amd import learned as L
 
@@ -169,35 +212,38 @@ def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain
 
     ticks = 300
     base_cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=plain_iterations)
-    plain = L.residual_trace(scene, base_cfg, ticks)
+    plain = L.residual_stats(scene, base_cfg, ticks)
     tried = {}
     chosen = None
+    stats = None
     for k in (3, 4, 5, 6):
-        r = L.residual_trace(scene, mh.GpuPhysicsConfig(friction=friction, solver_iterations=k), ticks, net=net)
-        tried[k] = float(r.mean())
-        if r.mean() <= plain.mean():
-            chosen = k
+        r = L.residual_stats(scene, mh.GpuPhysicsConfig(friction=friction, solver_iterations=k), ticks, net=net)
+        tried[k] = float(r["rms"].mean())
+        if r["rms"].mean() <= plain["rms"].mean():
+            chosen, stats = k, r
             break
-    out = {"plain_iterations": plain_iterations, "mean_residual_plain": round(float(plain.mean()), 5),
+    mean3 = lambda r: {key: round(float(v.mean()), 6) for key, v in r.items()}  # noqa: E731
+    out = {"plain_iterations": plain_iterations, "mean_residual_plain": round(float(plain["rms"].mean()), 5),
            "mean_residual_learned": {str(k): round(v, 5) for k, v in tried.items()},
+           "plain_means": mean3(plain), "learned_means": None if stats is None else mean3(stats),
            "residual": "rms over active contacts of |max(0, lambda + (bias - v_n)/d) - lambda| after the last "
-                       f"pass, mean over ticks 1..{ticks}",
+                       f"pass; penetration mean and max over active contacts; all averaged over ticks 1..{ticks}",
            "train_rows": int(F.shape[0]), "train_data": "seeds 10-19 of the same 100k pile, 32 solver passes, "
                                                          "200k rows sampled from ticks 40..120 per seed",
            "collect_s": round(collect_s, 2), "train_s": round(train_s, 3), "epochs": 20,
            "final_loss": round(float(losses[-1]), 5), "learned_iterations": chosen}
     if chosen is None:
         return out
-    cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=chosen, use_graph=True,
+    cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=chosen, use_graph=use_graph,
                               warm_start=mh._lib.WARM_START_LEARNED)
     gpu = mh.GpuPhysics.new_with_config(scene, cfg)
     if tuning:
         gpu.set_tuning(**tuning)
     gpu.set_mlp(net)
-    gpu.dispatch_ticks(1, 1)
+    gpu.dispatch_ticks(1, SETTLE_TICKS)
     gpu.synchronize()
     t2 = time.perf_counter()
-    gpu.dispatch_ticks(2, steps)
+    gpu.dispatch_ticks(SETTLE_TICKS + 1, steps)
     gpu.synchronize()
     v = steps / (time.perf_counter() - t2)
     flags = gpu.read_last_tick().error_flags
@@ -219,16 +265,17 @@ def save_tuning(autotune, bodies, tuned, workload):
     autotune.save_cache(entries, out_path)
 
 
-def timed_run(make, steps, tuning=None):
+def timed_run(make, steps, tuning=None, settle=0):
     """Steps/s of a fresh handle under the headline protocol (graph captured on
-    tick 1, ticks 2..steps+1 timed), with its last-tick diagnostics."""
+    the first tick; with `settle`, ticks 1..settle run untimed first; the next
+    `steps` ticks are timed), with its last-tick diagnostics."""
     gpu = make()
     if tuning:
         gpu.set_tuning(**tuning)
-    gpu.dispatch_ticks(1, 1)
+    gpu.dispatch_ticks(1, max(settle, 1))
     gpu.synchronize()
     t0 = time.perf_counter()
-    gpu.dispatch_ticks(2, steps)
+    gpu.dispatch_ticks(max(settle, 1) + 1, steps)
     gpu.synchronize()
     value = steps / (time.perf_counter() - t0)
     rb = gpu.read_last_tick()
@@ -301,9 +348,12 @@ def main():
                     help="launch shape: autotune now, the committed cache, or the defaults")
     ap.add_argument("--no-articulated", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every tick's kernels directly instead of replaying the captured graph")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--tuning-json", default="{}", help=argparse.SUPPRESS)
     ap.add_argument("--state", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--counts-out", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe(args)
@@ -313,14 +363,19 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
+
     import mechanic_amd as mh
 
     friction, iterations = 0.6, 8
+    steps = args.steps
+    use_graph = not args.no_graph
     scene = mh.named_pile("pile_100k")
-    cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=iterations, use_graph=True)
+    cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=iterations, use_graph=use_graph)
+    make = lambda: mh.GpuPhysics.new_with_config(scene, cfg)  # noqa: E731
 
     # Warm-up on a throwaway handle (code objects, clocks, graph path).
-    warm = mh.GpuPhysics.new_with_config(scene, cfg)
+    warm = make()
     for t in range(1, args.warmup + 1):
         warm.dispatch_tick(t)
     warm.synchronize()
@@ -329,13 +384,12 @@ def main():
     # Launch shape (results never depend on it): the autotuner's winner for this
     # scene, the committed cache entry for its size bucket, or the defaults.
     from mechanic_amd import autotune
-    make = lambda: mh.GpuPhysics.new_with_config(scene, cfg)  # noqa: E731
     t_tune = time.perf_counter()
     if args.tuning == "auto":
-        # Tune on the pile as it is while it is timed (after ~300 ticks), not
-        # while it still falls: every trial restores that state.
+        # Tune on the settled pile, as it is in the timed window: every trial
+        # restores the state after SETTLE_TICKS ticks.
         pre = make()
-        pre.dispatch_ticks(1, 300)
+        pre.dispatch_ticks(1, SETTLE_TICKS)
         pre.synchronize()
         state = pre.read_bodies()
         pre.close()
@@ -347,37 +401,56 @@ def main():
     else:
         tuned = {"tuning": dict(autotune.DEFAULTS)}
     tune_s = time.perf_counter() - t_tune
-    default_value = timed_run(make, args.steps)[0] if args.tuning == "auto" else None
+    default_value = timed_run(make, steps, settle=SETTLE_TICKS)[0] if args.tuning == "auto" else None
 
-    gpu = mh.GpuPhysics.new_with_config(scene, cfg)
+    # The headline: ticks SETTLE_TICKS+1 .. SETTLE_TICKS+steps of one handle.
+    gpu = make()
     gpu.set_tuning(**tuned["tuning"])
-    gpu.dispatch_ticks(1, 1)  # captures the tick graph
+    gpu.dispatch_ticks(1, SETTLE_TICKS)  # the first tick captures the tick graph
     gpu.synchronize()
     t0 = time.perf_counter()
-    gpu.dispatch_ticks(2, args.steps)
+    gpu.dispatch_ticks(SETTLE_TICKS + 1, steps)
     gpu.synchronize()
     wall = time.perf_counter() - t0
-    steps = args.steps
     readback = gpu.read_last_tick()
+    value = steps / wall
+    tick = SETTLE_TICKS + steps
 
-    # Event-bracketed (uncaptured) ticks continuing the same run: ms per kernel group.
+    # Reference-style synchronous engine TPS: dispatch + diagnostics readback per tick.
+    sync_ticks = 60
+    t1 = time.perf_counter()
+    for i in range(sync_ticks):
+        gpu.dispatch_tick(tick + 1 + i)
+        gpu.read_last_tick()
+    engine_tps = sync_ticks / (time.perf_counter() - t1)
+    tick += sync_ticks
+
+    # The state the profiled ticks start from: the traffic children and the CPU
+    # baseline restore it.
+    state_fd, state_path = tempfile.mkstemp(suffix=".npz", dir="/tmp")
+    os.close(state_fd)
+    poses0, velocities0 = gpu.read_bodies()
+    np.savez(state_path, poses=poses0, velocities=velocities0, tick=np.int64(tick))
+
+    # Event-bracketed (uncaptured) ticks continuing the same run: ms per kernel
+    # group, and the device wall-clock spans of the solver-pass kernels.
     groups = ["integrate_ms", "morton_ms", "sort_ms", "lbvh_ms", "traverse_ms", "narrowphase_ms", "prepare_ms",
               "solve_ms"]
     acc = {g: 0.0 for g in groups}
-    kernel_ms = {"solve_kernel_ms": 0.0, "apply_kernel_ms": 0.0}
+    kernel_ms = {"solve_kernel_ms": 0.0, "apply_kernel_ms": 0.0, "solve_kernel_event_ms": 0.0}
     total = 0.0
     counts = []
-    first = steps + 2
     for i in range(args.profile_ticks):
-        t = gpu.profile_tick(first + i)
+        t = gpu.profile_tick(tick + 1 + i)
         r = gpu.read_last_tick()
-        slots = numpy.frombuffer(gpu.read_buffer(mh._lib.BUFFER_BODY_SLOTS, 4 * scene.body_count), numpy.uint32)
-        counts.append((r.pair_count, r.contact_count, r.active_contact_count, int(slots.sum(dtype=numpy.int64))))
+        slots = np.frombuffer(gpu.read_buffer(mh._lib.BUFFER_BODY_SLOTS, 4 * scene.body_count), np.uint32)
+        counts.append((r.pair_count, r.contact_count, r.active_contact_count, int(slots.sum(dtype=np.int64))))
         for g in groups:
             acc[g] += t[g]
         for g in kernel_ms:
             kernel_ms[g] += t[g]
         total += t["total_ms"]
+    gpu.close()
     npt = max(args.profile_ticks, 1)
     phase_ms = {g[:-3]: acc[g] / npt for g in groups}
     p = sum(c[0] for c in counts) / npt
@@ -389,40 +462,28 @@ def main():
     s = 1 << (c - 1).bit_length()
     algo = algorithmic_bytes(n, c, s, p, k, a, slots, iterations)
     # Roofline of the dominant kernel, k_solve (8 launches a tick): compulsory
-    # bytes per launch over its mean launch span in the profiled ticks, stamped
-    # on the device wall clock by the kernel's own workgroups (first start to
-    # last end). HIP events bracket the kernel groups; around a single kernel
-    # they add several microseconds of their own on this runtime.
-    solve_ms = kernel_ms["solve_kernel_ms"] / npt
+    # bytes per launch over its mean launch duration in the profiled ticks,
+    # timed by HIP events recorded on the kernel's own stream right before and
+    # right after each launch (what rocprofv3's dispatch records measure too).
+    # The device wall-clock span of the kernel's workgroups (first start to
+    # last end) is reported beside it.
+    solve_ms = kernel_ms["solve_kernel_event_ms"] / npt
+    span_ms = kernel_ms["solve_kernel_ms"] / npt
     apply_ms = kernel_ms["apply_kernel_ms"] / npt
     solve_bytes = solve_kernel_bytes(scene.body_count, a, slots)
     achieved = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     per_phase_gbs = {g: (algo[g] / (phase_ms[g] * 1e-3) / 1e9 if phase_ms[g] > 0 else 0.0) for g in phase_ms}
 
-    # Reference-style synchronous engine TPS: dispatch + diagnostics readback per tick.
-    sync_ticks = 60
-    t1 = time.perf_counter()
-    base = first + args.profile_ticks
-    for i in range(sync_ticks):
-        gpu.dispatch_tick(base + i)
-        gpu.read_last_tick()
-    engine_tps = sync_ticks / (time.perf_counter() - t1)
-    import tempfile
+    # The falling window (ticks 2..steps+1) and config 1 as specified: 600 steps
+    # from the drop (ticks 2..601, the graph captured on tick 1).
+    fall_value, _ = timed_run(make, steps, tuned["tuning"])
+    run600_value, run600_rb = timed_run(make, 600, tuned["tuning"])
 
-    import numpy as np
-
-    state_fd, state_path = tempfile.mkstemp(suffix=".npz", dir="/tmp")
-    os.close(state_fd)
-    final_poses, final_velocities = gpu.read_bodies()
-    np.savez(state_path, poses=final_poses, velocities=final_velocities)
-    gpu.close()
-
-    value = steps / wall
     learned = None if args.no_learned else learned_warm_start(mh, scene, friction, iterations, steps, value,
-                                                                  tuned["tuning"])
+                                                                  tuned["tuning"], use_graph)
     batch = None if args.no_batched else batched(mh, steps)
     art = None if args.no_articulated else articulated(mh, min(steps, 300))
-    cpu = None if args.no_cpu else cpu_baseline(scene, friction, iterations)
+    cpu = None if args.no_cpu else cpu_baseline(scene, friction, iterations, state_path)
     traffic, traffic_note = (None, "skipped (--no-traffic)") if args.no_traffic else \
         measure_traffic(args, tuned["tuning"], iterations, state_path)
     os.remove(state_path)
@@ -441,12 +502,20 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 jittered cuboid lattice)",
         "config": {"workload": "pile_100k: 100,000 cuboids 0.25 m, 50x40x50 lattice 0.27 m spacing, "
-                               "jitter 0.02 m, uniform yaw, 0.5 m drop, seed 1, friction 0.6, 8 iterations, "
-                               f"ticks 2..{steps + 1} after the drop",
-                   "bodies": n, "solver_iterations": iterations, "friction": friction, "parallelism": "1 GPU"},
+                               "jitter 0.02 m, uniform yaw, 0.5 m drop, seed 1, friction 0.6, 8 iterations; "
+                               f"settled window: ticks {SETTLE_TICKS + 1}..{SETTLE_TICKS + steps} after the drop",
+                   "bodies": n, "solver_iterations": iterations, "friction": friction, "parallelism": "1 GPU",
+                   "launch": "captured hipGraph per tick" if use_graph else "direct launches"},
+        "windows": {"settled_steps_per_s": round(value, 2),
+                    "falling_steps_per_s": round(fall_value, 2), "falling_ticks": f"2..{steps + 1}",
+                    "config1_600_steps_per_s": round(run600_value, 2), "config1_ticks": "2..601",
+                    "config1_last_tick": {"pairs": run600_rb.pair_count, "contacts": run600_rb.contact_count,
+                                          "active_contacts": run600_rb.active_contact_count,
+                                          "error_flags": run600_rb.error_flags}},
         "phase_ms": {g: round(v, 4) for g, v in phase_ms.items()},
         "phase_gbs": {g: round(v, 1) for g, v in per_phase_gbs.items()},
         "profiled_tick_ms": round(total / npt, 4),
+        "profiled_ticks": f"{tick + 1}..{tick + npt}",
         "counts": {"pairs": round(p), "contacts": round(k), "active_contacts": round(a), "delta_slots": round(slots)},
         "error_flags": readback.error_flags,
         "engine_tps_sync": round(engine_tps, 2),
@@ -460,9 +529,15 @@ def main():
                      "traffic": None if traffic is None else round(traffic["bytes"]),
                      "traffic_read": None if traffic is None else round(traffic["read"]),
                      "traffic_written": None if traffic is None else round(traffic["written"]),
-                     "traffic_note": traffic_note or "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_solve, continuing "
-                                                     "the final state, x2.0 / x1.0 KiB (copy calibration)"},
-        "kernel_us": {"k_solve": round(solve_ms * 1e3, 2), "k_apply": round(apply_ms * 1e3, 2)},
+                     "traffic_algorithmic_bytes": None if traffic is None else round(traffic["algorithmic"]),
+                     "traffic_ratio": None if traffic is None else round(traffic["bytes"] / traffic["algorithmic"], 3),
+                     "traffic_note": traffic_note or "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_solve over the "
+                                                     f"{npt} ticks after {TRAFFIC_SETTLE} ticks from the profiled "
+                                                     "ticks' starting state, x2.0 / x1.0 KiB (copy calibration); "
+                                                     "traffic_ratio divides by the algorithmic bytes of the same "
+                                                     "launches"},
+        "kernel_us": {"k_solve": round(solve_ms * 1e3, 2), "k_solve_span": round(span_ms * 1e3, 2),
+                      "k_apply_span": round(apply_ms * 1e3, 2)},
         "cpu_baseline": cpu,
         "learned_warm_start": learned,
         "batched": batch,

@@ -199,6 +199,8 @@ typedef struct {
   double validation_ms;  /* articulated-state and bearing validation (with the snapshot) */
   double solve_kernel_ms;  /* mean span of one contact-solve pass kernel (k_solve), device clock */
   double apply_kernel_ms;  /* mean span of one velocity-gather kernel (k_apply), device clock */
+  double solve_kernel_event_ms;  /* mean time between HIP events recorded on the handle's stream right
+                                    before and right after each k_solve launch */
 } mh_phase_timings;
 
 typedef struct {

@@ -178,7 +178,7 @@ class PhaseTimings(ctypes.Structure):
     _fields_ = [(n, c_double) for n in ("integrate_ms", "morton_ms", "sort_ms", "lbvh_ms", "traverse_ms",
                                         "narrowphase_ms", "prepare_ms", "solve_ms", "persist_ms", "snapshot_ms",
                                         "total_ms", "mechanism_ms", "validation_ms", "solve_kernel_ms",
-                                        "apply_kernel_ms")]
+                                        "apply_kernel_ms", "solve_kernel_event_ms")]
 
 
 class ErrorDetail(ctypes.Structure):

@@ -241,7 +241,9 @@ void launch_prepare(hipStream_t s, const Dev& d);
 // With `clock`, the pass kernels stamp the device wall clock per workgroup:
 // pass i of k_solve at clock + 2 * i * solve_grid(d) (start, end per block),
 // pass i of k_apply at clock + 2 * (passes * solve_grid(d) + i * apply_grid(d)).
-void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock = nullptr);
+// With `events` (2 per pass), events[2 i] and events[2 i + 1] are recorded on
+// `s` right before and right after pass i's solve kernel.
+void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock = nullptr, hipEvent_t* events = nullptr);
 uint32_t solve_grid(const Dev& d);
 uint32_t apply_grid(const Dev& d);
 void launch_features(hipStream_t s, const Dev& d);

@@ -1034,14 +1034,16 @@ uint32_t apply_grid(const Dev& d) {
   return d.body_count ? (d.body_count * kApplyLanes + kBlock - 1) / kBlock + kXcds : 0;
 }
 
-void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock) {
+void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock, hipEvent_t* events) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t sgrid = solve_grid(d), bgrid = apply_grid(d);
   if (d.t_solve_kind != 1u) {
     k_group_sides<<<grid_for(d.pair_capacity / 64u * kBlock, 4096u), kBlock, 0, s>>>(d);
     for (uint32_t it = 0; it < iterations; ++it) {
       const int last = it + 1 == iterations ? 1 : 0;
+      if (events) (void)hipEventRecord(events[2 * it], s);
       k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, clock ? clock + 2 * (size_t)it * sgrid : nullptr);
+      if (events) (void)hipEventRecord(events[2 * it + 1], s);
       if (bgrid)
         k_apply<<<bgrid, kBlock, 0, s>>>(
             d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr);
@@ -1057,8 +1059,10 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock) {
     const int last = it + 1 == iterations ? 1 : 0;
     float4* src = records[it & 1u];
     float4* dst = records[(it + 1u) & 1u];
+    if (events) (void)hipEventRecord(events[2 * it], s);
     k_solve_bodies<<<sgrid, kBlock, 0, s>>>(d, src, dst, it == 0 ? 1 : 0, last,
                                             clock ? clock + 2 * (size_t)it * sgrid : nullptr);
+    if (events) (void)hipEventRecord(events[2 * it + 1], s);
     if (bearings) {
       // The bearing projection works on the records the pass just wrote.
       Dev dk = d;

@@ -44,6 +44,7 @@ struct mh_physics {
   bool has_last_tick = false;
   uint32_t* host_diag = nullptr;  // pinned
   hipEvent_t events[kEvents] = {};
+  std::vector<hipEvent_t> solve_events;  // 2 per solver pass of a profiled tick
   uint64_t* kclock = nullptr;  // device: per-workgroup clock stamps of profiled solver passes
   size_t kclock_words = 0;
   Timing timing;
@@ -108,7 +109,8 @@ uint32_t next_pow2(uint32_t v) {
 // Enqueues one tick in the pass order of dispatch_tick (device.rs:886-951).
 // `stop` ends it early for phase tests (1: after integration and the mechanism
 // passes, 2: after the broadphase, 3: after contact generation).
-void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uint64_t* clock = nullptr) {
+void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uint64_t* clock = nullptr,
+                 hipEvent_t* solve_events = nullptr) {
   const Dev& d = p->d;
   const bool mech = p->mech_active;
   auto mark = [&](int i) {
@@ -145,7 +147,7 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uin
     if (d.learn_mode == kLearnLog) launch_features(s, d);
     if (d.learn_mode == kLearnInfer) launch_mlp_infer(s, d);
     mark(kMarkPrepare);
-    launch_solve(s, d, mech && d.has_dynamic_root, clock);
+    launch_solve(s, d, mech && d.has_dynamic_root, clock, solve_events);
     if (d.learn_mode == kLearnLog) launch_targets(s, d);
     if (d.residual_on) launch_residual(s, d);
     if (mech) {
@@ -878,6 +880,8 @@ void mh_physics_destroy(mh_physics* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   for (auto& e : p->events)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : p->solve_events)
+    if (e) (void)hipEventDestroy(e);
   for (void* a : p->allocations) (void)hipFree(a);
   if (p->kclock) (void)hipFree(p->kclock);
   if (p->host_diag) (void)hipHostFree(p->host_diag);
@@ -941,11 +945,16 @@ int mh_physics_profile_tick(mh_physics* p, uint64_t tick, mh_phase_timings* out)
     HIP_TRY_NOERR(hipMalloc(&p->kclock, words * sizeof(uint64_t)));
     p->kclock_words = words;
   }
+  while (p->solve_events.size() < 2 * (size_t)passes) {
+    hipEvent_t e = nullptr;
+    HIP_TRY_NOERR(hipEventCreate(&e));
+    p->solve_events.push_back(e);
+  }
   int rc = begin_tick(p, tick);
   if (rc != MH_OK) return rc;
   uint64_t* clock = p->collisions ? p->kclock : nullptr;
   if (clock) HIP_TRY_NOERR(hipMemsetAsync(clock, 0, words * sizeof(uint64_t), p->stream));
-  encode_tick(p, p->stream, p->events, 0, clock);
+  encode_tick(p, p->stream, p->events, 0, clock, clock ? p->solve_events.data() : nullptr);
   HIP_TRY_NOERR(hipGetLastError());
   HIP_TRY_NOERR(hipEventSynchronize(p->events[kMarkEnd]));
   float ms[kEvents - 1];
@@ -993,6 +1002,13 @@ int mh_physics_profile_tick(mh_physics* p, uint64_t tick, mh_phase_timings* out)
     };
     t.solve_kernel_ms = mean_span(0, sgrid);
     t.apply_kernel_ms = mean_span(2 * (size_t)passes * sgrid, bgrid);
+    double event_sum = 0.0;
+    for (uint32_t i = 0; i < passes; ++i) {
+      float e = 0.0f;
+      HIP_TRY_NOERR(hipEventElapsedTime(&e, p->solve_events[2 * i], p->solve_events[2 * i + 1]));
+      event_sum += e;
+    }
+    t.solve_kernel_event_ms = event_sum / passes;
   }
   p->timing.t = t;
   p->timing.valid = true;

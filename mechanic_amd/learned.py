@@ -129,21 +129,31 @@ def collect_rows(scene: Scene, config: GpuPhysicsConfig, ticks: int, keep: Itera
     return np.concatenate(feats), np.concatenate(targs)
 
 
-def residual_trace(scene: Scene, config: GpuPhysicsConfig, ticks: int, net: Optional[WarmStartNet] = None,
-                   every: int = 1) -> np.ndarray:
-    """Per-tick contact residual (rms) of a run of `ticks` ticks; with `net`
-    the run uses the learned warm start."""
+def residual_stats(scene: Scene, config: GpuPhysicsConfig, ticks: int, net: Optional[WarmStartNet] = None,
+                   every: int = 1) -> dict:
+    """Per-tick contact residual (rms), mean and maximum penetration of the
+    active contacts, over a run of `ticks` ticks; with `net` the run uses the
+    learned warm start. Arrays keyed rms, mean_penetration, max_penetration."""
     cfg = GpuPhysicsConfig(**{**config.__dict__, "residual": True,
                               "warm_start": _lib.WARM_START_LEARNED if net is not None else config.warm_start})
-    out = []
+    out = {"rms": [], "mean_penetration": [], "max_penetration": []}
     with GpuPhysics.new_with_config(scene, cfg) as gpu:
         if net is not None:
             gpu.set_mlp(net)
         for tick in range(1, ticks + 1):
             gpu.dispatch_tick(tick)
             if tick % every == 0:
-                out.append(gpu.read_residual()["rms"])
+                r = gpu.read_residual()
+                for k in out:
+                    out[k].append(r[k])
         flags = gpu.read_last_tick().error_flags
         if flags:
             raise DeviceError(f"run reported error flags {flags}")
-    return np.asarray(out)
+    return {k: np.asarray(v) for k, v in out.items()}
+
+
+def residual_trace(scene: Scene, config: GpuPhysicsConfig, ticks: int, net: Optional[WarmStartNet] = None,
+                   every: int = 1) -> np.ndarray:
+    """Per-tick contact residual (rms) of a run of `ticks` ticks; with `net`
+    the run uses the learned warm start."""
+    return residual_stats(scene, config, ticks, net, every)["rms"]

@@ -1,10 +1,11 @@
 """Summarises a rocprofv3 results database (rocpd SQLite) as text.
 
-    python -m mechanic_amd.rocprof_summary RESULTS.db [--top 30]
+    python -m mechanic_amd.rocprof_summary RESULTS.db [--top 30] [--window KERNEL FIRST COUNT]
 
 Prints, per kernel: calls, average / total duration and, when the run
 collected hardware counters (--pmc), the average value of each counter per
-dispatch.
+dispatch. --window also prints the mean duration of dispatches FIRST ..
+FIRST+COUNT-1 (in dispatch order) of the kernels whose name contains KERNEL.
 """
 from __future__ import annotations
 
@@ -53,10 +54,20 @@ def counter_tail_average(db: str, kernel_part: str, counter: str, last: int):
     return sum(vals) / len(vals) if vals else None
 
 
+def window_average(cur, kernel_part: str, first: int, count: int):
+    """Mean duration (ns) and count of dispatches first..first+count-1 (dispatch
+    order) of the kernels whose name contains `kernel_part`."""
+    rows = cur.execute("select end - start from kernels where name like ? order by dispatch_id",
+                       (f"%{kernel_part}%",)).fetchall()
+    vals = [r[0] for r in rows][first:first + count]
+    return (sum(vals) / len(vals) if vals else None), len(vals), len(rows)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=30)
+    ap.add_argument("--window", nargs=3, metavar=("KERNEL", "FIRST", "COUNT"))
     args = ap.parse_args(argv)
     cur = sqlite3.connect(args.db).cursor()
     counters = counter_stats(cur)
@@ -70,6 +81,13 @@ def main(argv=None):
         for kernel in sorted(counters, key=lambda k: -sum(v[0] for v in counters[k].values())):
             vals = ", ".join(f"{c}={v:.4g}" for c, (v, _) in sorted(counters[kernel].items()))
             print(f"{kernel[:64]:64s} {vals}")
+    if args.window:
+        name, first, count = args.window[0], int(args.window[1]), int(args.window[2])
+        avg, n, total_n = window_average(cur, name, first, count)
+        if avg is None:
+            print(f"\nwindow {name} [{first}, {first + count}): no dispatches (of {total_n})")
+        else:
+            print(f"\nwindow {name} dispatches [{first}, {first + n}) of {total_n}: mean {avg / 1e3:.2f} us")
     return 0
 
 

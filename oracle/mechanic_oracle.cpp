@@ -25,6 +25,7 @@
 // Pinning: see tests/test_oracle.py (closed forms, brute force, the reference's
 // own SAT known-answer vectors, collision.rs:199-232).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -146,6 +147,12 @@ struct Sat {
   uint32_t near_face_axes;
 };
 
+// Phases of a tick, for the per-phase wall time of the CPU baseline.
+enum Phase {
+  kPhaseIntegrate, kPhaseMechanism, kPhaseWorldMasses, kPhaseMorton, kPhaseSort, kPhaseLbvh, kPhaseTraverse,
+  kPhaseNarrowphase, kPhasePrepare, kPhaseSolve, kPhaseValidate, kPhaseCount
+};
+
 template <typename R>
 struct World {
   // configuration (GpuTickConfig, device.rs:866-883)
@@ -191,6 +198,7 @@ struct World {
   std::vector<Drive> drives;
   std::vector<Link> links_a, links_b;
   std::vector<int32_t> mdeltas;
+  double phase_ns[kPhaseCount] = {};  // accumulated wall time per tick phase
 };
 
 template <typename R> V4<R> load4(const float* f) { return {R(f[0]), R(f[1]), R(f[2]), R(f[3])}; }
@@ -1226,19 +1234,34 @@ template <typename R> void mechanism_passes(World<R>& w) {
 template <typename R> void step(World<R>& w, uint64_t tick64, int stop) {
   const uint32_t tick = (uint32_t)tick64;
   std::memset(w.diag, 0, sizeof(w.diag));
+  auto t0 = std::chrono::steady_clock::now();
+  // Wall time of each phase of the tick, for the CPU baseline's per-phase report.
+  auto lap = [&](int phase) {
+    const auto t1 = std::chrono::steady_clock::now();
+    w.phase_ns[phase] += (double)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+    t0 = t1;
+  };
   integrate(w);
+  lap(kPhaseIntegrate);
   if (w.mech_active) mechanism_passes(w);
+  lap(kPhaseMechanism);
   if (stop == 1) return;
   if (w.collisions && w.collider_count > 0) {
     update_world_masses(w);
+    lap(kPhaseWorldMasses);
     compute_morton(w);
+    lap(kPhaseMorton);
     sort_keys(w);
+    lap(kPhaseSort);
     build_topology(w);
     build_bounds(w);
+    lap(kPhaseLbvh);
     traverse(w);
+    lap(kPhaseTraverse);
     if (stop == 2) return;
     narrowphase(w);
     generate_ground_contacts(w);
+    lap(kPhaseNarrowphase);
     if (stop == 3) return;
     // Collider pair of each contact before prepare overwrites metadata z/w.
     std::vector<uint64_t> keys(w.contacts.size());
@@ -1246,6 +1269,7 @@ template <typename R> void step(World<R>& w, uint64_t tick64, int stop) {
       keys[i] = ((uint64_t)std::min(w.contacts[i].m_z, w.contacts[i].m_w) << 32) |
                 std::max(w.contacts[i].m_z, w.contacts[i].m_w);
     prepare_contacts(w, tick);
+    lap(kPhasePrepare);
     w.deltas.assign(6 * (size_t)w.body_count, 0);
     const uint32_t iterations = std::max(1u, w.iterations);
     const bool interleave = w.mech_active && w.has_dynamic_root;
@@ -1270,9 +1294,11 @@ template <typename R> void step(World<R>& w, uint64_t tick64, int stop) {
       capture_coordinates(w);
       if (!w.has_dynamic_root) reconstruct(w);
     }
+    lap(kPhaseSolve);
   }
   if (w.mech_active) validate_articulated(w);
   if (w.bearing_count > 0) validate_bearings(w);
+  lap(kPhaseValidate);
 }
 
 }  // namespace oracle
@@ -1418,6 +1444,17 @@ void oracle_set_bodies(OracleHandle* h, const float* poses, const float* velocit
   };
   if (h->fp64) set(h->d);
   else set(h->f);
+}
+
+// Accumulated wall time (ms) of each tick phase since the last reset, in Phase
+// order; returns the phase count.
+int oracle_phase_ms(OracleHandle* h, double* out, int reset) {
+  double* ns = h->fp64 ? h->d.phase_ns : h->f.phase_ns;
+  for (int k = 0; k < kPhaseCount; ++k) {
+    if (out) out[k] = ns[k] * 1e-6;
+    if (reset) ns[k] = 0.0;
+  }
+  return kPhaseCount;
 }
 
 void oracle_get_diag(OracleHandle* h, uint32_t* out) {

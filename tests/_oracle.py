@@ -33,6 +33,8 @@ def load():
     lib.oracle_get_bodies_f64.argtypes = [P, P, P]
     lib.oracle_set_bodies.argtypes = [P, P, P]
     lib.oracle_get_diag.argtypes = [P, P]
+    lib.oracle_phase_ms.restype = i32
+    lib.oracle_phase_ms.argtypes = [P, P, i32]
     for name in ("oracle_get_keys", "oracle_get_children", "oracle_get_parents", "oracle_get_pairs",
                  "oracle_get_contacts", "oracle_get_world_masses"):
         getattr(lib, name).restype = sz
@@ -140,6 +142,15 @@ class Oracle:
         out = np.zeros((max(n, 1), 6), np.float64)
         lib.oracle_closure_residuals(self.h, out.ctypes.data, n)
         return out[:n]
+
+    PHASES = ("integrate", "mechanism", "world_masses", "morton", "sort", "lbvh", "traverse", "narrowphase",
+              "prepare", "solve", "validate")
+
+    def phase_ms(self, reset=False):
+        """Wall time (ms) of each tick phase accumulated since the last reset."""
+        out = np.zeros(len(self.PHASES), np.float64)
+        load().oracle_phase_ms(self.h, out.ctypes.data, int(reset))
+        return dict(zip(self.PHASES, out.tolist()))
 
     def diag(self):
         d = np.zeros(8, np.uint32)

@@ -308,31 +308,54 @@ def batched(mh, steps):
             "error_flags": rb.error_flags, "contacts": rb.contact_count}
 
 
-def articulated(mh, steps):
-    """The articulated 100,000-body scene: steps/s and ms per phase."""
+# Config 2's closure solve: one PCG per closure component to a 1e-6 tolerance
+# (at most 64 iterations), 24 Newton steps a tick.
+CONFIG2_CLOSURES = dict(closure_components=True, closure_pcg_tolerance=1e-6, closure_pcg_iterations=64,
+                        closure_newton_steps=24)
+
+
+def articulated(mh, steps, use_graph=True):
+    """The articulated 100,000-body scene: steps/s, ms per phase, the
+    non-convergence flag and the worst closure anchor residual, with config 2's
+    closure solve and with the reference's (one global PCG, 8 iterations, 12
+    Newton steps)."""
     from mechanic_amd import articulated as A
 
     scene = A.articulated_pile()
-    cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8, mechanism_self_collisions=False)
-    make = lambda: mh.GpuPhysics.new_with_config(scene, cfg)  # noqa: E731
-    value, rb = timed_run(make, steps)
-    gpu = make()
-    gpu.dispatch_ticks(1, steps)
-    acc = {}
-    n = 10
-    for i in range(n):
-        t = gpu.profile_tick(steps + 1 + i)
-        for k, v in t.items():
-            acc[k] = acc.get(k, 0.0) + v / n
-    prb = gpu.read_last_tick()
-    gpu.close()
-    return {"workload": "90,000 cuboids + 10,000 cylinders (OBB), 4,500 tree bearings in chains of 2-20 links, "
-                        "500 loop closures, 1,000 drives at +-2 rad/s, friction 0.6, 8 iterations, "
-                        f"mechanism self-collisions off, ticks 2..{steps + 1}",
-            "steps": steps, "steps_per_s": round(value, 2),
-            "phase_ms": {k[:-3]: round(v, 4) for k, v in acc.items() if k.endswith("_ms")},
-            "error_flags": rb.error_flags, "anchor_residual_m": round(prb.anchor_residual_meters, 6),
-            "contacts": prb.contact_count}
+    out = {"workload": "90,000 cuboids + 10,000 cylinders (OBB), 4,500 tree bearings in chains of 2-20 links, "
+                       "500 loop closures, 1,000 drives at +-2 rad/s, friction 0.6, 8 iterations, "
+                       f"mechanism self-collisions off, ticks 2..{steps + 1}"}
+    for label, closures in (("config2", CONFIG2_CLOSURES), ("reference_closures", {})):
+        cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8, mechanism_self_collisions=False,
+                                  use_graph=use_graph, **closures)
+        make = lambda: mh.GpuPhysics.new_with_config(scene, cfg)  # noqa: E731
+        value, rb = timed_run(make, steps)
+        gpu = make()
+        worst = 0.0
+        flags = 0
+        for t in range(1, steps + 1):
+            gpu.dispatch_tick(t)
+            if t % 10 == 0 or t == steps:
+                r = gpu.read_last_tick()
+                worst = max(worst, r.anchor_residual_meters)
+                flags |= r.error_flags
+        acc = {}
+        n = 10
+        for i in range(n):
+            t = gpu.profile_tick(steps + 1 + i)
+            for k, v in t.items():
+                acc[k] = acc.get(k, 0.0) + v / n
+        prb = gpu.read_last_tick()
+        gpu.close()
+        out[label] = {"closure_solve": "per component, PCG tolerance 1e-6, max 64 iterations, 24 Newton steps"
+                      if closures else "global PCG, 8 iterations, 12 Newton steps (the reference's)",
+                      "steps_per_s": round(value, 2),
+                      "phase_ms": {k[:-3]: round(v, 4) for k, v in acc.items() if k.endswith("_ms")},
+                      "error_flags": flags, "worst_anchor_residual_m": round(worst, 7),
+                      "contacts": prb.contact_count}
+    out["steps"] = steps
+    out["steps_per_s"] = out["config2"]["steps_per_s"]
+    return out
 
 
 def main():
@@ -463,10 +486,10 @@ def main():
     algo = algorithmic_bytes(n, c, s, p, k, a, slots, iterations)
     # Roofline of the dominant kernel, k_solve (8 launches a tick): compulsory
     # bytes per launch over its mean launch duration in the profiled ticks,
-    # timed by HIP events recorded on the kernel's own stream right before and
-    # right after each launch (what rocprofv3's dispatch records measure too).
-    # The device wall-clock span of the kernel's workgroups (first start to
-    # last end) is reported beside it.
+    # timed by the start and stop HIP events each launch carries on its stream
+    # (hipExtLaunchKernelGGL: the dispatch's own start and end, what rocprofv3's
+    # dispatch records measure). The device wall-clock span of the kernel's
+    # workgroups (first start to last end) is reported beside it.
     solve_ms = kernel_ms["solve_kernel_event_ms"] / npt
     span_ms = kernel_ms["solve_kernel_ms"] / npt
     apply_ms = kernel_ms["apply_kernel_ms"] / npt
@@ -482,7 +505,7 @@ def main():
     learned = None if args.no_learned else learned_warm_start(mh, scene, friction, iterations, steps, value,
                                                                   tuned["tuning"], use_graph)
     batch = None if args.no_batched else batched(mh, steps)
-    art = None if args.no_articulated else articulated(mh, min(steps, 300))
+    art = None if args.no_articulated else articulated(mh, min(steps, 300), use_graph)
     cpu = None if args.no_cpu else cpu_baseline(scene, friction, iterations, state_path)
     traffic, traffic_note = (None, "skipped (--no-traffic)") if args.no_traffic else \
         measure_traffic(args, tuned["tuning"], iterations, state_path)

@@ -171,6 +171,26 @@ typedef struct {
   uint32_t warm_start_mode;           /* MH_WARM_START_*; default MH_WARM_START_CACHE */
   uint32_t use_graph;                 /* 1 = replay each tick as a captured hipGraph */
   uint32_t residual;                  /* 1 = measure the contact residual after the solve */
+  /* Learned warm start fallback: after the learned passes the RMS contact
+   * residual is measured on the device; a tick above this limit (N s) runs
+   * learned_fallback_passes more passes. 0 disables the check. */
+  float learned_residual_limit;
+  uint32_t learned_fallback_passes;   /* 0: as many as the learned run's passes */
+  /* Loop-closure Newton solve (device.rs:976-1020, closure.wgsl:333-427): Newton
+   * steps per tick (0: the reference's 12), PCG iterations per step (0: the
+   * reference's 8), and a PCG tolerance on the preconditioned residual
+   * sqrt(r.z / r0.z) (0: run every iteration, as the reference does). */
+  uint32_t closure_newton_steps;
+  uint32_t closure_pcg_iterations;
+  float closure_pcg_tolerance;
+  /* 1: one PCG per closure component (closures linked by shared coordinates),
+   * each with its own step lengths, tolerance stop and Newton convergence
+   * test, one small workgroup per component running every Newton step and its
+   * forward kinematics in one launch. The system is block diagonal over
+   * components, so both organisations solve the same equations. Components may
+   * hold at most 256 path bodies, 256 coordinates and 128 closures
+   * (MH_ERR_UNSUPPORTED otherwise). 0: one global PCG, as the reference. */
+  uint32_t closure_components;
 } mh_config;
 
 /* Warm start of the contact solver's first pass. */
@@ -199,8 +219,9 @@ typedef struct {
   double validation_ms;  /* articulated-state and bearing validation (with the snapshot) */
   double solve_kernel_ms;  /* mean span of one contact-solve pass kernel (k_solve), device clock */
   double apply_kernel_ms;  /* mean span of one velocity-gather kernel (k_apply), device clock */
-  double solve_kernel_event_ms;  /* mean time between HIP events recorded on the handle's stream right
-                                    before and right after each k_solve launch */
+  double solve_kernel_event_ms;  /* mean time between the start and stop HIP events each k_solve
+                                    dispatch carries (hipExtLaunchKernelGGL): the dispatch's own
+                                    start and end, what rocprofv3 records */
 } mh_phase_timings;
 
 typedef struct {
@@ -356,11 +377,20 @@ typedef struct {
   double mean_penetration;    /* (m) */
   double max_penetration;     /* (m) */
   uint32_t active_contacts;
-  uint32_t reserved;
+  uint32_t fallback_passes;   /* learned warm start: extra passes this tick ran (0 when within the limit) */
 } mh_contact_residual;
 
 /* Handles created with config.residual = 1. */
 MH_API int mh_physics_read_residual(mh_physics* physics, mh_contact_residual* out);
+
+/* Learned warm start fallback counts of a MH_WARM_START_LEARNED handle. */
+typedef struct {
+  uint32_t fallback_this_tick;   /* 1 if the last tick's residual check ran the extra passes */
+  uint32_t fallback_ticks;       /* ticks that ran them since creation */
+  uint32_t nonfinite_this_tick;  /* contacts whose prediction was not finite (cache warm start instead) */
+  uint32_t nonfinite_total;      /* such contacts since creation */
+} mh_learned_stats;
+MH_API int mh_physics_read_learned_stats(mh_physics* physics, mh_learned_stats* out);
 
 enum {
   MH_BUFFER_MORTON_KEYS = 1,     /* u64 (code << 32 | collider) per sort slot, sorted after a tick */

@@ -152,7 +152,10 @@ class Creation(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [("collisions_enabled", c_uint32), ("mechanism_self_collisions", c_uint32),
                 ("solver_iterations", c_uint32), ("friction", c_float), ("pair_capacity", c_uint32),
-                ("warm_start_mode", c_uint32), ("use_graph", c_uint32), ("residual", c_uint32)]
+                ("warm_start_mode", c_uint32), ("use_graph", c_uint32), ("residual", c_uint32),
+                ("learned_residual_limit", c_float), ("learned_fallback_passes", c_uint32),
+                ("closure_newton_steps", c_uint32), ("closure_pcg_iterations", c_uint32),
+                ("closure_pcg_tolerance", c_float), ("closure_components", c_uint32)]
 
 
 class Tuning(ctypes.Structure):
@@ -163,7 +166,12 @@ class Tuning(ctypes.Structure):
 
 class ContactResidual(ctypes.Structure):
     _fields_ = [("rms", c_double), ("mean_penetration", c_double), ("max_penetration", c_double),
-                ("active_contacts", c_uint32), ("reserved", c_uint32)]
+                ("active_contacts", c_uint32), ("fallback_passes", c_uint32)]
+
+
+class LearnedStats(ctypes.Structure):
+    _fields_ = [("fallback_this_tick", c_uint32), ("fallback_ticks", c_uint32), ("nonfinite_this_tick", c_uint32),
+                ("nonfinite_total", c_uint32)]
 
 
 class TickReadback(ctypes.Structure):
@@ -261,6 +269,7 @@ def load() -> ctypes.CDLL:
         "mh_physics_read_training_rows": (c_int, [H, POINTER(c_float), POINTER(c_float), c_size_t,
                                                   POINTER(c_size_t)]),
         "mh_physics_read_residual": (c_int, [H, POINTER(ContactResidual)]),
+        "mh_physics_read_learned_stats": (c_int, [H, POINTER(LearnedStats)]),
         "mh_scene_cuboid_pile": (c_int, [POINTER(PileParams), POINTER(Transform), POINTER(c_float), POINTER(Mass),
                                          POINTER(SpatialInertia), POINTER(Collider), c_size_t]),
     }

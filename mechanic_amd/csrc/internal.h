@@ -27,6 +27,8 @@ constexpr uint32_t kRadixTicket = kRadixMaxPasses * 256;  // rs_hist word of pas
 constexpr uint32_t kRadixHistWords = kRadixTicket + kRadixMaxPasses;
 constexpr uint32_t kFeatures = 16;      // learned warm-start inputs per contact
 constexpr uint32_t kResidBlocks = 512;  // residual kernel grid
+// Largest closure component the per-component solve holds in LDS.
+constexpr uint32_t kCompMaxBodies = 256, kCompMaxCoords = 256, kCompMaxClosures = 128, kCompMaxPath = 2048;
 enum LearnMode : uint32_t { kLearnOff = 0, kLearnLog = 1, kLearnInfer = 2 };
 
 // Diagnostics words (abi.rs:56-73) plus private counters after them.
@@ -137,6 +139,10 @@ struct Dev {
   float4* finit;          // predicted initial friction impulse, per active contact
   double* resid;          // 4 per residual block: sum r^2, active contacts, sum penetration, max penetration
   const float* mlp;       // network parameters (mlp.hip layout), or null
+  float learned_limit;    // RMS residual above which a learned tick runs learned_extra more passes (0: never)
+  uint32_t learned_extra; // extra passes of such a tick
+  uint32_t* learned_state;  // [0] extra passes run this tick, [1] ticks that ran them, [2] non-finite
+                            // predictions this tick, [3] non-finite predictions since creation
 
   // ---- scene batches: several independent scenes in one handle ----
   uint32_t scene_count;       // 1 for a single scene
@@ -195,6 +201,21 @@ struct Dev {
   float4* joints;             // 2 per coordinate: joint axis, joint anchor (component-root frame)
   float* pcg;                 // 8 per coordinate: solution, residual, preconditioned, direction,
                               // operator product, gradient, diagonal, unused
+  // Per-component closure solve (closure_components): per component two uint4,
+  // (first path body, path bodies, first coordinate, coordinates) and (first
+  // closure, closures, 0, 0), into the flattened lists below. Bodies: global
+  // id, local parent (kInvalid: a root), local coordinate; coordinates: global
+  // id, local child body, (closure << 1 | side) entries in (closure, side)
+  // order; closures: global closure index, local end body per side (kInvalid: a
+  // root), local coordinates along each side's path to the root.
+  uint32_t cc_count, cc_max_bodies, cc_max_coords, cc_max_closures, cc_max_path, pad_cc;
+  const uint4* cc_off;
+  const uint32_t *cc_body, *cc_body_parent, *cc_body_coord, *cc_coord, *cc_coord_body, *cc_entry_off, *cc_entries,
+      *cc_closure, *cc_end, *cc_path_off, *cc_path;
+  uint32_t closure_steps;     // Newton steps per tick (the reference's 12 by default)
+  uint32_t closure_iters;     // PCG iterations per Newton step (the reference's 8 by default)
+  float closure_tol;          // PCG stop when sqrt(r.z / r0.z) falls below it (0: run every iteration)
+  uint32_t pad_closure;
 
   // ---- publication ----
   float4* snap_pos[3];
@@ -241,8 +262,9 @@ void launch_prepare(hipStream_t s, const Dev& d);
 // With `clock`, the pass kernels stamp the device wall clock per workgroup:
 // pass i of k_solve at clock + 2 * i * solve_grid(d) (start, end per block),
 // pass i of k_apply at clock + 2 * (passes * solve_grid(d) + i * apply_grid(d)).
-// With `events` (2 per pass), events[2 i] and events[2 i + 1] are recorded on
-// `s` right before and right after pass i's solve kernel.
+// With `events` (2 per pass), pass i's solve kernel is launched with
+// events[2 i] and events[2 i + 1] as its start and stop events: they carry the
+// dispatch's own start and end timestamps.
 void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock = nullptr, hipEvent_t* events = nullptr);
 uint32_t solve_grid(const Dev& d);
 uint32_t apply_grid(const Dev& d);
@@ -254,13 +276,20 @@ void launch_targets(hipStream_t s, const Dev& d);
 void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back);
 void launch_advance_coordinates(hipStream_t s, const Dev& d);
 void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures);
+// LDS bytes of the per-component closure kernel for `d`'s largest component;
+// raises the kernel's dynamic LDS limit when it needs more than 64 KiB.
+size_t closure_component_lds(const Dev& d);
+hipError_t prepare_closure_components(const Dev& d);
 void launch_capture_coordinates(hipStream_t s, const Dev& d);
 void launch_reconstruct(hipStream_t s, const Dev& d);
 void launch_world_masses(hipStream_t s, const Dev& d);
 void launch_validate_snapshot(hipStream_t s, const Dev& d);
 void launch_validate_bearings(hipStream_t s, const Dev& d);
 void launch_mlp_infer(hipStream_t s, const Dev& d);
-void launch_residual(hipStream_t s, const Dev& d);
+// With `gate`, the kernel returns at once unless *gate is non-zero.
+void launch_residual(hipStream_t s, const Dev& d, const uint32_t* gate = nullptr);
+// Whether a tick's solve ends with the learned warm start's residual check.
+inline bool learned_fallback_on(const Dev& d) { return d.learn_mode == kLearnInfer && d.learned_limit > 0.0f; }
 
 }  // namespace mh
 

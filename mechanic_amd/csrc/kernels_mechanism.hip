@@ -18,6 +18,8 @@
 //   from a CSR list in (closure, side) order -- the order in which the reference's
 //   serial PCG adds them -- and the PCG dot products are accumulated in FP64
 //   with a fixed reduction tree.
+#include <algorithm>
+
 #include "internal.h"
 #include "mh_math.h"
 
@@ -37,8 +39,6 @@ constexpr uint32_t kAnchorTolerance = 10u;       // closure.wgsl:69
 constexpr uint32_t kAxisTolerance = 1000u;       // closure.wgsl:70
 constexpr float kStepScale = 0.35f;              // closure.wgsl:72
 constexpr float kMaxStep = 0.2f;                 // closure.wgsl:73
-constexpr int kNewtonSteps = 12;                 // device.rs:977
-constexpr int kPcgIterations = 8;                // closure.wgsl:362
 constexpr uint32_t kPoseThreads = 1024;
 
 __device__ __forceinline__ uint64_t tick_m(const Dev& d) {
@@ -285,7 +285,23 @@ __device__ __forceinline__ float4 axis_rotation(v3 axis, float angle) {
   return f4(a, cosf(half_angle));
 }
 
-// prepare_links (mechanism.wgsl:75-107) for one non-root body.
+// prepare_links (mechanism.wgsl:75-107) for one non-root body at a given angle:
+// its link relative to its parent.
+__device__ __forceinline__ Link make_link(const Dev& d, uint32_t body, const MechBody& m, const BearingRow& r,
+                                          float angle) {
+  const float4 bind_rotation = d.mbodies[4 * (size_t)body + 3];
+  float4 rr;
+  v3 rp;
+  if (m.direction == 0u) {
+    rr = quat_multiply(axis_rotation(r.axis_a, angle), bind_rotation);
+    rp = r.anchor_a - quat_rotate(rr, r.anchor_b);
+  } else {
+    rr = quat_multiply(axis_rotation(r.axis_b, -angle), bind_rotation);
+    rp = r.anchor_b - quat_rotate(rr, r.anchor_a);
+  }
+  return Link{f4(rp, 0.0f), rr, m.parent, 0u};
+}
+
 __device__ __forceinline__ void prepare_link(const Dev& d, uint32_t body) {
   const MechBody m = load_mbody(d, body);
   const BearingRow r = load_bearing(d, m.tree_bearing);
@@ -579,6 +595,9 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       rz_part += (double)residual * (double)pre;
     }
     double rz = block_sum(rz_part, scratch);
+    // With a tolerance, the PCG stops once sqrt(r.z / r0.z) is at most it. rz is
+    // the same in every thread, so the whole workgroup leaves together.
+    const double rz_stop = (double)d.closure_tol * (double)d.closure_tol * rz;
     // The search direction of iteration `it` is p = z + beta p_prev (the
     // preconditioned start at it = 0); each phase forms it from the previous
     // iteration's z and p, which the update phase then replaces, so no pass
@@ -597,7 +616,8 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       my_e0 = d.cpath_offset[2 * k + side];
       my_e1 = d.cpath_offset[2 * k + side + 1];
     }
-    for (int it = 0; it < kPcgIterations; ++it) {
+    for (uint32_t it = 0; it < d.closure_iters; ++it) {
+      if (d.closure_tol > 0.0f && rz <= rz_stop) break;
       auto direction_of = [&](uint32_t c) {
         float z, p;
         if (in_lds) {
@@ -693,6 +713,352 @@ __global__ __launch_bounds__(kPoseThreads) void k_newton_step(Dev d) {
       else atomicOr(&diag[kDiagFlags], kFlagInvalid);
     }
   }
+}
+
+// The closure Newton loop per closure component (closure_components): one
+// workgroup per component runs every Newton step of the tick -- forward
+// kinematics of the component's path bodies by pointer jumping (the global
+// passes' composition order, so the same link bits), evaluation, the gradient
+// and Jacobi diagonal, a PCG with the component's own step lengths and
+// tolerance stop, and the clamped step -- and stops once its own closures are
+// inside tolerance. Everything the loop reads (bearing rows, bind rotations,
+// parents, paths, per-coordinate entry lists) is staged in LDS once, so the
+// dependent steps of the loop wait on LDS, not on memory. Components share no
+// coordinate, so they run independently; one forward-kinematics pass over
+// every tree follows the launch.
+struct CompLds {
+  float4 *body;   // 5 per path body: (anchor a, direction) (anchor b, parent) (axis a, coordinate) (axis b) (bind rotation)
+  float4 *lpos, *lrot;  // 2 * NB each: link position / rotation, two pointer-jumping buffers
+  uint2* lmeta;   // 2 * NB: local ancestor, resolved
+  float4* clo;    // 4 per closure: (anchor a, end a) (anchor b, end b) (axis a) (axis b) of its bearing
+  // PCG vectors in FP64: solution, residual, preconditioned residual, direction, product, diagonal;
+  // 6 per closure: J d (position, axis)
+  double *px, *pr, *pz, *pp, *pop, *pdg, *kprod;
+  float *angle, *jax, *jan, *kend, *kres, *kact;
+  uint32_t *entry_off, *entries, *path_off, *path, *coord_body;
+};
+
+__device__ __forceinline__ CompLds carve_comp(float4* base, const Dev& d) {
+  const uint32_t NB = d.cc_max_bodies, NQ = d.cc_max_coords, NK = d.cc_max_closures, NP = d.cc_max_path;
+  CompLds c;
+  c.body = base;
+  c.lpos = c.body + 5 * NB;
+  c.lrot = c.lpos + 2 * NB;
+  c.lmeta = reinterpret_cast<uint2*>(c.lrot + 2 * NB);  // 2 * NB uint2 = NB float4
+  c.clo = c.lrot + 3 * NB;
+  double* g = reinterpret_cast<double*>(c.clo + 4 * NK);
+  c.px = g;
+  c.pr = c.px + NQ;
+  c.pz = c.pr + NQ;
+  c.pp = c.pz + NQ;
+  c.pop = c.pp + NQ;
+  c.pdg = c.pop + NQ;
+  c.kprod = c.pdg + NQ;
+  float* f = reinterpret_cast<float*>(c.kprod + 6 * NK);
+  c.angle = f;
+  c.jax = c.angle + NQ;
+  c.jan = c.jax + 3 * NQ;
+  c.kend = c.jan + 3 * NQ;
+  c.kres = c.kend + 12 * NK;
+  c.kact = c.kres + 6 * NK;
+  uint32_t* u = reinterpret_cast<uint32_t*>(c.kact + NK);
+  c.entry_off = u;
+  c.entries = c.entry_off + NQ + 1;
+  c.path_off = c.entries + NP;
+  c.path = c.path_off + 2 * NK + 1;
+  c.coord_body = c.path + NP;
+  return c;
+}
+
+// (x*x' + y*y') + z*z' in FP64 of an FP32 column and an FP64 vector.
+__device__ __forceinline__ double dot_d(v3 a, double bx, double by, double bz) {
+  return ((double)a.x * bx + (double)a.y * by) + (double)a.z * bz;
+}
+
+__global__ __launch_bounds__(256) void k_closure_components(Dev d) {
+  extern __shared__ float4 lds4[];
+  __shared__ uint32_t state[2];
+  __shared__ double scratch[4];
+  const uint4 o0 = d.cc_off[2 * (size_t)blockIdx.x], o1 = d.cc_off[2 * (size_t)blockIdx.x + 1];
+  const uint32_t b0 = o0.x, nb = o0.y, q0 = o0.z, nq = o0.w, k0 = o1.x, nk = o1.y;
+  const uint32_t NB = d.cc_max_bodies;
+  const CompLds L = carve_comp(lds4, d);
+  uint32_t* diag = diag_now(d);
+  const uint32_t t = threadIdx.x, T = blockDim.x;
+  // Stage the component's rows and lists.
+  for (uint32_t i = t; i < nb; i += T) {
+    const MechBody m = load_mbody(d, d.cc_body[b0 + i]);
+    const BearingRow r = load_bearing(d, m.tree_bearing);
+    float4* b = L.body + 5 * i;
+    b[0] = f4(r.anchor_a, __uint_as_float(m.direction));
+    b[1] = f4(r.anchor_b, __uint_as_float(d.cc_body_parent[b0 + i]));
+    b[2] = f4(r.axis_a, __uint_as_float(d.cc_body_coord[b0 + i]));
+    b[3] = f4(r.axis_b, 0.0f);
+    b[4] = d.mbodies[4 * (size_t)d.cc_body[b0 + i] + 3];
+  }
+  for (uint32_t kk = t; kk < nk; kk += T) {
+    const BearingRow r = load_bearing(d, d.closure_rows[d.cc_closure[k0 + kk]]);
+    float4* c = L.clo + 4 * kk;
+    c[0] = f4(r.anchor_a, __uint_as_float(d.cc_end[2 * (size_t)(k0 + kk)]));
+    c[1] = f4(r.anchor_b, __uint_as_float(d.cc_end[2 * (size_t)(k0 + kk) + 1]));
+    c[2] = f4(r.axis_a, 0.0f);
+    c[3] = f4(r.axis_b, 0.0f);
+  }
+  const uint32_t e_base = d.cc_entry_off[q0], p_base = d.cc_path_off[2 * k0];
+  for (uint32_t l = t; l <= nq; l += T) L.entry_off[l] = d.cc_entry_off[q0 + l] - e_base;
+  for (uint32_t e = t; e < d.cc_entry_off[q0 + nq] - e_base; e += T) L.entries[e] = d.cc_entries[e_base + e];
+  for (uint32_t j = t; j <= 2 * nk; j += T) L.path_off[j] = d.cc_path_off[2 * k0 + j] - p_base;
+  for (uint32_t e = t; e < d.cc_path_off[2 * (k0 + nk)] - p_base; e += T) L.path[e] = d.cc_path[p_base + e];
+  for (uint32_t l = t; l < nq; l += T) {
+    L.angle[l] = d.coords[d.cc_coord[q0 + l]].x;
+    L.coord_body[l] = d.cc_coord_body[q0 + l];
+  }
+  __syncthreads();
+  uint32_t fb = 0;  // buffer holding the final links
+  auto link = [&](uint32_t local, float4& position, float4& rotation) {
+    if (local == kInvalid) {  // a root: the identity link of prepare_links' root rows
+      position = make_float4(0.f, 0.f, 0.f, 0.f);
+      rotation = make_float4(0.f, 0.f, 0.f, 1.f);
+    } else {
+      position = L.lpos[fb * NB + local];
+      rotation = L.lrot[fb * NB + local];
+    }
+  };
+  auto joint = [&](uint32_t l, v3& axis, v3& anchor) {
+    axis = mk(L.jax[3 * l], L.jax[3 * l + 1], L.jax[3 * l + 2]);
+    anchor = mk(L.jan[3 * l], L.jan[3 * l + 1], L.jan[3 * l + 2]);
+  };
+  auto end_of = [&](uint32_t kk, uint32_t side, v3& anchor, v3& axis) {
+    const float* e = L.kend + 12 * kk + (side ? 6 : 0);
+    anchor = mk(e[0], e[1], e[2]);
+    axis = mk(e[3], e[4], e[5]);
+  };
+  for (uint32_t step = 0; step < d.closure_steps; ++step) {
+    // Forward kinematics of the path bodies (prepare_links + pointer jumping).
+    for (uint32_t i = t; i < nb; i += T) {
+      const float4* b = L.body + 5 * i;
+      const MechBody m{0u, 0u, __float_as_uint(b[0].w), 0u};
+      BearingRow r;
+      r.anchor_a = xyz(b[0]);
+      r.anchor_b = xyz(b[1]);
+      r.axis_a = xyz(b[2]);
+      r.axis_b = xyz(b[3]);
+      const float angle = L.angle[__float_as_uint(b[2].w)];
+      // make_link's arithmetic with the staged rows (bind rotation in b[4]).
+      float4 rr;
+      v3 rp;
+      if (m.direction == 0u) {
+        rr = quat_multiply(axis_rotation(r.axis_a, angle), b[4]);
+        rp = r.anchor_a - quat_rotate(rr, r.anchor_b);
+      } else {
+        rr = quat_multiply(axis_rotation(r.axis_b, -angle), b[4]);
+        rp = r.anchor_b - quat_rotate(rr, r.anchor_a);
+      }
+      L.lpos[i] = f4(rp, 0.0f);
+      L.lrot[i] = rr;
+      L.lmeta[i] = make_uint2(__float_as_uint(b[1].w), 0u);
+    }
+    __syncthreads();
+    uint32_t src = 0;
+    for (uint32_t round = 0; round < d.jump_rounds; ++round) {
+      for (uint32_t i = t; i < nb; i += T) {
+        const uint2 meta = L.lmeta[src * NB + i];
+        const Link child{L.lpos[src * NB + i], L.lrot[src * NB + i], meta.x, meta.y};
+        Link out = child;
+        if (child.resolved == 0u) {
+          Link parent;
+          if (child.ancestor == kInvalid) {
+            parent = Link{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 1.f), kInvalid, 1u};
+          } else {
+            const uint2 pm = L.lmeta[src * NB + child.ancestor];
+            parent = Link{L.lpos[src * NB + child.ancestor], L.lrot[src * NB + child.ancestor], pm.x, pm.y};
+          }
+          out = compose_link(parent, child);
+        }
+        L.lpos[(src ^ 1u) * NB + i] = out.position;
+        L.lrot[(src ^ 1u) * NB + i] = out.rotation;
+        L.lmeta[(src ^ 1u) * NB + i] = make_uint2(out.ancestor, out.resolved);
+      }
+      __syncthreads();
+      src ^= 1u;
+    }
+    fb = src;
+    // evaluate_closures for the component's closures.
+    if (t == 0) {
+      state[0] = 0u;
+      state[1] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t kk = t; kk < nk; kk += T) {
+      const float4* c = L.clo + 4 * kk;
+      float4 pa_p, pa_r, pb_p, pb_r;
+      link(__float_as_uint(c[0].w), pa_p, pa_r);
+      link(__float_as_uint(c[1].w), pb_p, pb_r);
+      const v3 anchor_a = xyz(pa_p) + quat_rotate(pa_r, xyz(c[0]));
+      const v3 anchor_b = xyz(pb_p) + quat_rotate(pb_r, xyz(c[1]));
+      const v3 axis_a = normalize_v(quat_rotate(pa_r, xyz(c[2])));
+      const v3 axis_b = normalize_v(quat_rotate(pb_r, xyz(c[3])));
+      const v3 res_pos = anchor_a - anchor_b, res_axis = axis_a - axis_b;
+      float* e = L.kend + 12 * kk;
+      e[0] = anchor_a.x; e[1] = anchor_a.y; e[2] = anchor_a.z;
+      e[3] = axis_a.x; e[4] = axis_a.y; e[5] = axis_a.z;
+      e[6] = anchor_b.x; e[7] = anchor_b.y; e[8] = anchor_b.z;
+      e[9] = axis_b.x; e[10] = axis_b.y; e[11] = axis_b.z;
+      float* rs = L.kres + 6 * kk;
+      rs[0] = res_pos.x; rs[1] = res_pos.y; rs[2] = res_pos.z;
+      rs[3] = res_axis.x; rs[4] = res_axis.y; rs[5] = res_axis.z;
+      float active = 0.0f;
+      if (!finite3(res_pos) || !finite3(res_axis)) {
+        atomicOr(&diag[kDiagFlags], kFlagInvalid);
+        atomicMax(&state[0], 0xffffffffu);
+        atomicMax(&state[1], 0xffffffffu);
+      } else {
+        const uint32_t um = round_to_u32(length(res_pos) * 1000000.0f);
+        const float degrees = acosf(clampf(dot(axis_a, axis_b), -1.0f, 1.0f)) * 57.295779513f;
+        const uint32_t ud = round_to_u32(degrees * 1000000.0f);
+        atomicMax(&state[0], um);
+        atomicMax(&state[1], ud);
+        active = (um <= kAnchorTolerance && ud <= kAxisTolerance) ? 0.0f : 1.0f;
+      }
+      L.kact[kk] = active;
+    }
+    __syncthreads();
+    if (!(state[0] > kAnchorTolerance || state[1] > kAxisTolerance)) break;  // the same for every thread
+    // Joint axes and anchors of the component's coordinates (closure.wgsl:172-185).
+    for (uint32_t l = t; l < nq; l += T) {
+      const float4* b = L.body + 5 * L.coord_body[l];
+      float4 pp4, pr4;
+      link(__float_as_uint(b[1].w), pp4, pr4);
+      v3 axis, anchor;
+      if (__float_as_uint(b[0].w) == 0u) {
+        axis = normalize_v(quat_rotate(pr4, xyz(b[2])));
+        anchor = xyz(pp4) + quat_rotate(pr4, xyz(b[0]));
+      } else {
+        axis = -normalize_v(quat_rotate(pr4, xyz(b[3])));
+        anchor = xyz(pp4) + quat_rotate(pr4, xyz(b[1]));
+      }
+      L.jax[3 * l] = axis.x; L.jax[3 * l + 1] = axis.y; L.jax[3 * l + 2] = axis.z;
+      L.jan[3 * l] = anchor.x; L.jan[3 * l + 1] = anchor.y; L.jan[3 * l + 2] = anchor.z;
+    }
+    __syncthreads();
+    // Gradient and Jacobi diagonal, gathered per coordinate in (closure, side) order.
+    double rz_part = 0.0;
+    for (uint32_t l = t; l < nq; l += T) {
+      v3 jaxis, janchor;
+      joint(l, jaxis, janchor);
+      double g = 0.0, dg = 0.0;
+      for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
+        const uint32_t entry = L.entries[e], kk = entry >> 1, side = entry & 1u;
+        if (L.kact[kk] == 0.0f) continue;
+        v3 end_anchor, end_axis;
+        end_of(kk, side, end_anchor, end_axis);
+        const float sign = side ? -1.0f : 1.0f;
+        const v3 pd = cross(jaxis, end_anchor - janchor) * sign;
+        const v3 ad = cross(jaxis, end_axis) * sign;
+        const float* rs = L.kres + 6 * kk;
+        const float gradient = dot(mk(rs[0], rs[1], rs[2]), pd) + 1.0f * dot(mk(rs[3], rs[4], rs[5]), ad);
+        const float diagonal = dot(pd, pd) + 1.0f * dot(ad, ad);
+        if (gradient == gradient && diagonal == diagonal) {
+          g += (double)gradient;
+          dg += (double)diagonal;
+        } else {
+          atomicOr(&diag[kDiagFlags], kFlagInvalid);
+        }
+      }
+      const double residual = dg > (double)1.0e-12f ? -g : 0.0;
+      const double pre = dg > (double)1.0e-12f ? residual / dg : 0.0;
+      L.px[l] = 0.0;
+      L.pr[l] = residual;
+      L.pz[l] = pre;
+      L.pp[l] = pre;
+      L.pop[l] = 0.0;
+      L.pdg[l] = dg;
+      rz_part += residual * pre;
+    }
+    double rz = block_sum(rz_part, scratch);
+    const double rz_stop = (double)d.closure_tol * (double)d.closure_tol * rz;
+    double beta = 0.0;
+    const uint32_t slots = 2u * ((nk + 31u) / 32u) * 32u;  // whole lane pairs for the side shuffle
+    for (uint32_t it = 0; it < d.closure_iters; ++it) {
+      if (d.closure_tol > 0.0f && rz <= rz_stop) break;
+      auto direction_of = [&](uint32_t l) { return it == 0 ? L.pp[l] : L.pz[l] + beta * L.pp[l]; };
+      // branch_product per closure side, summed in path order (FP32 columns times
+      // the FP64 direction, FP64 sums); side 0 adds side 1's.
+      for (uint32_t slot = t; slot < slots; slot += T) {
+        const uint32_t kk = slot >> 1, side = slot & 1u;
+        double spx = 0.0, spy = 0.0, spz = 0.0, sax = 0.0, say = 0.0, saz = 0.0;
+        if (kk < nk) {
+          v3 end_anchor, end_axis;
+          end_of(kk, side, end_anchor, end_axis);
+          const float sign = side ? -1.0f : 1.0f;
+          for (uint32_t e = L.path_off[2 * kk + side]; e < L.path_off[2 * kk + side + 1]; ++e) {
+            const uint32_t l = L.path[e];
+            v3 jaxis, janchor;
+            joint(l, jaxis, janchor);
+            const double dir = direction_of(l);
+            const v3 pd = cross(jaxis, end_anchor - janchor) * sign, ad = cross(jaxis, end_axis) * sign;
+            spx = spx + (double)pd.x * dir;
+            spy = spy + (double)pd.y * dir;
+            spz = spz + (double)pd.z * dir;
+            sax = sax + (double)ad.x * dir;
+            say = say + (double)ad.y * dir;
+            saz = saz + (double)ad.z * dir;
+          }
+        }
+        const double s1[6] = {__shfl_xor(spx, 1), __shfl_xor(spy, 1), __shfl_xor(spz, 1),
+                              __shfl_xor(sax, 1), __shfl_xor(say, 1), __shfl_xor(saz, 1)};
+        if (kk < nk && side == 0u) {
+          double* q = L.kprod + 6 * kk;
+          q[0] = spx + s1[0]; q[1] = spy + s1[1]; q[2] = spz + s1[2];
+          q[3] = sax + s1[3]; q[4] = say + s1[4]; q[5] = saz + s1[5];
+        }
+      }
+      __syncthreads();
+      // branch_transpose gathered per coordinate.
+      double dad_part = 0.0;
+      for (uint32_t l = t; l < nq; l += T) {
+        v3 jaxis, janchor;
+        joint(l, jaxis, janchor);
+        double op = 0.0;
+        for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
+          const uint32_t entry = L.entries[e], kk = entry >> 1, side = entry & 1u;
+          v3 end_anchor, end_axis;
+          end_of(kk, side, end_anchor, end_axis);
+          const float sign = side ? -1.0f : 1.0f;
+          const v3 pd = cross(jaxis, end_anchor - janchor) * sign;
+          const v3 ad = cross(jaxis, end_axis) * sign;
+          const double* q = L.kprod + 6 * kk;
+          op = op + (dot_d(pd, q[0], q[1], q[2]) + 1.0 * dot_d(ad, q[3], q[4], q[5]));
+        }
+        L.pop[l] = op;
+        dad_part += direction_of(l) * op;
+      }
+      const double dad = block_sum(dad_part, scratch);
+      const double alpha = fabs(dad) > 1.0e-20 ? rz / dad : 0.0;
+      double rz_next_part = 0.0;
+      for (uint32_t l = t; l < nq; l += T) {
+        const double p = direction_of(l);
+        L.px[l] = L.px[l] + alpha * p;
+        L.pr[l] = L.pr[l] - alpha * L.pop[l];
+        const double df = L.pdg[l];
+        const double z = df > (double)1.0e-12f ? L.pr[l] / df : 0.0;
+        L.pz[l] = z;
+        L.pp[l] = p;
+        rz_next_part += L.pr[l] * z;
+      }
+      const double rz_next = block_sum(rz_next_part, scratch);  // its barriers publish the update
+      beta = fabs(rz) > 1.0e-20 ? rz_next / rz : 0.0;
+      rz = rz_next;
+    }
+    // The clamped Newton step (closure.wgsl:416-426), in FP32.
+    for (uint32_t l = t; l < nq; l += T) {
+      const float step_c = clampf(kStepScale * (float)L.px[l], -kMaxStep, kMaxStep);
+      if (step_c == step_c) L.angle[l] = L.angle[l] + step_c;
+      else atomicOr(&diag[kDiagFlags], kFlagInvalid);
+    }
+    __syncthreads();
+  }
+  for (uint32_t l = t; l < nq; l += T) d.coords[d.cc_coord[q0 + l]].x = L.angle[l];
 }
 
 // Pose publication and the parent-to-child velocity reconstruction of
@@ -810,14 +1176,39 @@ void launch_advance_coordinates(hipStream_t s, const Dev& d) {
 // kinematics), publication and reconstruction (device.rs:963-1048).
 void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures) {
   if (d.fk_groups == 0) return;
+  if (closures && d.cc_count) {
+    // One workgroup per closure component, then forward kinematics of every tree.
+    const uint32_t items = std::max(std::max(d.cc_max_bodies, d.cc_max_coords), 2u * d.cc_max_closures);
+    const uint32_t threads = items <= 64u ? 64u : 256u;
+    k_closure_components<<<d.cc_count, threads, closure_component_lds(d), s>>>(d);
+    k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, -1);
+    k_publish<<<1, kPoseThreads, 0, s>>>(d);
+    return;
+  }
   k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, -1);
   if (closures) {
-    for (int step = 0; step < kNewtonSteps; ++step) {
+    for (uint32_t step = 0; step < d.closure_steps; ++step) {
       k_newton_step<<<1, kPoseThreads, 0, s>>>(d);
-      k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, step);
+      k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, (int)step);
     }
   }
   k_publish<<<1, kPoseThreads, 0, s>>>(d);
+}
+
+size_t closure_component_lds(const Dev& d) {
+  const size_t NB = d.cc_max_bodies, NQ = d.cc_max_coords, NK = d.cc_max_closures, NP = d.cc_max_path;
+  // float4 region: bodies (5), links (2 + 2) and link meta (1) per path body, 4 per closure;
+  // doubles: 6 per coordinate and per closure; floats: 7 per coordinate, 19 per closure;
+  // u32: entry offsets, entries, path offsets, paths, coordinate bodies.
+  return 16 * (10 * NB + 4 * NK) + 8 * (6 * NQ + 6 * NK) + 4 * (7 * NQ + 19 * NK) +
+         4 * ((NQ + 1) + NP + (2 * NK + 1) + NP + NQ);
+}
+
+hipError_t prepare_closure_components(const Dev& d) {
+  const size_t lds = closure_component_lds(d);
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_closure_components),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 void launch_capture_coordinates(hipStream_t s, const Dev& d) {

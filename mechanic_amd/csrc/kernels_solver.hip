@@ -7,6 +7,8 @@
 // looks pairs up in the index of the previous tick, so whether a contact finds
 // its cached impulse depends only on the previous tick's contact set, never on
 // the order in which threads ran.
+#include <hip/hip_ext.h>
+
 #include "internal.h"
 #include "mh_geometry.h"
 #include "mh_math.h"
@@ -151,6 +153,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   const float4* rows_prev = d.mrows[(tick64 + 1) & 1];
   const uint32_t count = read_count(&diag[kDiagContacts], d.pair_capacity);
   const bool per_body = d.t_solve_kind == 1u;
+  if (d.learn_mode == kLearnInfer && blockIdx.x == 0 && threadIdx.x == 0) d.learned_state[2] = 0u;
   if (per_body)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.side_tiles; i += gridDim.x * blockDim.x)
       d.side_status[i] = 0u;
@@ -273,6 +276,11 @@ __device__ __forceinline__ bool contact_pass(const Dev& d, const SolverBody& sa,
   const v3 velocity_b = has_b ? sb.lin + cross(sb.ang, r.arm_b) : mk(0.0f, 0.0f, 0.0f);
   impulse = mk(0.0f, 0.0f, 0.0f);
   bool applies = false;
+  // The learned warm start's first pass, unless the contact's prediction was
+  // not finite (then finit.w is 1 and the contact takes the cache warm start).
+  float4 predicted = make_float4(0.f, 0.f, 0.f, 1.f);
+  if (first && d.learn_mode == kLearnInfer) predicted = d.finit[ai];
+  const bool learned_first = first && d.learn_mode == kLearnInfer && predicted.w == 0.0f;
   if (r.denominator > 0.0f) {
     const v3 relative = velocity_b - (sa.lin + cross(sa.ang, r.arm_a));
     const float normal_speed = dot(relative, r.normal);
@@ -285,7 +293,7 @@ __device__ __forceinline__ bool contact_pass(const Dev& d, const SolverBody& sa,
       float bound;
       if (first) {
         // A learned prediction replaces the halved cached impulse in full.
-        const float scale = (r.analytic || d.learn_mode == kLearnInfer) ? 1.0f : kWarmStartScale;
+        const float scale = (r.analytic || learned_first) ? 1.0f : kWarmStartScale;
         const float warmed = lambda * scale;
         const float accumulated = fmx(warmed + step, 0.0f);
         lambda = accumulated;
@@ -302,9 +310,9 @@ __device__ __forceinline__ bool contact_pass(const Dev& d, const SolverBody& sa,
       const v3 tangent_velocity = relative - r.normal * normal_speed;
       const float tangent_speed = length(tangent_velocity);
       v3 friction_vec = mk(0.0f, 0.0f, 0.0f);
-      if (first && d.learn_mode == kLearnInfer) {
+      if (learned_first) {
         // Learned warm start: the predicted friction impulse, inside the cone.
-        const float4 f = d.finit[ai];
+        const float4 f = predicted;
         const float mag = length(xyz(f));
         const float cap = bound;
         friction_vec = mag > cap && mag > 0.0f ? xyz(f) * (cap / mag) : xyz(f);
@@ -414,8 +422,9 @@ __global__ __launch_bounds__(256) void k_group_sides(Dev d) {
 // (collision.wgsl:841-906) otherwise, per contact, with the group's sides
 // summed per body in LDS; the last pass also runs persist_contacts
 // (collision.wgsl:1071-1088). Each XCD takes a contiguous run of whole groups.
-__global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint64_t* clock) {
+__global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint64_t* clock, const uint32_t* gate) {
   __shared__ int32_t lds_q[kBlock / 64][128][6];
+  if (gate && *reinterpret_cast<const volatile uint32_t*>(gate) == 0u) return;
   if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
@@ -626,10 +635,11 @@ __global__ __launch_bounds__(256) void k_scatter_sides(Dev d) {
 constexpr uint32_t kGroupBodies = 16;
 
 __global__ __launch_bounds__(256) void k_solve_bodies(Dev d, const float4* __restrict__ src, float4* __restrict__ dst,
-                                                      int first, int last, uint64_t* clock) {
+                                                      int first, int last, uint64_t* clock, const uint32_t* gate) {
   __shared__ float4 lds_own[kGroupBodies][4];
   __shared__ uint32_t lds_off[kGroupBodies + 1];
   __shared__ int32_t lds_acc[6][kGroupBodies];
+  if (gate && *reinterpret_cast<const volatile uint32_t*>(gate) == 0u) return;
   if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
@@ -725,7 +735,8 @@ __global__ __launch_bounds__(256) void k_solve_bodies(Dev d, const float4* __res
 
 // The records after an odd number of per-body passes, back into sbody.
 __global__ __launch_bounds__(256) void k_copy_records(const float4* __restrict__ src, float4* __restrict__ dst,
-                                                      uint32_t words) {
+                                                      uint32_t words, const uint32_t* gate) {
+  if (gate && *reinterpret_cast<const volatile uint32_t*>(gate) == 0u) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < words) dst[i] = src[i];
 }
@@ -742,7 +753,8 @@ __global__ __launch_bounds__(256) void k_copy_records(const float4* __restrict__
 constexpr uint32_t kApplyLanes = 4;
 constexpr uint32_t kApplyPerLane = (kSlotOverflow + kApplyLanes - 1) / kApplyLanes;
 
-__global__ __launch_bounds__(256) void k_apply(Dev d, int last, uint64_t* clock) {
+__global__ __launch_bounds__(256) void k_apply(Dev d, int last, uint64_t* clock, const uint32_t* gate) {
+  if (gate && *reinterpret_cast<const volatile uint32_t*>(gate) == 0u) return;
   if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
@@ -936,8 +948,15 @@ __global__ __launch_bounds__(256) void k_learned(Dev d) {
     __builtin_amdgcn_wave_barrier();
     if (ai < count) {
       const float y0 = sout[wave][lane][0], y1 = sout[wave][lane][1], y2 = sout[wave][lane][2];
-      d.lambda[ai] = fmx(y0, 0.0f);
-      d.finit[ai] = f4(e1 * y1 + e2 * y2, 0.0f);
+      if (isfinite(y0) && isfinite(y1) && isfinite(y2)) {
+        d.lambda[ai] = fmx(y0, 0.0f);
+        d.finit[ai] = f4(e1 * y1 + e2 * y2, 0.0f);
+      } else {
+        // Fallback for this contact: keep its cached impulse (the cache warm
+        // start), flagged for the first pass, and count it.
+        d.finit[ai] = make_float4(0.f, 0.f, 0.f, 1.f);
+        atomicAdd(&d.learned_state[2], 1u);
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -948,8 +967,9 @@ __global__ __launch_bounds__(256) void k_learned(Dev d) {
 // unrelaxed projected step would still make (zero when the contact is either
 // separating with no impulse or holding exactly its bias speed). Per-block
 // partial sums in FP64; the host adds the blocks in order.
-__global__ __launch_bounds__(256) void k_residual(Dev d) {
+__global__ __launch_bounds__(256) void k_residual(Dev d, const uint32_t* gate) {
   __shared__ double part[4][kBlock / 64];
+  if (gate && *reinterpret_cast<const volatile uint32_t*>(gate) == 0u) return;
   const uint64_t tick64 = tick_sv(d);
   uint32_t* diag = d.diag[tick64 & 1];
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
@@ -1010,6 +1030,25 @@ __global__ __launch_bounds__(256) void k_residual(Dev d) {
   }
 }
 
+// The learned warm start's judge: the RMS residual after the learned passes,
+// from the residual kernel's block partials summed in block order. Above the
+// limit, or with any non-finite prediction, the tick's extra passes run.
+__global__ void k_learned_judge(Dev d) {
+  if (threadIdx.x != 0) return;
+  double r2 = 0.0, n = 0.0;
+  for (uint32_t b = 0; b < kResidBlocks; ++b) {
+    r2 += d.resid[4 * b];
+    n += d.resid[4 * b + 1];
+  }
+  const double rms = n > 0.0 ? sqrt(r2 / n) : 0.0;
+  uint32_t* st = d.learned_state;
+  const uint32_t nonfinite = *reinterpret_cast<volatile uint32_t*>(&st[2]);
+  const uint32_t run = (rms > (double)d.learned_limit || nonfinite > 0u) ? 1u : 0u;
+  st[0] = run;
+  st[1] += run;
+  st[3] += nonfinite;
+}
+
 static inline uint32_t grid_for(uint32_t n, uint32_t cap) {
   uint32_t b = (n + kBlock - 1) / kBlock;
   return b < cap ? b : cap;
@@ -1037,17 +1076,36 @@ uint32_t apply_grid(const Dev& d) {
 void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock, hipEvent_t* events) {
   const uint32_t iterations = d.solver_iterations < 1 ? 1 : d.solver_iterations;
   const uint32_t sgrid = solve_grid(d), bgrid = apply_grid(d);
+  // The learned warm start's fallback: after the passes, the residual is judged
+  // on the device and `extra` more passes run only when it is over the limit
+  // (every kernel of theirs returns at once otherwise, so the captured graph
+  // is the same every tick). Not with bearing projections between passes.
+  const bool fallback = learned_fallback_on(d) && !bearings;
+  const uint32_t extra = fallback ? (d.learned_extra ? d.learned_extra : iterations) : 0u;
+  const uint32_t* gate = d.learned_state;
   if (d.t_solve_kind != 1u) {
     k_group_sides<<<grid_for(d.pair_capacity / 64u * kBlock, 4096u), kBlock, 0, s>>>(d);
     for (uint32_t it = 0; it < iterations; ++it) {
       const int last = it + 1 == iterations ? 1 : 0;
-      if (events) (void)hipEventRecord(events[2 * it], s);
-      k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, clock ? clock + 2 * (size_t)it * sgrid : nullptr);
-      if (events) (void)hipEventRecord(events[2 * it + 1], s);
+      uint64_t* pass_clock = clock ? clock + 2 * (size_t)it * sgrid : nullptr;
+      if (events)  // start and stop events carry this dispatch's own timestamps
+        hipExtLaunchKernelGGL(k_solve, dim3(sgrid), dim3(kBlock), 0, s, events[2 * it], events[2 * it + 1], 0, d,
+                              it == 0 ? 1 : 0, last, pass_clock, static_cast<const uint32_t*>(nullptr));
+      else
+        k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, pass_clock, nullptr);
       if (bgrid)
         k_apply<<<bgrid, kBlock, 0, s>>>(
-            d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr);
+            d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr, nullptr);
       if (bearings) launch_bearing_projection(s, d, 1, last);
+    }
+    if (fallback) {
+      launch_residual(s, d);
+      k_learned_judge<<<1, 64, 0, s>>>(d);
+      for (uint32_t it = 0; it < extra; ++it) {
+        const int last = it + 1 == extra ? 1 : 0;
+        k_solve<<<sgrid, kBlock, 0, s>>>(d, 0, last, nullptr, gate);
+        if (bgrid) k_apply<<<bgrid, kBlock, 0, s>>>(d, last, nullptr, gate);
+      }
     }
     return;
   }
@@ -1055,14 +1113,18 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock, hi
   k_side_offsets<<<d.side_tiles, kBlock, 0, s>>>(d);
   k_scatter_sides<<<grid_for(d.pair_capacity, d.t_prepare_blocks ? d.t_prepare_blocks : 1024u), kBlock, 0, s>>>(d);
   float4* records[2] = {d.sbody, d.sbody_alt};
+  const uint32_t words = 4 * d.body_count;
   for (uint32_t it = 0; it < iterations; ++it) {
     const int last = it + 1 == iterations ? 1 : 0;
     float4* src = records[it & 1u];
     float4* dst = records[(it + 1u) & 1u];
-    if (events) (void)hipEventRecord(events[2 * it], s);
-    k_solve_bodies<<<sgrid, kBlock, 0, s>>>(d, src, dst, it == 0 ? 1 : 0, last,
-                                            clock ? clock + 2 * (size_t)it * sgrid : nullptr);
-    if (events) (void)hipEventRecord(events[2 * it + 1], s);
+    uint64_t* pass_clock = clock ? clock + 2 * (size_t)it * sgrid : nullptr;
+    if (events)
+      hipExtLaunchKernelGGL(k_solve_bodies, dim3(sgrid), dim3(kBlock), 0, s, events[2 * it], events[2 * it + 1], 0, d,
+                            static_cast<const float4*>(src), dst, it == 0 ? 1 : 0, last, pass_clock,
+                            static_cast<const uint32_t*>(nullptr));
+    else
+      k_solve_bodies<<<sgrid, kBlock, 0, s>>>(d, src, dst, it == 0 ? 1 : 0, last, pass_clock, nullptr);
     if (bearings) {
       // The bearing projection works on the records the pass just wrote.
       Dev dk = d;
@@ -1070,9 +1132,17 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock, hi
       launch_bearing_projection(s, dk, 1, last);
     }
   }
-  if (iterations & 1u) {
-    const uint32_t words = 4 * d.body_count;
-    k_copy_records<<<(words + kBlock - 1) / kBlock, kBlock, 0, s>>>(d.sbody_alt, d.sbody, words);
+  if (iterations & 1u)
+    k_copy_records<<<(words + kBlock - 1) / kBlock, kBlock, 0, s>>>(d.sbody_alt, d.sbody, words, nullptr);
+  if (fallback) {
+    launch_residual(s, d);
+    k_learned_judge<<<1, 64, 0, s>>>(d);
+    for (uint32_t it = 0; it < extra; ++it) {
+      const int last = it + 1 == extra ? 1 : 0;
+      k_solve_bodies<<<sgrid, kBlock, 0, s>>>(d, records[it & 1u], records[(it + 1u) & 1u], 0, last, nullptr, gate);
+    }
+    if (extra & 1u)
+      k_copy_records<<<(words + kBlock - 1) / kBlock, kBlock, 0, s>>>(d.sbody_alt, d.sbody, words, gate);
   }
 }
 
@@ -1088,6 +1158,8 @@ void launch_targets(hipStream_t s, const Dev& d) { k_targets<<<grid_for(d.pair_c
 
 void launch_mlp_infer(hipStream_t s, const Dev& d) { k_learned<<<1024, kBlock, 0, s>>>(d); }
 
-void launch_residual(hipStream_t s, const Dev& d) { k_residual<<<kResidBlocks, kBlock, 0, s>>>(d); }
+void launch_residual(hipStream_t s, const Dev& d, const uint32_t* gate) {
+  k_residual<<<kResidBlocks, kBlock, 0, s>>>(d, gate);
+}
 
 }  // namespace mh

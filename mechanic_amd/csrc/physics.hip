@@ -149,7 +149,10 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uin
     mark(kMarkPrepare);
     launch_solve(s, d, mech && d.has_dynamic_root, clock, solve_events);
     if (d.learn_mode == kLearnLog) launch_targets(s, d);
-    if (d.residual_on) launch_residual(s, d);
+    // With the learned warm start's fallback the solve already measured the
+    // residual; it is measured again only if the extra passes ran.
+    if (d.residual_on)
+      launch_residual(s, d, learned_fallback_on(d) && !(mech && d.has_dynamic_root) ? d.learned_state : nullptr);
     if (mech) {
       // encode_post_contact_mechanism (device.rs:1128-1155).
       if (!d.has_dynamic_root) {
@@ -204,7 +207,102 @@ struct Forest {
       coord_body, fk_list, fk_offset;
   uint32_t max_depth = 0, jump_rounds = 0;
   bool has_dynamic_root = false;
+  // Closure components (closures linked by shared coordinates), each with its
+  // path bodies, coordinates and closures in local numbering (see Dev).
+  std::vector<uint32_t> cc_off, cc_body, cc_body_parent, cc_body_coord, cc_coord, cc_coord_body, cc_entry_off,
+      cc_entries, cc_closure, cc_end, cc_path_off, cc_path;
+  uint32_t cc_count = 0, cc_max_bodies = 0, cc_max_coords = 0, cc_max_closures = 0, cc_max_path = 0;
 };
+
+// Closure components of a forest: closures joined when their branches share a
+// coordinate, with every component's path bodies (closed under parents up to
+// the roots), coordinates and closures renumbered locally.
+void build_components(const mh_creation* c, Forest& f, const std::vector<uint32_t>& parent,
+                      const std::vector<uint32_t>& root, const std::vector<uint32_t>& tree_row) {
+  const uint32_t nk = (uint32_t)f.closure_rows.size(), nq = c->coordinate_count;
+  if (nk == 0) return;
+  std::vector<uint32_t> up(nk);
+  for (uint32_t k = 0; k < nk; ++k) up[k] = k;
+  auto find = [&](uint32_t x) {
+    while (up[x] != x) x = up[x] = up[up[x]];
+    return x;
+  };
+  std::vector<uint32_t> owner(nq, kInvalid);
+  for (uint32_t k = 0; k < nk; ++k)
+    for (uint32_t e = f.cpath_offset[2 * k]; e < f.cpath_offset[2 * k + 2]; ++e) {
+      const uint32_t q = f.cpath[e];
+      if (owner[q] == kInvalid) owner[q] = k;
+      else up[find(k)] = find(owner[q]);
+    }
+  std::vector<std::vector<uint32_t>> members;
+  std::vector<uint32_t> id(nk, kInvalid);
+  for (uint32_t k = 0; k < nk; ++k) {
+    const uint32_t r = find(k);
+    if (id[r] == kInvalid) {
+      id[r] = (uint32_t)members.size();
+      members.emplace_back();
+    }
+    members[id[r]].push_back(k);
+  }
+  f.cc_count = (uint32_t)members.size();
+  std::vector<uint32_t> local_body(c->body_count, kInvalid), local_coord(nq, kInvalid);
+  f.cc_entry_off.push_back(0);
+  f.cc_path_off.push_back(0);
+  for (const auto& ks : members) {
+    const uint32_t body_first = (uint32_t)f.cc_body.size(), coord_first = (uint32_t)f.cc_coord.size();
+    const uint32_t closure_first = (uint32_t)f.cc_closure.size();
+    // Path bodies: every non-root body from a closure end up to its root.
+    std::vector<uint32_t> bodies, coords;
+    for (uint32_t k : ks)
+      for (uint32_t side = 0; side < 2; ++side) {
+        uint32_t b = c->bearings[f.closure_rows[k]].metadata[side];
+        while (!root[b] && local_body[b] == kInvalid) {
+          local_body[b] = 0u;  // marked; numbered below
+          bodies.push_back(b);
+          b = parent[b];
+        }
+      }
+    std::sort(bodies.begin(), bodies.end());
+    for (uint32_t i = 0; i < (uint32_t)bodies.size(); ++i) local_body[bodies[i]] = i;
+    for (uint32_t b : bodies) coords.push_back(c->bearings[tree_row[b]].metadata[2]);
+    std::sort(coords.begin(), coords.end());
+    for (uint32_t i = 0; i < (uint32_t)coords.size(); ++i) local_coord[coords[i]] = i;
+    for (uint32_t b : bodies) {
+      f.cc_body.push_back(b);
+      f.cc_body_parent.push_back(root[parent[b]] ? kInvalid : local_body[parent[b]]);
+      f.cc_body_coord.push_back(local_coord[c->bearings[tree_row[b]].metadata[2]]);
+    }
+    std::vector<std::vector<uint32_t>> entries(coords.size());
+    for (uint32_t kk = 0; kk < (uint32_t)ks.size(); ++kk) {
+      const uint32_t k = ks[kk];
+      f.cc_closure.push_back(k);
+      for (uint32_t side = 0; side < 2; ++side) {
+        const uint32_t b = c->bearings[f.closure_rows[k]].metadata[side];
+        f.cc_end.push_back(root[b] ? kInvalid : local_body[b]);
+        for (uint32_t e = f.cpath_offset[2 * k + side]; e < f.cpath_offset[2 * k + side + 1]; ++e) {
+          const uint32_t l = local_coord[f.cpath[e]];
+          f.cc_path.push_back(l);
+          entries[l].push_back((kk << 1) | side);
+        }
+        f.cc_path_off.push_back((uint32_t)f.cc_path.size());
+      }
+    }
+    for (uint32_t l = 0; l < (uint32_t)coords.size(); ++l) {
+      f.cc_coord.push_back(coords[l]);
+      f.cc_coord_body.push_back(local_body[f.coord_body[coords[l]]]);
+      f.cc_entries.insert(f.cc_entries.end(), entries[l].begin(), entries[l].end());
+      f.cc_entry_off.push_back((uint32_t)f.cc_entries.size());
+    }
+    const uint32_t nb = (uint32_t)bodies.size(), ncq = (uint32_t)coords.size(), ncl = (uint32_t)ks.size();
+    f.cc_off.insert(f.cc_off.end(), {body_first, nb, coord_first, ncq, closure_first, ncl, 0u, 0u});
+    f.cc_max_bodies = std::max(f.cc_max_bodies, nb);
+    f.cc_max_coords = std::max(f.cc_max_coords, ncq);
+    f.cc_max_closures = std::max(f.cc_max_closures, ncl);
+    f.cc_max_path = std::max(f.cc_max_path, f.cc_path_off.back() - f.cc_path_off[2 * closure_first]);
+    for (uint32_t b : bodies) local_body[b] = kInvalid;
+    for (uint32_t q : coords) local_coord[q] = kInvalid;
+  }
+}
 
 int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
   const uint32_t n = c->body_count, nb = c->bearing_count, nq = c->coordinate_count;
@@ -305,6 +403,7 @@ int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
     f.coord_entries.insert(f.coord_entries.end(), per_coord[q].begin(), per_coord[q].end());
     f.coord_offset.push_back((uint32_t)f.coord_entries.size());
   }
+  build_components(c, f, parent, root, tree_row);
   // The serial small-mechanism solvers (device.rs:1063-1075, 1370-1400) are not built.
   if (f.max_depth > 0 && f.has_dynamic_root && nb <= 64)
     return fail(error, MH_ERR_UNSUPPORTED,
@@ -392,6 +491,46 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
   TRY(alloc(p, &cl_product, 2 * (size_t)d.closure_count, error));
   TRY(alloc(p, &joints, 2 * (size_t)nq, error));
   TRY(alloc(p, &pcg, 8 * (size_t)nq, error));
+  if (p->config.closure_components && f.cc_count > 0) {
+    if (f.cc_max_bodies > kCompMaxBodies || f.cc_max_coords > kCompMaxCoords || f.cc_max_closures > kCompMaxClosures ||
+        f.cc_max_path > kCompMaxPath)
+      return fail(error, MH_ERR_UNSUPPORTED,
+                  "a closure component exceeds 256 path bodies, 256 coordinates, 128 closures or 2048 path entries: use "
+                  "closure_components = 0");
+    d.cc_count = f.cc_count;
+    d.cc_max_bodies = f.cc_max_bodies;
+    d.cc_max_coords = f.cc_max_coords;
+    d.cc_max_closures = f.cc_max_closures;
+    d.cc_max_path = f.cc_max_path;
+    if (prepare_closure_components(d) != hipSuccess)
+      return fail(error, MH_ERR_DEVICE, "closure component kernel: LDS attribute");
+    uint32_t *off, *body, *body_parent, *body_coord, *coord, *coord_body, *entry_off, *entries, *closure, *end, *path_off,
+        *path;
+    TRY(alloc_upload(p, &off, f.cc_off, error));
+    TRY(alloc_upload(p, &body, f.cc_body, error));
+    TRY(alloc_upload(p, &body_parent, f.cc_body_parent, error));
+    TRY(alloc_upload(p, &body_coord, f.cc_body_coord, error));
+    TRY(alloc_upload(p, &coord, f.cc_coord, error));
+    TRY(alloc_upload(p, &coord_body, f.cc_coord_body, error));
+    TRY(alloc_upload(p, &entry_off, f.cc_entry_off, error));
+    TRY(alloc_upload(p, &entries, f.cc_entries.empty() ? std::vector<uint32_t>{0u} : f.cc_entries, error));
+    TRY(alloc_upload(p, &closure, f.cc_closure, error));
+    TRY(alloc_upload(p, &end, f.cc_end, error));
+    TRY(alloc_upload(p, &path_off, f.cc_path_off, error));
+    TRY(alloc_upload(p, &path, f.cc_path.empty() ? std::vector<uint32_t>{0u} : f.cc_path, error));
+    d.cc_off = reinterpret_cast<const uint4*>(off);
+    d.cc_body = body;
+    d.cc_body_parent = body_parent;
+    d.cc_body_coord = body_coord;
+    d.cc_coord = coord;
+    d.cc_coord_body = coord_body;
+    d.cc_entry_off = entry_off;
+    d.cc_entries = entries;
+    d.cc_closure = closure;
+    d.cc_end = end;
+    d.cc_path_off = path_off;
+    d.cc_path = path;
+  }
 #undef TRY
   d.bearings = bearings;
   d.mbodies = mbodies;
@@ -519,6 +658,12 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
     return fail(error, MH_ERR_INVALID_ARGUMENT, "friction must be finite and non-negative");
   if (cfg.warm_start_mode > MH_WARM_START_LEARNED)
     return fail(error, MH_ERR_INVALID_ARGUMENT, "unknown warm_start_mode");
+  if (!(cfg.learned_residual_limit >= 0.0f) || !std::isfinite(cfg.learned_residual_limit) ||
+      cfg.learned_fallback_passes > 64)
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "learned_residual_limit must be finite and >= 0, passes <= 64");
+  if (cfg.closure_newton_steps > 64 || cfg.closure_pcg_iterations > 256 || !(cfg.closure_pcg_tolerance >= 0.0f) ||
+      !std::isfinite(cfg.closure_pcg_tolerance))
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "closure steps <= 64, PCG iterations <= 256, tolerance >= 0");
   if ((cfg.pair_capacity & (cfg.pair_capacity - 1)) != 0 || cfg.pair_capacity < 256 ||
       cfg.pair_capacity > (1u << 22))
     return fail(error, MH_ERR_INVALID_ARGUMENT, "pair_capacity must be a power of two in [256, 2^22]");
@@ -564,6 +709,11 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   d.friction = cfg.friction;
   d.learn_mode = cfg.warm_start_mode;
   d.residual_on = cfg.residual ? 1u : 0u;
+  d.learned_limit = cfg.learned_residual_limit;
+  d.learned_extra = cfg.learned_fallback_passes;
+  d.closure_steps = cfg.closure_newton_steps ? cfg.closure_newton_steps : 12u;  // device.rs:977
+  d.closure_iters = cfg.closure_pcg_iterations ? cfg.closure_pcg_iterations : 8u;  // closure.wgsl:358
+  d.closure_tol = cfg.closure_pcg_tolerance;
 
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
   float* inv_mass;
@@ -628,6 +778,7 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
     ALLOC(d.finit, cfg.pair_capacity);
   }
   ALLOC(d.resid, 4 * (size_t)kResidBlocks);
+  ALLOC(d.learned_state, 4);
   d.pos = pos; d.rot = rot; d.lin = lin; d.ang = ang;
   d.inv_mass = inv_mass; d.masses = masses; d.world_masses = wm;
   d.root_flags = roots; d.components = comps;
@@ -1272,11 +1423,31 @@ int mh_physics_read_training_rows(mh_physics* p, float* features, float* targets
   return MH_OK;
 }
 
+int mh_physics_read_learned_stats(mh_physics* p, mh_learned_stats* out) {
+  if (!p || !out) return MH_ERR_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  if (p->d.learn_mode != kLearnInfer) return MH_ERR_INVALID_ARGUMENT;
+  HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  uint32_t st[4] = {};
+  HIP_TRY_NOERR(hipMemcpy(st, p->d.learned_state, sizeof(st), hipMemcpyDeviceToHost));
+  out->fallback_this_tick = st[0];
+  out->fallback_ticks = st[1];
+  out->nonfinite_this_tick = st[2];
+  out->nonfinite_total = st[3];
+  return MH_OK;
+}
+
 int mh_physics_read_residual(mh_physics* p, mh_contact_residual* out) {
   if (!p || !out) return MH_ERR_INVALID_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
   if (!p->d.residual_on) return MH_ERR_INVALID_ARGUMENT;
   HIP_TRY_NOERR(hipStreamSynchronize(p->stream));
+  if (learned_fallback_on(p->d)) {
+    uint32_t gate = 0;
+    HIP_TRY_NOERR(hipMemcpy(&gate, p->d.learned_state, sizeof(gate), hipMemcpyDeviceToHost));
+    const uint32_t passes = p->d.solver_iterations ? p->d.solver_iterations : 1u;
+    out->fallback_passes = gate ? (p->d.learned_extra ? p->d.learned_extra : passes) : 0u;
+  }
   std::vector<double> part(4 * (size_t)kResidBlocks);
   HIP_TRY_NOERR(hipMemcpy(part.data(), p->d.resid, part.size() * sizeof(double), hipMemcpyDeviceToHost));
   double r2 = 0.0, n = 0.0, pen = 0.0, mx = 0.0;

@@ -67,11 +67,25 @@ class GpuPhysicsConfig:
     use_graph: bool = True
     warm_start: int = _lib.WARM_START_CACHE  # WARM_START_CACHE / _LOG / _LEARNED
     residual: bool = False  # measure the contact residual after each tick's solve
+    # Learned warm start fallback: a tick whose RMS residual after the learned
+    # passes exceeds the limit (N s) runs more passes (0 disables the check;
+    # 0 passes: as many as the learned run's).
+    learned_residual_limit: float = 0.0
+    learned_fallback_passes: int = 0
+    # Loop-closure Newton solve: Newton steps (0: 12), PCG iterations per step
+    # (0: 8) and PCG tolerance on sqrt(r.z / r0.z) (0: every iteration).
+    closure_newton_steps: int = 0
+    closure_pcg_iterations: int = 0
+    closure_pcg_tolerance: float = 0.0
+    closure_components: bool = False  # one PCG per closure component (one workgroup each)
 
     def to_c(self) -> _lib.Config:
         return _lib.Config(int(self.collisions_enabled), int(self.mechanism_self_collisions),
                            int(self.solver_iterations), float(self.friction), int(self.pair_capacity),
-                           int(self.warm_start), int(self.use_graph), int(self.residual))
+                           int(self.warm_start), int(self.use_graph), int(self.residual),
+                           float(self.learned_residual_limit), int(self.learned_fallback_passes),
+                           int(self.closure_newton_steps), int(self.closure_pcg_iterations),
+                           float(self.closure_pcg_tolerance), int(self.closure_components))
 
 
 @dataclass
@@ -365,7 +379,13 @@ class GpuPhysics:
         r = _lib.ContactResidual()
         _check(_lib.load().mh_physics_read_residual(self._h, ctypes.byref(r)), "read_residual")
         return dict(rms=r.rms, mean_penetration=r.mean_penetration, max_penetration=r.max_penetration,
-                    active_contacts=r.active_contacts)
+                    active_contacts=r.active_contacts, fallback_passes=r.fallback_passes)
+
+    def read_learned_stats(self) -> dict:
+        """Fallback counts of a learned-warm-start handle."""
+        r = _lib.LearnedStats()
+        _check(_lib.load().mh_physics_read_learned_stats(self._h, ctypes.byref(r)), "read_learned_stats")
+        return {name: getattr(r, name) for name, _ in _lib.LearnedStats._fields_}
 
 
     def read_coordinates(self) -> np.ndarray:

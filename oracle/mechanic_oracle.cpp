@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstring>
 #include <unordered_map>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -199,6 +200,12 @@ struct World {
   std::vector<Link> links_a, links_b;
   std::vector<int32_t> mdeltas;
   double phase_ns[kPhaseCount] = {};  // accumulated wall time per tick phase
+  // Closure Newton solve: steps per tick (device.rs:977), PCG iterations per
+  // step (closure.wgsl:358) and the optional PCG tolerance.
+  uint32_t closure_steps = 12, closure_iters = 8;
+  float closure_tol = 0.0f;
+  bool closure_components = false;  // one PCG per closure component instead of one global system
+  uint64_t pcg_iterations_run = 0;  // PCG iterations run since creation
 };
 
 template <typename R> V4<R> load4(const float* f) { return {R(f[0]), R(f[1]), R(f[2]), R(f[3])}; }
@@ -1043,14 +1050,24 @@ template <typename R> void joint_of(const World<R>& w, uint32_t body, V3<R>& axi
 
 // The closure Newton loop (device.rs:976-1020; closure.wgsl:233-427). Gradient
 // and diagonal sums and the PCG dot products accumulate in FP64; vectors stay in R.
-template <typename R> void closure_newton(World<R>& w, int max_steps = 12) {
+//
+// Two organisations of the PCG: one system over every coordinate (the
+// reference's), or, with `closure_components`, one independent PCG per closure
+// component -- closures linked by shared coordinates -- each with its own
+// step lengths, tolerance stop and Newton convergence test. JᵀJ is block
+// diagonal over components, so both solve the same system; the second
+// converges each block in a few iterations instead of coupling every loop's
+// step lengths.
+template <typename R> void closure_newton(World<R>& w, int max_steps = -1) {
+  if (max_steps < 0) max_steps = (int)w.closure_steps;
   const uint32_t nq = w.coordinate_count;
   std::vector<uint32_t> closures;
   for (uint32_t i = 0; i < w.bearing_count; ++i)
     if (w.bearings[i].closure != 0u) closures.push_back(i);
   if (closures.empty()) return;
+  const size_t nk = closures.size();
   struct End { V3<R> anchor_a, axis_a, anchor_b, axis_b, res_pos, res_axis; bool active; };
-  std::vector<End> ends(closures.size());
+  std::vector<End> ends(nk);
   std::vector<R> solution(nq), residual(nq), pre(nq), direction(nq), op(nq), diagonal(nq);
   // Coordinates on the walk from `body` to its root.
   auto walk = [&](uint32_t body, auto&& visit) {
@@ -1059,14 +1076,53 @@ template <typename R> void closure_newton(World<R>& w, int max_steps = 12) {
       body = w.mbodies[body].parent;
     }
   };
+  // Component of each closure and coordinate (one component for the whole
+  // system unless closure_components); coordinates on no closure path: none.
+  const uint32_t kNone = 0xffffffffu;
+  std::vector<uint32_t> comp_closure(nk, 0), comp_coord(nq, kNone);
+  uint32_t ncomp = 1;
+  {
+    std::vector<uint32_t> parent(nk);
+    for (size_t k = 0; k < nk; ++k) parent[k] = (uint32_t)k;
+    auto find = [&](uint32_t x) {
+      while (parent[x] != x) x = parent[x] = parent[parent[x]];
+      return x;
+    };
+    std::vector<uint32_t> owner(nq, kNone);
+    for (size_t k = 0; k < nk; ++k) {
+      const auto& r = w.bearings[closures[k]];
+      for (uint32_t side = 0; side < 2; ++side)
+        walk(side ? r.b : r.a, [&](uint32_t, uint32_t c) {
+          if (owner[c] == kNone) owner[c] = (uint32_t)k;
+          else if (w.closure_components) parent[find((uint32_t)k)] = find(owner[c]);
+        });
+    }
+    if (w.closure_components) {
+      std::vector<uint32_t> id(nk, kNone);
+      ncomp = 0;
+      for (size_t k = 0; k < nk; ++k) {
+        const uint32_t root = find((uint32_t)k);
+        if (id[root] == kNone) id[root] = ncomp++;
+        comp_closure[k] = id[root];
+      }
+    }
+    for (uint32_t c = 0; c < nq; ++c)
+      if (owner[c] != kNone) comp_coord[c] = comp_closure[owner[c]];
+  }
+  std::vector<uint8_t> comp_active(ncomp), comp_done(ncomp);
+  std::vector<uint32_t> um_max(ncomp), ud_max(ncomp);
+  std::vector<double> rz(ncomp), rz_stop(ncomp), dad(ncomp), rz_next(ncomp);
+  std::vector<R> alpha(ncomp), beta(ncomp);
   for (int step = 0; step < max_steps; ++step) {
-    uint32_t state0 = 0, state1 = 0;
+    std::fill(um_max.begin(), um_max.end(), 0u);
+    std::fill(ud_max.begin(), ud_max.end(), 0u);
     const auto& links = final_links(w);
-    for (size_t k = 0; k < closures.size(); ++k) {
+    for (size_t k = 0; k < nk; ++k) {
       const auto& r = w.bearings[closures[k]];
       const auto& pa = links[r.a];
       const auto& pb = links[r.b];
       End& e = ends[k];
+      const uint32_t cp = comp_closure[k];
       e.anchor_a = add(xyz(pa.position), quat_rotate(pa.rotation, r.anchor_a));
       e.anchor_b = add(xyz(pb.position), quat_rotate(pb.rotation, r.anchor_b));
       e.axis_a = normalize_v(quat_rotate(pa.rotation, r.axis_a));
@@ -1076,21 +1132,32 @@ template <typename R> void closure_newton(World<R>& w, int max_steps = 12) {
       e.active = false;
       if (!finite3v(e.res_pos) || !finite3v(e.res_axis)) {
         w.diag[0] |= INVALID_NUMERIC;
-        state0 = state1 = 0xffffffffu;
+        um_max[cp] = ud_max[cp] = 0xffffffffu;
         continue;
       }
       uint32_t um = round_u32(length(e.res_pos) * R(1000000.0f));
       R degrees = std::acos(clampr(dot(e.axis_a, e.axis_b), R(-1), R(1))) * R(57.295779513f);
       uint32_t ud = round_u32(degrees * R(1000000.0f));
-      state0 = std::max(state0, um);
-      state1 = std::max(state1, ud);
+      um_max[cp] = std::max(um_max[cp], um);
+      ud_max[cp] = std::max(ud_max[cp], ud);
       e.active = !(um <= 10u && ud <= 1000u);
     }
-    if (!(state0 > 10u || state1 > 1000u)) break;
+    bool any = false;
+    for (uint32_t cp = 0; cp < ncomp; ++cp) {
+      comp_active[cp] = (um_max[cp] > 10u || ud_max[cp] > 1000u) ? 1 : 0;
+      any = any || comp_active[cp];
+    }
+    if (!any) break;
+    // Coordinates the step moves: all (one global system), or those of the
+    // components still outside tolerance.
+    auto live = [&](uint32_t c) {
+      if (!w.closure_components) return true;
+      return comp_coord[c] != kNone && comp_active[comp_coord[c]] != 0;
+    };
     std::vector<double> g(nq, 0.0), dg(nq, 0.0);
-    for (size_t k = 0; k < closures.size(); ++k) {
+    for (size_t k = 0; k < nk; ++k) {
       const End& e = ends[k];
-      if (!e.active) continue;
+      if (!e.active || !comp_active[comp_closure[k]]) continue;
       const auto& r = w.bearings[closures[k]];
       for (uint32_t side = 0; side < 2; ++side) {
         V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
@@ -1111,63 +1178,100 @@ template <typename R> void closure_newton(World<R>& w, int max_steps = 12) {
         });
       }
     }
-    double rz = 0.0;
-    for (uint32_t c = 0; c < nq; ++c) {
-      R gf = R(g[c]), df = R(dg[c]);
-      diagonal[c] = df;
-      residual[c] = df > R(1.0e-12f) ? -gf : R(0);
-      pre[c] = df > R(1.0e-12f) ? residual[c] / df : R(0);
-      solution[c] = R(0);
-      direction[c] = pre[c];
-      rz += (double)residual[c] * (double)pre[c];
-    }
-    for (int it = 0; it < 8; ++it) {
-      std::fill(op.begin(), op.end(), R(0));
-      for (size_t k = 0; k < closures.size(); ++k) {
-        const End& e = ends[k];
-        const auto& r = w.bearings[closures[k]];
-        V3<R> prod_pos[2], prod_axis[2];
-        for (uint32_t side = 0; side < 2; ++side) {
-          V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
-          R sign = side ? R(-1) : R(1);
-          V3<R> pp = v3<R>(0, 0, 0), pa = pp;
-          walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
-            V3<R> jaxis, janchor;
-            joint_of(w, body, jaxis, janchor);
-            pp = add(pp, mul(mul(cross(jaxis, sub(end_anchor, janchor)), sign), direction[c]));
-            pa = add(pa, mul(mul(cross(jaxis, end_axis), sign), direction[c]));
-          });
-          prod_pos[side] = pp;
-          prod_axis[side] = pa;
-        }
-        V3<R> product_pos = add(prod_pos[0], prod_pos[1]), product_axis = add(prod_axis[0], prod_axis[1]);
-        for (uint32_t side = 0; side < 2; ++side) {
-          V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
-          R sign = side ? R(-1) : R(1);
-          walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
-            V3<R> jaxis, janchor;
-            joint_of(w, body, jaxis, janchor);
-            V3<R> pd = mul(cross(jaxis, sub(end_anchor, janchor)), sign);
-            V3<R> ad = mul(cross(jaxis, end_axis), sign);
-            op[c] = op[c] + (dot(pd, product_pos) + R(1) * dot(ad, product_axis));
-          });
-        }
-      }
-      double dad = 0.0;
-      for (uint32_t c = 0; c < nq; ++c) dad += (double)direction[c] * (double)op[c];
-      R alpha = std::fabs(dad) > 1.0e-20 ? R(rz / dad) : R(0);
-      double rz_next = 0.0;
+    // The PCG: vectors of type P -- R for the global system (the reference's
+    // FP32 vectors, FP64 dots), double for the per-component solve, whose
+    // tolerance stop FP32 recursive residuals do not reach.
+    auto pcg = [&](auto zero) {
+      using P = decltype(zero);
+      std::vector<P> sol(nq), res(nq), pz(nq), dir(nq), opp(nq), dia(nq);
+      std::fill(rz.begin(), rz.end(), 0.0);
       for (uint32_t c = 0; c < nq; ++c) {
-        solution[c] = solution[c] + alpha * direction[c];
-        residual[c] = residual[c] - alpha * op[c];
-        pre[c] = diagonal[c] > R(1.0e-12f) ? residual[c] / diagonal[c] : R(0);
-        rz_next += (double)residual[c] * (double)pre[c];
+        const P gf = P(std::is_same<P, double>::value ? g[c] : (double)R(g[c]));
+        const P df = P(std::is_same<P, double>::value ? dg[c] : (double)R(dg[c]));
+        dia[c] = df;
+        res[c] = df > P(1.0e-12f) ? -gf : P(0);
+        pz[c] = df > P(1.0e-12f) ? res[c] / df : P(0);
+        sol[c] = P(0);
+        dir[c] = pz[c];
+        if (comp_coord[c] != kNone) rz[comp_coord[c]] += (double)res[c] * (double)pz[c];
       }
-      R beta = std::fabs(rz) > 1.0e-20 ? R(rz_next / rz) : R(0);
-      for (uint32_t c = 0; c < nq; ++c) direction[c] = pre[c] + beta * direction[c];
-      rz = rz_next;
-    }
+      std::vector<P> al(ncomp), be(ncomp);
+      for (uint32_t cp = 0; cp < ncomp; ++cp) {
+        rz_stop[cp] = (double)w.closure_tol * (double)w.closure_tol * rz[cp];
+        comp_done[cp] = comp_active[cp] ? 0 : 1;
+      }
+      for (uint32_t it = 0; it < w.closure_iters; ++it) {
+        bool running = false;
+        for (uint32_t cp = 0; cp < ncomp; ++cp) {
+          if (!comp_done[cp] && w.closure_tol > 0.0f && rz[cp] <= rz_stop[cp]) comp_done[cp] = 1;
+          running = running || !comp_done[cp];
+        }
+        if (!running) break;
+        w.pcg_iterations_run += 1;
+        std::fill(opp.begin(), opp.end(), P(0));
+        for (size_t k = 0; k < nk; ++k) {
+          if (comp_done[comp_closure[k]]) continue;
+          const End& e = ends[k];
+          const auto& r = w.bearings[closures[k]];
+          V3<P> prod_pos[2], prod_axis[2];
+          for (uint32_t side = 0; side < 2; ++side) {
+            V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
+            R sign = side ? R(-1) : R(1);
+            V3<P> pp = v3<P>(0, 0, 0), pa = pp;
+            walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+              V3<R> jaxis, janchor;
+              joint_of(w, body, jaxis, janchor);
+              const V3<R> pd = mul(cross(jaxis, sub(end_anchor, janchor)), sign), ad = mul(cross(jaxis, end_axis), sign);
+              pp = add(pp, mul(v3<P>(P(pd.x), P(pd.y), P(pd.z)), dir[c]));
+              pa = add(pa, mul(v3<P>(P(ad.x), P(ad.y), P(ad.z)), dir[c]));
+            });
+            prod_pos[side] = pp;
+            prod_axis[side] = pa;
+          }
+          V3<P> product_pos = add(prod_pos[0], prod_pos[1]), product_axis = add(prod_axis[0], prod_axis[1]);
+          for (uint32_t side = 0; side < 2; ++side) {
+            V3<R> end_anchor = side ? e.anchor_b : e.anchor_a, end_axis = side ? e.axis_b : e.axis_a;
+            R sign = side ? R(-1) : R(1);
+            walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+              V3<R> jaxis, janchor;
+              joint_of(w, body, jaxis, janchor);
+              const V3<R> pd = mul(cross(jaxis, sub(end_anchor, janchor)), sign), ad = mul(cross(jaxis, end_axis), sign);
+              opp[c] = opp[c] + (dot(v3<P>(P(pd.x), P(pd.y), P(pd.z)), product_pos) +
+                                 P(1) * dot(v3<P>(P(ad.x), P(ad.y), P(ad.z)), product_axis));
+            });
+          }
+        }
+        std::fill(dad.begin(), dad.end(), 0.0);
+        for (uint32_t c = 0; c < nq; ++c)
+          if (comp_coord[c] != kNone) dad[comp_coord[c]] += (double)dir[c] * (double)opp[c];
+        for (uint32_t cp = 0; cp < ncomp; ++cp) al[cp] = std::fabs(dad[cp]) > 1.0e-20 ? P(rz[cp] / dad[cp]) : P(0);
+        std::fill(rz_next.begin(), rz_next.end(), 0.0);
+        for (uint32_t c = 0; c < nq; ++c) {
+          const uint32_t cp = comp_coord[c];
+          if (cp == kNone || comp_done[cp]) continue;
+          sol[c] = sol[c] + al[cp] * dir[c];
+          res[c] = res[c] - al[cp] * opp[c];
+          pz[c] = dia[c] > P(1.0e-12f) ? res[c] / dia[c] : P(0);
+          rz_next[cp] += (double)res[c] * (double)pz[c];
+        }
+        for (uint32_t cp = 0; cp < ncomp; ++cp) {
+          if (comp_done[cp]) continue;
+          be[cp] = std::fabs(rz[cp]) > 1.0e-20 ? P(rz_next[cp] / rz[cp]) : P(0);
+          rz[cp] = rz_next[cp];
+        }
+        for (uint32_t c = 0; c < nq; ++c) {
+          const uint32_t cp = comp_coord[c];
+          if (cp == kNone || comp_done[cp]) continue;
+          dir[c] = pz[c] + be[cp] * dir[c];
+        }
+      }
+      // The solution in the step's type: FP32 for the FP32 oracle.
+      for (uint32_t c = 0; c < nq; ++c) solution[c] = R(sol[c]);
+    };
+    if (w.closure_components) pcg(0.0);
+    else pcg(R(0));
     for (uint32_t c = 0; c < nq; ++c) {
+      if (!live(c)) continue;
       R step_c = clampr(R(0.35f) * solution[c], R(-0.2f), R(0.2f));
       if (step_c == step_c) w.coord_angle[c] = w.coord_angle[c] + step_c;
       else w.diag[0] |= INVALID_NUMERIC;
@@ -1621,6 +1725,19 @@ void oracle_fk_newton(OracleHandle* h, int steps) {
   };
   if (h->fp64) run(h->d);
   else run(h->f);
+}
+
+// Closure solve settings (0 keeps the reference's 12 steps / 8 iterations; a
+// tolerance of 0 runs every iteration); returns the PCG iterations run so far.
+uint64_t oracle_set_closure(OracleHandle* h, uint32_t steps, uint32_t iterations, float tolerance, int components) {
+  auto set = [&](auto& w) {
+    if (steps) w.closure_steps = steps;
+    if (iterations) w.closure_iters = iterations;
+    w.closure_tol = tolerance;
+    w.closure_components = components != 0;
+    return w.pcg_iterations_run;
+  };
+  return h->fp64 ? set(h->d) : set(h->f);
 }
 
 // Per closure: anchor residual (3) and axis residual (3) from the final links.

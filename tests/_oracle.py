@@ -33,6 +33,8 @@ def load():
     lib.oracle_get_bodies_f64.argtypes = [P, P, P]
     lib.oracle_set_bodies.argtypes = [P, P, P]
     lib.oracle_get_diag.argtypes = [P, P]
+    lib.oracle_set_closure.restype = u64
+    lib.oracle_set_closure.argtypes = [P, u32, u32, f32, i32]
     lib.oracle_phase_ms.restype = i32
     lib.oracle_phase_ms.argtypes = [P, P, i32]
     for name in ("oracle_get_keys", "oracle_get_children", "oracle_get_parents", "oracle_get_pairs",
@@ -132,6 +134,12 @@ class Oracle:
     def set_coordinates(self, rows):
         a = np.ascontiguousarray(rows, np.float32).reshape(-1, 2)
         load().oracle_set_coordinates(self.h, a.ctypes.data)
+
+    def set_closure(self, steps=0, iterations=0, tolerance=0.0, components=False):
+        """Closure Newton steps, PCG iterations per step and PCG tolerance
+        (0 keeps 12 / 8 / none), and one PCG per closure component instead of
+        one global system; returns the PCG iterations run so far."""
+        return load().oracle_set_closure(self.h, steps, iterations, tolerance, int(components))
 
     def fk_newton(self, steps=0):
         load().oracle_fk_newton(self.h, steps)

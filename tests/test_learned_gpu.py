@@ -191,3 +191,76 @@ def test_learned_warm_start_halves_solver_passes(L):
     print(f"mean residual: plain 8 passes {plain.mean():.4f}, learned 4 passes {learned.mean():.4f}")
     assert np.all(np.isfinite(learned))
     assert learned.mean() < plain.mean()
+
+
+def _trained(L, seed=7):
+    train = mh.cuboid_pile(12, 8, 12, seed=seed)
+    F, T = L.collect_rows(train, GpuPhysicsConfig(friction=0.6, solver_iterations=32), 120, range(20, 121, 10))
+    net = L.WarmStartNet(3)
+    net.fit(F, T, epochs=10, batch=4096, learning_rate=1e-3)
+    return net
+
+
+@pytest.mark.parametrize("solve_kind", [0, 1])
+def test_non_finite_predictions_fall_back_to_the_cache_warm_start(L, solve_kind):
+    """A network whose outputs are not finite (NaN weights): every contact takes
+    the cache warm start and the tick runs its fallback passes, so 4 learned
+    passes plus 4 fallback passes give a plain 8-pass run bit for bit, and the
+    fallback counts say so."""
+    scene = mh.cuboid_pile(8, 6, 8, seed=3)
+    net = L.WarmStartNet(1)
+    bad = net.get_net()
+    bad[:net.param_count] = np.nan
+    net.set_net(bad)
+    plain = GpuPhysicsConfig(friction=0.6, solver_iterations=8)
+    learned = GpuPhysicsConfig(friction=0.6, solver_iterations=4, warm_start=_lib.WARM_START_LEARNED,
+                               learned_residual_limit=1e9, learned_fallback_passes=4)
+    with GpuPhysics.new_with_config(scene, plain) as g:
+        g.set_tuning(solve_kind=solve_kind)
+        for t in range(1, 41):
+            g.dispatch_tick(t)
+        p0, v0 = g.read_bodies()
+    with GpuPhysics.new_with_config(scene, learned) as g:
+        g.set_tuning(solve_kind=solve_kind)
+        g.set_mlp(net)
+        for t in range(1, 41):
+            g.dispatch_tick(t)
+        p1, v1 = g.read_bodies()
+        stats = g.read_learned_stats()
+        assert g.read_last_tick().error_flags == 0
+    assert np.array_equal(p0.view(np.uint32), p1.view(np.uint32))
+    assert np.array_equal(v0.view(np.uint32), v1.view(np.uint32))
+    assert stats["fallback_ticks"] == 40 and stats["fallback_this_tick"] == 1
+    assert stats["nonfinite_this_tick"] > 0 and stats["nonfinite_total"] >= stats["nonfinite_this_tick"]
+
+
+def test_residual_limit_decides_the_fallback(L):
+    """With a limit no tick reaches the extra passes never run (the run equals
+    one without the check); with a zero-width margin below every tick's
+    residual they run every tick, and the residual the tick reports drops."""
+    net = _trained(L)
+    scene = mh.cuboid_pile(12, 8, 12, seed=1)
+    base = dict(friction=0.6, solver_iterations=4, warm_start=_lib.WARM_START_LEARNED, residual=True)
+    runs = {}
+    for name, extra in (("none", {}), ("never", dict(learned_residual_limit=1e9)),
+                        ("always", dict(learned_residual_limit=1e-12, learned_fallback_passes=4))):
+        with GpuPhysics.new_with_config(scene, GpuPhysicsConfig(**base, **extra)) as g:
+            g.set_mlp(net)
+            rms = []
+            passes = []
+            for t in range(1, 61):
+                g.dispatch_tick(t)
+                r = g.read_residual()
+                rms.append(r["rms"])
+                passes.append(r["fallback_passes"])
+            runs[name] = (g.read_bodies(), np.asarray(rms), passes, g.read_learned_stats())
+    (p_none, v_none), rms_none, _, _ = runs["none"]
+    (p_never, v_never), rms_never, passes_never, st_never = runs["never"]
+    (_, _), rms_always, passes_always, st_always = runs["always"]
+    assert np.array_equal(p_none.view(np.uint32), p_never.view(np.uint32))
+    assert np.array_equal(v_none.view(np.uint32), v_never.view(np.uint32))
+    assert st_never["fallback_ticks"] == 0 and set(passes_never) == {0}
+    ticks_with_contacts = int((rms_none > 0).sum())
+    assert st_always["fallback_ticks"] >= ticks_with_contacts and set(passes_always) <= {0, 4}
+    print(f"mean residual: learned 4 {rms_none.mean():.4f}, with 4 fallback passes every tick {rms_always.mean():.4f}")
+    assert rms_always.mean() < rms_none.mean()

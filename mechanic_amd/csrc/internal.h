@@ -21,6 +21,8 @@ constexpr uint32_t kSideTile = 2048;  // bodies per tile of the contact-side off
 constexpr uint32_t kMaxSlots = 32;      // contact-side delta slots per body (body-major)
 constexpr uint32_t kSlotOverflow = 31;  // slot code of a side past the body's slots (atomics instead)
 constexpr uint32_t kContactMask = 0x3fffffu;  // contact index bits of a solver row's word (2^22 pairs)
+constexpr uint32_t kSerialBearingLimit = 64u;    // device.rs:19: serial sweeps up to this many bearings
+constexpr uint32_t kSerialPassMultiplier = 12u;  // device.rs:20: their passes per solver iteration
 constexpr uint32_t kRadixTile = 1024;  // keys per radix-sort workgroup
 constexpr uint32_t kRadixMaxPasses = 8;  // 8-bit digit passes at most (64-bit keys)
 constexpr uint32_t kRadixTicket = kRadixMaxPasses * 256;  // rs_hist word of pass 0's tile ticket
@@ -177,6 +179,9 @@ struct Dev {
   uint32_t level_count;       // depth levels 1..max in mlist
   uint32_t jump_rounds;       // pointer-jump rounds of one forward-kinematics pass
   uint32_t has_dynamic_root;  // a root with inverse mass > 0 exists
+  uint32_t serial_mech;       // dynamic root and <= 64 bearings: serial contact and bearing sweeps
+  uint4* serial_keys;         // 2 per active contact: (body a, body b, manifold slot, low), (high, active index)
+  uint32_t* serial_order;     // active contacts in the serial sweep order
   const float4* bearings;     // 5 per bearing: anchor a, anchor b, axis a, axis b, metadata bits
   const float4* mbodies;      // 4 per body (mh_mechanism_body): metadata, traversal, bind position, bind rotation
   float2* coords;             // angle, angular velocity per coordinate
@@ -274,6 +279,10 @@ void launch_targets(hipStream_t s, const Dev& d);
 // solver records during the contact solve; `write_back` also stores them to the
 // state arrays.
 void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back);
+// The serial small-mechanism sweeps after the parallel warm-start pass:
+// passes - 1 contact sweeps, each bearing sweep after the warm start and after
+// every contact sweep, then the final velocities and accumulated impulses.
+void launch_serial_solve(hipStream_t s, const Dev& d, uint32_t passes);
 void launch_advance_coordinates(hipStream_t s, const Dev& d);
 void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures);
 // LDS bytes of the per-component closure kernel for `d`'s largest component;

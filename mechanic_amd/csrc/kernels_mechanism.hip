@@ -22,6 +22,7 @@
 
 #include "internal.h"
 #include "mh_math.h"
+#include "mh_solver.h"
 
 namespace mh {
 
@@ -1160,12 +1161,226 @@ __global__ __launch_bounds__(256) void k_validate_bearings(Dev d, int mechanism)
 
 inline uint32_t blocks_for(uint32_t n) { return (n + 255u) / 256u; }
 
+// ---------------------------------------------------------------------------
+// Serial small-mechanism solvers (device.rs:19-20, 1370-1416). With a dynamic
+// mechanism root and at most 64 bearings in the scene, the passes after the
+// parallel warm start are 12x as many immediate Gauss-Seidel sweeps: the active
+// contacts forward then backward in their sort order (solve_accumulate_serial,
+// collision.wgsl:979-1048), then the bearing rows forward then backward
+// (project_bearing_velocities_serial, articulated.wgsl:277-285). The sweeps are
+// one thread's work by definition; here they run as ONE launch for all passes
+// (the reference dispatches two per pass) on the solver records, which hold
+// the velocities while the contact solve runs.
+// ---------------------------------------------------------------------------
+
+// Sort key of an active contact: (body A, body B, manifold slot) as the
+// reference selects them (collision.wgsl:991-1017), then the collider pair. The
+// slot is where acquire_manifold's probe starts, pair_hash(low, high) &
+// (capacity - 1) (collision.wgsl:499-504): the reference's slot whenever its
+// table holds no colliding entry. Above 64 active contacts the reference sweeps
+// in append order, which its atomics leave unspecified; the sorted order is
+// used there too.
+__global__ __launch_bounds__(256) void k_serial_keys(Dev d) {
+  const uint64_t tick = tick_m(d);
+  const uint32_t n = read_count(&d.diag[tick & 1][kDiagActive], d.pair_capacity);
+  const float4* mrows = d.mrows[tick & 1];
+  for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < n; ai += gridDim.x * blockDim.x) {
+    const float4 r3 = row_word(d, 3, ai);
+    const uint32_t ci = __float_as_uint(r3.z) & kContactMask;
+    const float4 m0 = mrows[3 * (size_t)ci];
+    const uint32_t low = __float_as_uint(m0.x), high = __float_as_uint(m0.y);
+    d.serial_keys[2 * (size_t)ai] =
+        make_uint4(__float_as_uint(r3.x), __float_as_uint(r3.y), pair_hash(low, high) & d.hash_mask, low);
+    d.serial_keys[2 * (size_t)ai + 1] = make_uint4(high, ai, 0u, 0u);
+  }
+}
+
+__device__ __forceinline__ bool key_less(uint4 x, uint32_t xh, uint4 y, uint32_t yh) {
+  if (x.x != y.x) return x.x < y.x;
+  if (x.y != y.y) return x.y < y.y;
+  if (x.z != y.z) return x.z < y.z;
+  if (x.w != y.w) return x.w < y.w;
+  return xh < yh;
+}
+
+// Rank of every key among all keys (the keys are distinct: one contact per
+// collider pair), so serial_order is the sorted permutation. Keys stream
+// through LDS in tiles of 256.
+__global__ __launch_bounds__(256) void k_serial_rank(Dev d) {
+  __shared__ uint4 tile_key[256];
+  __shared__ uint32_t tile_high[256];
+  const uint64_t tick = tick_m(d);
+  const uint32_t n = read_count(&d.diag[tick & 1][kDiagActive], d.pair_capacity);
+  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    uint4 mine = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t mine_high = 0u;
+    if (valid) {
+      mine = d.serial_keys[2 * (size_t)i];
+      mine_high = d.serial_keys[2 * (size_t)i + 1].x;
+    }
+    uint32_t rank = 0;
+    for (uint32_t j0 = 0; j0 < n; j0 += 256u) {
+      __syncthreads();
+      if (j0 + threadIdx.x < n) {
+        tile_key[threadIdx.x] = d.serial_keys[2 * (size_t)(j0 + threadIdx.x)];
+        tile_high[threadIdx.x] = d.serial_keys[2 * (size_t)(j0 + threadIdx.x) + 1].x;
+      }
+      __syncthreads();
+      const uint32_t m = n - j0 < 256u ? n - j0 : 256u;
+      for (uint32_t t = 0; t < m; ++t) rank += key_less(tile_key[t], tile_high[t], mine, mine_high) ? 1u : 0u;
+    }
+    if (valid) d.serial_order[rank] = i;
+  }
+}
+
+__device__ __forceinline__ void store_velocity(const Dev& d, uint32_t b, v3 lin, v3 ang) {
+  float4* r = d.sbody + 4 * (size_t)b;
+  r[0] = f4(lin, r[0].w);
+  r[1] = f4(ang, r[1].w);
+}
+
+// solve_contact_immediate (collision.wgsl:908-977) on active contact `ai`: the
+// full projected impulse (no relaxation) against the current velocities,
+// applied at once.
+__device__ void contact_immediate(const Dev& d, uint32_t ai) {
+  const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
+  const uint32_t a = __float_as_uint(r3.x), b = __float_as_uint(r3.y);
+  const bool has_b = b != kInvalid;
+  const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
+  const float penetration = r0.w, denominator = r3.w;
+  const SolverBody sa = load_body(d.sbody, a);
+  const SolverBody sb = has_b ? load_body(d.sbody, b) : sa;
+  const v3 velocity_b = has_b ? sb.lin + cross(sb.ang, arm_b) : mk(0.0f, 0.0f, 0.0f);
+  if (denominator <= 0.0f) return;
+  const v3 relative = velocity_b - (sa.lin + cross(sa.ang, arm_a));
+  const float normal_speed = dot(relative, normal);
+  const float lambda = d.lambda[ai];
+  if (penetration <= 1.0e-5f && normal_speed >= -1.0e-5f && lambda <= 1.0e-6f) {
+    d.lambda[ai] = 0.0f;
+    return;
+  }
+  const float previous = fmx(lambda, 0.0f);
+  const float recovery = r2.w < 0.0f ? 1.0f : 0.2f;
+  const float bias = fmn((fmx(penetration - kSlop, 0.0f) * recovery) / d.delta_seconds, kMaxCorrection);
+  const float accumulated = fmx(previous + (-normal_speed + bias) / denominator, 0.0f);
+  const float delta = accumulated - previous;
+  d.lambda[ai] = accumulated;
+  v3 impulse = normal * delta;
+  const v3 tangent_velocity = relative - normal * normal_speed;
+  const float tangent_speed = length(tangent_velocity);
+  if (tangent_speed > 1.0e-6f) {
+    const v3 tangent = tangent_velocity / tangent_speed;
+    const float tangent_denominator = denom_of(sa, sb, has_b, arm_a, arm_b, tangent);
+    const float friction_impulse = fmn(tangent_speed / tangent_denominator, fabsf(delta) * d.friction);
+    impulse = impulse - tangent * friction_impulse;
+  }
+  store_velocity(d, a, sa.lin - impulse * sa.inv_mass, sa.ang + inv_inertia(sa, cross(arm_a, -impulse)));
+  if (has_b) store_velocity(d, b, sb.lin + impulse * sb.inv_mass, sb.ang + inv_inertia(sb, cross(arm_b, impulse)));
+}
+
+// solve_linear_axis_immediate (articulated.wgsl:157-191).
+__device__ void linear_immediate(const Dev& d, uint32_t a, uint32_t b, v3 arm_a, v3 arm_b, v3 direction) {
+  const v3 velocity_a = lin_of(d, a, 1) + cross(ang_of(d, a, 1), arm_a);
+  const v3 velocity_b = lin_of(d, b, 1) + cross(ang_of(d, b, 1), arm_b);
+  const v3 relative = velocity_b - velocity_a;
+  const v3 angular_a = cross(arm_a, direction);
+  const v3 angular_b = cross(arm_b, direction);
+  const float denominator =
+      ((inv_mass_of(d, a) + inv_mass_of(d, b)) + dot(angular_a, wii(d, a, angular_a))) + dot(angular_b, wii(d, b, angular_b));
+  if (denominator <= 1.0e-12f) return;
+  const v3 impulse = direction * (dot(relative, direction) / denominator);
+  store_velocity(d, a, lin_of(d, a, 1) + impulse * inv_mass_of(d, a), ang_of(d, a, 1) + wii(d, a, cross(arm_a, impulse)));
+  store_velocity(d, b, lin_of(d, b, 1) - impulse * inv_mass_of(d, b),
+                 ang_of(d, b, 1) + wii(d, b, cross(arm_b, -impulse)));
+}
+
+// solve_angular_axis_immediate (articulated.wgsl:193-214).
+__device__ void angular_immediate(const Dev& d, uint32_t a, uint32_t b, v3 axis) {
+  const v3 relative = ang_of(d, b, 1) - ang_of(d, a, 1);
+  const v3 inverse_a = wii(d, a, axis);
+  const v3 inverse_b = wii(d, b, axis);
+  const float denominator = dot(axis, inverse_a + inverse_b);
+  if (denominator <= 1.0e-12f) return;
+  const float impulse = dot(relative, axis) / denominator;
+  store_velocity(d, a, lin_of(d, a, 1), ang_of(d, a, 1) + inverse_a * impulse);
+  store_velocity(d, b, lin_of(d, b, 1), ang_of(d, b, 1) - inverse_b * impulse);
+}
+
+// project_bearing_velocity_row (articulated.wgsl:216-276).
+__device__ void bearing_row_immediate(const Dev& d, uint32_t i) {
+  const BearingRow r = load_bearing(d, i);
+  const float4 rot_a = d.rot[r.a], rot_b = d.rot[r.b];
+  const v3 arm_a = quat_rotate(rot_a, r.anchor_a);
+  const v3 arm_b = quat_rotate(rot_b, r.anchor_b);
+  linear_immediate(d, r.a, r.b, arm_a, arm_b, mk(1.0f, 0.0f, 0.0f));
+  linear_immediate(d, r.a, r.b, arm_a, arm_b, mk(0.0f, 1.0f, 0.0f));
+  linear_immediate(d, r.a, r.b, arm_a, arm_b, mk(0.0f, 0.0f, 1.0f));
+  const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
+  const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
+  const v3 hinge = normalize_v(axis_a + axis_b);
+  const v3 helper = fabsf(hinge.x) > 0.8f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+  const v3 tangent_a = normalize_v(cross(hinge, helper));
+  const v3 tangent_b = cross(hinge, tangent_a);
+  angular_immediate(d, r.a, r.b, tangent_a);
+  angular_immediate(d, r.a, r.b, tangent_b);
+}
+
+__device__ void bearing_sweep(const Dev& d) {
+  for (uint32_t i = 0; i < d.bearing_count; ++i) bearing_row_immediate(d, i);
+  for (uint32_t i = d.bearing_count; i > 0; --i) bearing_row_immediate(d, i - 1);
+}
+
+// All serial passes of one tick in one thread. Without an active contact the
+// reference skips its bearing sweeps (indirect arguments 12..14,
+// collision.wgsl:747) and its contact sweeps have nothing to do.
+__global__ __launch_bounds__(64) void k_serial_passes(Dev d, uint32_t passes) {
+  if (threadIdx.x != 0) return;
+  const uint64_t tick = tick_m(d);
+  const uint32_t n = read_count(&d.diag[tick & 1][kDiagActive], d.pair_capacity);
+  if (n == 0) return;
+  bearing_sweep(d);
+  for (uint32_t p = 1; p < passes; ++p) {
+    for (uint32_t k = 0; k < n; ++k) contact_immediate(d, d.serial_order[k]);
+    for (uint32_t k = n; k > 0; --k) contact_immediate(d, d.serial_order[k - 1]);
+    bearing_sweep(d);
+  }
+}
+
+// The end of the last pass: accumulated impulses into this tick's manifold rows
+// (persist_contacts, collision.wgsl:1071-1088) and the records' velocities into
+// the state.
+__global__ __launch_bounds__(256) void k_serial_finish(Dev d) {
+  const uint64_t tick = tick_m(d);
+  const uint32_t n = read_count(&d.diag[tick & 1][kDiagActive], d.pair_capacity);
+  if (n == 0) return;
+  float4* rows_now = d.mrows[tick & 1];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < n; ai += stride) {
+    const uint32_t ci = __float_as_uint(row_word(d, 3, ai).z) & kContactMask;
+    rows_now[3 * (size_t)ci + 2].w = d.lambda[ai];
+  }
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < d.body_count; b += stride) {
+    d.lin[b] = f4(xyz(d.sbody[4 * (size_t)b]), 0.0f);
+    d.ang[b] = f4(xyz(d.sbody[4 * (size_t)b + 1]), 0.0f);
+  }
+}
+
 }  // namespace
 
 void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back) {
   if (d.bearing_count == 0) return;
   k_bearing_project<<<blocks_for(d.bearing_count), 256, 0, s>>>(d, in_contact);
   k_bearing_apply<<<blocks_for(d.bearing_body_count), 256, 0, s>>>(d, in_contact, write_back);
+}
+
+void launch_serial_solve(hipStream_t s, const Dev& d, uint32_t passes) {
+  const uint32_t grid = std::min(std::max(d.pair_capacity, d.body_count) / 256u + 1u, 1024u);
+  k_serial_keys<<<grid, 256, 0, s>>>(d);
+  k_serial_rank<<<grid, 256, 0, s>>>(d);
+  k_serial_passes<<<1, 64, 0, s>>>(d, passes);
+  k_serial_finish<<<grid, 256, 0, s>>>(d);
 }
 
 void launch_advance_coordinates(hipStream_t s, const Dev& d) {

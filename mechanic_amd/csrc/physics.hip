@@ -404,10 +404,6 @@ int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
     f.coord_offset.push_back((uint32_t)f.coord_entries.size());
   }
   build_components(c, f, parent, root, tree_row);
-  // The serial small-mechanism solvers (device.rs:1063-1075, 1370-1400) are not built.
-  if (f.max_depth > 0 && f.has_dynamic_root && nb <= 64)
-    return fail(error, MH_ERR_UNSUPPORTED,
-                "mechanisms with <= 64 bearings next to a dynamic root use the serial solver, not built");
   return MH_OK;
 }
 
@@ -431,6 +427,8 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
   d.jump_rounds = f.jump_rounds;
   d.has_dynamic_root = f.has_dynamic_root ? 1u : 0u;
   p->mech_active = f.max_depth > 0;
+  // The serial small-mechanism solvers (device.rs:1370-1373).
+  d.serial_mech = (p->mech_active && f.has_dynamic_root && nb <= kSerialBearingLimit) ? 1u : 0u;
   int rc;
 #define TRY(x)                  \
   do {                          \
@@ -468,6 +466,10 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
   TRY(alloc_upload(p, &l0, links, error));
   TRY(alloc_upload(p, &l1, links, error));
   TRY(alloc(p, &mdeltas, 6 * (size_t)n, error));
+  if (d.serial_mech) {
+    TRY(alloc(p, &d.serial_keys, 2 * (size_t)d.pair_capacity, error));
+    TRY(alloc(p, &d.serial_order, d.pair_capacity, error));
+  }
   uint32_t *mlist, *level_offset, *blist, *closure_rows, *cpath_offset, *cpath, *coord_offset, *coord_entries,
       *coord_body;
   TRY(alloc_upload(p, &mlist, f.mlist, error));
@@ -1000,7 +1002,8 @@ int mh_physics_set_tuning(mh_physics* p, const mh_tuning* t) {
   d.t_solve_blocks = t->solve_blocks;
   d.t_scan_blocks = t->scan_blocks;
   d.t_traverse_kind = t->traverse_kind;
-  d.t_solve_kind = t->solve_kind;
+  // The serial small-mechanism path warms up with the per-contact pass.
+  d.t_solve_kind = d.serial_mech ? 0u : t->solve_kind;
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->graph_exec = nullptr;

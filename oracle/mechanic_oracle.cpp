@@ -206,6 +206,9 @@ struct World {
   float closure_tol = 0.0f;
   bool closure_components = false;  // one PCG per closure component instead of one global system
   uint64_t pcg_iterations_run = 0;  // PCG iterations run since creation
+  // Largest bearing count that takes the serial small-mechanism sweeps (the
+  // reference's 64; tests lower it to run the same scene without them).
+  uint32_t serial_limit = 64;  // device.rs:19
 };
 
 template <typename R> V4<R> load4(const float* f) { return {R(f[0]), R(f[1]), R(f[2]), R(f[3])}; }
@@ -887,6 +890,157 @@ template <typename R> void bearing_projection(World<R>& w) {
   }
 }
 
+// ---- serial small-mechanism solvers (device.rs:19-20, 1370-1416) ----------
+// A scene whose mechanisms have a dynamic root and at most 64 bearings in all
+// runs, after the parallel warm-start pass, 12x the passes as immediate
+// Gauss-Seidel sweeps: contacts forward then backward in their sort order
+// (collision.wgsl:979-1048), each followed by a forward-then-backward sweep of
+// the bearing rows (articulated.wgsl:216-285) when any contact is active.
+constexpr uint32_t kSerialPassMultiplier = 12;  // device.rs:20
+
+// pair_hash (collision.wgsl:488-497).
+inline uint32_t pair_hash(uint32_t a, uint32_t b) {
+  uint32_t v = a * 0x9e3779b9u;
+  v ^= b * 0x85ebca6bu;
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  v ^= v >> 16;
+  return v;
+}
+
+// Order of the serial contact sweep: ascending (body A, body B, manifold slot)
+// as solve_accumulate_serial selects them (collision.wgsl:991-1017). The slot
+// is where acquire_manifold's probe starts, pair_hash(low, high) & (capacity - 1)
+// (collision.wgsl:499-504): the reference's slot whenever its table holds no
+// colliding entry; the collider pair breaks the remaining ties. Above 64
+// active contacts the reference sweeps them in append order, which its atomics
+// leave unspecified; the same sorted order is used there.
+template <typename R>
+std::vector<uint32_t> serial_order(const World<R>& w, const std::vector<uint64_t>& pair_keys) {
+  struct Key { uint32_t a, b, slot, low, high, ci; };
+  std::vector<Key> k;
+  k.reserve(w.active.size());
+  for (uint32_t ci : w.active) {
+    const Contact<R>& c = w.contacts[ci];
+    const uint32_t low = (uint32_t)(pair_keys[ci] >> 32), high = (uint32_t)pair_keys[ci];
+    k.push_back({c.body_a, c.body_b, pair_hash(low, high) & (w.pair_capacity - 1u), low, high, ci});
+  }
+  std::sort(k.begin(), k.end(), [](const Key& x, const Key& y) {
+    if (x.a != y.a) return x.a < y.a;
+    if (x.b != y.b) return x.b < y.b;
+    if (x.slot != y.slot) return x.slot < y.slot;
+    if (x.low != y.low) return x.low < y.low;
+    return x.high < y.high;
+  });
+  std::vector<uint32_t> out;
+  out.reserve(k.size());
+  for (const Key& x : k) out.push_back(x.ci);
+  return out;
+}
+
+// solve_contact_immediate (collision.wgsl:908-977): the full projected impulse
+// (no relaxation) against the current velocities, applied at once.
+template <typename R> void contact_immediate(World<R>& w, Contact<R>& c) {
+  const uint32_t a = c.body_a, b = c.body_b;
+  const V3<R> arm_a = xyz(c.arm_a_impulse), arm_b = xyz(c.arm_b), normal = xyz(c.normal_penetration);
+  V3<R> velocity_b = v3<R>(0, 0, 0);
+  if (b != INVALID) velocity_b = contact_velocity(w, b, arm_b);
+  const R denominator = impulse_denominator(w, a, b, arm_a, arm_b, normal);
+  if (denominator <= R(0)) return;
+  const V3<R> relative = sub(velocity_b, contact_velocity(w, a, arm_a));
+  const R normal_speed = dot(relative, normal);
+  if (c.normal_penetration.w <= R(1.0e-5f) && normal_speed >= R(-1.0e-5f) && c.arm_a_impulse.w <= R(1.0e-6f)) {
+    c.arm_a_impulse.w = R(0);
+    return;
+  }
+  const R previous = mx(c.arm_a_impulse.w, R(0));
+  const R bias = penetration_bias(w, c.normal_penetration.w, c.arm_b.w < R(0));
+  const R accumulated = mx(previous + (-normal_speed + bias) / denominator, R(0));
+  const R delta = accumulated - previous;
+  c.arm_a_impulse.w = accumulated;
+  V3<R> impulse = mul(normal, delta);
+  const V3<R> tangent_velocity = sub(relative, mul(normal, normal_speed));
+  const R tangent_speed = length(tangent_velocity);
+  if (tangent_speed > R(1.0e-6f)) {
+    const V3<R> tangent = divs(tangent_velocity, tangent_speed);
+    const R tangent_denominator = impulse_denominator(w, a, b, arm_a, arm_b, tangent);
+    const R friction_impulse = mn(tangent_speed / tangent_denominator, std::fabs(delta) * w.friction);
+    impulse = sub(impulse, mul(tangent, friction_impulse));
+  }
+  const V3<R> la = sub(xyz(w.lin[a]), mul(impulse, w.world_masses[3 * (size_t)a].w));
+  w.lin[a] = {la.x, la.y, la.z, R(0)};
+  const V3<R> aa = add(xyz(w.ang[a]), inverse_inertia(w, a, cross(arm_a, neg(impulse))));
+  w.ang[a] = {aa.x, aa.y, aa.z, R(0)};
+  if (b != INVALID) {
+    const V3<R> lb = add(xyz(w.lin[b]), mul(impulse, w.world_masses[3 * (size_t)b].w));
+    w.lin[b] = {lb.x, lb.y, lb.z, R(0)};
+    const V3<R> ab = add(xyz(w.ang[b]), inverse_inertia(w, b, cross(arm_b, impulse)));
+    w.ang[b] = {ab.x, ab.y, ab.z, R(0)};
+  }
+}
+
+// solve_accumulate_serial (collision.wgsl:979-1048) over the sorted order.
+template <typename R> void serial_contacts(World<R>& w, const std::vector<uint32_t>& order) {
+  for (size_t i = 0; i < order.size(); ++i) contact_immediate(w, w.contacts[order[i]]);
+  for (size_t i = order.size(); i > 0; --i) contact_immediate(w, w.contacts[order[i - 1]]);
+}
+
+// solve_linear_axis_immediate / solve_angular_axis_immediate (articulated.wgsl:157-214).
+template <typename R> void bearing_linear_immediate(World<R>& w, uint32_t a, uint32_t b, V3<R> arm_a, V3<R> arm_b,
+                                                    V3<R> rel, V3<R> dir) {
+  const V3<R> angular_a = cross(arm_a, dir), angular_b = cross(arm_b, dir);
+  const R den =
+      ((mass_inv(w, a) + mass_inv(w, b)) + dot(angular_a, wii(w, a, angular_a))) + dot(angular_b, wii(w, b, angular_b));
+  if (den <= R(1.0e-12f)) return;
+  const V3<R> impulse = mul(dir, dot(rel, dir) / den);
+  const V3<R> la = add(xyz(w.lin[a]), mul(impulse, mass_inv(w, a)));
+  w.lin[a] = {la.x, la.y, la.z, R(0)};
+  const V3<R> aa = add(xyz(w.ang[a]), wii(w, a, cross(arm_a, impulse)));
+  w.ang[a] = {aa.x, aa.y, aa.z, R(0)};
+  const V3<R> lb = sub(xyz(w.lin[b]), mul(impulse, mass_inv(w, b)));
+  w.lin[b] = {lb.x, lb.y, lb.z, R(0)};
+  const V3<R> ab = add(xyz(w.ang[b]), wii(w, b, cross(arm_b, neg(impulse))));
+  w.ang[b] = {ab.x, ab.y, ab.z, R(0)};
+}
+template <typename R> void bearing_angular_immediate(World<R>& w, uint32_t a, uint32_t b, V3<R> rel, V3<R> axis) {
+  const V3<R> ia = wii(w, a, axis), ib = wii(w, b, axis);
+  const R den = dot(axis, add(ia, ib));
+  if (den <= R(1.0e-12f)) return;
+  const R impulse = dot(rel, axis) / den;
+  const V3<R> aa = add(xyz(w.ang[a]), mul(ia, impulse));
+  w.ang[a] = {aa.x, aa.y, aa.z, R(0)};
+  const V3<R> ab = sub(xyz(w.ang[b]), mul(ib, impulse));
+  w.ang[b] = {ab.x, ab.y, ab.z, R(0)};
+}
+
+// project_bearing_velocity_row (articulated.wgsl:216-276).
+template <typename R> void bearing_row_immediate(World<R>& w, uint32_t i) {
+  const auto& r = w.bearings[i];
+  const V3<R> arm_a = quat_rotate(w.rot[r.a], r.anchor_a), arm_b = quat_rotate(w.rot[r.b], r.anchor_b);
+  const V3<R> dirs[3] = {v3<R>(1, 0, 0), v3<R>(0, 1, 0), v3<R>(0, 0, 1)};
+  for (const V3<R>& dir : dirs) {
+    const V3<R> va = add(xyz(w.lin[r.a]), cross(xyz(w.ang[r.a]), arm_a));
+    const V3<R> vb = add(xyz(w.lin[r.b]), cross(xyz(w.ang[r.b]), arm_b));
+    bearing_linear_immediate(w, r.a, r.b, arm_a, arm_b, sub(vb, va), dir);
+  }
+  const V3<R> axis_a = normalize_v(quat_rotate(w.rot[r.a], r.axis_a));
+  const V3<R> axis_b = normalize_v(quat_rotate(w.rot[r.b], r.axis_b));
+  const V3<R> hinge = normalize_v(add(axis_a, axis_b));
+  const V3<R> helper = std::fabs(hinge.x) > R(0.8f) ? v3<R>(0, 1, 0) : v3<R>(1, 0, 0);
+  const V3<R> ta = normalize_v(cross(hinge, helper));
+  const V3<R> tb = cross(hinge, ta);
+  bearing_angular_immediate(w, r.a, r.b, sub(xyz(w.ang[r.b]), xyz(w.ang[r.a])), ta);
+  bearing_angular_immediate(w, r.a, r.b, sub(xyz(w.ang[r.b]), xyz(w.ang[r.a])), tb);
+}
+
+// project_bearing_velocities_serial (articulated.wgsl:277-285).
+template <typename R> void serial_bearings(World<R>& w) {
+  for (uint32_t i = 0; i < w.bearing_count; ++i) bearing_row_immediate(w, i);
+  for (uint32_t i = w.bearing_count; i > 0; --i) bearing_row_immediate(w, i - 1);
+}
+
 template <typename R> V3<R> permitted_axis(const World<R>& w, uint32_t body) {
   const auto& m = w.mbodies[body];
   const auto& r = w.bearings[m.tree_bearing];
@@ -1377,10 +1531,25 @@ template <typename R> void step(World<R>& w, uint64_t tick64, int stop) {
     w.deltas.assign(6 * (size_t)w.body_count, 0);
     const uint32_t iterations = std::max(1u, w.iterations);
     const bool interleave = w.mech_active && w.has_dynamic_root;
-    for (uint32_t it = 0; it < iterations; ++it) {
-      solve_pass(w, it == 0);
-      if (!w.active.empty()) solve_apply(w);
-      if (interleave) bearing_projection(w);
+    if (interleave && w.bearing_count <= w.serial_limit) {
+      // device.rs:1352-1416 with serial_mechanism: the parallel warm start, then
+      // the serial sweeps; the bearing sweeps run only with active contacts.
+      solve_pass(w, true);
+      if (!w.active.empty()) {
+        solve_apply(w);
+        const std::vector<uint32_t> order = serial_order(w, keys);
+        serial_bearings(w);
+        for (uint32_t p = 1; p < iterations * kSerialPassMultiplier; ++p) {
+          serial_contacts(w, order);
+          serial_bearings(w);
+        }
+      }
+    } else {
+      for (uint32_t it = 0; it < iterations; ++it) {
+        solve_pass(w, it == 0);
+        if (!w.active.empty()) solve_apply(w);
+        if (interleave) bearing_projection(w);
+      }
     }
     // persist_contacts (collision.wgsl:1071-1088)
     for (uint32_t ci : w.active) {
@@ -1738,6 +1907,11 @@ uint64_t oracle_set_closure(OracleHandle* h, uint32_t steps, uint32_t iterations
     return w.pcg_iterations_run;
   };
   return h->fp64 ? set(h->d) : set(h->f);
+}
+
+void oracle_set_serial_limit(OracleHandle* h, uint32_t bearings) {
+  if (h->fp64) h->d.serial_limit = bearings;
+  else h->f.serial_limit = bearings;
 }
 
 // Per closure: anchor residual (3) and axis residual (3) from the final links.

@@ -35,6 +35,7 @@ def load():
     lib.oracle_get_diag.argtypes = [P, P]
     lib.oracle_set_closure.restype = u64
     lib.oracle_set_closure.argtypes = [P, u32, u32, f32, i32]
+    lib.oracle_set_serial_limit.argtypes = [P, u32]
     lib.oracle_phase_ms.restype = i32
     lib.oracle_phase_ms.argtypes = [P, P, i32]
     for name in ("oracle_get_keys", "oracle_get_children", "oracle_get_parents", "oracle_get_pairs",
@@ -134,6 +135,10 @@ class Oracle:
     def set_coordinates(self, rows):
         a = np.ascontiguousarray(rows, np.float32).reshape(-1, 2)
         load().oracle_set_coordinates(self.h, a.ctypes.data)
+
+    def set_serial_limit(self, bearings):
+        """Largest bearing count that takes the serial small-mechanism sweeps."""
+        load().oracle_set_serial_limit(self.h, bearings)
 
     def set_closure(self, steps=0, iterations=0, tolerance=0.0, components=False):
         """Closure Newton steps, PCG iterations per step and PCG tolerance

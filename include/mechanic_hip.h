@@ -191,6 +191,13 @@ typedef struct {
    * hold at most 256 path bodies, 256 coordinates and 128 closures
    * (MH_ERR_UNSUPPORTED otherwise). 0: one global PCG, as the reference. */
   uint32_t closure_components;
+  /* 1 (needs closure_components): after the Newton steps, the coordinate speeds
+   * are projected onto the loops' velocity constraint J v = 0 (minimum-norm
+   * correction, the same per-component PCG and tolerance) before the tree
+   * reconstructs body velocities from them. The reference reconstructs from the
+   * unprojected speeds (articulated.wgsl:423-456), which leaves loop-violating
+   * velocities for the bearing projection to absorb. 0: as the reference. */
+  uint32_t closure_velocity_projection;
 } mh_config;
 
 /* Warm start of the contact solver's first pass. */

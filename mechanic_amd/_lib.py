@@ -155,7 +155,8 @@ class Config(ctypes.Structure):
                 ("warm_start_mode", c_uint32), ("use_graph", c_uint32), ("residual", c_uint32),
                 ("learned_residual_limit", c_float), ("learned_fallback_passes", c_uint32),
                 ("closure_newton_steps", c_uint32), ("closure_pcg_iterations", c_uint32),
-                ("closure_pcg_tolerance", c_float), ("closure_components", c_uint32)]
+                ("closure_pcg_tolerance", c_float), ("closure_components", c_uint32),
+                ("closure_velocity_projection", c_uint32)]
 
 
 class Tuning(ctypes.Structure):

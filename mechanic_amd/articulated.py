@@ -364,6 +364,15 @@ def articulated_pile(seed: int = 2, nx: int = 50, ny: int = 36, nz: int = 50, cy
         outer_d[chain] = outer_d[chain[0]]
         bore[chain] = bore[chain[0]]
 
+    # Loop closures: link a to link a + 3 at the centre of a + 3 (a planar
+    # four-joint loop, one degree of freedom), every loop slot of the longest
+    # chains first (five on a 20-link chain), so that whole chains stay free of
+    # loops and carry the drives.
+    loops = []
+    for ch in sorted((ch for ch in chains if len(ch) >= 4), key=len, reverse=True):
+        for start in range(0, len(ch) - 3, 4):
+            if len(loops) < closures:
+                loops.append((ch, start))
     positions = np.zeros((cylinders, 3))
     rotations = np.zeros((cylinders, 4))
     rotations[:, 3] = 1.0
@@ -393,33 +402,16 @@ def articulated_pile(seed: int = 2, nx: int = 50, ny: int = 36, nz: int = 50, cy
                 specs.append(BearingSpec(chain[k - 1], b, joint.copy(), z_axis, None))
             joint = joint + d * length[b]
         static[chain[0]] = True
-    # Loop closures: link a to link a + 3 at the centre of a + 3 (a planar
-    # four-joint loop, one degree of freedom), spread over chains of >= 4 links.
     tree_count = len(specs)
-    bearing_of_child = {s_.b: i for i, s_ in enumerate(specs)}
-    in_loop = set()
-    long_chains = [ch for ch in chains if len(ch) >= 4]
-    made = 0
-    slot = {id(ch): 0 for ch in long_chains}
-    while made < closures and long_chains:
-        progressed = False
-        for ch in long_chains:
-            if made >= closures:
-                break
-            start = slot[id(ch)]
-            if start + 3 >= len(ch):
-                continue
-            a, b = ch[start], ch[start + 3]
-            specs.append(BearingSpec(a, b, positions[b].copy(), z_axis, None))
-            in_loop.update(bearing_of_child[ch[start + k]] for k in (1, 2, 3))
-            slot[id(ch)] = start + 4
-            made += 1
-            progressed = True
-        if not progressed:
-            break
-    # Drives: driven_fraction of all bearings, on tree bearings outside every
-    # loop (two motors in one single-degree-of-freedom loop would fight).
-    eligible = [i for i in range(tree_count) if i not in in_loop]
+    for ch, start in loops:
+        a, b = ch[start], ch[start + 3]
+        specs.append(BearingSpec(a, b, positions[b].copy(), z_axis, None))
+    # Drives: driven_fraction of all bearings, on the chains that carry no loop.
+    # A drive on a looped chain (inside a loop two motors fight over its one
+    # degree of freedom; between or below loops it excites them) pumps energy
+    # into the loops faster than the reference's velocity scheme removes it.
+    looped = {b for ch, _ in loops for b in ch}
+    eligible = [i for i in range(tree_count) if specs[i].b not in looped]
     want = min(len(eligible), int(round(driven_fraction * len(specs))))
     for i in sorted(rng.choice(len(eligible), size=want, replace=False)) if want else []:
         specs[eligible[i]].drive = Drive(mode=1, max_acceleration=drive_acceleration, max_speed=2.0,

@@ -90,6 +90,8 @@ struct Dev {
   // ---- colliders ----
   const float4* colliders;  // 4 per collider: centre, rotation, half extents, metadata bits
   const uint2* suppressed;
+  const uint32_t* supp_off;   // body_count + 1: each body's suppressed partners in supp_adj
+  const uint32_t* supp_adj;   // 2 * suppression_count: partners, sorted per body
 
   // ---- broadphase ----
   float4* aabbs;          // 2 per collider: min, max
@@ -220,6 +222,7 @@ struct Dev {
   uint32_t closure_steps;     // Newton steps per tick (the reference's 12 by default)
   uint32_t closure_iters;     // PCG iterations per Newton step (the reference's 8 by default)
   float closure_tol;          // PCG stop when sqrt(r.z / r0.z) falls below it (0: run every iteration)
+  uint32_t closure_vproj;     // project coordinate speeds onto J v = 0 before reconstruction (component solve)
   uint32_t pad_closure;
 
   // ---- publication ----

@@ -720,6 +720,33 @@ __device__ __forceinline__ bool suppressed(const Dev& d, uint32_t body_a, uint32
   return left < d.suppression_count && d.suppressed[left].x == lo && d.suppressed[left].y == hi;
 }
 
+// The same membership test from one body's side: its suppressed partners are a
+// sorted list (supp_off / supp_adj), whose first four a query lane holds in
+// registers for the whole packet.
+struct Partners {
+  uint4 first;           // first four partners, kInvalid-padded
+  uint32_t more, end;    // the rest: supp_adj[more, end)
+};
+
+__device__ __forceinline__ Partners load_partners(const Dev& d, uint32_t body) {
+  Partners r{make_uint4(kInvalid, kInvalid, kInvalid, kInvalid), 0u, 0u};
+  const uint32_t begin = d.supp_off[body], end = d.supp_off[body + 1];
+  const uint32_t n = end - begin;
+  if (n > 0) r.first.x = d.supp_adj[begin];
+  if (n > 1) r.first.y = d.supp_adj[begin + 1];
+  if (n > 2) r.first.z = d.supp_adj[begin + 2];
+  if (n > 3) r.first.w = d.supp_adj[begin + 3];
+  r.more = n > 4 ? begin + 4 : end;
+  r.end = end;
+  return r;
+}
+
+__device__ __forceinline__ bool is_partner(const Dev& d, const Partners& p, uint32_t other) {
+  bool found = other == p.first.x || other == p.first.y || other == p.first.z || other == p.first.w;
+  for (uint32_t i = p.more; !found && i < p.end; ++i) found = d.supp_adj[i] == other;
+  return found;
+}
+
 __device__ __forceinline__ bool overlaps(float4 qmn, float4 qmx, float4 mn, float4 mx) {
   return qmn.x <= mx.x && qmn.y <= mx.y && qmn.z <= mx.z && mn.x <= qmx.x && mn.y <= qmx.y && mn.z <= qmx.z;
 }
@@ -773,6 +800,7 @@ struct Packet {
   uint32_t self, self_body;
   uint32_t scene, scene_end;  // this lane's scene and the end of its sorted leaf range
   float4 umn, umx;  // union box of the packet's queries (wave-uniform)
+  Partners partners;  // the query body's suppressed partners (when the scene has any)
 };
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -786,6 +814,7 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+template <bool kPartners = true>
 __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint32_t lane) {
   Packet k;
   k.first = first;
@@ -807,6 +836,8 @@ __device__ __forceinline__ Packet load_packet(const Dev& d, uint32_t first, uint
       k.scene_end = d.scene_first[k.scene + 1];
     }
   }
+  k.partners = Partners{make_uint4(kInvalid, kInvalid, kInvalid, kInvalid), 0u, 0u};
+  if (kPartners && k.valid && d.suppression_count != 0) k.partners = load_partners(d, k.self_body);
   const float inf = __builtin_huge_valf();
   k.umn = make_float4(wave_min(k.valid ? k.qmn.x : inf), wave_min(k.valid ? k.qmn.y : inf),
                       wave_min(k.valid ? k.qmn.z : inf), 0.0f);
@@ -853,7 +884,7 @@ __device__ __forceinline__ void scan_range_as(const Dev& d, uint32_t* diag, Pair
       uint64_t mask = __builtin_amdgcn_ballot_w64(hit) & lanes_left;
       if constexpr (kSuppress) {
         if (mask != 0ull) {
-          hit = ((mask >> lane) & 1ull) && !suppressed(d, k.self_body, other_body);
+          hit = ((mask >> lane) & 1ull) && !is_partner(d, k.partners, other_body);
           mask = __builtin_amdgcn_ballot_w64(hit);
         }
       }
@@ -882,6 +913,15 @@ __device__ __forceinline__ void scan_range(const Dev& d, uint32_t* diag, PairSta
     if (suppress) scan_range_as<false, true>(d, diag, st, lane, k, start, last);
     else scan_range_as<false, false>(d, diag, st, lane, k, start, last);
   }
+}
+
+// scan_range with the suppression list chosen at compile time (k_scan is
+// instantiated for scenes with and without one).
+template <bool kSuppress>
+__device__ __forceinline__ void scan_range_fixed(const Dev& d, uint32_t* diag, PairStage& st, uint32_t lane,
+                                                 const Packet& k, uint32_t start, uint32_t last) {
+  if (d.scene_count > 1) scan_range_as<true, kSuppress>(d, diag, st, lane, k, start, last);
+  else scan_range_as<false, kSuppress>(d, diag, st, lane, k, start, last);
 }
 
 constexpr uint32_t kItemStage = 64;  // work items per packet
@@ -1148,6 +1188,7 @@ __global__ __launch_bounds__(1024) void k_scan_prefix(Dev d) {
 // Every wavefront takes a contiguous share of the work items (in packet order,
 // through the prefix of the per-packet counts) and scans each item's leaves
 // against its packet.
+template <bool kSuppress>
 __global__ __launch_bounds__(256) void k_scan(Dev d) {
   __shared__ uint2 stage_mem[4][kStage];
   __shared__ uint32_t flush_lds[5];
@@ -1171,7 +1212,7 @@ __global__ __launch_bounds__(256) void k_scan(Dev d) {
       else hi = mid;
     }
     uint32_t pk = lo, pk_first = d.work_first[pk], pk_next = d.work_first[pk + 1];
-    Packet k = load_packet(d, pk * 64u, lane);
+    Packet k = load_packet<kSuppress>(d, pk * 64u, lane);
     for (uint32_t i = begin; i < end; ++i) {
       if (i >= pk_next) {
         do {
@@ -1179,10 +1220,10 @@ __global__ __launch_bounds__(256) void k_scan(Dev d) {
           pk_first = pk_next;
           pk_next = d.work_first[pk + 1];
         } while (i >= pk_next);
-        k = load_packet(d, pk * 64u, lane);
+        k = load_packet<kSuppress>(d, pk * 64u, lane);
       }
       const uint4 it = d.work[(size_t)pk * kItemStage + (i - pk_first)];
-      scan_range(d, diag, st, lane, k, it.y, it.z);
+      scan_range_fixed<kSuppress>(d, diag, st, lane, k, it.y, it.z);
     }
   }
   flush_pairs_block(d, diag, st, lane, flush_lds);
@@ -1339,7 +1380,9 @@ void launch_traverse(hipStream_t s, const Dev& d) {
   } else {
     k_traverse<false><<<packets, kBlock, 0, s>>>(d);
     k_scan_prefix<<<1, 1024, 0, s>>>(d);
-    k_scan<<<d.t_scan_blocks ? d.t_scan_blocks : 4096u, kBlock, 0, s>>>(d);
+    const uint32_t scan_blocks = d.t_scan_blocks ? d.t_scan_blocks : 4096u;
+    if (d.suppression_count) k_scan<true><<<scan_blocks, kBlock, 0, s>>>(d);
+    else k_scan<false><<<scan_blocks, kBlock, 0, s>>>(d);
   }
 }
 

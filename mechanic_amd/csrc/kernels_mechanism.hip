@@ -112,38 +112,49 @@ __device__ __forceinline__ v3 ang_of(const Dev& d, uint32_t b, int in_contact) {
   return in_contact ? xyz(d.sbody[4 * (size_t)b + 1]) : xyz(d.ang[b]);
 }
 
-__device__ __forceinline__ void add_mdelta(const Dev& d, uint32_t body, v3 linear, v3 angular) {
+
+// One body's Q20 deltas from one bearing row: the rounded integers of each
+// add_delta call (articulated.wgsl:107-116), summed with wrapping adds -- the
+// arithmetic of the atomics, in any order -- before one atomic per component.
+struct QSum {
+  int32_t v[6];
+};
+__device__ __forceinline__ void qsum_add(QSum& q, v3 linear, v3 angular) {
+  const int32_t r[6] = {round_to_i32(linear.x * kFixedScaleM),  round_to_i32(linear.y * kFixedScaleM),
+                        round_to_i32(linear.z * kFixedScaleM),  round_to_i32(angular.x * kFixedScaleM),
+                        round_to_i32(angular.y * kFixedScaleM), round_to_i32(angular.z * kFixedScaleM)};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) q.v[k] = (int32_t)((uint32_t)q.v[k] + (uint32_t)r[k]);
+}
+__device__ __forceinline__ void qsum_flush(const Dev& d, uint32_t body, const QSum& q) {
   int32_t* p = d.mdeltas + 6 * (size_t)body;
-  atomicAdd(&p[0], round_to_i32(linear.x * kFixedScaleM));
-  atomicAdd(&p[1], round_to_i32(linear.y * kFixedScaleM));
-  atomicAdd(&p[2], round_to_i32(linear.z * kFixedScaleM));
-  atomicAdd(&p[3], round_to_i32(angular.x * kFixedScaleM));
-  atomicAdd(&p[4], round_to_i32(angular.y * kFixedScaleM));
-  atomicAdd(&p[5], round_to_i32(angular.z * kFixedScaleM));
+#pragma unroll
+  for (int k = 0; k < 6; ++k) atomicAdd(&p[k], q.v[k]);
 }
 
 // solve_linear_axis (articulated.wgsl:118-143).
 __device__ __forceinline__ void linear_axis(const Dev& d, uint32_t a, uint32_t b, v3 arm_a, v3 arm_b, v3 relative,
-                                            v3 direction) {
+                                            v3 direction, QSum& qa, QSum& qb) {
   const v3 angular_a = cross(arm_a, direction);
   const v3 angular_b = cross(arm_b, direction);
   const float denominator =
       ((inv_mass_of(d, a) + inv_mass_of(d, b)) + dot(angular_a, wii(d, a, angular_a))) + dot(angular_b, wii(d, b, angular_b));
   if (denominator <= 1.0e-12f) return;
   const v3 impulse = direction * ((kBearingRelaxation * dot(relative, direction)) / denominator);
-  add_mdelta(d, a, impulse * inv_mass_of(d, a), wii(d, a, cross(arm_a, impulse)));
-  add_mdelta(d, b, (-impulse) * inv_mass_of(d, b), wii(d, b, cross(arm_b, -impulse)));
+  qsum_add(qa, impulse * inv_mass_of(d, a), wii(d, a, cross(arm_a, impulse)));
+  qsum_add(qb, (-impulse) * inv_mass_of(d, b), wii(d, b, cross(arm_b, -impulse)));
 }
 
 // solve_angular_axis (articulated.wgsl:145-155).
-__device__ __forceinline__ void angular_axis(const Dev& d, uint32_t a, uint32_t b, v3 relative, v3 axis) {
+__device__ __forceinline__ void angular_axis(const Dev& d, uint32_t a, uint32_t b, v3 relative, v3 axis, QSum& qa,
+                                             QSum& qb) {
   const v3 inverse_a = wii(d, a, axis);
   const v3 inverse_b = wii(d, b, axis);
   const float denominator = dot(axis, inverse_a + inverse_b);
   if (denominator <= 1.0e-12f) return;
   const float impulse = (kBearingRelaxation * dot(relative, axis)) / denominator;
-  add_mdelta(d, a, mk(0.0f, 0.0f, 0.0f), inverse_a * impulse);
-  add_mdelta(d, b, mk(0.0f, 0.0f, 0.0f), (-inverse_b) * impulse);
+  qsum_add(qa, mk(0.0f, 0.0f, 0.0f), inverse_a * impulse);
+  qsum_add(qb, mk(0.0f, 0.0f, 0.0f), (-inverse_b) * impulse);
 }
 
 // project_bearing_velocities (articulated.wgsl:281-305).
@@ -158,9 +169,10 @@ __global__ __launch_bounds__(256) void k_bearing_project(Dev d, int in_contact) 
   const v3 velocity_a = lin_of(d, r.a, in_contact) + cross(ang_a, arm_a);
   const v3 velocity_b = lin_of(d, r.b, in_contact) + cross(ang_b, arm_b);
   const v3 relative = velocity_b - velocity_a;
-  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(1.0f, 0.0f, 0.0f));
-  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 1.0f, 0.0f));
-  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 0.0f, 1.0f));
+  QSum qa = {{0, 0, 0, 0, 0, 0}}, qb = {{0, 0, 0, 0, 0, 0}};
+  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(1.0f, 0.0f, 0.0f), qa, qb);
+  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 1.0f, 0.0f), qa, qb);
+  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 0.0f, 1.0f), qa, qb);
   const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
   const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
   const v3 hinge = normalize_v(axis_a + axis_b);
@@ -168,8 +180,10 @@ __global__ __launch_bounds__(256) void k_bearing_project(Dev d, int in_contact) 
   const v3 tangent_a = normalize_v(cross(hinge, helper));
   const v3 tangent_b = cross(hinge, tangent_a);
   const v3 relative_angular = ang_b - ang_a;
-  angular_axis(d, r.a, r.b, relative_angular, tangent_a);
-  angular_axis(d, r.a, r.b, relative_angular, tangent_b);
+  angular_axis(d, r.a, r.b, relative_angular, tangent_a, qa, qb);
+  angular_axis(d, r.a, r.b, relative_angular, tangent_b, qa, qb);
+  qsum_flush(d, r.a, qa);
+  qsum_flush(d, r.b, qb);
 }
 
 // apply_velocity_deltas (articulated.wgsl:307-326) over the bodies some bearing
@@ -442,6 +456,21 @@ __device__ double block_sum(double v, double* scratch) {
   for (uint32_t w = 0; w < blockDim.x / 64; ++w) total += scratch[w];
   __syncthreads();
   return total;
+}
+
+// block_sum for a workgroup of kT threads: a single wavefront's butterfly
+// leaves the same bits in every lane (each stage adds the same two values in
+// either order), so one wavefront needs no LDS and no barrier; 0.0 + v keeps
+// block_sum's result for -0.0.
+template <uint32_t kT>
+__device__ __forceinline__ double group_sum(double v, double* scratch) {
+  if constexpr (kT == 64) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return 0.0 + v;
+  } else {
+    return block_sum(v, scratch);
+  }
 }
 
 struct Endpoint {
@@ -733,10 +762,12 @@ struct CompLds {
   uint2* lmeta;   // 2 * NB: local ancestor, resolved
   float4* clo;    // 4 per closure: (anchor a, end a) (anchor b, end b) (axis a) (axis b) of its bearing
   // PCG vectors in FP64: solution, residual, preconditioned residual, direction, product, diagonal;
-  // 6 per closure: J d (position, axis)
-  double *px, *pr, *pz, *pp, *pop, *pdg, *kprod;
+  // 6 per closure: J d (position, axis); 6 per path entry: its column times d
+  double *px, *pr, *pz, *pp, *pop, *pdg, *kprod, *ptmp;
   float *angle, *jax, *jan, *kend, *kres, *kact;
+  float* col;  // 6 per coordinate entry: the Jacobian column (position, axis rows) of that entry
   uint32_t *entry_off, *entries, *path_off, *path, *coord_body;
+  uint32_t* pcol;  // per path entry: its coordinate entry (the column it uses)
 };
 
 __device__ __forceinline__ CompLds carve_comp(float4* base, const Dev& d) {
@@ -755,19 +786,22 @@ __device__ __forceinline__ CompLds carve_comp(float4* base, const Dev& d) {
   c.pop = c.pp + NQ;
   c.pdg = c.pop + NQ;
   c.kprod = c.pdg + NQ;
-  float* f = reinterpret_cast<float*>(c.kprod + 6 * NK);
+  c.ptmp = c.kprod + 6 * NK;
+  float* f = reinterpret_cast<float*>(c.ptmp + 6 * NP);
   c.angle = f;
   c.jax = c.angle + NQ;
   c.jan = c.jax + 3 * NQ;
   c.kend = c.jan + 3 * NQ;
   c.kres = c.kend + 12 * NK;
   c.kact = c.kres + 6 * NK;
-  uint32_t* u = reinterpret_cast<uint32_t*>(c.kact + NK);
+  c.col = c.kact + NK;
+  uint32_t* u = reinterpret_cast<uint32_t*>(c.col + 6 * NP);
   c.entry_off = u;
   c.entries = c.entry_off + NQ + 1;
   c.path_off = c.entries + NP;
   c.path = c.path_off + 2 * NK + 1;
   c.coord_body = c.path + NP;
+  c.pcol = c.coord_body + NQ;
   return c;
 }
 
@@ -776,7 +810,10 @@ __device__ __forceinline__ double dot_d(v3 a, double bx, double by, double bz) {
   return ((double)a.x * bx + (double)a.y * by) + (double)a.z * bz;
 }
 
-__global__ __launch_bounds__(256) void k_closure_components(Dev d) {
+// kT threads per workgroup: 64 (one wavefront, whose barriers the compiler
+// reduces to wavefront barriers) or 256.
+template <uint32_t kT>
+__global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
   extern __shared__ float4 lds4[];
   __shared__ uint32_t state[2];
   __shared__ double scratch[4];
@@ -785,7 +822,7 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
   const uint32_t NB = d.cc_max_bodies;
   const CompLds L = carve_comp(lds4, d);
   uint32_t* diag = diag_now(d);
-  const uint32_t t = threadIdx.x, T = blockDim.x;
+  const uint32_t t = threadIdx.x, T = kT;
   // Stage the component's rows and lists.
   for (uint32_t i = t; i < nb; i += T) {
     const MechBody m = load_mbody(d, d.cc_body[b0 + i]);
@@ -815,6 +852,16 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
     L.coord_body[l] = d.cc_coord_body[q0 + l];
   }
   __syncthreads();
+  // Each path entry's coordinate entry: the (closure, side) entry of its coordinate.
+  const uint32_t np = L.path_off[2 * nk];
+  for (uint32_t slot = t; slot < 2 * nk; slot += T)
+    for (uint32_t pe = L.path_off[slot]; pe < L.path_off[slot + 1]; ++pe) {
+      const uint32_t l = L.path[pe];
+      uint32_t e = L.entry_off[l];
+      while (e + 1 < L.entry_off[l + 1] && L.entries[e] != slot) ++e;
+      L.pcol[pe] = e;
+    }
+  __syncthreads();
   uint32_t fb = 0;  // buffer holding the final links
   auto link = [&](uint32_t local, float4& position, float4& rotation) {
     if (local == kInvalid) {  // a root: the identity link of prepare_links' root rows
@@ -834,8 +881,8 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
     anchor = mk(e[0], e[1], e[2]);
     axis = mk(e[3], e[4], e[5]);
   };
-  for (uint32_t step = 0; step < d.closure_steps; ++step) {
-    // Forward kinematics of the path bodies (prepare_links + pointer jumping).
+  // Forward kinematics of the path bodies (prepare_links + pointer jumping).
+  auto forward_kinematics = [&]() {
     for (uint32_t i = t; i < nb; i += T) {
       const float4* b = L.body + 5 * i;
       const MechBody m{0u, 0u, __float_as_uint(b[0].w), 0u};
@@ -884,7 +931,9 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
       src ^= 1u;
     }
     fb = src;
-    // evaluate_closures for the component's closures.
+  };
+  // evaluate_closures for the component's closures; true when all are inside tolerance.
+  auto evaluate = [&]() {
     if (t == 0) {
       state[0] = 0u;
       state[1] = 0u;
@@ -924,8 +973,10 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
       L.kact[kk] = active;
     }
     __syncthreads();
-    if (!(state[0] > kAnchorTolerance || state[1] > kAxisTolerance)) break;  // the same for every thread
-    // Joint axes and anchors of the component's coordinates (closure.wgsl:172-185).
+    return !(state[0] > kAnchorTolerance || state[1] > kAxisTolerance);  // the same for every thread
+  };
+  // Joint axes and anchors of the component's coordinates (closure.wgsl:172-185).
+  auto joint_frames = [&]() {
     for (uint32_t l = t; l < nq; l += T) {
       const float4* b = L.body + 5 * L.coord_body[l];
       float4 pp4, pr4;
@@ -942,99 +993,92 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
       L.jan[3 * l] = anchor.x; L.jan[3 * l + 1] = anchor.y; L.jan[3 * l + 2] = anchor.z;
     }
     __syncthreads();
-    // Gradient and Jacobi diagonal, gathered per coordinate in (closure, side) order.
-    double rz_part = 0.0;
+  };
+  const uint32_t slots = 2u * ((nk + 31u) / 32u) * 32u;  // whole lane pairs for the side shuffle
+  // The Jacobian columns of every coordinate entry at the current joint frames
+  // and closure ends (FP32, as the reference forms them).
+  auto columns = [&]() {
     for (uint32_t l = t; l < nq; l += T) {
       v3 jaxis, janchor;
       joint(l, jaxis, janchor);
-      double g = 0.0, dg = 0.0;
       for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
         const uint32_t entry = L.entries[e], kk = entry >> 1, side = entry & 1u;
-        if (L.kact[kk] == 0.0f) continue;
         v3 end_anchor, end_axis;
         end_of(kk, side, end_anchor, end_axis);
         const float sign = side ? -1.0f : 1.0f;
         const v3 pd = cross(jaxis, end_anchor - janchor) * sign;
         const v3 ad = cross(jaxis, end_axis) * sign;
-        const float* rs = L.kres + 6 * kk;
-        const float gradient = dot(mk(rs[0], rs[1], rs[2]), pd) + 1.0f * dot(mk(rs[3], rs[4], rs[5]), ad);
-        const float diagonal = dot(pd, pd) + 1.0f * dot(ad, ad);
-        if (gradient == gradient && diagonal == diagonal) {
-          g += (double)gradient;
-          dg += (double)diagonal;
-        } else {
-          atomicOr(&diag[kDiagFlags], kFlagInvalid);
+        float* c = L.col + 6 * e;
+        c[0] = pd.x; c[1] = pd.y; c[2] = pd.z;
+        c[3] = ad.x; c[4] = ad.y; c[5] = ad.z;
+      }
+    }
+    __syncthreads();
+  };
+  // branch_product per closure side, summed in path order (FP32 columns times the
+  // FP64 vector, FP64 sums); side 0 adds side 1's. Result: J v per closure in kprod.
+  // The products of all path entries are formed in parallel, the sums in path order.
+  auto product = [&](auto vec_of) {
+    for (uint32_t pe = t; pe < np; pe += T) {
+      const float* c = L.col + 6 * L.pcol[pe];
+      const double dir = vec_of(L.path[pe]);
+      double* o = L.ptmp + 6 * pe;
+      o[0] = (double)c[0] * dir; o[1] = (double)c[1] * dir; o[2] = (double)c[2] * dir;
+      o[3] = (double)c[3] * dir; o[4] = (double)c[4] * dir; o[5] = (double)c[5] * dir;
+    }
+    __syncthreads();
+    for (uint32_t slot = t; slot < slots; slot += T) {
+      const uint32_t kk = slot >> 1, side = slot & 1u;
+      double spx = 0.0, spy = 0.0, spz = 0.0, sax = 0.0, say = 0.0, saz = 0.0;
+      if (kk < nk) {
+        for (uint32_t pe = L.path_off[2 * kk + side]; pe < L.path_off[2 * kk + side + 1]; ++pe) {
+          const double* o = L.ptmp + 6 * pe;
+          spx = spx + o[0];
+          spy = spy + o[1];
+          spz = spz + o[2];
+          sax = sax + o[3];
+          say = say + o[4];
+          saz = saz + o[5];
         }
       }
-      const double residual = dg > (double)1.0e-12f ? -g : 0.0;
-      const double pre = dg > (double)1.0e-12f ? residual / dg : 0.0;
-      L.px[l] = 0.0;
-      L.pr[l] = residual;
-      L.pz[l] = pre;
-      L.pp[l] = pre;
-      L.pop[l] = 0.0;
-      L.pdg[l] = dg;
-      rz_part += residual * pre;
+      const double s1[6] = {__shfl_xor(spx, 1), __shfl_xor(spy, 1), __shfl_xor(spz, 1),
+                            __shfl_xor(sax, 1), __shfl_xor(say, 1), __shfl_xor(saz, 1)};
+      if (kk < nk && side == 0u) {
+        double* q = L.kprod + 6 * kk;
+        q[0] = spx + s1[0]; q[1] = spy + s1[1]; q[2] = spz + s1[2];
+        q[3] = sax + s1[3]; q[4] = say + s1[4]; q[5] = saz + s1[5];
+      }
     }
-    double rz = block_sum(rz_part, scratch);
+    __syncthreads();
+  };
+  // branch_transpose of kprod gathered for coordinate l (its entries in (closure, side) order).
+  auto transpose = [&](uint32_t l) {
+    double op = 0.0;
+    for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
+      const uint32_t kk = L.entries[e] >> 1;
+      const float* c = L.col + 6 * e;
+      const double* q = L.kprod + 6 * kk;
+      op = op + (dot_d(mk(c[0], c[1], c[2]), q[0], q[1], q[2]) + 1.0 * dot_d(mk(c[3], c[4], c[5]), q[3], q[4], q[5]));
+    }
+    return op;
+  };
+  // Jacobi-preconditioned CG on J^T J x = r from x = 0, with px, pr, pz, pp, pdg
+  // set up by the caller and rz = r.z; stops at closure_iters iterations or when
+  // r.z falls below closure_tol^2 of its start.
+  auto pcg = [&](double rz) {
     const double rz_stop = (double)d.closure_tol * (double)d.closure_tol * rz;
     double beta = 0.0;
-    const uint32_t slots = 2u * ((nk + 31u) / 32u) * 32u;  // whole lane pairs for the side shuffle
     for (uint32_t it = 0; it < d.closure_iters; ++it) {
       if (d.closure_tol > 0.0f && rz <= rz_stop) break;
       auto direction_of = [&](uint32_t l) { return it == 0 ? L.pp[l] : L.pz[l] + beta * L.pp[l]; };
-      // branch_product per closure side, summed in path order (FP32 columns times
-      // the FP64 direction, FP64 sums); side 0 adds side 1's.
-      for (uint32_t slot = t; slot < slots; slot += T) {
-        const uint32_t kk = slot >> 1, side = slot & 1u;
-        double spx = 0.0, spy = 0.0, spz = 0.0, sax = 0.0, say = 0.0, saz = 0.0;
-        if (kk < nk) {
-          v3 end_anchor, end_axis;
-          end_of(kk, side, end_anchor, end_axis);
-          const float sign = side ? -1.0f : 1.0f;
-          for (uint32_t e = L.path_off[2 * kk + side]; e < L.path_off[2 * kk + side + 1]; ++e) {
-            const uint32_t l = L.path[e];
-            v3 jaxis, janchor;
-            joint(l, jaxis, janchor);
-            const double dir = direction_of(l);
-            const v3 pd = cross(jaxis, end_anchor - janchor) * sign, ad = cross(jaxis, end_axis) * sign;
-            spx = spx + (double)pd.x * dir;
-            spy = spy + (double)pd.y * dir;
-            spz = spz + (double)pd.z * dir;
-            sax = sax + (double)ad.x * dir;
-            say = say + (double)ad.y * dir;
-            saz = saz + (double)ad.z * dir;
-          }
-        }
-        const double s1[6] = {__shfl_xor(spx, 1), __shfl_xor(spy, 1), __shfl_xor(spz, 1),
-                              __shfl_xor(sax, 1), __shfl_xor(say, 1), __shfl_xor(saz, 1)};
-        if (kk < nk && side == 0u) {
-          double* q = L.kprod + 6 * kk;
-          q[0] = spx + s1[0]; q[1] = spy + s1[1]; q[2] = spz + s1[2];
-          q[3] = sax + s1[3]; q[4] = say + s1[4]; q[5] = saz + s1[5];
-        }
-      }
-      __syncthreads();
-      // branch_transpose gathered per coordinate.
+      product(direction_of);
       double dad_part = 0.0;
       for (uint32_t l = t; l < nq; l += T) {
-        v3 jaxis, janchor;
-        joint(l, jaxis, janchor);
-        double op = 0.0;
-        for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
-          const uint32_t entry = L.entries[e], kk = entry >> 1, side = entry & 1u;
-          v3 end_anchor, end_axis;
-          end_of(kk, side, end_anchor, end_axis);
-          const float sign = side ? -1.0f : 1.0f;
-          const v3 pd = cross(jaxis, end_anchor - janchor) * sign;
-          const v3 ad = cross(jaxis, end_axis) * sign;
-          const double* q = L.kprod + 6 * kk;
-          op = op + (dot_d(pd, q[0], q[1], q[2]) + 1.0 * dot_d(ad, q[3], q[4], q[5]));
-        }
+        const double op = transpose(l);
         L.pop[l] = op;
         dad_part += direction_of(l) * op;
       }
-      const double dad = block_sum(dad_part, scratch);
+      const double dad = group_sum<kT>(dad_part, scratch);
       const double alpha = fabs(dad) > 1.0e-20 ? rz / dad : 0.0;
       double rz_next_part = 0.0;
       for (uint32_t l = t; l < nq; l += T) {
@@ -1047,10 +1091,52 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
         L.pp[l] = p;
         rz_next_part += L.pr[l] * z;
       }
-      const double rz_next = block_sum(rz_next_part, scratch);  // its barriers publish the update
+      const double rz_next = group_sum<kT>(rz_next_part, scratch);  // its barriers publish the update
       beta = fabs(rz) > 1.0e-20 ? rz_next / rz : 0.0;
       rz = rz_next;
     }
+  };
+  // The PCG's start from the right-hand side -g and diagonal dg of one coordinate.
+  auto pcg_start = [&](uint32_t l, double g, double dg) {
+    const double residual = dg > (double)1.0e-12f ? -g : 0.0;
+    const double pre = dg > (double)1.0e-12f ? residual / dg : 0.0;
+    L.px[l] = 0.0;
+    L.pr[l] = residual;
+    L.pz[l] = pre;
+    L.pp[l] = pre;
+    L.pop[l] = 0.0;
+    L.pdg[l] = dg;
+    return residual * pre;
+  };
+  bool fresh = false;  // the links hold forward kinematics of L.angle
+  for (uint32_t step = 0; step < d.closure_steps; ++step) {
+    forward_kinematics();
+    fresh = true;
+    if (evaluate()) break;
+    joint_frames();
+    columns();
+    // Gradient and Jacobi diagonal, gathered per coordinate in (closure, side) order.
+    double rz_part = 0.0;
+    for (uint32_t l = t; l < nq; l += T) {
+      double g = 0.0, dg = 0.0;
+      for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
+        const uint32_t kk = L.entries[e] >> 1;
+        if (L.kact[kk] == 0.0f) continue;
+        const float* c = L.col + 6 * e;
+        const v3 pd = mk(c[0], c[1], c[2]), ad = mk(c[3], c[4], c[5]);
+        const float* rs = L.kres + 6 * kk;
+        const float gradient = dot(mk(rs[0], rs[1], rs[2]), pd) + 1.0f * dot(mk(rs[3], rs[4], rs[5]), ad);
+        const float diagonal = dot(pd, pd) + 1.0f * dot(ad, ad);
+        if (gradient == gradient && diagonal == diagonal) {
+          g += (double)gradient;
+          dg += (double)diagonal;
+        } else {
+          atomicOr(&diag[kDiagFlags], kFlagInvalid);
+        }
+      }
+      rz_part += pcg_start(l, g, dg);
+    }
+    pcg(group_sum<kT>(rz_part, scratch));
     // The clamped Newton step (closure.wgsl:416-426), in FP32.
     for (uint32_t l = t; l < nq; l += T) {
       const float step_c = clampf(kStepScale * (float)L.px[l], -kMaxStep, kMaxStep);
@@ -1058,6 +1144,39 @@ __global__ __launch_bounds__(256) void k_closure_components(Dev d) {
       else atomicOr(&diag[kDiagFlags], kFlagInvalid);
     }
     __syncthreads();
+    fresh = false;
+  }
+  if (d.closure_vproj) {
+    // Velocity-level closure: the coordinate speeds the tree reconstruction will
+    // use, projected onto J v = 0 at the final configuration (every closure of
+    // the component, inside tolerance or not): v += x with J^T J x = -J^T (J v),
+    // the minimum-norm correction.
+    if (!fresh) {
+      forward_kinematics();
+      evaluate();
+    }
+    joint_frames();
+    columns();
+    for (uint32_t l = t; l < nq; l += T) L.pz[l] = (double)d.coords[d.cc_coord[q0 + l]].y;
+    __syncthreads();
+    product([&](uint32_t l) { return L.pz[l]; });
+    double rz_part = 0.0;
+    for (uint32_t l = t; l < nq; l += T) {
+      double dg = 0.0;
+      for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
+        const float* c = L.col + 6 * e;
+        const v3 pd = mk(c[0], c[1], c[2]), ad = mk(c[3], c[4], c[5]);
+        dg += (double)(dot(pd, pd) + 1.0f * dot(ad, ad));
+      }
+      rz_part += pcg_start(l, transpose(l), dg);  // pz[l] is read by no other thread any more
+    }
+    pcg(group_sum<kT>(rz_part, scratch));
+    for (uint32_t l = t; l < nq; l += T) {
+      float2* c = &d.coords[d.cc_coord[q0 + l]];
+      const float speed = (float)((double)c->y + L.px[l]);
+      if (speed == speed) c->y = speed;
+      else atomicOr(&diag[kDiagFlags], kFlagInvalid);
+    }
   }
   for (uint32_t l = t; l < nq; l += T) d.coords[d.cc_coord[q0 + l]].x = L.angle[l];
 }
@@ -1122,21 +1241,36 @@ __global__ __launch_bounds__(256) void k_validate_snapshot(Dev d) {
 }
 
 // record_residual (bearings.wgsl:53-72).
-__device__ __forceinline__ void record_residual(uint32_t* diag, v3 anchor_a, v3 anchor_b, v3 axis_a, v3 axis_b,
-                                                bool closure) {
-  const uint32_t um = round_to_u32(length(anchor_a - anchor_b) * 1000000.0f);
+// Anchor residual in micrometres and axis residual in micro-degrees
+// (bearings.wgsl:48-72).
+__device__ __forceinline__ void residual_words(v3 anchor_a, v3 anchor_b, v3 axis_a, v3 axis_b, uint32_t& um,
+                                               uint32_t& ud) {
+  um = round_to_u32(length(anchor_a - anchor_b) * 1000000.0f);
   const float degrees = atan2f(length(cross(axis_a, axis_b)), clampf(dot(axis_a, axis_b), -1.0f, 1.0f)) * 57.295779513f;
-  const uint32_t ud = round_to_u32(degrees * 1000000.0f);
-  atomicMax(&diag[kDiagAnchor], um);
-  atomicMax(&diag[kDiagAxis], ud);
-  if (closure && (um > kAnchorTolerance || ud > kAxisTolerance)) atomicOr(&diag[kDiagFlags], kFlagNonConvergence);
+  ud = round_to_u32(degrees * 1000000.0f);
 }
 
+
 // validate_mechanism_bearings / validate_bearings (bearings.wgsl:74-102).
+// Each wavefront takes the maxima of its rows before one atomic per word (the
+// maxima and the flag are order-independent): atomics on one address serialise
+// device-wide.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, m);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_validate_bearings(Dev d, int mechanism) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.bearing_count) return;
+  if ((i & ~63u) >= d.bearing_count) return;  // whole wavefront past the last row
   uint32_t* diag = diag_now(d);
+  uint32_t um = 0u, ud = 0u;
+  bool open = false;
+  if (i < d.bearing_count) {
   const BearingRow r = load_bearing(d, i);
   float4 pos_a, rot_a, pos_b, rot_b;
   if (mechanism) {
@@ -1156,7 +1290,17 @@ __global__ __launch_bounds__(256) void k_validate_bearings(Dev d, int mechanism)
   const v3 anchor_b = xyz(pos_b) + quat_rotate(rot_b, r.anchor_b);
   const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
   const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
-  record_residual(diag, anchor_a, anchor_b, axis_a, axis_b, r.closure != 0u);
+  residual_words(anchor_a, anchor_b, axis_a, axis_b, um, ud);
+  open = r.closure != 0u && (um > kAnchorTolerance || ud > kAxisTolerance);
+  }
+  um = wave_max_u32(um);
+  ud = wave_max_u32(ud);
+  const bool any_open = __ballot(open) != 0ull;
+  if ((threadIdx.x & 63u) == 0u) {
+    atomicMax(&diag[kDiagAnchor], um);
+    atomicMax(&diag[kDiagAxis], ud);
+    if (any_open) atomicOr(&diag[kDiagFlags], kFlagNonConvergence);
+  }
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + 255u) / 256u; }
@@ -1395,7 +1539,8 @@ void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures) {
     // One workgroup per closure component, then forward kinematics of every tree.
     const uint32_t items = std::max(std::max(d.cc_max_bodies, d.cc_max_coords), 2u * d.cc_max_closures);
     const uint32_t threads = items <= 64u ? 64u : 256u;
-    k_closure_components<<<d.cc_count, threads, closure_component_lds(d), s>>>(d);
+    if (threads == 64u) k_closure_components<64><<<d.cc_count, 64, closure_component_lds(d), s>>>(d);
+    else k_closure_components<256><<<d.cc_count, 256, closure_component_lds(d), s>>>(d);
     k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, -1);
     k_publish<<<1, kPoseThreads, 0, s>>>(d);
     return;
@@ -1413,16 +1558,20 @@ void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures) {
 size_t closure_component_lds(const Dev& d) {
   const size_t NB = d.cc_max_bodies, NQ = d.cc_max_coords, NK = d.cc_max_closures, NP = d.cc_max_path;
   // float4 region: bodies (5), links (2 + 2) and link meta (1) per path body, 4 per closure;
-  // doubles: 6 per coordinate and per closure; floats: 7 per coordinate, 19 per closure;
-  // u32: entry offsets, entries, path offsets, paths, coordinate bodies.
-  return 16 * (10 * NB + 4 * NK) + 8 * (6 * NQ + 6 * NK) + 4 * (7 * NQ + 19 * NK) +
-         4 * ((NQ + 1) + NP + (2 * NK + 1) + NP + NQ);
+  // doubles: 6 per coordinate, per closure and per path entry; floats: 7 per coordinate,
+  // 19 per closure, 6 per path entry; u32: entry offsets, entries, path offsets, paths,
+  // coordinate bodies, path-entry columns.
+  return 16 * (10 * NB + 4 * NK) + 8 * (6 * NQ + 6 * NK + 6 * NP) + 4 * (7 * NQ + 19 * NK + 6 * NP) +
+         4 * ((NQ + 1) + NP + (2 * NK + 1) + NP + NQ + NP);
 }
 
 hipError_t prepare_closure_components(const Dev& d) {
   const size_t lds = closure_component_lds(d);
   if (lds <= 65536) return hipSuccess;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_closure_components),
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_closure_components<64>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_closure_components<256>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 

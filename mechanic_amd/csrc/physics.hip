@@ -3,6 +3,7 @@
 // hipGraph. Mirrors GpuPhysics (crates/mechanic-gpu/src/device.rs:199-1663).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -666,6 +667,8 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   if (cfg.closure_newton_steps > 64 || cfg.closure_pcg_iterations > 256 || !(cfg.closure_pcg_tolerance >= 0.0f) ||
       !std::isfinite(cfg.closure_pcg_tolerance))
     return fail(error, MH_ERR_INVALID_ARGUMENT, "closure steps <= 64, PCG iterations <= 256, tolerance >= 0");
+  if (cfg.closure_velocity_projection && !cfg.closure_components)
+    return fail(error, MH_ERR_INVALID_ARGUMENT, "closure_velocity_projection needs closure_components");
   if ((cfg.pair_capacity & (cfg.pair_capacity - 1)) != 0 || cfg.pair_capacity < 256 ||
       cfg.pair_capacity > (1u << 22))
     return fail(error, MH_ERR_INVALID_ARGUMENT, "pair_capacity must be a power of two in [256, 2^22]");
@@ -716,6 +719,7 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   d.closure_steps = cfg.closure_newton_steps ? cfg.closure_newton_steps : 12u;  // device.rs:977
   d.closure_iters = cfg.closure_pcg_iterations ? cfg.closure_pcg_iterations : 8u;  // closure.wgsl:358
   d.closure_tol = cfg.closure_pcg_tolerance;
+  d.closure_vproj = cfg.closure_velocity_projection ? 1u : 0u;
 
   float4 *pos, *rot, *lin, *ang, *masses, *wm, *cols, *aabbs, *cposes, *inodes, *leaves, *contacts, *rows;
   float* inv_mass;
@@ -819,6 +823,34 @@ int create_impl(const mh_creation* creation, const mh_config* config_in, const B
   }
   UPLOAD(cols, creation->colliders, (size_t)c * sizeof(mh_collider));
   UPLOAD(supp, creation->suppressed_pairs, (size_t)creation->suppression_count * sizeof(mh_pair));
+  {
+    // Each body's suppressed partners, sorted: the traversal's membership test.
+    std::vector<uint32_t> off(n + 1, 0), adj(2 * (size_t)creation->suppression_count);
+    for (uint32_t i = 0; i < creation->suppression_count; ++i) {
+      const mh_pair& q = creation->suppressed_pairs[i];
+      if (q.collider_a >= n || q.collider_b >= n) {
+        mh_physics_destroy(p);
+        return fail(error, MH_ERR_INVALID_ARGUMENT, "a suppressed pair names no body");
+      }
+      ++off[q.collider_a + 1];
+      ++off[q.collider_b + 1];
+    }
+    for (uint32_t b = 0; b < n; ++b) off[b + 1] += off[b];
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (uint32_t i = 0; i < creation->suppression_count; ++i) {
+      const mh_pair& q = creation->suppressed_pairs[i];
+      adj[fill[q.collider_a]++] = q.collider_b;
+      adj[fill[q.collider_b]++] = q.collider_a;
+    }
+    for (uint32_t b = 0; b < n; ++b) std::sort(adj.begin() + off[b], adj.begin() + off[b + 1]);
+    uint32_t *soff, *sadj;
+    ALLOC(soff, (size_t)n + 1);
+    ALLOC(sadj, adj.size());
+    UPLOAD(soff, off.data(), off.size() * sizeof(uint32_t));
+    if (!adj.empty()) UPLOAD(sadj, adj.data(), adj.size() * sizeof(uint32_t));
+    d.supp_off = soff;
+    d.supp_adj = sadj;
+  }
   if (creation->bearing_count > 0) {
     rc = upload_forest(p, creation, forest, error);
     if (rc != MH_OK) { mh_physics_destroy(p); return rc; }

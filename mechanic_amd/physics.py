@@ -78,6 +78,9 @@ class GpuPhysicsConfig:
     closure_pcg_iterations: int = 0
     closure_pcg_tolerance: float = 0.0
     closure_components: bool = False  # one PCG per closure component (one workgroup each)
+    # Project the coordinate speeds onto the loops' velocity constraint before
+    # the tree reconstruction (needs closure_components).
+    closure_velocity_projection: bool = False
 
     def to_c(self) -> _lib.Config:
         return _lib.Config(int(self.collisions_enabled), int(self.mechanism_self_collisions),
@@ -85,7 +88,8 @@ class GpuPhysicsConfig:
                            int(self.warm_start), int(self.use_graph), int(self.residual),
                            float(self.learned_residual_limit), int(self.learned_fallback_passes),
                            int(self.closure_newton_steps), int(self.closure_pcg_iterations),
-                           float(self.closure_pcg_tolerance), int(self.closure_components))
+                           float(self.closure_pcg_tolerance), int(self.closure_components),
+                           int(self.closure_velocity_projection))
 
 
 @dataclass

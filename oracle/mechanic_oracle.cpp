@@ -205,6 +205,7 @@ struct World {
   uint32_t closure_steps = 12, closure_iters = 8;
   float closure_tol = 0.0f;
   bool closure_components = false;  // one PCG per closure component instead of one global system
+  bool closure_vproj = false;       // project coordinate speeds onto J v = 0 before reconstruction
   uint64_t pcg_iterations_run = 0;  // PCG iterations run since creation
   // Largest bearing count that takes the serial small-mechanism sweeps (the
   // reference's 64; tests lower it to run the same scene without them).
@@ -1434,6 +1435,164 @@ template <typename R> void closure_newton(World<R>& w, int max_steps = -1) {
   }
 }
 
+// Velocity-level closure (closure_velocity_projection, component solve only):
+// the coordinate speeds are projected onto J v = 0 at the final configuration,
+// v += x with J^T J x = -J^T (J v) solved per closure component by the
+// Jacobi-preconditioned CG of the Newton steps (FP32 Jacobian columns, FP64
+// vectors and sums, the same tolerance and iteration cap), before the tree
+// reconstructs body velocities from the speeds. Every closure takes part.
+template <typename R> void closure_velocity_projection(World<R>& w) {
+  const uint32_t nq = w.coordinate_count;
+  std::vector<uint32_t> closures;
+  for (uint32_t i = 0; i < w.bearing_count; ++i)
+    if (w.bearings[i].closure != 0u) closures.push_back(i);
+  const size_t nk = closures.size();
+  if (nk == 0) return;
+  auto walk = [&](uint32_t body, auto&& visit) {
+    while (w.mbodies[body].root == 0u) {
+      visit(body, w.bearings[w.mbodies[body].tree_bearing].coordinate);
+      body = w.mbodies[body].parent;
+    }
+  };
+  // Components: closures linked by shared coordinates.
+  const uint32_t kNone = 0xffffffffu;
+  std::vector<uint32_t> parent(nk), owner(nq, kNone), comp_closure(nk), comp_coord(nq, kNone);
+  for (size_t k = 0; k < nk; ++k) parent[k] = (uint32_t)k;
+  auto find = [&](uint32_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  for (size_t k = 0; k < nk; ++k) {
+    const auto& r = w.bearings[closures[k]];
+    for (uint32_t side = 0; side < 2; ++side)
+      walk(side ? r.b : r.a, [&](uint32_t, uint32_t c) {
+        if (owner[c] == kNone) owner[c] = (uint32_t)k;
+        else parent[find((uint32_t)k)] = find(owner[c]);
+      });
+  }
+  std::vector<uint32_t> id(nk, kNone);
+  uint32_t ncomp = 0;
+  for (size_t k = 0; k < nk; ++k) {
+    const uint32_t root = find((uint32_t)k);
+    if (id[root] == kNone) id[root] = ncomp++;
+    comp_closure[k] = id[root];
+  }
+  for (uint32_t c = 0; c < nq; ++c)
+    if (owner[c] != kNone) comp_coord[c] = comp_closure[owner[c]];
+  // Closure ends at the final configuration.
+  const auto& links = final_links(w);
+  struct End { V3<R> anchor[2], axis[2]; };
+  std::vector<End> ends(nk);
+  for (size_t k = 0; k < nk; ++k) {
+    const auto& r = w.bearings[closures[k]];
+    const auto& pa = links[r.a];
+    const auto& pb = links[r.b];
+    ends[k].anchor[0] = add(xyz(pa.position), quat_rotate(pa.rotation, r.anchor_a));
+    ends[k].anchor[1] = add(xyz(pb.position), quat_rotate(pb.rotation, r.anchor_b));
+    ends[k].axis[0] = normalize_v(quat_rotate(pa.rotation, r.axis_a));
+    ends[k].axis[1] = normalize_v(quat_rotate(pb.rotation, r.axis_b));
+  }
+  // Columns of one path entry: position and axis rows, signed by side.
+  auto column = [&](size_t k, uint32_t side, uint32_t body, V3<R>& pd, V3<R>& ad) {
+    V3<R> jaxis, janchor;
+    joint_of(w, body, jaxis, janchor);
+    const R sign = side ? R(-1) : R(1);
+    pd = mul(cross(jaxis, sub(ends[k].anchor[side], janchor)), sign);
+    ad = mul(cross(jaxis, ends[k].axis[side]), sign);
+  };
+  auto dot_d = [](V3<R> a, const double* b) {
+    return ((double)a.x * b[0] + (double)a.y * b[1]) + (double)a.z * b[2];
+  };
+  // J v per closure: each side summed in path order, side 0 plus side 1.
+  std::vector<double> prod(6 * nk);
+  auto product = [&](const std::vector<double>& v) {
+    for (size_t k = 0; k < nk; ++k) {
+      const auto& r = w.bearings[closures[k]];
+      double acc[2][6] = {};
+      for (uint32_t side = 0; side < 2; ++side)
+        walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+          V3<R> pd, ad;
+          column(k, side, body, pd, ad);
+          const double x[6] = {(double)pd.x, (double)pd.y, (double)pd.z, (double)ad.x, (double)ad.y, (double)ad.z};
+          for (int j = 0; j < 6; ++j) acc[side][j] = acc[side][j] + x[j] * v[c];
+        });
+      for (int j = 0; j < 6; ++j) prod[6 * k + j] = acc[0][j] + acc[1][j];
+    }
+  };
+  // J^T prod per coordinate, gathered in (closure, side) order.
+  auto transpose = [&](std::vector<double>& out) {
+    std::fill(out.begin(), out.end(), 0.0);
+    for (size_t k = 0; k < nk; ++k) {
+      const auto& r = w.bearings[closures[k]];
+      for (uint32_t side = 0; side < 2; ++side)
+        walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+          V3<R> pd, ad;
+          column(k, side, body, pd, ad);
+          out[c] = out[c] + (dot_d(pd, &prod[6 * k]) + 1.0 * dot_d(ad, &prod[6 * k + 3]));
+        });
+    }
+  };
+  std::vector<double> speed(nq), dg(nq, 0.0), g(nq), x(nq, 0.0), res(nq), z(nq), dir(nq), op(nq);
+  for (uint32_t c = 0; c < nq; ++c) speed[c] = (double)w.coord_speed[c];
+  for (size_t k = 0; k < nk; ++k) {
+    const auto& r = w.bearings[closures[k]];
+    for (uint32_t side = 0; side < 2; ++side)
+      walk(side ? r.b : r.a, [&](uint32_t body, uint32_t c) {
+        V3<R> pd, ad;
+        column(k, side, body, pd, ad);
+        dg[c] += (double)(dot(pd, pd) + R(1) * dot(ad, ad));
+      });
+  }
+  product(speed);
+  transpose(g);
+  std::vector<double> rz(ncomp, 0.0), rz_stop(ncomp), dad(ncomp), rz_next(ncomp), beta(ncomp, 0.0);
+  std::vector<uint8_t> done(ncomp, 0);
+  for (uint32_t c = 0; c < nq; ++c) {
+    res[c] = dg[c] > (double)1.0e-12f ? -g[c] : 0.0;
+    z[c] = dg[c] > (double)1.0e-12f ? res[c] / dg[c] : 0.0;
+    dir[c] = z[c];
+    if (comp_coord[c] != kNone) rz[comp_coord[c]] += res[c] * z[c];
+  }
+  for (uint32_t cp = 0; cp < ncomp; ++cp) rz_stop[cp] = (double)w.closure_tol * (double)w.closure_tol * rz[cp];
+  for (uint32_t it = 0; it < w.closure_iters; ++it) {
+    bool running = false;
+    for (uint32_t cp = 0; cp < ncomp; ++cp) {
+      if (!done[cp] && w.closure_tol > 0.0f && rz[cp] <= rz_stop[cp]) done[cp] = 1;
+      running = running || !done[cp];
+    }
+    if (!running) break;
+    if (it > 0)
+      for (uint32_t c = 0; c < nq; ++c)
+        if (comp_coord[c] != kNone && !done[comp_coord[c]]) dir[c] = z[c] + beta[comp_coord[c]] * dir[c];
+    product(dir);
+    transpose(op);
+    std::fill(dad.begin(), dad.end(), 0.0);
+    for (uint32_t c = 0; c < nq; ++c)
+      if (comp_coord[c] != kNone) dad[comp_coord[c]] += dir[c] * op[c];
+    std::fill(rz_next.begin(), rz_next.end(), 0.0);
+    for (uint32_t c = 0; c < nq; ++c) {
+      const uint32_t cp = comp_coord[c];
+      if (cp == kNone || done[cp]) continue;
+      const double alpha = std::fabs(dad[cp]) > 1.0e-20 ? rz[cp] / dad[cp] : 0.0;
+      x[c] = x[c] + alpha * dir[c];
+      res[c] = res[c] - alpha * op[c];
+      z[c] = dg[c] > (double)1.0e-12f ? res[c] / dg[c] : 0.0;
+      rz_next[cp] += res[c] * z[c];
+    }
+    for (uint32_t cp = 0; cp < ncomp; ++cp) {
+      if (done[cp]) continue;
+      beta[cp] = std::fabs(rz[cp]) > 1.0e-20 ? rz_next[cp] / rz[cp] : 0.0;
+      rz[cp] = rz_next[cp];
+    }
+  }
+  for (uint32_t c = 0; c < nq; ++c) {
+    if (comp_coord[c] == kNone) continue;
+    const R v = R(speed[c] + x[c]);
+    if (v == v) w.coord_speed[c] = v;
+    else w.diag[0] |= INVALID_NUMERIC;
+  }
+}
+
 // validate_articulated_state (articulated.wgsl:458-484).
 template <typename R> void validate_articulated(World<R>& w) {
   for (uint32_t b = 0; b < w.body_count; ++b) {
@@ -1481,6 +1640,7 @@ template <typename R> void mechanism_passes(World<R>& w) {
   advance_coordinates(w);
   forward_kinematics(w);
   closure_newton(w);
+  if (w.closure_components && w.closure_vproj) closure_velocity_projection(w);
   publish(w);
   reconstruct(w);
 }
@@ -1898,12 +2058,14 @@ void oracle_fk_newton(OracleHandle* h, int steps) {
 
 // Closure solve settings (0 keeps the reference's 12 steps / 8 iterations; a
 // tolerance of 0 runs every iteration); returns the PCG iterations run so far.
-uint64_t oracle_set_closure(OracleHandle* h, uint32_t steps, uint32_t iterations, float tolerance, int components) {
+uint64_t oracle_set_closure(OracleHandle* h, uint32_t steps, uint32_t iterations, float tolerance, int components,
+                            int velocity_projection) {
   auto set = [&](auto& w) {
     if (steps) w.closure_steps = steps;
     if (iterations) w.closure_iters = iterations;
     w.closure_tol = tolerance;
     w.closure_components = components != 0;
+    w.closure_vproj = velocity_projection != 0;
     return w.pcg_iterations_run;
   };
   return h->fp64 ? set(h->d) : set(h->f);

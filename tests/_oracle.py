@@ -34,7 +34,7 @@ def load():
     lib.oracle_set_bodies.argtypes = [P, P, P]
     lib.oracle_get_diag.argtypes = [P, P]
     lib.oracle_set_closure.restype = u64
-    lib.oracle_set_closure.argtypes = [P, u32, u32, f32, i32]
+    lib.oracle_set_closure.argtypes = [P, u32, u32, f32, i32, i32]
     lib.oracle_set_serial_limit.argtypes = [P, u32]
     lib.oracle_phase_ms.restype = i32
     lib.oracle_phase_ms.argtypes = [P, P, i32]
@@ -140,11 +140,13 @@ class Oracle:
         """Largest bearing count that takes the serial small-mechanism sweeps."""
         load().oracle_set_serial_limit(self.h, bearings)
 
-    def set_closure(self, steps=0, iterations=0, tolerance=0.0, components=False):
+    def set_closure(self, steps=0, iterations=0, tolerance=0.0, components=False, velocity_projection=False):
         """Closure Newton steps, PCG iterations per step and PCG tolerance
-        (0 keeps 12 / 8 / none), and one PCG per closure component instead of
-        one global system; returns the PCG iterations run so far."""
-        return load().oracle_set_closure(self.h, steps, iterations, tolerance, int(components))
+        (0 keeps 12 / 8 / none), one PCG per closure component instead of one
+        global system, and the velocity-level closure projection before the
+        tree reconstruction; returns the PCG iterations run so far."""
+        return load().oracle_set_closure(self.h, steps, iterations, tolerance, int(components),
+                                         int(velocity_projection))
 
     def fk_newton(self, steps=0):
         load().oracle_fk_newton(self.h, steps)

@@ -234,23 +234,57 @@ def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain
            "final_loss": round(float(losses[-1]), 5), "learned_iterations": chosen}
     if chosen is None:
         return out
-    cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=chosen, use_graph=use_graph,
-                              warm_start=mh._lib.WARM_START_LEARNED)
-    gpu = mh.GpuPhysics.new_with_config(scene, cfg)
-    if tuning:
-        gpu.set_tuning(**tuning)
-    gpu.set_mlp(net)
-    gpu.dispatch_ticks(1, SETTLE_TICKS)
-    gpu.synchronize()
-    t2 = time.perf_counter()
-    gpu.dispatch_ticks(SETTLE_TICKS + 1, steps)
-    gpu.synchronize()
-    v = steps / (time.perf_counter() - t2)
-    flags = gpu.read_last_tick().error_flags
-    gpu.close()
+    def learned_rate(extra):
+        cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=chosen, use_graph=use_graph,
+                                  warm_start=mh._lib.WARM_START_LEARNED, **extra)
+        gpu = mh.GpuPhysics.new_with_config(scene, cfg)
+        if tuning:
+            gpu.set_tuning(**tuning)
+        gpu.set_mlp(net)
+        gpu.dispatch_ticks(1, SETTLE_TICKS)
+        gpu.synchronize()
+        before = gpu.read_learned_stats()["fallback_ticks"]
+        t2 = time.perf_counter()
+        gpu.dispatch_ticks(SETTLE_TICKS + 1, steps)
+        gpu.synchronize()
+        rate = steps / (time.perf_counter() - t2)
+        flags = gpu.read_last_tick().error_flags
+        fired = gpu.read_learned_stats()["fallback_ticks"] - before
+        gpu.close()
+        return rate, flags, fired
+
+    v, flags, _ = learned_rate({})
     out.update({"iteration_ratio": round(plain_iterations / chosen, 3), "steps_per_s": round(v, 2),
                 "steps_per_s_plain": round(value_plain, 2), "speedup": round(v / value_plain, 3),
                 "error_flags": flags})
+    # The fallback: a tick whose residual after the learned passes exceeds the
+    # plain run's mean runs `chosen` more passes (judged on the device).
+    limit = float(plain["rms"].mean())
+    fb = {"learned_residual_limit": limit, "learned_fallback_passes": chosen}
+    fb_cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=chosen, residual=True,
+                                 warm_start=mh._lib.WARM_START_LEARNED, **fb)
+    per_tick = {"rms": [], "mean_penetration": [], "max_penetration": []}
+    fired = []
+    with mh.GpuPhysics.new_with_config(scene, fb_cfg) as gpu:
+        gpu.set_mlp(net)
+        for tick in range(1, ticks + 1):
+            gpu.dispatch_tick(tick)
+            r = gpu.read_residual()
+            for key in per_tick:
+                per_tick[key].append(r[key])
+            fired.append(gpu.read_learned_stats()["fallback_this_tick"])
+        stats = gpu.read_learned_stats()
+    fb_rate, fb_flags, fb_fired = learned_rate(fb)
+    out["fallback"] = {
+        "rule": f"a tick whose rms residual after the {chosen} learned passes exceeds {limit:.4f} (the plain "
+                f"{plain_iterations}-pass mean over ticks 1..{ticks}) runs {chosen} more passes; a non-finite "
+                "prediction takes the cache warm start",
+        "fired_ticks": int(stats["fallback_ticks"]), "of_ticks": ticks,
+        "fired_in_ticks_1_100": int(sum(fired[:100])), "fired_in_ticks_101_300": int(sum(fired[100:])),
+        "nonfinite_predictions": int(stats["nonfinite_total"]),
+        "means": {key: round(float(np.mean(v)), 6) for key, v in per_tick.items()},
+        "steps_per_s_settled": round(fb_rate, 2), "fired_in_settled_window": int(fb_fired),
+        "settled_window_ticks": f"{SETTLE_TICKS + 1}..{SETTLE_TICKS + steps}", "error_flags": fb_flags}
     return out
 
 

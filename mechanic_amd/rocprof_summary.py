@@ -1,11 +1,14 @@
 """Summarises a rocprofv3 results database (rocpd SQLite) as text.
 
     python -m mechanic_amd.rocprof_summary RESULTS.db [--top 30] [--window KERNEL FIRST COUNT]
+                                              [--last-ticks N --total-ticks T]
 
 Prints, per kernel: calls, average / total duration and, when the run
 collected hardware counters (--pmc), the average value of each counter per
 dispatch. --window also prints the mean duration of dispatches FIRST ..
 FIRST+COUNT-1 (in dispatch order) of the kernels whose name contains KERNEL.
+--last-ticks N --total-ticks T averages each counter over the dispatches of
+the last N of T ticks only (a kernel's calls per tick: its calls / T).
 """
 from __future__ import annotations
 
@@ -54,6 +57,27 @@ def counter_tail_average(db: str, kernel_part: str, counter: str, last: int):
     return sum(vals) / len(vals) if vals else None
 
 
+def tail_counters(cur, last_ticks: int, total_ticks: int):
+    """Per kernel and counter: the mean over the dispatches of the last
+    `last_ticks` of `total_ticks` ticks, and the dispatches averaged."""
+    try:
+        cols = [c[1] for c in cur.execute("pragma table_info(counters_collection)").fetchall()]
+        order = next((c for c in ("dispatch_id", "correlation_id", "start") if c in cols), "rowid")
+        rows = cur.execute(f"select kernel_name, counter_name, value from counters_collection "
+                           f"order by {order}").fetchall()
+    except sqlite3.Error:
+        return {}
+    series = defaultdict(list)
+    for kernel, counter, value in rows:
+        series[(kernel, counter)].append(value)
+    out = defaultdict(dict)
+    for (kernel, counter), vals in series.items():
+        per_tick = max(1, round(len(vals) / total_ticks))
+        tail = vals[-last_ticks * per_tick:]
+        out[kernel][counter] = (sum(tail) / len(tail), len(tail))
+    return out
+
+
 def window_average(cur, kernel_part: str, first: int, count: int):
     """Mean duration (ns) and count of dispatches first..first+count-1 (dispatch
     order) of the kernels whose name contains `kernel_part`."""
@@ -68,6 +92,8 @@ def main(argv=None):
     ap.add_argument("db")
     ap.add_argument("--top", type=int, default=30)
     ap.add_argument("--window", nargs=3, metavar=("KERNEL", "FIRST", "COUNT"))
+    ap.add_argument("--last-ticks", type=int, default=0)
+    ap.add_argument("--total-ticks", type=int, default=0)
     args = ap.parse_args(argv)
     cur = sqlite3.connect(args.db).cursor()
     counters = counter_stats(cur)
@@ -80,6 +106,12 @@ def main(argv=None):
         print("\ncounters (average per dispatch)")
         for kernel in sorted(counters, key=lambda k: -sum(v[0] for v in counters[k].values())):
             vals = ", ".join(f"{c}={v:.4g}" for c, (v, _) in sorted(counters[kernel].items()))
+            print(f"{kernel[:64]:64s} {vals}")
+    if args.last_ticks and args.total_ticks:
+        tail = tail_counters(cur, args.last_ticks, args.total_ticks)
+        print(f"\ncounters (average per dispatch over the last {args.last_ticks} of {args.total_ticks} ticks)")
+        for kernel in sorted(tail, key=lambda k: -sum(v[0] for v in tail[k].values())):
+            vals = ", ".join(f"{c}={v:.4g} (n={n})" for c, (v, n) in sorted(tail[kernel].items()))
             print(f"{kernel[:64]:64s} {vals}")
     if args.window:
         name, first, count = args.window[0], int(args.window[1]), int(args.window[2])

@@ -58,11 +58,14 @@ def solve_kernel_bytes(n, a, r):
     return 76 * a + 24 * r + 64 * n
 
 
-# What FETCH_SIZE and WRITE_SIZE (KiB per dispatch) read for a plain device copy
-# of a known 256 MiB: FETCH_SIZE counts half the bytes read, WRITE_SIZE all the
-# bytes written (rocprofv3 on MI355X; the copy's summary is kept with the profiles).
-FETCH_KIB_SCALE = 2.0
-WRITE_KIB_SCALE = 1.0
+# HBM traffic of a launch from the L2's memory-side request counters, one
+# rocprofv3 --pmc pass each, with every request counted at its own size. On
+# MI355X the derived FETCH_SIZE counts a 128-byte read as 64 bytes (its
+# 128-byte term, TCC_BUBBLE, reads 0 on gfx950), so it reports half of a plain
+# copy's reads; these counters read a 256 MiB copy's 268435456 bytes exactly
+# (the calibration summary is kept with the profiles).
+READ_COUNTERS = {"TCC_EA0_RDREQ_32B": 32, "TCC_EA0_RDREQ_64B": 64, "TCC_EA0_RDREQ_128B": 128}
+WRITE_COUNTERS = ("TCC_EA0_WRREQ", "TCC_EA0_WRREQ_64B")  # all writes; the 64-byte ones (the rest are 32 B)
 
 
 TRAFFIC_SETTLE = 5  # ticks that rebuild the contact cache after the state is restored
@@ -98,11 +101,11 @@ def traffic_probe(args):
 
 
 def measure_traffic(args, tuning, iterations, state):
-    """HBM bytes per k_solve launch: FETCH_SIZE and WRITE_SIZE from two
-    rocprofv3 --pmc runs (child processes) continuing the settled state,
-    averaged over the launches of the child's profile_ticks ticks and scaled by
-    the copy calibration, with the algorithmic bytes of the same launches.
-    A child that ends with a non-zero status is a failed measurement."""
+    """HBM bytes per k_solve launch from two rocprofv3 --pmc runs (child
+    processes) continuing the settled state: the read requests by size, then
+    the write requests and their 64-byte share, averaged over the launches of
+    the child's profile_ticks ticks, with the algorithmic bytes of the same
+    launches. A child that ends with a non-zero status is a failed measurement."""
     import shutil
     import subprocess
     import tempfile
@@ -111,13 +114,13 @@ def measure_traffic(args, tuning, iterations, state):
 
     if shutil.which("rocprofv3") is None:
         return None, "rocprofv3 not found"
-    kib = {}
+    mean = {}
     algorithmic = []
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    for counters in (tuple(READ_COUNTERS), WRITE_COUNTERS):
         with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
             counts_path = os.path.join(tmp, "counts.json")
-            cmd = ["rocprofv3", "--pmc", counter, "--kernel-include-regex", "k_solve", "-d", tmp, "-o", "run", "--",
-                   sys.executable, os.path.abspath(__file__), "--traffic-probe", "--state", state,
+            cmd = ["rocprofv3", "--pmc", *counters, "--kernel-include-regex", "k_solve", "-d", tmp, "-o", "run",
+                   "--", sys.executable, os.path.abspath(__file__), "--traffic-probe", "--state", state,
                    "--profile-ticks", str(args.profile_ticks), "--tuning-json", json.dumps(tuning),
                    "--counts-out", counts_path]
             env = dict(os.environ, TMPDIR="/tmp")
@@ -125,22 +128,25 @@ def measure_traffic(args, tuning, iterations, state):
                 res = subprocess.run(cmd, cwd="/tmp", env=env, timeout=300, stdout=subprocess.DEVNULL,
                                      stderr=subprocess.PIPE, text=True)
             except (subprocess.SubprocessError, OSError) as e:
-                return None, f"{counter} run failed: {type(e).__name__}"
+                return None, f"{counters[0]} run failed: {type(e).__name__}"
             tail = " | ".join(l for l in res.stderr.strip().splitlines()[-3:] if l)[-300:]
             if res.returncode != 0 or not os.path.exists(counts_path):
-                return None, f"{counter} run exited with status {res.returncode}: {tail}"
+                return None, f"{counters[0]} run exited with status {res.returncode}: {tail}"
             dbs = [os.path.join(r, f) for r, _, fs in os.walk(tmp) for f in fs if f.endswith("results.db")]
-            value = counter_tail_average(dbs[0], "k_solve", counter, iterations * args.profile_ticks) if dbs else None
-            if value is None:
-                return None, f"{counter}: no k_solve samples ({tail})"
-            kib[counter] = value
+            for counter in counters:
+                value = counter_tail_average(dbs[0], "k_solve", counter, iterations * args.profile_ticks) \
+                    if dbs else None
+                if value is None:
+                    return None, f"{counter}: no k_solve samples ({tail})"
+                mean[counter] = value
             with open(counts_path) as f:
                 c = json.load(f)
             algorithmic += [solve_kernel_bytes(c["bodies"], a, r) for a, r in c["ticks"]]
-    fetch = kib["FETCH_SIZE"] * FETCH_KIB_SCALE * 1024.0
-    write = kib["WRITE_SIZE"] * WRITE_KIB_SCALE * 1024.0
+    read = sum(mean[c] * size for c, size in READ_COUNTERS.items())
+    all_writes, writes_64 = (mean[c] for c in WRITE_COUNTERS)
+    written = 64.0 * writes_64 + 32.0 * (all_writes - writes_64)
     algo = sum(algorithmic) / len(algorithmic)
-    return {"bytes": fetch + write, "read": fetch, "written": write, "algorithmic": algo}, None
+    return {"bytes": read + written, "read": read, "written": written, "algorithmic": algo}, None
 
 
 def cpu_baseline(scene, friction, iterations, state):
@@ -591,11 +597,11 @@ def main():
                      "traffic_written": None if traffic is None else round(traffic["written"]),
                      "traffic_algorithmic_bytes": None if traffic is None else round(traffic["algorithmic"]),
                      "traffic_ratio": None if traffic is None else round(traffic["bytes"] / traffic["algorithmic"], 3),
-                     "traffic_note": traffic_note or "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_solve over the "
-                                                     f"{npt} ticks after {TRAFFIC_SETTLE} ticks from the profiled "
-                                                     "ticks' starting state, x2.0 / x1.0 KiB (copy calibration); "
-                                                     "traffic_ratio divides by the algorithmic bytes of the same "
-                                                     "launches"},
+                     "traffic_note": traffic_note or "rocprofv3 --pmc of k_solve's L2-to-memory requests by "
+                                                     "size (TCC_EA0_RDREQ_32B/64B/128B, TCC_EA0_WRREQ and "
+                                                     f"_64B) over the {npt} ticks after {TRAFFIC_SETTLE} ticks "
+                                                     "from the profiled ticks' starting state; traffic_ratio "
+                                                     "divides by the algorithmic bytes of the same launches"},
         "kernel_us": {"k_solve": round(solve_ms * 1e3, 2), "k_solve_span": round(span_ms * 1e3, 2),
                       "k_apply_span": round(apply_ms * 1e3, 2)},
         "cpu_baseline": cpu,

@@ -2045,6 +2045,30 @@ void oracle_set_coordinates(OracleHandle* h, const float* in) {
   else set(h->f);
 }
 
+// write_mechanism_drives (device.rs:1622-1644): replaces every coordinate's
+// drive row (8 words each, the GpuMechanismDrive layout). Returns 0, or -1 when
+// the row count differs from the coordinate count (DriveStateCount).
+int oracle_set_drives(OracleHandle* h, uint32_t nq, const float* drives) {
+  int rc = 0;
+  auto set = [&](auto& w) {
+    using W = typename std::remove_reference<decltype(w)>::type;
+    using R = typename std::remove_reference<decltype(w.dt)>::type;
+    if (nq != w.coordinate_count) {
+      rc = -1;
+      return;
+    }
+    for (uint32_t q = 0; q < nq; ++q) {
+      const float* r = drives + 8 * (size_t)q;
+      uint32_t mode;
+      std::memcpy(&mode, r, 4);
+      w.drives[q] = typename W::Drive{mode, R(r[1]), R(r[2]), R(r[3]), R(r[4]), R(r[5]), R(r[6])};
+    }
+  };
+  if (h->fp64) set(h->d);
+  else set(h->f);
+  return rc;
+}
+
 // Forward kinematics from the current coordinates, then `steps` closure Newton
 // steps (no dynamics), for the closure-solver checks.
 void oracle_fk_newton(OracleHandle* h, int steps) {

@@ -49,6 +49,8 @@ def load():
     lib.oracle_set_mechanisms.argtypes = [P, u32, P, P, P, u32, P]
     lib.oracle_get_coordinates.argtypes = [P, P]
     lib.oracle_set_coordinates.argtypes = [P, P]
+    lib.oracle_set_drives.restype = i32
+    lib.oracle_set_drives.argtypes = [P, u32, P]
     lib.oracle_fk_newton.argtypes = [P, i32]
     lib.oracle_closure_residuals.restype = sz
     lib.oracle_closure_residuals.argtypes = [P, P, sz]
@@ -135,6 +137,15 @@ class Oracle:
     def set_coordinates(self, rows):
         a = np.ascontiguousarray(rows, np.float32).reshape(-1, 2)
         load().oracle_set_coordinates(self.h, a.ctypes.data)
+
+    def set_drives(self, rows):
+        """write_mechanism_drives (device.rs:1622-1644): one GpuMechanismDrive
+        row per coordinate."""
+        arr = np.zeros((max(len(rows), 1), 8), np.float32)
+        for q, r in enumerate(rows):
+            arr[q] = np.frombuffer(bytes(r), np.float32)
+        if load().oracle_set_drives(self.h, len(rows), arr.ctypes.data) != 0:
+            raise ValueError(f"DriveStateCount: {len(rows)} rows for {self.q} coordinates")
 
     def set_serial_limit(self, bearings):
         """Largest bearing count that takes the serial small-mechanism sweeps."""

@@ -50,6 +50,7 @@ enum CounterWord : uint32_t {
   kCountReserved = 1,
   kCountTop = 2,
   kCountSideTicket = 3,  // tile tickets of the contact-side offset scan
+  kCountWalked = 4,      // packets whose walk has finished (the last one forms the item prefix)
   kCounterWords = 16,
 };
 
@@ -108,7 +109,7 @@ struct Dev {
   uint4* work;            // traversal work items, kItemStage per packet: (packet first leaf, first, last, 0)
   uint32_t work_capacity;
   uint32_t* work_count;   // items recorded per packet
-  uint32_t* work_first;   // packets + 1: exclusive prefix of work_count (k_scan_prefix)
+  uint32_t* work_first;   // packets + 1: exclusive prefix of work_count (the last walk workgroup forms it)
   uint32_t* counters;     // per-tick private counters, see CounterWord
   uint32_t* parents;      // reference encoding of node_parents (parent + 1)
   uint2* pairs;
@@ -252,6 +253,20 @@ __device__ __forceinline__ XcdSplit xcd_split(uint32_t n) {
   s.end = (uint32_t)(((uint64_t)n * (xcd + 1)) / kXcds);
   s.first = s.begin + local * blockDim.x;
   s.step = blocks * blockDim.x;
+  return s;
+}
+
+// Plain grid order for kernels that append what they produce (narrowphase,
+// contact preparation): all XCDs work on neighbouring items at any moment, so
+// the appended rows come out in nearly the items' own (spatial) order, and the
+// solver's contiguous XCD eighths of them stay spatially compact. Split by XCD
+// instead, the eight streams of appends interleave rows from eight regions.
+__device__ __forceinline__ XcdSplit grid_split(uint32_t n) {
+  XcdSplit s;
+  s.begin = 0;
+  s.end = n;
+  s.first = blockIdx.x * blockDim.x;
+  s.step = gridDim.x * blockDim.x;
   return s;
 }
 

@@ -45,6 +45,7 @@ __global__ __launch_bounds__(256) void k_morton(Dev d) {
   uint32_t* diag = d.diag[tick & 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kCounterWords) d.counters[i] = 0u;  // sort_count >= 256 threads always run
+  if (i < (d.collider_count + 63u) / 64u) d.work_count[i] = 0u;  // packets' item counts, unpublished
   {
     // This tick's manifold index starts empty (prepare_contacts inserts into it;
     // the previous tick's index, read by the lookups, is the other one).
@@ -263,6 +264,9 @@ constexpr uint32_t kRsAggregate = 1u << 30;  // status word: the tile's own coun
 constexpr uint32_t kRsPrefix = 2u << 30;     // status word: count over tiles 0..this one
 constexpr uint32_t kRsCount = (1u << 30) - 1u;
 constexpr uint32_t kRsSpinLimit = 1u << 22;  // reads of an unpublished word before giving up
+// A packet's published item count: the count in the low bits and a published bit.
+constexpr uint32_t kWalkCountMask = 0xffu;
+constexpr uint32_t kWalkPublished = 0x100u;
 
 // Exclusive prefix sum over a 256-thread workgroup.
 __device__ __forceinline__ uint32_t exclusive_scan256(uint32_t v, uint32_t* wave_sums) {
@@ -1153,36 +1157,50 @@ __global__ __launch_bounds__(256) void k_traverse(Dev d) {
   if (run_first <= run_last) emit_range(run_first, run_last);
   if (st.count) flush_pairs(d, diag, st, lane);
   if constexpr (!kInline) {
+    // Publish the packet's item count with its published bit as one word (the
+    // Morton kernel zeroes the words each tick); the last workgroup to arrive
+    // forms the exclusive prefix of all the counts, which the scan distributes
+    // by. Counts and arrivals are relaxed device atomics (a release fence per
+    // workgroup would write the XCD's L2 back each time): the last workgroup
+    // reads each word until it is published, as every workgroup published its
+    // own before arriving.
+    __shared__ uint32_t last;
     __syncthreads();
-    if (threadIdx.x == 0) d.work_count[packet] = slots < kItemStage ? slots : kItemStage;
+    const uint32_t tag = kWalkPublished;
+    if (threadIdx.x == 0) {
+      const uint32_t count = slots < kItemStage ? slots : kItemStage;
+      store_status(&d.work_count[packet], tag | count);
+      const uint32_t arrived =
+          __hip_atomic_fetch_add(&d.counters[kCountWalked], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = arrived == gridDim.x - 1u ? 1u : 0u;
+    }
+    __syncthreads();
+    if (last) {
+      __shared__ uint32_t wave_sums[kBlock / 64];
+      const uint32_t packets = gridDim.x;
+      const uint32_t per = (packets + kBlock - 1u) / kBlock;
+      const uint32_t begin = threadIdx.x * per < packets ? threadIdx.x * per : packets;
+      const uint32_t end = begin + per < packets ? begin + per : packets;
+      uint32_t sum = 0;
+      for (uint32_t i = begin; i < end; ++i) {
+        uint32_t v = load_status(&d.work_count[i]);
+        for (uint32_t spins = 0; (v & ~kWalkCountMask) != tag; v = load_status(&d.work_count[i])) {
+          if (++spins >= kRsSpinLimit) {
+            atomicOr(&diag[kDiagFlags], 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        sum += v & kWalkCountMask;
+      }
+      uint32_t run = exclusive_scan256(sum, wave_sums);
+      for (uint32_t i = begin; i < end; ++i) {
+        d.work_first[i] = run;
+        run += load_status(&d.work_count[i]) & kWalkCountMask;
+      }
+      if (threadIdx.x == kBlock - 1u) d.work_first[packets] = run;
+    }
   }
-}
-
-// Exclusive prefix of the per-packet item counts (one workgroup).
-__global__ __launch_bounds__(1024) void k_scan_prefix(Dev d) {
-  __shared__ uint32_t wave_sums[16];
-  const uint32_t packets = (d.collider_count + 63u) / 64u;
-  const uint32_t per = (packets + 1023u) / 1024u;
-  const uint32_t begin = threadIdx.x * per;
-  const uint32_t end = begin + per < packets ? begin + per : packets;
-  uint32_t sum = 0;
-  for (uint32_t i = begin; i < end; ++i) sum += d.work_count[i];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t x = sum;
-#pragma unroll
-  for (uint32_t off = 1; off < 64u; off <<= 1) {
-    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
-    if (lane >= off) x += y;
-  }
-  if (lane == 63u) wave_sums[wave] = x;
-  __syncthreads();
-  uint32_t run = x - sum;
-  for (uint32_t w = 0; w < wave; ++w) run += wave_sums[w];
-  for (uint32_t i = begin; i < end; ++i) {
-    d.work_first[i] = run;
-    run += d.work_count[i];
-  }
-  if (threadIdx.x == 1023u) d.work_first[packets] = run;
 }
 
 // Every wavefront takes a contiguous share of the work items (in packet order,
@@ -1379,7 +1397,6 @@ void launch_traverse(hipStream_t s, const Dev& d) {
     k_traverse<true><<<packets, kBlock, 0, s>>>(d);
   } else {
     k_traverse<false><<<packets, kBlock, 0, s>>>(d);
-    k_scan_prefix<<<1, 1024, 0, s>>>(d);
     const uint32_t scan_blocks = d.t_scan_blocks ? d.t_scan_blocks : 4096u;
     if (d.suppression_count) k_scan<true><<<scan_blocks, kBlock, 0, s>>>(d);
     else k_scan<false><<<scan_blocks, kBlock, 0, s>>>(d);

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_narrowphase(Dev d) {
   uint32_t* diag = d.diag[tick & 1];
   uint32_t requested = *reinterpret_cast<volatile uint32_t*>(&diag[kDiagPairs]);
   const uint32_t count = requested < d.pair_capacity ? requested : d.pair_capacity;
-  const XcdSplit xs = xcd_split(count);
+  const XcdSplit xs = grid_split(count);
   for (uint32_t base = xs.first; base < xs.end; base += xs.step) {
     const uint32_t index = base + threadIdx.x;
     bool emit = false;

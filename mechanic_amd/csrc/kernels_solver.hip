@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   if (per_body)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.side_tiles; i += gridDim.x * blockDim.x)
       d.side_status[i] = 0u;
-  const XcdSplit xs = xcd_split(count);
+  const XcdSplit xs = grid_split(count);
   for (uint32_t base = xs.first; base < xs.end; base += xs.step) {
     const uint32_t ci = base + threadIdx.x;
     bool active = false;

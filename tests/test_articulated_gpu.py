@@ -152,6 +152,22 @@ def test_small_articulated_pile_rollout_statistics():
     assert abs(np.abs(gv[:, :3]).mean() - np.abs(ov[:, :3]).mean()) < 0.05 * max(np.abs(ov[:, :3]).mean(), 0.1)
 
 
+def test_reference_closure_solve_beyond_lds_and_one_product_slot():
+    """The reference's global Newton step past its two fast paths: more than
+    4,608 coordinates (joint frames and PCG vectors read from global memory,
+    not LDS) and more than 512 closures (threads take a second closure-side
+    product slot, reloading its end points and path ranges), against the
+    oracle over the first ticks."""
+    scene = A.articulated_pile(seed=9, nx=2, ny=2, nz=2, cylinders=6000, tree_bearings=5000, closures=600)
+    assert scene.coordinate_count > 4608 and scene.bearing_count - scene.coordinate_count > 512
+    for ticks in (1, 3):
+        (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, ticks)
+        print(f"[articulated] {scene.coordinate_count} coordinates, ticks {ticks}: flags {gf} / {of}", flush=True)
+        assert gf == of
+        np.testing.assert_allclose(gp, op, rtol=0, atol=POSE_TOL * ticks)
+        np.testing.assert_allclose(gc, oc, rtol=0, atol=1e-4)
+
+
 def test_graph_replay_matches_direct_dispatch():
     scene = small_articulated_pile()
     out = []

@@ -169,6 +169,32 @@ def test_contacts_bit_exact(ticks):
     compare_contacts(gpu, ora)
 
 
+def test_manifold_rows_follow_their_contacts():
+    """MH_BUFFER_MANIFOLDS: each contact's persistent-manifold row of the tick
+    (prepare_contacts, collision.wgsl:720-727, with persist_contacts'
+    impulse, collision.wgsl:1071-1088): the
+    ordered collider pair, the tick and one point, the contact's normal,
+    penetration and arm, and a non-negative impulse. A contact the
+    preparation culled writes no row this tick."""
+    scene = mh.named_pile("pile_1k")
+    gpu, _ = make(scene, friction=0.6)
+    for tick in range(1, 61):
+        gpu.dispatch_tick(tick)
+    r = gpu.read_last_tick()
+    k = r.contact_count
+    c = np.frombuffer(gpu.read_buffer(_lib.BUFFER_CONTACTS, k * 64), np.float32).reshape(-1, 16)
+    m = np.frombuffer(gpu.read_buffer(_lib.BUFFER_MANIFOLDS, k * 48), np.float32).reshape(-1, 12)
+    cu, mu = c.view(np.uint32), m.view(np.uint32)
+    kept = mu[:, 2] == 60
+    assert kept.sum() >= r.active_contact_count > 0
+    low, high = np.minimum(cu[:, 2], cu[:, 3]), np.maximum(cu[:, 2], cu[:, 3])
+    assert np.array_equal(mu[kept, 0], low[kept]) and np.array_equal(mu[kept, 1], high[kept])
+    assert np.all(mu[kept, 3] == 1)
+    assert np.array_equal(mu[kept, 4:8], cu[kept, 4:8])
+    assert np.array_equal(mu[kept, 8:11], cu[kept, 8:11])
+    assert np.all(m[kept, 11] >= 0.0) and m[kept, 11].sum() > 0.0
+
+
 @pytest.mark.parametrize("solver", list(SOLVER))
 @pytest.mark.parametrize("friction,iterations", [(0.05, 8), (0.6, 8), (0.6, 3), (0.05, 1)])
 def test_full_step_pile_1k_bit_exact(friction, iterations, solver):

@@ -15,7 +15,18 @@ FP64 oracle shows the same, and the bounds below are on the agreement.
 
 The four-bar's loop closure is compared the same way: the closure anchor
 residual after each of 300 ticks, GPU (the validation pass's maximum) against
-the FP64 oracle's final links."""
+the FP64 oracle's final links.
+
+The articulated configuration's closure solve (a PCG per closure component to
+a 1e-6 tolerance, at most 64 iterations, 24 Newton steps, speeds projected on
+the loops) is rolled out 300 ticks on a reduced articulated scene built by the
+same rules (cylinder chains of 2-20 links with loops, 20% of the bearings
+driven at +-2 rad/s within +-pi/2, cuboids beside them): the same statistics
+plus the closure residuals and the FP64 oracle's PCG iterations per tick.
+MH_ROLLOUT_FULL=1 runs it on the 100,000-body scene instead (the FP64 oracle
+then takes several minutes)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -122,3 +133,74 @@ def test_four_bar_closure_residual_matches_fp64_oracle():
     # Below the reference's 10 um tolerance throughout, in both.
     assert flags & mh.CONSTRAINT_NON_CONVERGENCE_FLAG == 0
     assert g_res.max() < 10.0 and o_res.max() < 10.0
+
+
+# The articulated configuration's closure solve.
+CONFIG2_CLOSURES = dict(closure_components=True, closure_pcg_tolerance=1e-6, closure_pcg_iterations=64,
+                        closure_newton_steps=24, closure_velocity_projection=True)
+
+
+def test_articulated_rollout_statistics_match_fp64_oracle():
+    full = os.environ.get("MH_ROLLOUT_FULL") == "1"
+    scene = A.articulated_pile() if full else \
+        A.articulated_pile(seed=2, nx=12, ny=8, nz=12, cylinders=2000, tree_bearings=900, closures=100)
+    cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8, mechanism_self_collisions=False, **CONFIG2_CLOSURES)
+    gpu = mh.GpuPhysics.new_with_config(scene, cfg)
+    ora = Oracle(scene, iterations=8, friction=0.6, self_collisions=False, fp64=True, threads=16 if full else 8)
+    c = CONFIG2_CLOSURES
+    pcg = ora.set_closure(c["closure_newton_steps"], c["closure_pcg_iterations"], c["closure_pcg_tolerance"],
+                          True, True)
+    g_stats, o_stats, g_anchor, o_anchor, o_closure, o_pcg = [], [], [], [], [], []
+    flags = 0
+    for t in range(1, TICKS + 1):
+        gpu.dispatch_tick(t)
+        ora.step(t)
+        r = gpu.read_last_tick()
+        flags |= r.error_flags
+        rows = np.frombuffer(gpu.read_buffer(_lib.BUFFER_CONTACTS, r.contact_count * 64), np.float32).reshape(-1, 16)
+        gp, gv = gpu.read_bodies()
+        g_stats.append(tick_stats(scene, gp, gv, rows[:, 7]))
+        op, ov = ora.bodies_f64()
+        o_stats.append(tick_stats(scene, op, ov, ora.contacts()[:, 7]))
+        g_anchor.append(1e6 * r.anchor_residual_meters)
+        od = ora.diag()
+        flags_o = int(od[0])
+        o_anchor.append(float(od[3]))
+        o_closure.append(1e6 * np.linalg.norm(ora.closure_residuals()[:, :3], axis=1).max())
+        total = ora.set_closure(c["closure_newton_steps"], c["closure_pcg_iterations"], c["closure_pcg_tolerance"],
+                                True, True)
+        o_pcg.append(total - pcg)
+        pcg = total
+        if full and t % 10 == 0:
+            print(f"[articulated rollout] tick {t}", flush=True)
+    gpu.close()
+    print(f"[articulated rollout] {scene.body_count} bodies, {scene.bearing_count} bearings "
+          f"({scene.bearing_count - scene.coordinate_count} closures)")
+    for k in (1, 30, 100, 200, TICKS):
+        gs, os_ = g_stats[k - 1], o_stats[k - 1]
+        print(f"tick {k:3d}: com_y gpu {gs['com_y']:.4f} fp64 {os_['com_y']:.4f} | KE gpu {gs['kinetic']:.4g} "
+              f"fp64 {os_['kinetic']:.4g} J | contacts gpu {gs['contacts']} fp64 {os_['contacts']} | pen mean gpu "
+              f"{1e3 * gs['pen_mean']:.2f} fp64 {1e3 * os_['pen_mean']:.2f} mm, max gpu {1e3 * gs['pen_max']:.1f} "
+              f"fp64 {1e3 * os_['pen_max']:.1f} mm | max anchor residual gpu {g_anchor[k - 1]:.0f} fp64 "
+              f"{o_anchor[k - 1]:.0f} um (fp64 closures {o_closure[k - 1]:.3f} um) | fp64 PCG iterations "
+              f"{o_pcg[k - 1]}", flush=True)
+    g_anchor, o_closure, o_pcg = np.array(g_anchor), np.array(o_closure), np.array(o_pcg)
+    print(f"closure residual over {TICKS} ticks: gpu (all bearings, whole um) mean {g_anchor.mean():.2f} max "
+          f"{g_anchor.max():.0f}; fp64 closures mean {o_closure.mean():.3f} max {o_closure.max():.3f} um; "
+          f"fp64 PCG iterations per tick mean {o_pcg.mean():.0f} max {o_pcg.max()}", flush=True)
+    assert flags == 0 and flags_o == 0
+    # The closures stay inside the reference's 10 um tolerance (the GPU figure
+    # also covers tree bearings far from the origin, rounded to whole um).
+    assert o_closure.max() <= 10.5 and g_anchor.max() <= 50.0
+    # The first tick parts only at rounding level (a contact at the emission
+    # threshold can fall either way in FP32 and FP64).
+    assert abs(g_stats[0]["com_y"] - o_stats[0]["com_y"]) < 1e-4
+    assert abs(g_stats[0]["contacts"] - o_stats[0]["contacts"]) <= 1e-4 * o_stats[0]["contacts"]
+    # The last 100 ticks, as means.
+    lo, hi = TICKS - 100, TICKS
+    mean = lambda s, k: window(s, k, lo, hi).mean()  # noqa: E731
+    assert abs(mean(g_stats, "com_y") - mean(o_stats, "com_y")) < 0.05 * abs(mean(o_stats, "com_y"))
+    assert abs(mean(g_stats, "kinetic") - mean(o_stats, "kinetic")) < 0.3 * mean(o_stats, "kinetic")
+    assert abs(mean(g_stats, "contacts") - mean(o_stats, "contacts")) < 0.05 * mean(o_stats, "contacts")
+    assert abs(mean(g_stats, "pen_mean") - mean(o_stats, "pen_mean")) < 0.2 * mean(o_stats, "pen_mean")
+    assert abs(mean(g_stats, "pen_max") - mean(o_stats, "pen_max")) < 0.3 * mean(o_stats, "pen_max")

@@ -42,20 +42,22 @@ def algorithmic_bytes(n, c, s, p, k, a, r, iterations):
         "traverse": 40 * c + 72 * (2 * c - 1) + 8 * p,
         "narrowphase": 200 * p + 64 * k,
         "prepare": 340 * k,
-        "solve": 12 * a + 4 * r + iterations * (solve_kernel_bytes(n, a, r) + 24 * r + 112 * n),
+        "solve": iterations * (solve_kernel_bytes(n, a, r, iterations) + 24 * r + 112 * n),
     }
 
 
 LAUNCHES = {"integrate": 1, "morton": 2, "lbvh": 2, "traverse": 1, "narrowphase": 1, "prepare": 1}
 
 
-def solve_kernel_bytes(n, a, r):
-    """Compulsory HBM bytes of one contact-solve pass kernel (k_solve): per active
-    contact its 64-byte solver row, its accumulated impulse read and written
-    (4 + 4) and its sides' group word (4); per delta slot the Q20 velocity deltas
-    of one body summed over a 64-contact group (24); per body its 64-byte solver
-    state read once."""
-    return 76 * a + 24 * r + 64 * n
+def solve_kernel_bytes(n, a, r, iterations):
+    """Compulsory HBM bytes of one contact-solve pass kernel (k_solve), as the
+    mean over a tick's passes: per active contact its 64-byte solver row, its
+    accumulated impulse read and written (4 + 4) and its sides' group word (4,
+    written by the first pass and read by the others); per delta slot the Q20
+    velocity deltas of one body summed over a 64-contact group (24), plus the
+    first pass's slot counter increment (4, spread over the passes); per body its
+    64-byte solver state read once."""
+    return 76 * a + 24 * r + 64 * n + 4 * r / iterations
 
 
 # HBM traffic of a launch from the L2's memory-side request counters, one
@@ -141,7 +143,7 @@ def measure_traffic(args, tuning, iterations, state):
                 mean[counter] = value
             with open(counts_path) as f:
                 c = json.load(f)
-            algorithmic += [solve_kernel_bytes(c["bodies"], a, r) for a, r in c["ticks"]]
+            algorithmic += [solve_kernel_bytes(c["bodies"], a, r, iterations) for a, r in c["ticks"]]
     read = sum(mean[c] * size for c, size in READ_COUNTERS.items())
     all_writes, writes_64 = (mean[c] for c in WRITE_COUNTERS)
     written = 64.0 * writes_64 + 32.0 * (all_writes - writes_64)
@@ -536,7 +538,7 @@ def main():
     solve_ms = kernel_ms["solve_kernel_event_ms"] / npt
     span_ms = kernel_ms["solve_kernel_ms"] / npt
     apply_ms = kernel_ms["apply_kernel_ms"] / npt
-    solve_bytes = solve_kernel_bytes(scene.body_count, a, slots)
+    solve_bytes = solve_kernel_bytes(scene.body_count, a, slots, iterations)
     achieved = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     per_phase_gbs = {g: (algo[g] / (phase_ms[g] * 1e-3) / 1e9 if phase_ms[g] > 0 else 0.0) for g in phase_ms}
 

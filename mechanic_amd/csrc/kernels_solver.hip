@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
       const uint32_t ci = __float_as_uint(r3.z);
       const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
       // Per-body passes: both contact sides take a slot of their body, their row.
-      // (The per-contact passes take slots per group of sides, k_group_sides.)
+      // (The per-contact passes take slots per group of sides in their first pass.)
       if (per_body) {
         const uint32_t slot_a = atomicAdd(&d.slot_count[body_a], 1u);
         const uint32_t slot_b = body_b != kInvalid ? atomicAdd(&d.slot_count[body_b], 1u) : 0u;
@@ -249,9 +249,9 @@ __device__ __forceinline__ bool contact_pass(const Dev& d, const SolverBody& sa,
 // per-contact passes run one wavefront per aligned group of 64 active contacts;
 // the group's 128 sides touch far fewer bodies (about two in five on a settled
 // pile), so each body of a group gets one delta slot, and its sides' Q20
-// integers are summed in LDS before one store. k_group_sides finds, for every
-// side, the group's first side of the same body (its representative) and gives
-// each representative a slot of its body. Packed per active contact:
+// integers are summed in LDS before one store. The tick's first pass finds, for
+// every side, the group's first side of the same body (its representative) and
+// gives each representative a slot of its body. Packed per active contact:
 // representative of side A | of side B << 7 | slot A << 14 | slot B << 19
 // (kSlotOverflow: add with atomics instead).
 constexpr uint32_t kGroupSlotShift = 14;
@@ -271,69 +271,56 @@ __device__ __forceinline__ uint32_t side_slot_of(uint32_t* bodies, uint32_t body
   return kInvalid;  // unreachable: at most 128 bodies in 256 entries
 }
 
-__global__ __launch_bounds__(256) void k_group_sides(Dev d) {
-  __shared__ uint32_t lds_body[kBlock / 64][kSideTable];
-  __shared__ uint32_t lds_first[kBlock / 64][kSideTable];
-  const uint64_t tick64 = tick_sv(d);
-  uint32_t* diag = d.diag[tick64 & 1];
-  const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
-  const uint32_t groups = (count + 63u) / 64u;
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  uint32_t* bodies = lds_body[wave];
-  uint32_t* firsts = lds_first[wave];
-  for (uint32_t g = blockIdx.x * (blockDim.x >> 6) + wave; g < groups; g += waves) {
-    const uint32_t ai = g * 64u + lane;
-    const bool valid = ai < count;
-    uint32_t ba = kInvalid, bb = kInvalid;
-    if (valid) {
-      const float4 r3 = row_word(d, 3, ai);
-      ba = __float_as_uint(r3.x);
-      bb = __float_as_uint(r3.y);
-    }
-    // Side positions: 2 * lane for A, 2 * lane + 1 for B. First position of the
-    // same body, over the group's sides in order: the minimum position any side
-    // of the body records in the body's table entry.
-    for (uint32_t e = lane; e < kSideTable; e += 64u) {
-      bodies[e] = kInvalid;
-      firsts[e] = kInvalid;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t ha = ba != kInvalid ? side_slot_of(bodies, ba) : kInvalid;
-    const uint32_t hb = bb != kInvalid ? side_slot_of(bodies, bb) : kInvalid;
-    if (ha != kInvalid) atomicMin(&firsts[ha], 2u * lane);
-    if (hb != kInvalid) atomicMin(&firsts[hb], 2u * lane + 1u);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t rep_a = ha != kInvalid ? firsts[ha] : 2u * lane;
-    const uint32_t rep_b = hb != kInvalid ? firsts[hb] : 2u * lane + 1u;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();  // the next group clears the same table
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    uint32_t own_a = 0u, own_b = 0u;
-    if (valid && rep_a == 2u * lane) own_a = atomicAdd(&d.slot_count[ba], 1u);
-    if (valid && bb != kInvalid && rep_b == 2u * lane + 1u) own_b = atomicAdd(&d.slot_count[bb], 1u);
-    const uint32_t a_from_a = (uint32_t)__shfl((int)own_a, (int)(rep_a >> 1));
-    const uint32_t a_from_b = (uint32_t)__shfl((int)own_b, (int)(rep_a >> 1));
-    const uint32_t b_from_a = (uint32_t)__shfl((int)own_a, (int)(rep_b >> 1));
-    const uint32_t b_from_b = (uint32_t)__shfl((int)own_b, (int)(rep_b >> 1));
-    uint32_t slot_a = (rep_a & 1u) ? a_from_b : a_from_a;
-    uint32_t slot_b = (rep_b & 1u) ? b_from_b : b_from_a;
-    slot_a = slot_a < kSlotOverflow ? slot_a : kSlotOverflow;
-    slot_b = bb != kInvalid && slot_b < kSlotOverflow ? slot_b : kSlotOverflow;
-    if (valid) d.side_group[ai] = rep_a | (rep_b << 7) | (slot_a << kGroupSlotShift) | (slot_b << (kGroupSlotShift + 5u));
+// The group word of one lane's contact (ba, bb: its bodies; kInvalid for no
+// contact or no body B) in a group of 64 on one wavefront: the first position
+// of each side's body over the group's sides in order (side A of lane l at 2l,
+// side B at 2l + 1), from the wavefront's LDS table, and a slot of its body for
+// each side that is that first position (one counter atomic per body of the
+// group). All 64 lanes of the wavefront call it together.
+__device__ __forceinline__ uint32_t group_word(const Dev& d, uint32_t* bodies, uint32_t* firsts, uint32_t lane,
+                                               bool valid, uint32_t ba, uint32_t bb) {
+  for (uint32_t e = lane; e < kSideTable; e += 64u) {
+    bodies[e] = kInvalid;
+    firsts[e] = kInvalid;
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t ha = ba != kInvalid ? side_slot_of(bodies, ba) : kInvalid;
+  const uint32_t hb = bb != kInvalid ? side_slot_of(bodies, bb) : kInvalid;
+  if (ha != kInvalid) atomicMin(&firsts[ha], 2u * lane);
+  if (hb != kInvalid) atomicMin(&firsts[hb], 2u * lane + 1u);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t rep_a = ha != kInvalid ? firsts[ha] : 2u * lane;
+  const uint32_t rep_b = hb != kInvalid ? firsts[hb] : 2u * lane + 1u;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();  // the next group clears the same table
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint32_t own_a = 0u, own_b = 0u;
+  if (valid && rep_a == 2u * lane) own_a = atomicAdd(&d.slot_count[ba], 1u);
+  if (valid && bb != kInvalid && rep_b == 2u * lane + 1u) own_b = atomicAdd(&d.slot_count[bb], 1u);
+  const uint32_t a_from_a = (uint32_t)__shfl((int)own_a, (int)(rep_a >> 1));
+  const uint32_t a_from_b = (uint32_t)__shfl((int)own_b, (int)(rep_a >> 1));
+  const uint32_t b_from_a = (uint32_t)__shfl((int)own_a, (int)(rep_b >> 1));
+  const uint32_t b_from_b = (uint32_t)__shfl((int)own_b, (int)(rep_b >> 1));
+  uint32_t slot_a = (rep_a & 1u) ? a_from_b : a_from_a;
+  uint32_t slot_b = (rep_b & 1u) ? b_from_b : b_from_a;
+  slot_a = slot_a < kSlotOverflow ? slot_a : kSlotOverflow;
+  slot_b = bb != kInvalid && slot_b < kSlotOverflow ? slot_b : kSlotOverflow;
+  return rep_a | (rep_b << 7) | (slot_a << kGroupSlotShift) | (slot_b << (kGroupSlotShift + 5u));
 }
 
 // warm_start (collision.wgsl:774-839) when first, solve_accumulate
 // (collision.wgsl:841-906) otherwise, per contact, with the group's sides
 // summed per body in LDS; the last pass also runs persist_contacts
 // (collision.wgsl:1071-1088). Each XCD takes a contiguous run of whole groups.
+template <bool kGroup>
 __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint64_t* clock, const uint32_t* gate) {
   __shared__ int32_t lds_q[kBlock / 64][128][6];
+  __shared__ uint32_t lds_body[kGroup ? kBlock / 64 : 1][kGroup ? kSideTable : 1];
+  __shared__ uint32_t lds_first[kGroup ? kBlock / 64 : 1][kGroup ? kSideTable : 1];
   if (gate && *reinterpret_cast<const volatile uint32_t*>(gate) == 0u) return;
   if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
   const uint64_t tick64 = tick_sv(d);
@@ -356,11 +343,21 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint6
       q[2 * lane + 1][c] = 0;
     }
     uint32_t body_a = kInvalid, body_b = kInvalid, grp = 0u;
+    if constexpr (kGroup) {
+      // The tick's first pass groups the sides (the later passes read the words).
+      if (valid) {
+        const float4 r3 = row_word(d, 3, ai);
+        body_a = __float_as_uint(r3.x);
+        body_b = __float_as_uint(r3.y);
+      }
+      grp = group_word(d, lds_body[wave], lds_first[wave], lane, valid, body_a, body_b);
+      if (valid) d.side_group[ai] = grp;
+    }
     if (valid) {
       const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
       body_a = __float_as_uint(r3.x);
       body_b = __float_as_uint(r3.y);
-      grp = d.side_group[ai];
+      if constexpr (!kGroup) grp = d.side_group[ai];
       const uint32_t word = __float_as_uint(r3.z);
       const bool has_b = body_b != kInvalid;
       const ContactRow row{xyz(r0), xyz(r1), xyz(r2), r0.w, r3.w, r2.w < 0.0f};
@@ -996,22 +993,24 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock, hi
   if (bearings && d.serial_mech) {
     // device.rs:1352-1416 with serial_mechanism: the parallel warm-start pass,
     // then 12x the passes as serial sweeps (per-contact passes only).
-    k_group_sides<<<grid_for(d.pair_capacity / 64u * kBlock, 4096u), kBlock, 0, s>>>(d);
-    k_solve<<<sgrid, kBlock, 0, s>>>(d, 1, 0, nullptr, nullptr);
+    k_solve<true><<<sgrid, kBlock, 0, s>>>(d, 1, 0, nullptr, nullptr);
     if (bgrid) k_apply<<<bgrid, kBlock, 0, s>>>(d, 0, nullptr, nullptr);
     launch_serial_solve(s, d, iterations * kSerialPassMultiplier);
     return;
   }
   if (d.t_solve_kind != 1u) {
-    k_group_sides<<<grid_for(d.pair_capacity / 64u * kBlock, 4096u), kBlock, 0, s>>>(d);
     for (uint32_t it = 0; it < iterations; ++it) {
       const int last = it + 1 == iterations ? 1 : 0;
       uint64_t* pass_clock = clock ? clock + 2 * (size_t)it * sgrid : nullptr;
+      // The first pass also groups the tick's contact sides.
       if (events)  // start and stop events carry this dispatch's own timestamps
-        hipExtLaunchKernelGGL(k_solve, dim3(sgrid), dim3(kBlock), 0, s, events[2 * it], events[2 * it + 1], 0, d,
-                              it == 0 ? 1 : 0, last, pass_clock, static_cast<const uint32_t*>(nullptr));
+        hipExtLaunchKernelGGL(it == 0 ? k_solve<true> : k_solve<false>, dim3(sgrid), dim3(kBlock), 0, s,
+                              events[2 * it], events[2 * it + 1], 0, d, it == 0 ? 1 : 0, last, pass_clock,
+                              static_cast<const uint32_t*>(nullptr));
+      else if (it == 0)
+        k_solve<true><<<sgrid, kBlock, 0, s>>>(d, 1, last, pass_clock, nullptr);
       else
-        k_solve<<<sgrid, kBlock, 0, s>>>(d, it == 0 ? 1 : 0, last, pass_clock, nullptr);
+        k_solve<false><<<sgrid, kBlock, 0, s>>>(d, 0, last, pass_clock, nullptr);
       if (bgrid)
         k_apply<<<bgrid, kBlock, 0, s>>>(
             d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr, nullptr);
@@ -1022,7 +1021,7 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock, hi
       k_learned_judge<<<1, 64, 0, s>>>(d);
       for (uint32_t it = 0; it < extra; ++it) {
         const int last = it + 1 == extra ? 1 : 0;
-        k_solve<<<sgrid, kBlock, 0, s>>>(d, 0, last, nullptr, gate);
+        k_solve<false><<<sgrid, kBlock, 0, s>>>(d, 0, last, nullptr, gate);
         if (bgrid) k_apply<<<bgrid, kBlock, 0, s>>>(d, last, nullptr, gate);
       }
     }

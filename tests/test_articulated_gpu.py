@@ -168,6 +168,39 @@ def test_reference_closure_solve_beyond_lds_and_one_product_slot():
         np.testing.assert_allclose(gc, oc, rtol=0, atol=1e-4)
 
 
+@pytest.mark.parametrize("case", ["nan_angle", "inf_angle", "scaled_root_rotation"])
+def test_articulated_validation_raises_invalid_numeric(case):
+    """validate_articulated_state (articulated.wgsl:458-484): a non-finite
+    coordinate angle, or a body rotation whose squared length leaves
+    (0.5, 1.5), raises INVALID_NUMERIC on the GPU exactly as on the oracle; the
+    untouched scene raises nothing."""
+    scene = A.hinge_chain(links=3, length=0.4)
+    coords = [(0.1, 0.0)] * scene.coordinate_count
+    if case == "nan_angle":
+        coords[1] = (float("nan"), 0.0)
+    elif case == "inf_angle":
+        coords[2] = (float("inf"), 0.0)
+    flags = []
+    for poisoned in (False, True):
+        cfg = GpuPhysicsConfig(collisions_enabled=False)
+        with GpuPhysics.new_with_config(scene, cfg) as gpu:
+            ora = Oracle(scene, collisions=False)
+            rows = coords if poisoned else [(0.1, 0.0)] * scene.coordinate_count
+            gpu.initialize_mechanism_coordinates([mh._lib.MechanismCoordinate(a, v) for a, v in rows])
+            ora.set_coordinates(np.asarray(rows, np.float32))
+            if poisoned and case == "scaled_root_rotation":
+                p, v = gpu.read_bodies()
+                p[0, 4:8] *= 2.0  # the static root: |q|^2 = 4
+                gpu.write_bodies(p, v)
+                ora.set_bodies(p, v)
+            gpu.dispatch_tick(1)
+            ora.step(1)
+            flags.append((gpu.read_last_tick().error_flags, int(ora.diag()[0])))
+    assert flags[0] == (0, 0)
+    g, o = flags[1]
+    assert g & mh.INVALID_NUMERIC_FLAG and g == o, flags
+
+
 def test_graph_replay_matches_direct_dispatch():
     scene = small_articulated_pile()
     out = []

@@ -315,3 +315,26 @@ def test_loops_stay_closed_only_with_the_velocity_projection():
                     break
     assert first_flag["projected"] is None, first_flag
     assert first_flag["reference"] is not None and first_flag["components"] is not None, first_flag
+
+
+@pytest.mark.parametrize("case", ["nan_angle", "inf_angle", "scaled_root_rotation", "clean"])
+def test_validation_flags_non_finite_state(case):
+    """validate_articulated_state (articulated.wgsl:458-484) in the oracle: a NaN
+    or infinite angle or a root rotation with |q|^2 = 4 raises
+    INVALID_NUMERIC (2) after one tick; a clean chain raises nothing."""
+    import mechanic_amd as mh
+    scene = A.hinge_chain(links=3, length=0.4)
+    coords = np.full((scene.coordinate_count, 2), [0.1, 0.0], np.float32)
+    if case == "nan_angle":
+        coords[1, 0] = np.nan
+    elif case == "inf_angle":
+        coords[2, 0] = np.inf
+    o = Oracle(scene, collisions=False)
+    o.set_coordinates(coords)
+    if case == "scaled_root_rotation":
+        p, v = o.bodies()
+        p[0, 4:8] *= 2.0
+        o.set_bodies(p, v)
+    o.step(1)
+    flags = int(o.diag()[0])
+    assert bool(flags & mh.INVALID_NUMERIC_FLAG) == (case != "clean"), flags

@@ -245,6 +245,11 @@ MH_API const char* mh_status_string(int code);
 MH_API int mh_device_count(int* count);
 /* Architecture name of the current device (e.g. "gfx950:sramecc+:xnack-"). */
 MH_API int mh_device_arch(char* out, size_t capacity);
+/* Converts `count` values on the current device with the rounding of the
+   solvers' Q20 velocity deltas (add_velocity_delta, collision.wgsl:764-772;
+   articulated.wgsl:104-112): round half to even, then a saturating cast (NaN ->
+   0). A self-test of the device conversion; no counterpart in the reference. */
+MH_API int mh_device_round_i32(const float* values, int32_t* out, size_t count);
 
 /* GpuPhysics::new_with_config (device.rs:399-811). */
 MH_API int mh_physics_create(const mh_creation* creation, const mh_config* config,

@@ -230,6 +230,7 @@ def load() -> ctypes.CDLL:
         "mh_status_string": (ctypes.c_char_p, [c_int]),
         "mh_device_count": (c_int, [POINTER(c_int)]),
         "mh_device_arch": (c_int, [ctypes.c_char_p, c_size_t]),
+        "mh_device_round_i32": (c_int, [POINTER(c_float), POINTER(ctypes.c_int32), c_size_t]),
         "mh_physics_create": (c_int, [POINTER(Creation), POINTER(Config), POINTER(H), POINTER(ErrorDetail)]),
         "mh_physics_destroy": (None, [H]),
         "mh_physics_body_count": (c_uint32, [H]),

@@ -282,6 +282,8 @@ void launch_lbvh(hipStream_t s, const Dev& d);
 void launch_traverse(hipStream_t s, const Dev& d);
 void launch_narrowphase(hipStream_t s, const Dev& d);
 void launch_prepare(hipStream_t s, const Dev& d);
+// round_to_i32 over `count` device values (the conversion's self-test).
+void launch_round_i32(const float* values, int32_t* out, uint32_t count);
 // With `clock`, the pass kernels stamp the device wall clock per workgroup:
 // pass i of k_solve at clock + 2 * i * solve_grid(d) (start, end per block),
 // pass i of k_apply at clock + 2 * (passes * solve_grid(d) + i * apply_grid(d)).

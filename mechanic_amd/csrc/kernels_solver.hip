@@ -961,6 +961,15 @@ static inline uint32_t grid_for(uint32_t n, uint32_t cap) {
   return b < cap ? b : cap;
 }
 
+__global__ __launch_bounds__(256) void k_round_i32(const float* values, int32_t* out, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = round_to_i32(values[i]);
+}
+
+void launch_round_i32(const float* values, int32_t* out, uint32_t count) {
+  k_round_i32<<<(count + kBlock - 1) / kBlock, kBlock>>>(values, out, count);
+}
+
 void launch_prepare(hipStream_t s, const Dev& d) {
   k_prepare<<<grid_for(d.pair_capacity, d.t_prepare_blocks ? d.t_prepare_blocks : 1024u), kBlock, 0, s>>>(d);
 }

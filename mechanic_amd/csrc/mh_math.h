@@ -59,13 +59,16 @@ __device__ __forceinline__ v3 quat_rotate(float4 q, v3 v) {
 }
 __device__ __forceinline__ float4 quat_conj(float4 q) { return make_float4(-q.x, -q.y, -q.z, q.w); }
 
-// f32 -> i32 after round-half-to-even, saturating like the GPU conversion.
+// f32 -> i32 after round-half-to-even, saturating like the GPU conversion: NaN
+// -> 0, values at or above 2^31 -> INT_MAX, below -2^31 -> INT_MIN. gfx950's
+// v_cvt_i32_f32 saturates exactly so (mh_device_round_i32 checks it against
+// these rules), in one instruction where the compare-and-select form takes
+// eight; the contact solver converts twelve values a contact a pass.
 __device__ __forceinline__ int32_t round_to_i32(float x) {
-  float r = __builtin_rintf(x);
-  if (!(r == r)) return 0;
-  if (r >= 2147483648.0f) return 2147483647;
-  if (r < -2147483648.0f) return (int32_t)0x80000000u;
-  return (int32_t)r;
+  const float r = __builtin_rintf(x);
+  int32_t out;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(out) : "v"(r));
+  return out;
 }
 
 __device__ __forceinline__ bool finite4(float4 v) {

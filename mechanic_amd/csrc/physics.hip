@@ -604,6 +604,26 @@ int mh_device_arch(char* out, size_t capacity) {
   return MH_OK;
 }
 
+int mh_device_round_i32(const float* values, int32_t* out, size_t count) {
+  if ((!values || !out) && count) return MH_ERR_INVALID_ARGUMENT;
+  if (count == 0) return MH_OK;
+  if (count > (1u << 24)) return MH_ERR_INVALID_ARGUMENT;
+  float* dv = nullptr;
+  int32_t* dq = nullptr;
+  HIP_TRY_NOERR(hipMalloc(&dv, count * sizeof(float)));
+  hipError_t e = hipMalloc(&dq, count * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMemcpy(dv, values, count * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_round_i32(dv, dq, (uint32_t)count);
+    e = hipDeviceSynchronize();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dq, count * sizeof(int32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(dv);
+  if (dq) (void)hipFree(dq);
+  HIP_TRY_NOERR(e);
+  return MH_OK;
+}
+
 int mh_device_count(int* count) {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);

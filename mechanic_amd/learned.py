@@ -110,7 +110,10 @@ class WarmStartNet:
 
 def collect_rows(scene: Scene, config: GpuPhysicsConfig, ticks: int, keep: Iterable[int]) -> Tuple[np.ndarray, np.ndarray]:
     """Steps `scene` for `ticks` ticks on a WARM_START_LOG handle and returns the
-    training rows of the ticks in `keep` (1-based tick indices)."""
+    training rows of the ticks in `keep` (1-based tick indices). Each tick's rows
+    come in a canonical order (sorted by their bits): the device appends active
+    contacts in whatever order its workgroups finish, so without it a sample of
+    the rows, and the network trained on it, would change from run to run."""
     cfg = GpuPhysicsConfig(**{**config.__dict__, "warm_start": _lib.WARM_START_LOG})
     keep = set(int(k) for k in keep)
     feats, targs = [], []
@@ -119,6 +122,9 @@ def collect_rows(scene: Scene, config: GpuPhysicsConfig, ticks: int, keep: Itera
             gpu.dispatch_tick(tick)
             if tick in keep:
                 f, t = gpu.read_training_rows()
+                bits = np.concatenate([f.view(np.uint32), t.view(np.uint32)], axis=1)
+                order = np.lexsort(bits.T[::-1])
+                f, t = f[order], t[order]
                 feats.append(f)
                 targs.append(t)
         flags = gpu.read_last_tick().error_flags

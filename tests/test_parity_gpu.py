@@ -169,6 +169,33 @@ def test_contacts_bit_exact(ticks):
     compare_contacts(gpu, ora)
 
 
+def test_device_fixed_point_rounding_matches_the_reference_rule():
+    """The device conversion of the Q20 velocity deltas (add_velocity_delta,
+    collision.wgsl:764-772): i32(round(x)) with round half to even and a
+    saturating cast (NaN -> 0, at or above 2^31 -> INT_MAX, below -2^31 ->
+    INT_MIN), which the oracle restates; checked on halfway cases, the
+    saturation edges, infinities, NaN, signed zeros, subnormals and random
+    values over the whole exponent range."""
+    import ctypes
+    special = np.array([0.0, -0.0, 0.5, 1.5, 2.5, -0.5, -1.5, -2.5, 3.5, -3.5, 8388607.5, -8388607.5,
+                        2147483520.0, -2147483520.0, 2147483648.0, -2147483648.0, -2147483904.0, 4294967296.0,
+                        1e30, -1e30, np.inf, -np.inf, np.nan, -np.nan, 1e-45, -1e-45, 1e-38, 0.49999997,
+                        -0.49999997, 1048576.0 * 0.7, -1048576.0 * 1.3], np.float32)
+    rng = np.random.default_rng(7)
+    bits = rng.integers(0, 1 << 32, 200_000, dtype=np.uint64).astype(np.uint32)
+    values = np.concatenate([special, bits.view(np.float32), (rng.standard_normal(100_000) * 3e6).astype(np.float32)])
+    out = np.zeros(values.size, np.int32)
+    rc = _lib.load().mh_device_round_i32(values.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), values.size)
+    assert rc == 0
+    with np.errstate(invalid="ignore"):  # signalling NaNs among the random bit patterns
+        r = np.rint(values.astype(np.float64))
+    expected = np.where(np.isnan(r), 0, np.clip(np.nan_to_num(r, nan=0.0, posinf=2.0 ** 31, neginf=-2.0 ** 32),
+                                                -2.0 ** 31, 2.0 ** 31 - 1)).astype(np.int64).astype(np.int32)
+    bad = np.nonzero(out != expected)[0]
+    assert bad.size == 0, f"{bad.size} differ, first {values[bad[0]]!r}: device {out[bad[0]]} rule {expected[bad[0]]}"
+
+
 def test_manifold_rows_follow_their_contacts():
     """MH_BUFFER_MANIFOLDS: each contact's persistent-manifold row of the tick
     (prepare_contacts, collision.wgsl:720-727, with persist_contacts'

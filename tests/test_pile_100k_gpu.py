@@ -67,15 +67,15 @@ def test_settled_pile_100k_bit_exact(settled):
 
 
 def test_learned_warm_start_on_the_100k_pile():
-    """The learned warm start at full size: trained on 600k contacts logged
-    from the plain solver (32 passes) on piles of seeds 10-12, it runs the
+    """The learned warm start at full size: trained on 2M contacts logged
+    from the plain solver (32 passes) on piles of seeds 10-19, it runs the
     100,000-cuboid pile (seed 1) with 4 passes against the cache warm start
     with 8, over 300 ticks: the mean contact residual is no higher, and the
     mean and maximum penetration no more than 5% higher."""
     rng = np.random.default_rng(0)
     log_cfg = GpuPhysicsConfig(friction=FRICTION, solver_iterations=32)
     feats, targs = [], []
-    for seed in (10, 11, 12):
+    for seed in range(10, 20):
         F, T = L.collect_rows(mh.named_pile("pile_100k", seed=seed), log_cfg, 120, range(40, 121, 20))
         pick = rng.choice(F.shape[0], size=min(200_000, F.shape[0]), replace=False)
         feats.append(F[pick])

@@ -216,7 +216,8 @@ struct Dev {
   // id, local child body, (closure << 1 | side) entries in (closure, side)
   // order; closures: global closure index, local end body per side (kInvalid: a
   // root), local coordinates along each side's path to the root.
-  uint32_t cc_count, cc_max_bodies, cc_max_coords, cc_max_closures, cc_max_path, pad_cc;
+  uint32_t cc_count, cc_max_bodies, cc_max_coords, cc_max_closures, cc_max_path;
+  uint32_t cc_dense;  // one-wavefront components hold J^T J densely in LDS for the PCG products
   const uint4* cc_off;
   const uint32_t *cc_body, *cc_body_parent, *cc_body_coord, *cc_coord, *cc_coord_body, *cc_entry_off, *cc_entries,
       *cc_closure, *cc_end, *cc_path_off, *cc_path;

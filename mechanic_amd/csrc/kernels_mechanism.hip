@@ -768,6 +768,9 @@ struct CompLds {
   float* col;  // 6 per coordinate entry: the Jacobian column (position, axis rows) of that entry
   uint32_t *entry_off, *entries, *path_off, *path, *coord_body;
   uint32_t* pcol;  // per path entry: its coordinate entry (the column it uses)
+  // Dense form (cc_dense): J per (closure, coordinate), 6 rows each, and J^T J,
+  // entry (l, m) at m * NQ + l, both FP64.
+  double *dJ, *dA;
 };
 
 __device__ __forceinline__ CompLds carve_comp(float4* base, const Dev& d) {
@@ -802,6 +805,9 @@ __device__ __forceinline__ CompLds carve_comp(float4* base, const Dev& d) {
   c.path = c.path_off + 2 * NK + 1;
   c.coord_body = c.path + NP;
   c.pcol = c.coord_body + NQ;
+  const uintptr_t dense = (reinterpret_cast<uintptr_t>(c.pcol + NP) + 15u) & ~uintptr_t(15u);
+  c.dJ = reinterpret_cast<double*>(dense);
+  c.dA = c.dJ + 6 * NK * NQ;
   return c;
 }
 
@@ -1062,6 +1068,38 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
     }
     return op;
   };
+  // J^T J of the current columns, formed once per Newton step for the PCG of a
+  // one-wavefront component: J per (closure, coordinate) is the sum of the
+  // coordinate's side columns of that closure (in (closure, side) order), and
+  // entry (l, m) the 6-row dot of J's columns l and m summed over the closures.
+  // The matrix-free product and transpose apply the same operator; this one
+  // costs one row of nq products per coordinate an iteration.
+  const bool dense = kT == 64 && d.cc_dense != 0u;
+  const uint32_t NQ = d.cc_max_coords;
+  auto form_dense = [&]() {
+    for (uint32_t i = t; i < 6 * nk * NQ; i += T) L.dJ[i] = 0.0;
+    __syncthreads();
+    for (uint32_t l = t; l < nq; l += T)
+      for (uint32_t e = L.entry_off[l]; e < L.entry_off[l + 1]; ++e) {
+        const float* c = L.col + 6 * e;
+        double* j = L.dJ + 6 * ((L.entries[e] >> 1) * NQ + l);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) j[r] = j[r] + (double)c[r];
+      }
+    __syncthreads();
+    for (uint32_t l = t; l < nq; l += T)
+      for (uint32_t m = 0; m < nq; ++m) {
+        double a = 0.0;
+        for (uint32_t kk = 0; kk < nk; ++kk) {
+          const double* jl = L.dJ + 6 * (kk * NQ + l);
+          const double* jm = L.dJ + 6 * (kk * NQ + m);
+          a = a + ((((((jl[0] * jm[0]) + jl[1] * jm[1]) + jl[2] * jm[2]) + jl[3] * jm[3]) + jl[4] * jm[4]) +
+                   jl[5] * jm[5]);
+        }
+        L.dA[m * NQ + l] = a;
+      }
+    __syncthreads();
+  };
   // Jacobi-preconditioned CG on J^T J x = r from x = 0, with px, pr, pz, pp, pdg
   // set up by the caller and rz = r.z; stops at closure_iters iterations or when
   // r.z falls below closure_tol^2 of its start.
@@ -1071,12 +1109,22 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
     for (uint32_t it = 0; it < d.closure_iters; ++it) {
       if (d.closure_tol > 0.0f && rz <= rz_stop) break;
       auto direction_of = [&](uint32_t l) { return it == 0 ? L.pp[l] : L.pz[l] + beta * L.pp[l]; };
-      product(direction_of);
       double dad_part = 0.0;
-      for (uint32_t l = t; l < nq; l += T) {
-        const double op = transpose(l);
-        L.pop[l] = op;
-        dad_part += direction_of(l) * op;
+      if (dense) {
+        __syncthreads();  // the previous iteration's z and p, written per coordinate
+        for (uint32_t l = t; l < nq; l += T) {
+          double op = 0.0;
+          for (uint32_t m = 0; m < nq; ++m) op = op + L.dA[m * NQ + l] * direction_of(m);
+          L.pop[l] = op;
+          dad_part += direction_of(l) * op;
+        }
+      } else {
+        product(direction_of);
+        for (uint32_t l = t; l < nq; l += T) {
+          const double op = transpose(l);
+          L.pop[l] = op;
+          dad_part += direction_of(l) * op;
+        }
       }
       const double dad = group_sum<kT>(dad_part, scratch);
       const double alpha = fabs(dad) > 1.0e-20 ? rz / dad : 0.0;
@@ -1115,6 +1163,7 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
     if (evaluate()) break;
     joint_frames();
     columns();
+    if (dense) form_dense();
     // Gradient and Jacobi diagonal, gathered per coordinate in (closure, side) order.
     double rz_part = 0.0;
     for (uint32_t l = t; l < nq; l += T) {
@@ -1157,6 +1206,7 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
     }
     joint_frames();
     columns();
+    if (dense) form_dense();
     for (uint32_t l = t; l < nq; l += T) L.pz[l] = (double)d.coords[d.cc_coord[q0 + l]].y;
     __syncthreads();
     product([&](uint32_t l) { return L.pz[l]; });
@@ -1561,8 +1611,9 @@ size_t closure_component_lds(const Dev& d) {
   // doubles: 6 per coordinate, per closure and per path entry; floats: 7 per coordinate,
   // 19 per closure, 6 per path entry; u32: entry offsets, entries, path offsets, paths,
   // coordinate bodies, path-entry columns.
+  const size_t dense = d.cc_dense ? 16 + 8 * (6 * NK * NQ + NQ * NQ) : 0;
   return 16 * (10 * NB + 4 * NK) + 8 * (6 * NQ + 6 * NK + 6 * NP) + 4 * (7 * NQ + 19 * NK + 6 * NP) +
-         4 * ((NQ + 1) + NP + (2 * NK + 1) + NP + NQ + NP);
+         4 * ((NQ + 1) + NP + (2 * NK + 1) + NP + NQ + NP) + dense;
 }
 
 hipError_t prepare_closure_components(const Dev& d) {

@@ -505,6 +505,13 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
     d.cc_max_coords = f.cc_max_coords;
     d.cc_max_closures = f.cc_max_closures;
     d.cc_max_path = f.cc_max_path;
+    // One-wavefront components keep J^T J in LDS when it (with J per closure and
+    // coordinate) takes at most 48 KiB.
+    const uint32_t items = std::max(std::max(f.cc_max_bodies, f.cc_max_coords), 2u * f.cc_max_closures);
+    const size_t dense_bytes = 8 * (6 * (size_t)f.cc_max_closures * f.cc_max_coords +
+                                    (size_t)f.cc_max_coords * f.cc_max_coords);
+    d.cc_dense = items <= 64u && dense_bytes <= 48u * 1024u ? 1u : 0u;
+    if (d.cc_dense && closure_component_lds(d) > 160u * 1024u) d.cc_dense = 0u;  // the CU's LDS
     if (prepare_closure_components(d) != hipSuccess)
       return fail(error, MH_ERR_DEVICE, "closure component kernel: LDS attribute");
     uint32_t *off, *body, *body_parent, *body_coord, *coord, *coord_body, *entry_off, *entries, *closure, *end, *path_off,

@@ -41,7 +41,7 @@ def algorithmic_bytes(n, c, s, p, k, a, r, iterations):
         "lbvh": 24 * (c - 1) + 136 * c,
         "traverse": 40 * c + 72 * (2 * c - 1) + 8 * p,
         "narrowphase": 200 * p + 64 * k,
-        "prepare": 340 * k,
+        "prepare": 328 * k,
         "solve": iterations * (solve_kernel_bytes(n, a, r, iterations) + 24 * r + 112 * n),
     }
 
@@ -51,13 +51,14 @@ LAUNCHES = {"integrate": 1, "morton": 2, "lbvh": 2, "traverse": 1, "narrowphase"
 
 def solve_kernel_bytes(n, a, r, iterations):
     """Compulsory HBM bytes of one contact-solve pass kernel (k_solve), as the
-    mean over a tick's passes: per active contact its 64-byte solver row, its
+    mean over a tick's passes: per active contact its 52-byte solver row, its
     accumulated impulse read and written (4 + 4) and its sides' group word (4,
-    written by the first pass and read by the others); per delta slot the Q20
+    written by the first pass and read by the others), plus its contact index
+    (4, read by the last pass to persist the impulse); per delta slot the Q20
     velocity deltas of one body summed over a 64-contact group (24), plus the
-    first pass's slot counter increment (4, spread over the passes); per body its
-    64-byte solver state read once."""
-    return 76 * a + 24 * r + 64 * n + 4 * r / iterations
+    first pass's slot counter increment (4); per body its 64-byte solver state
+    read once. The once-a-tick terms are spread over the passes."""
+    return 64 * a + 24 * r + 64 * n + (4 * a + 4 * r) / iterations
 
 
 # HBM traffic of a launch from the L2's memory-side request counters, one

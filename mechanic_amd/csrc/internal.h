@@ -20,7 +20,6 @@ constexpr uint32_t kSideTile = 2048;  // bodies per tile of the contact-side off
 // accumulate through the atomic overflow path.
 constexpr uint32_t kMaxSlots = 32;      // contact-side delta slots per body (body-major)
 constexpr uint32_t kSlotOverflow = 31;  // slot code of a side past the body's slots (atomics instead)
-constexpr uint32_t kContactMask = 0x3fffffu;  // contact index bits of a solver row's word (2^22 pairs)
 constexpr uint32_t kSerialBearingLimit = 64u;    // device.rs:19: serial sweeps up to this many bearings
 constexpr uint32_t kSerialPassMultiplier = 12u;  // device.rs:20: their passes per solver iteration
 constexpr uint32_t kRadixTile = 1024;  // keys per radix-sort workgroup
@@ -117,7 +116,7 @@ struct Dev {
   // ---- contacts and solver ----
   float4* contacts;       // 4 per contact, mh_contact layout
   uint32_t* active;       // contact index per active contact
-  float4* rows;           // 4 planes of pair_capacity: plane k holds solver row word k of each active contact
+  float4* rows;           // solver rows: 3 planes of pair_capacity float4, then a u32 plane (SolverRow)
   float* lambda;          // accumulated normal impulse per active contact
   int4* cdelta;           // slot-major: 2 int4 per (slot, body) -- one body's Q20 deltas of a group of sides, last pass
   float4* sbody;          // 4 per body: solver record (velocities, inverse mass and inertia)

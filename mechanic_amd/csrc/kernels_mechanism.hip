@@ -1379,12 +1379,10 @@ __global__ __launch_bounds__(256) void k_serial_keys(Dev d) {
   const uint32_t n = read_count(&d.diag[tick & 1][kDiagActive], d.pair_capacity);
   const float4* mrows = d.mrows[tick & 1];
   for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < n; ai += gridDim.x * blockDim.x) {
-    const float4 r3 = row_word(d, 3, ai);
-    const uint32_t ci = __float_as_uint(r3.z) & kContactMask;
-    const float4 m0 = mrows[3 * (size_t)ci];
+    const uint2 bodies = row_body_pair(d, ai);
+    const float4 m0 = mrows[3 * (size_t)d.active[ai]];
     const uint32_t low = __float_as_uint(m0.x), high = __float_as_uint(m0.y);
-    d.serial_keys[2 * (size_t)ai] =
-        make_uint4(__float_as_uint(r3.x), __float_as_uint(r3.y), pair_hash(low, high) & d.hash_mask, low);
+    d.serial_keys[2 * (size_t)ai] = make_uint4(bodies.x, bodies.y, pair_hash(low, high) & d.hash_mask, low);
     d.serial_keys[2 * (size_t)ai + 1] = make_uint4(high, ai, 0u, 0u);
   }
 }
@@ -1439,11 +1437,11 @@ __device__ __forceinline__ void store_velocity(const Dev& d, uint32_t b, v3 lin,
 // full projected impulse (no relaxation) against the current velocities,
 // applied at once.
 __device__ void contact_immediate(const Dev& d, uint32_t ai) {
-  const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
-  const uint32_t a = __float_as_uint(r3.x), b = __float_as_uint(r3.y);
+  const SolverRow sr = load_row(d, ai);
+  const uint32_t a = sr.body_a, b = sr.body_b;
   const bool has_b = b != kInvalid;
-  const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
-  const float penetration = r0.w, denominator = r3.w;
+  const v3 normal = sr.normal, arm_a = sr.arm_a, arm_b = sr.arm_b;
+  const float penetration = sr.penetration, denominator = sr.denominator;
   const SolverBody sa = load_body(d.sbody, a);
   const SolverBody sb = has_b ? load_body(d.sbody, b) : sa;
   const v3 velocity_b = has_b ? sb.lin + cross(sb.ang, arm_b) : mk(0.0f, 0.0f, 0.0f);
@@ -1456,7 +1454,7 @@ __device__ void contact_immediate(const Dev& d, uint32_t ai) {
     return;
   }
   const float previous = fmx(lambda, 0.0f);
-  const float recovery = r2.w < 0.0f ? 1.0f : 0.2f;
+  const float recovery = sr.analytic ? 1.0f : 0.2f;
   const float bias = fmn((fmx(penetration - kSlop, 0.0f) * recovery) / d.delta_seconds, kMaxCorrection);
   const float accumulated = fmx(previous + (-normal_speed + bias) / denominator, 0.0f);
   const float delta = accumulated - previous;
@@ -1552,8 +1550,7 @@ __global__ __launch_bounds__(256) void k_serial_finish(Dev d) {
   float4* rows_now = d.mrows[tick & 1];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < n; ai += stride) {
-    const uint32_t ci = __float_as_uint(row_word(d, 3, ai).z) & kContactMask;
-    rows_now[3 * (size_t)ci + 2].w = d.lambda[ai];
+    rows_now[3 * (size_t)d.active[ai] + 2].w = d.lambda[ai];
   }
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < d.body_count; b += stride) {
     d.lin[b] = f4(xyz(d.sbody[4 * (size_t)b]), 0.0f);

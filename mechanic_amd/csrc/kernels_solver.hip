@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
   for (uint32_t base = xs.first; base < xs.end; base += xs.step) {
     const uint32_t ci = base + threadIdx.x;
     bool active = false;
-    float4 r0, r1, r2, r3;
+    SolverRow row;
     float cached_lambda = 0.0f;
     if (ci < xs.end) {
       const float4* c = d.contacts + 4 * (size_t)ci;
@@ -138,20 +138,23 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
         mr[2] = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, cached_impulse);
         if (n_pen.w > 1.0e-5f || normal_speed < -1.0e-5f || cached_impulse > 1.0e-6f) {
           active = true;
-          r0 = n_pen;
-          r1 = make_float4(arm_a4.x, arm_a4.y, arm_a4.z, hit ? 1.0f : 0.0f);
+          row.normal = normal;
+          row.penetration = n_pen.w;
+          row.arm_a = arm_a;
+          row.denominator = denom_of(sa, sb, has_b, arm_a, arm_b, normal);
+          row.arm_b = arm_b;
+          row.body_a = body_a;
+          row.body_b = body_b;
+          row.analytic = arm_b4.w < 0.0f;  // generate_ground_contacts' analytic cylinder marker
+          row.hit = hit;
           cached_lambda = cached_impulse;
-          r2 = arm_b4;
-          float denom = denom_of(sa, sb, has_b, arm_a, arm_b, normal);
-          r3 = make_float4(m.x, m.y, __uint_as_float(ci), denom);
         }
       }
     }
   done:
     uint32_t a = block_append(&diag[kDiagActive], active, lds_append);
     if (active) {
-      const uint32_t ci = __float_as_uint(r3.z);
-      const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+      const uint32_t body_a = row.body_a, body_b = row.body_b;
       // Per-body passes: both contact sides take a slot of their body, their row.
       // (The per-contact passes take slots per group of sides in their first pass.)
       if (per_body) {
@@ -161,10 +164,7 @@ __global__ __launch_bounds__(256) void k_prepare(Dev d) {
       }
       d.active[a] = ci;
       d.lambda[a] = cached_lambda;
-      set_row_word(d, 0, a, r0);
-      set_row_word(d, 1, a, r1);
-      set_row_word(d, 2, a, r2);
-      set_row_word(d, 3, a, make_float4(r3.x, r3.y, __uint_as_float(ci), r3.w));
+      store_row(d, a, row);
     }
   }
 }
@@ -346,21 +346,20 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint6
     if constexpr (kGroup) {
       // The tick's first pass groups the sides (the later passes read the words).
       if (valid) {
-        const float4 r3 = row_word(d, 3, ai);
-        body_a = __float_as_uint(r3.x);
-        body_b = __float_as_uint(r3.y);
+        const uint2 bodies = row_body_pair(d, ai);
+        body_a = bodies.x;
+        body_b = bodies.y;
       }
       grp = group_word(d, lds_body[wave], lds_first[wave], lane, valid, body_a, body_b);
       if (valid) d.side_group[ai] = grp;
     }
     if (valid) {
-      const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
-      body_a = __float_as_uint(r3.x);
-      body_b = __float_as_uint(r3.y);
+      const SolverRow sr = load_row(d, ai);
+      body_a = sr.body_a;
+      body_b = sr.body_b;
       if constexpr (!kGroup) grp = d.side_group[ai];
-      const uint32_t word = __float_as_uint(r3.z);
       const bool has_b = body_b != kInvalid;
-      const ContactRow row{xyz(r0), xyz(r1), xyz(r2), r0.w, r3.w, r2.w < 0.0f};
+      const ContactRow row{sr.normal, sr.arm_a, sr.arm_b, sr.penetration, sr.denominator, sr.analytic};
       float lambda = d.lambda[ai];
       const SolverBody sa = load_body(d.sbody, body_a);
       const SolverBody sb = has_b ? load_body(d.sbody, body_b) : sa;
@@ -381,7 +380,7 @@ __global__ __launch_bounds__(256) void k_solve(Dev d, int first, int last, uint6
           for (int c = 0; c < 6; ++c) atomicAdd(&q[rb][c], qb.v[c]);
         }
       }
-      if (last) rows_now[3 * (size_t)(word & kContactMask) + 2].w = lambda;
+      if (last) rows_now[3 * (size_t)d.active[ai] + 2].w = lambda;
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -506,24 +505,25 @@ __global__ __launch_bounds__(256) void k_scatter_sides(Dev d) {
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
   const XcdSplit xs = xcd_split(count);
   for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
-    const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
-    const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
-    const uint32_t ci = __float_as_uint(r3.z) & kContactMask;
+    const SolverRow sr = load_row(d, ai);
+    const uint32_t body_a = sr.body_a, body_b = sr.body_b;
+    const uint32_t ci = d.active[ai];
     const uint2 slots = d.side_slot[ai];
     const float lambda = d.lambda[ai];
-    const float4 w1 = make_float4(r1.x, r1.y, r1.z, r3.w);
+    const float4 w0 = f4(sr.normal, sr.penetration), w1 = f4(sr.arm_a, sr.denominator);
+    const float4 w2 = f4(sr.arm_b, sr.analytic ? -1.0f : 0.0f);
     float4* ra = d.srow + 4 * (size_t)(d.side_off[body_a] + slots.x);
-    ra[0] = r0;
+    ra[0] = w0;
     ra[1] = w1;
-    ra[2] = r2;
-    ra[3] = make_float4(r3.y, __uint_as_float(ai), __uint_as_float(ci), __uint_as_float(0u));
+    ra[2] = w2;
+    ra[3] = make_float4(__uint_as_float(body_b), __uint_as_float(ai), __uint_as_float(ci), __uint_as_float(0u));
     d.slam[d.side_off[body_a] + slots.x] = lambda;
     if (body_b != kInvalid) {
       float4* rb = d.srow + 4 * (size_t)(d.side_off[body_b] + slots.y);
-      rb[0] = r0;
+      rb[0] = w0;
       rb[1] = w1;
-      rb[2] = r2;
-      rb[3] = make_float4(r3.x, __uint_as_float(ai), __uint_as_float(ci), __uint_as_float(1u));
+      rb[2] = w2;
+      rb[3] = make_float4(__uint_as_float(body_a), __uint_as_float(ai), __uint_as_float(ci), __uint_as_float(1u));
       d.slam[d.side_off[body_b] + slots.y] = lambda;
     }
   }
@@ -743,25 +743,26 @@ __global__ __launch_bounds__(256) void k_apply(Dev d, int last, uint64_t* clock,
 // matched, and whether the contact is against the ground. Also returns the
 // tangent basis.
 __device__ __forceinline__ void contact_features(const Dev& d, uint32_t ai, float f[kFeatures], v3& e1, v3& e2) {
-  const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
-  const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+  const SolverRow sr = load_row(d, ai);
+  const uint32_t body_a = sr.body_a, body_b = sr.body_b;
   const bool has_b = body_b != kInvalid;
-  const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
+  const v3 normal = sr.normal, arm_a = sr.arm_a, arm_b = sr.arm_b;
   const SolverBody sa = load_body(d, body_a);
   const SolverBody sb = has_b ? load_body(d, body_b) : sa;
   const v3 vb = has_b ? sb.lin + cross(sb.ang, arm_b) : mk(0.0f, 0.0f, 0.0f);
   const v3 relative = vb - (sa.lin + cross(sa.ang, arm_a));
   const float normal_speed = dot(relative, normal);
   const float tangent_speed = length(relative - normal * normal_speed);
-  const bool analytic = r2.w < 0.0f;
-  const float bias = fmn((fmx(r0.w - kSlop, 0.0f) * (analytic ? 1.0f : 0.2f)) / d.delta_seconds, kMaxCorrection);
+  const bool analytic = sr.analytic;
+  const float bias =
+      fmn((fmx(sr.penetration - kSlop, 0.0f) * (analytic ? 1.0f : 0.2f)) / d.delta_seconds, kMaxCorrection);
   float t1[3], t2[3];
   tangent_basis(normal.x, normal.y, normal.z, t1, t2);
   e1 = mk(t1[0], t1[1], t1[2]);
   e2 = mk(t2[0], t2[1], t2[2]);
   f[0] = normal_speed;
   f[1] = tangent_speed;
-  f[2] = r0.w;
+  f[2] = sr.penetration;
   f[3] = bias;
   f[4] = sa.inv_mass;
   f[5] = has_b ? sb.inv_mass : 0.0f;
@@ -771,9 +772,9 @@ __device__ __forceinline__ void contact_features(const Dev& d, uint32_t ai, floa
   f[9] = has_b ? dot(inv_inertia(sb, normal), normal) : 0.0f;
   f[10] = has_b ? dot(inv_inertia(sb, e1), e1) : 0.0f;
   f[11] = has_b ? dot(inv_inertia(sb, e2), e2) : 0.0f;
-  f[12] = r3.w;
+  f[12] = sr.denominator;
   f[13] = d.lambda[ai];
-  f[14] = r1.w;
+  f[14] = sr.hit ? 1.0f : 0.0f;
   f[15] = has_b ? 0.0f : 1.0f;
 }
 
@@ -802,7 +803,7 @@ __global__ __launch_bounds__(256) void k_targets(Dev d) {
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
   const XcdSplit xs = xcd_split(count);
   for (uint32_t ai = xs.first + threadIdx.x; ai < xs.end; ai += xs.step) {
-    const float4 n = row_word(d, 0, ai);
+    const float4 n = d.rows[ai];  // plane 0: the normal
     float t1[3], t2[3];
     tangent_basis(n.x, n.y, n.z, t1, t2);
     const v3 f = xyz(d.fsum[ai]);
@@ -882,28 +883,28 @@ __global__ __launch_bounds__(256) void k_residual(Dev d, const uint32_t* gate) {
   const uint32_t count = read_count(&diag[kDiagActive], d.pair_capacity);
   double sum_r2 = 0.0, n = 0.0, sum_pen = 0.0, max_pen = 0.0;
   for (uint32_t ai = blockIdx.x * blockDim.x + threadIdx.x; ai < count; ai += gridDim.x * blockDim.x) {
-    const float4 r0 = row_word(d, 0, ai), r1 = row_word(d, 1, ai), r2 = row_word(d, 2, ai), r3 = row_word(d, 3, ai);
-    const uint32_t body_a = __float_as_uint(r3.x), body_b = __float_as_uint(r3.y);
+    const SolverRow sr = load_row(d, ai);
+    const uint32_t body_a = sr.body_a, body_b = sr.body_b;
     const bool has_b = body_b != kInvalid;
-    const v3 normal = xyz(r0), arm_a = xyz(r1), arm_b = xyz(r2);
+    const v3 normal = sr.normal, arm_a = sr.arm_a, arm_b = sr.arm_b;
     const SolverBody sa = load_body(d, body_a);
     const v3 vb = has_b ? [&] { const SolverBody sb = load_body(d, body_b); return sb.lin + cross(sb.ang, arm_b); }()
                         : mk(0.0f, 0.0f, 0.0f);
     const float normal_speed = dot(vb - (sa.lin + cross(sa.ang, arm_a)), normal);
-    const bool analytic = r2.w < 0.0f;
-    const double bias = (double)fmn((fmx(r0.w - kSlop, 0.0f) * (analytic ? 1.0f : 0.2f)) / d.delta_seconds,
+    const bool analytic = sr.analytic;
+    const double bias = (double)fmn((fmx(sr.penetration - kSlop, 0.0f) * (analytic ? 1.0f : 0.2f)) / d.delta_seconds,
                                     kMaxCorrection);
     const double lambda = (double)d.lambda[ai];
     double r = 0.0;
-    if (r3.w > 0.0f) {
-      double next = lambda + (bias - (double)normal_speed) / (double)r3.w;
+    if (sr.denominator > 0.0f) {
+      double next = lambda + (bias - (double)normal_speed) / (double)sr.denominator;
       if (next < 0.0) next = 0.0;
       r = next - lambda;
     }
     sum_r2 += r * r;
     n += 1.0;
-    sum_pen += (double)r0.w;
-    max_pen = (double)r0.w > max_pen ? (double)r0.w : max_pen;
+    sum_pen += (double)sr.penetration;
+    max_pen = (double)sr.penetration > max_pen ? (double)sr.penetration : max_pen;
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {

@@ -90,13 +90,52 @@ __device__ __forceinline__ float denom_of(const SolverBody& a, const SolverBody&
   return result;
 }
 
-// Solver rows as four planes (structure of arrays): a wavefront's load of one
-// word is 1 KiB of consecutive rows.
-__device__ __forceinline__ float4 row_word(const Dev& d, uint32_t k, uint32_t ai) {
-  return d.rows[(size_t)k * d.pair_capacity + ai];
+// An active contact's solver row, stored as three 16-byte planes and one 4-byte
+// plane (structure of arrays: a wavefront's load of one plane is consecutive
+// rows, 1 KiB for a 16-byte plane):
+//   plane 0: normal, penetration
+//   plane 1: arm from body A, normal denominator
+//   plane 2: arm from body B, body B
+//   plane 3: body A | analytic ground contact << 30 | cache hit << 31
+// 52 bytes a contact. The contact's index in the contact list is d.active.
+struct SolverRow {
+  v3 normal, arm_a, arm_b;
+  float penetration, denominator;
+  uint32_t body_a, body_b;
+  bool analytic, hit;
+};
+constexpr uint32_t kRowAnalytic = 1u << 30, kRowHit = 1u << 31, kRowBody = kRowAnalytic - 1u;
+
+__device__ __forceinline__ const uint32_t* row_bodies(const Dev& d) {
+  return reinterpret_cast<const uint32_t*>(d.rows + 3 * (size_t)d.pair_capacity);
 }
-__device__ __forceinline__ void set_row_word(const Dev& d, uint32_t k, uint32_t ai, float4 v) {
-  d.rows[(size_t)k * d.pair_capacity + ai] = v;
+// Body A and body B of a row (the grouping and sort keys need no more).
+__device__ __forceinline__ uint2 row_body_pair(const Dev& d, uint32_t ai) {
+  return make_uint2(row_bodies(d)[ai] & kRowBody, __float_as_uint(d.rows[2 * (size_t)d.pair_capacity + ai].w));
+}
+__device__ __forceinline__ SolverRow load_row(const Dev& d, uint32_t ai) {
+  const size_t cap = d.pair_capacity;
+  const float4 p0 = d.rows[ai], p1 = d.rows[cap + ai], p2 = d.rows[2 * cap + ai];
+  const uint32_t w = row_bodies(d)[ai];
+  SolverRow r;
+  r.normal = xyz(p0);
+  r.penetration = p0.w;
+  r.arm_a = xyz(p1);
+  r.denominator = p1.w;
+  r.arm_b = xyz(p2);
+  r.body_b = __float_as_uint(p2.w);
+  r.body_a = w & kRowBody;
+  r.analytic = (w & kRowAnalytic) != 0u;
+  r.hit = (w & kRowHit) != 0u;
+  return r;
+}
+__device__ __forceinline__ void store_row(const Dev& d, uint32_t ai, const SolverRow& r) {
+  const size_t cap = d.pair_capacity;
+  d.rows[ai] = f4(r.normal, r.penetration);
+  d.rows[cap + ai] = f4(r.arm_a, r.denominator);
+  d.rows[2 * cap + ai] = f4(r.arm_b, __uint_as_float(r.body_b));
+  reinterpret_cast<uint32_t*>(d.rows + 3 * cap)[ai] =
+      r.body_a | (r.analytic ? kRowAnalytic : 0u) | (r.hit ? kRowHit : 0u);
 }
 
 }  // namespace mh

@@ -103,6 +103,11 @@ def traffic_probe(args):
         json.dump({"bodies": scene.body_count, "ticks": counts}, f)
 
 
+def progress(message):
+    """One line on stderr per finished phase (stdout carries only the JSON line)."""
+    print(f"bench: {message}", file=sys.stderr, flush=True)
+
+
 def measure_traffic(args, tuning, iterations, state):
     """HBM bytes per k_solve launch from two rocprofv3 --pmc runs (child
     processes) continuing the settled state: the read requests by size, then
@@ -470,6 +475,7 @@ def main():
     else:
         tuned = {"tuning": dict(autotune.DEFAULTS)}
     tune_s = time.perf_counter() - t_tune
+    progress(f"launch shape chosen ({args.tuning}, {tune_s:.1f} s)")
     default_value = timed_run(make, steps, settle=SETTLE_TICKS)[0] if args.tuning == "auto" else None
 
     # The headline: ticks SETTLE_TICKS+1 .. SETTLE_TICKS+steps of one handle.
@@ -548,13 +554,19 @@ def main():
     fall_value, _ = timed_run(make, steps, tuned["tuning"])
     run600_value, run600_rb = timed_run(make, 600, tuned["tuning"])
 
+    progress(f"headline {value:.1f} steps/s; falling and 600-step windows done")
     learned = None if args.no_learned else learned_warm_start(mh, scene, friction, iterations, steps, value,
                                                                   tuned["tuning"], use_graph)
+    progress("learned warm start done")
     batch = None if args.no_batched else batched(mh, steps)
+    progress("batched handles done")
     art = None if args.no_articulated else articulated(mh, min(steps, 300), use_graph)
+    progress("articulated pile done")
     cpu = None if args.no_cpu else cpu_baseline(scene, friction, iterations, state_path)
+    progress("cpu baseline done")
     traffic, traffic_note = (None, "skipped (--no-traffic)") if args.no_traffic else \
         measure_traffic(args, tuned["tuning"], iterations, state_path)
+    progress("traffic counters done")
     os.remove(state_path)
 
     out = {

@@ -473,6 +473,32 @@ __device__ __forceinline__ double group_sum(double v, double* scratch) {
   }
 }
 
+// One FP64 word of the lane `kCtrl` names, by DPP (both halves).
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), kCtrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over a whole wavefront (every lane active), the same bits in every lane:
+// within each row of 16 lanes, four DPP rotations by 8, 4, 2 and 1 (each adds a
+// pair that both of its lanes form, so the row's lanes agree at every step),
+// then the four row sums as (row 0 + row 1) + (row 2 + row 3).
+__device__ __forceinline__ double wave_sum_rows(double v) {
+  v = v + dpp_f64<0x128>(v);  // row_ror:8
+  v = v + dpp_f64<0x124>(v);  // row_ror:4
+  v = v + dpp_f64<0x122>(v);  // row_ror:2
+  v = v + dpp_f64<0x121>(v);  // row_ror:1
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
 struct Endpoint {
   v3 anchor, axis;
 };
@@ -1103,28 +1129,81 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
   // Jacobi-preconditioned CG on J^T J x = r from x = 0, with px, pr, pz, pp, pdg
   // set up by the caller and rz = r.z; stops at closure_iters iterations or when
   // r.z falls below closure_tol^2 of its start.
+  // With the dense J^T J (one wavefront, nq <= 64) lane l keeps coordinate l's
+  // vectors in registers; only the direction is shared, through LDS, for the
+  // product, whose row sums run in coordinate order as in the general form.
+  auto pcg_wave = [&](double rz) {
+    const bool own = t < nq;
+    double x = 0.0, r = 0.0, z = 0.0, p = 0.0, dg = 0.0;
+    if (own) {
+      r = L.pr[t];
+      z = L.pz[t];
+      p = L.pp[t];
+      dg = L.pdg[t];
+    }
+    const double rz_stop = (double)d.closure_tol * (double)d.closure_tol * rz;
+    double beta = 0.0;
+    double* shared_dir = L.pop;
+    const double* row = L.dA + t;
+    for (uint32_t it = 0; it < d.closure_iters; ++it) {
+      if (d.closure_tol > 0.0f && rz <= rz_stop) break;
+      const double dir = it == 0 ? p : z + beta * p;
+      if (own) shared_dir[t] = dir;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      double op = 0.0;
+      if (own) {
+        uint32_t m = 0;
+        for (; m + 4u <= nq; m += 4u) {
+          const double a0 = row[m * NQ], a1 = row[(m + 1u) * NQ], a2 = row[(m + 2u) * NQ], a3 = row[(m + 3u) * NQ];
+          const double d0 = shared_dir[m], d1 = shared_dir[m + 1u], d2 = shared_dir[m + 2u], d3 = shared_dir[m + 3u];
+          op = op + a0 * d0;
+          op = op + a1 * d1;
+          op = op + a2 * d2;
+          op = op + a3 * d3;
+        }
+        for (; m < nq; ++m) op = op + row[m * NQ] * shared_dir[m];
+      }
+      double dad_part = 0.0;
+      if (own) dad_part += dir * op;
+      const double dad = wave_sum_rows(dad_part);
+      const double alpha = fabs(dad) > 1.0e-20 ? rz / dad : 0.0;
+      double rz_next_part = 0.0;
+      if (own) {
+        x = x + alpha * dir;
+        r = r - alpha * op;
+        z = dg > (double)1.0e-12f ? r / dg : 0.0;
+        p = dir;
+        rz_next_part += r * z;
+      }
+      const double rz_next = wave_sum_rows(rz_next_part);
+      beta = fabs(rz) > 1.0e-20 ? rz_next / rz : 0.0;
+      rz = rz_next;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();  // every lane has read the direction before the next is stored
+    }
+    if (own) L.px[t] = x;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
   auto pcg = [&](double rz) {
+    if constexpr (kT == 64) {
+      if (dense) {
+        pcg_wave(rz);
+        return;
+      }
+    }
     const double rz_stop = (double)d.closure_tol * (double)d.closure_tol * rz;
     double beta = 0.0;
     for (uint32_t it = 0; it < d.closure_iters; ++it) {
       if (d.closure_tol > 0.0f && rz <= rz_stop) break;
       auto direction_of = [&](uint32_t l) { return it == 0 ? L.pp[l] : L.pz[l] + beta * L.pp[l]; };
       double dad_part = 0.0;
-      if (dense) {
-        __syncthreads();  // the previous iteration's z and p, written per coordinate
-        for (uint32_t l = t; l < nq; l += T) {
-          double op = 0.0;
-          for (uint32_t m = 0; m < nq; ++m) op = op + L.dA[m * NQ + l] * direction_of(m);
-          L.pop[l] = op;
-          dad_part += direction_of(l) * op;
-        }
-      } else {
-        product(direction_of);
-        for (uint32_t l = t; l < nq; l += T) {
-          const double op = transpose(l);
-          L.pop[l] = op;
-          dad_part += direction_of(l) * op;
-        }
+      product(direction_of);
+      for (uint32_t l = t; l < nq; l += T) {
+        const double op = transpose(l);
+        L.pop[l] = op;
+        dad_part += direction_of(l) * op;
       }
       const double dad = group_sum<kT>(dad_part, scratch);
       const double alpha = fabs(dad) > 1.0e-20 ? rz / dad : 0.0;

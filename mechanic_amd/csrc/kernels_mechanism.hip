@@ -831,8 +831,10 @@ __device__ __forceinline__ CompLds carve_comp(float4* base, const Dev& d) {
   c.path = c.path_off + 2 * NK + 1;
   c.coord_body = c.path + NP;
   c.pcol = c.coord_body + NQ;
-  const uintptr_t dense = (reinterpret_cast<uintptr_t>(c.pcol + NP) + 15u) & ~uintptr_t(15u);
-  c.dJ = reinterpret_cast<double*>(dense);
+  // 16-byte aligned past the lists (base is), as an offset from base so the
+  // compiler still sees an LDS pointer and emits LDS instructions for it.
+  const size_t used = reinterpret_cast<const char*>(c.pcol + NP) - reinterpret_cast<const char*>(base);
+  c.dJ = reinterpret_cast<double*>(reinterpret_cast<char*>(base) + ((used + 15u) & ~size_t(15u)));
   c.dA = c.dJ + 6 * NK * NQ;
   return c;
 }
@@ -1113,17 +1115,24 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
         for (int r = 0; r < 6; ++r) j[r] = j[r] + (double)c[r];
       }
     __syncthreads();
-    for (uint32_t l = t; l < nq; l += T)
-      for (uint32_t m = 0; m < nq; ++m) {
-        double a = 0.0;
-        for (uint32_t kk = 0; kk < nk; ++kk) {
-          const double* jl = L.dJ + 6 * (kk * NQ + l);
-          const double* jm = L.dJ + 6 * (kk * NQ + m);
-          a = a + ((((((jl[0] * jm[0]) + jl[1] * jm[1]) + jl[2] * jm[2]) + jl[3] * jm[3]) + jl[4] * jm[4]) +
-                   jl[5] * jm[5]);
-        }
-        L.dA[m * NQ + l] = a;
+    // One entry of the lower triangle per thread and step, stored at (l, m) and
+    // (m, l): the products commute, so the two entries are the same bits.
+    const uint32_t pairs = nq * (nq + 1u) / 2u;
+    for (uint32_t pr = t; pr < pairs; pr += T) {
+      uint32_t l = (uint32_t)((sqrtf(8.0f * (float)pr + 1.0f) - 1.0f) * 0.5f);  // pr = l (l + 1) / 2 + m, m <= l
+      while (l * (l + 1u) / 2u > pr) --l;
+      while ((l + 1u) * (l + 2u) / 2u <= pr) ++l;
+      const uint32_t m = pr - l * (l + 1u) / 2u;
+      double a = 0.0;
+      for (uint32_t kk = 0; kk < nk; ++kk) {
+        const double* jl = L.dJ + 6 * (kk * NQ + l);
+        const double* jm = L.dJ + 6 * (kk * NQ + m);
+        a = a + ((((((jl[0] * jm[0]) + jl[1] * jm[1]) + jl[2] * jm[2]) + jl[3] * jm[3]) + jl[4] * jm[4]) +
+                 jl[5] * jm[5]);
       }
+      L.dA[m * NQ + l] = a;
+      L.dA[l * NQ + m] = a;
+    }
     __syncthreads();
   };
   // Jacobi-preconditioned CG on J^T J x = r from x = 0, with px, pr, pz, pp, pdg

@@ -388,8 +388,12 @@ constexpr uint16_t kNoParent = 0xffffu;
 constexpr uint16_t kLocalLeaf = 0x8000u;  // local child word: a leaf, else an internal node
 
 // Sorted keys, read from the LDS window when the position lies in it.
+// The window as an LDS pointer: with a generic one the compiler folds the two
+// loads of at() into one flat load from a selected address.
+typedef const __attribute__((address_space(3))) uint64_t* LdsKeys;
+
 struct KeyView {
-  const uint64_t* lds;
+  LdsKeys lds;
   const uint64_t* keys;
   int base, c;
   __device__ __forceinline__ uint64_t at(int j) const {
@@ -506,7 +510,7 @@ __global__ __launch_bounds__(256) void k_build(Dev d) {
   const int first_pos = (int)(blockIdx.x * kBlock);
   const int i = first_pos + (int)t;  // this thread's leaf and internal node
   const int block_last = min(first_pos + (int)kBlock, c) - 1;
-  const KeyView kv{s.keys, d.keys, first_pos - (int)kBlock, c};
+  const KeyView kv{(LdsKeys)s.keys, d.keys, first_pos - (int)kBlock, c};
   for (int w = (int)t; w < kKeyWindow; w += (int)kBlock) {
     const int j = kv.base + w;
     if (j >= 0 && j < c) s.keys[w] = d.keys[j];

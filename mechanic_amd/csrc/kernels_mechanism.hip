@@ -157,33 +157,53 @@ __device__ __forceinline__ void angular_axis(const Dev& d, uint32_t a, uint32_t 
   qsum_add(qb, mk(0.0f, 0.0f, 0.0f), (-inverse_b) * impulse);
 }
 
-// project_bearing_velocities (articulated.wgsl:281-305).
+// project_bearing_velocities (articulated.wgsl:281-305). The five axes of a
+// bearing row all start from the velocities before it (none sees another's
+// impulse), so lanes 0..4 of a group of kBearingLanes take one axis each; their
+// rounded Q20 integers are summed across the group with wrapping adds (the
+// arithmetic of the atomics, in any order) before lane 0's atomics.
+constexpr uint32_t kBearingLanes = 8;
 __global__ __launch_bounds__(256) void k_bearing_project(Dev d, int in_contact) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.bearing_count) return;
-  const BearingRow r = load_bearing(d, i);
-  const float4 rot_a = d.rot[r.a], rot_b = d.rot[r.b];
-  const v3 arm_a = quat_rotate(rot_a, r.anchor_a);
-  const v3 arm_b = quat_rotate(rot_b, r.anchor_b);
-  const v3 ang_a = ang_of(d, r.a, in_contact), ang_b = ang_of(d, r.b, in_contact);
-  const v3 velocity_a = lin_of(d, r.a, in_contact) + cross(ang_a, arm_a);
-  const v3 velocity_b = lin_of(d, r.b, in_contact) + cross(ang_b, arm_b);
-  const v3 relative = velocity_b - velocity_a;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = gid / kBearingLanes, part = gid % kBearingLanes;
+  const bool valid = i < d.bearing_count;
   QSum qa = {{0, 0, 0, 0, 0, 0}}, qb = {{0, 0, 0, 0, 0, 0}};
-  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(1.0f, 0.0f, 0.0f), qa, qb);
-  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 1.0f, 0.0f), qa, qb);
-  linear_axis(d, r.a, r.b, arm_a, arm_b, relative, mk(0.0f, 0.0f, 1.0f), qa, qb);
-  const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
-  const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
-  const v3 hinge = normalize_v(axis_a + axis_b);
-  const v3 helper = fabsf(hinge.x) > 0.8f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
-  const v3 tangent_a = normalize_v(cross(hinge, helper));
-  const v3 tangent_b = cross(hinge, tangent_a);
-  const v3 relative_angular = ang_b - ang_a;
-  angular_axis(d, r.a, r.b, relative_angular, tangent_a, qa, qb);
-  angular_axis(d, r.a, r.b, relative_angular, tangent_b, qa, qb);
-  qsum_flush(d, r.a, qa);
-  qsum_flush(d, r.b, qb);
+  uint32_t body_a = 0, body_b = 0;
+  if (valid && part < 5u) {
+    const BearingRow r = load_bearing(d, i);
+    body_a = r.a;
+    body_b = r.b;
+    const float4 rot_a = d.rot[r.a], rot_b = d.rot[r.b];
+    const v3 ang_a = ang_of(d, r.a, in_contact), ang_b = ang_of(d, r.b, in_contact);
+    if (part < 3u) {
+      const v3 arm_a = quat_rotate(rot_a, r.anchor_a);
+      const v3 arm_b = quat_rotate(rot_b, r.anchor_b);
+      const v3 velocity_a = lin_of(d, r.a, in_contact) + cross(ang_a, arm_a);
+      const v3 velocity_b = lin_of(d, r.b, in_contact) + cross(ang_b, arm_b);
+      const v3 relative = velocity_b - velocity_a;
+      const v3 direction = part == 0u ? mk(1.0f, 0.0f, 0.0f) : part == 1u ? mk(0.0f, 1.0f, 0.0f) : mk(0.0f, 0.0f, 1.0f);
+      linear_axis(d, r.a, r.b, arm_a, arm_b, relative, direction, qa, qb);
+    } else {
+      const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
+      const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
+      const v3 hinge = normalize_v(axis_a + axis_b);
+      const v3 helper = fabsf(hinge.x) > 0.8f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+      const v3 tangent_a = normalize_v(cross(hinge, helper));
+      const v3 tangent = part == 3u ? tangent_a : cross(hinge, tangent_a);
+      angular_axis(d, r.a, r.b, ang_b - ang_a, tangent, qa, qb);
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < (int)kBearingLanes; m <<= 1)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      qa.v[k] = (int32_t)((uint32_t)qa.v[k] + (uint32_t)__shfl_xor(qa.v[k], m));
+      qb.v[k] = (int32_t)((uint32_t)qb.v[k] + (uint32_t)__shfl_xor(qb.v[k], m));
+    }
+  if (valid && part == 0u) {
+    qsum_flush(d, body_a, qa);
+    qsum_flush(d, body_b, qb);
+  }
 }
 
 // apply_velocity_deltas (articulated.wgsl:307-326) over the bodies some bearing
@@ -1650,7 +1670,7 @@ __global__ __launch_bounds__(256) void k_serial_finish(Dev d) {
 
 void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back) {
   if (d.bearing_count == 0) return;
-  k_bearing_project<<<blocks_for(d.bearing_count), 256, 0, s>>>(d, in_contact);
+  k_bearing_project<<<blocks_for(d.bearing_count * kBearingLanes), 256, 0, s>>>(d, in_contact);
   k_bearing_apply<<<blocks_for(d.bearing_body_count), 256, 0, s>>>(d, in_contact, write_back);
 }
 

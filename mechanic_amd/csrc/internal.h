@@ -195,6 +195,8 @@ struct Dev {
   const uint32_t* fk_list;    // non-root mechanism bodies grouped by tree
   const uint32_t* fk_offset;  // fk_groups + 1 offsets into fk_list: whole trees per forward-kinematics workgroup
   uint32_t fk_groups;
+  const uint32_t* fk_meta;    // per fk_list entry: depth << 16 | its parent's index in the group (0xffff: a root)
+  uint32_t fk_small;          // every group holds at most 256 bodies (publication per group)
   uint32_t* mech_state;       // [0]: this tick's closure Newton loop has converged
   const uint32_t* blist;      // bodies referenced by bearings
   const uint32_t* closure_rows;  // bearing row of each closure

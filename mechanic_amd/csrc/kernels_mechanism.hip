@@ -1348,6 +1348,73 @@ __global__ __launch_bounds__(kPoseThreads) void k_publish(Dev d) {
 
 __global__ __launch_bounds__(kPoseThreads) void k_reconstruct(Dev d) { reconstruct_levels(d); }
 
+// k_publish per forward-kinematics group (whole trees, at most 256 bodies, one
+// per thread): the same per-body arithmetic as publish and reconstruct_body,
+// with the group's published rotations and reconstructed velocities kept in
+// LDS for the children that read them, so a depth level costs no round trip
+// to memory. A parent outside the group is a root, read from the state arrays.
+__global__ __launch_bounds__(256) void k_publish_groups(Dev d) {
+  __shared__ float4 s_rot[256], s_lin[256], s_ang[256];
+  const uint32_t begin = d.fk_offset[blockIdx.x], n = d.fk_offset[blockIdx.x + 1] - begin;
+  const uint32_t t = threadIdx.x;
+  const bool own = t < n;
+  uint32_t body = 0, meta = 0;
+  MechBody m{};
+  BearingRow r{};
+  if (own) {
+    body = d.fk_list[begin + t];
+    meta = d.fk_meta[begin + t];
+    m = load_mbody(d, body);
+    r = load_bearing(d, m.tree_bearing);
+    // publish_a / publish_b (mechanism.wgsl:149-185).
+    const Link l = load_link(d.links[d.jump_rounds & 1u], body);
+    const uint32_t root = l.ancestor;
+    float4 rot = d.rot[body];
+    if (body != root) {
+      const float4 root_rotation = d.rot[root];
+      d.pos[body] = f4(xyz(d.pos[root]) + quat_rotate(root_rotation, xyz(l.position)), 0.0f);
+      rot = normalize_q(quat_multiply(root_rotation, l.rotation));
+      d.rot[body] = rot;
+    }
+    s_rot[t] = rot;
+  }
+  __syncthreads();
+  const uint32_t depth = meta >> 16, pl = meta & 0xffffu;
+  const bool local = pl != 0xffffu;
+  for (uint32_t level = 1; level <= d.level_count; ++level) {
+    if (own && depth == level) {
+      // reconstruct_body_velocities (articulated.wgsl:423-456).
+      const float4 rp = local ? s_rot[pl] : d.rot[m.parent], rb = s_rot[t];
+      v3 axis, parent_anchor, child_anchor;
+      if (m.direction != 0u) {
+        axis = -normalize_v(quat_rotate(rp, r.axis_b));
+        parent_anchor = quat_rotate(rp, r.anchor_b);
+        child_anchor = quat_rotate(rb, r.anchor_a);
+      } else {
+        axis = normalize_v(quat_rotate(rp, r.axis_a));
+        parent_anchor = quat_rotate(rp, r.anchor_a);
+        child_anchor = quat_rotate(rb, r.anchor_b);
+      }
+      const float speed = d.coords[r.coordinate].y;
+      const v3 parent_ang = local ? xyz(s_ang[pl]) : xyz(d.ang[m.parent]);
+      const v3 parent_lin = local ? xyz(s_lin[pl]) : xyz(d.lin[m.parent]);
+      const v3 angular = parent_ang + axis * speed;
+      const v3 anchor_velocity = parent_lin + cross(parent_ang, parent_anchor);
+      const v3 linear = anchor_velocity - cross(angular, child_anchor);
+      d.lin[body] = f4(linear, 0.0f);
+      d.ang[body] = f4(angular, 0.0f);
+      s_lin[t] = f4(linear, 0.0f);
+      s_ang[t] = f4(angular, 0.0f);
+    }
+    __syncthreads();
+  }
+}
+
+void launch_publish(hipStream_t s, const Dev& d) {
+  if (d.fk_small) k_publish_groups<<<d.fk_groups, 256, 0, s>>>(d);
+  else k_publish<<<1, kPoseThreads, 0, s>>>(d);
+}
+
 // update_world_masses (collision.wgsl:144-162) and the solver record, for ticks
 // whose poses and velocities change after integration.
 __device__ __forceinline__ v3 world_inverse_column(const float4* m, float4 rotation, v3 axis) {
@@ -1697,7 +1764,7 @@ void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures) {
     if (threads == 64u) k_closure_components<64><<<d.cc_count, 64, closure_component_lds(d), s>>>(d);
     else k_closure_components<256><<<d.cc_count, 256, closure_component_lds(d), s>>>(d);
     k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, -1);
-    k_publish<<<1, kPoseThreads, 0, s>>>(d);
+    launch_publish(s, d);
     return;
   }
   k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, -1);
@@ -1707,7 +1774,7 @@ void launch_mechanism_pose(hipStream_t s, const Dev& d, int closures) {
       k_forward_kinematics<<<d.fk_groups, 256, 0, s>>>(d, (int)step);
     }
   }
-  k_publish<<<1, kPoseThreads, 0, s>>>(d);
+  launch_publish(s, d);
 }
 
 size_t closure_component_lds(const Dev& d) {

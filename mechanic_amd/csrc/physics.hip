@@ -205,8 +205,9 @@ bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1])
 // (what create_mechanism_resources derives, device.rs:1684-2232).
 struct Forest {
   std::vector<uint32_t> mlist, level_offset, blist, closure_rows, cpath_offset, cpath, coord_offset, coord_entries,
-      coord_body, fk_list, fk_offset;
+      coord_body, fk_list, fk_offset, fk_meta;
   uint32_t max_depth = 0, jump_rounds = 0;
+  bool fk_small = false;
   bool has_dynamic_root = false;
   // Closure components (closures linked by shared coordinates), each with its
   // path bodies, coordinates and closures in local numbering (see Dev).
@@ -360,6 +361,27 @@ int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
     i = j;
   }
   if (!f.fk_list.empty()) f.fk_offset.push_back((uint32_t)f.fk_list.size());
+  // Per group entry: its depth and its parent's index in the group (a tree's
+  // bodies share a group; a parent outside it is a root), for the publication
+  // kernel that keeps a group's rotations and velocities in LDS. Only when
+  // every group holds at most 256 bodies (one per thread).
+  f.fk_meta.assign(f.fk_list.size(), 0u);
+  f.fk_small = f.max_depth < 0xffffu;
+  {
+    std::vector<uint32_t> local(n, kInvalid);
+    for (size_t g = 0; g + 1 < f.fk_offset.size(); ++g) {
+      const uint32_t g0 = f.fk_offset[g], g1 = f.fk_offset[g + 1];
+      if (g1 - g0 > 256u) f.fk_small = false;
+      for (uint32_t i = g0; i < g1; ++i) local[f.fk_list[i]] = i - g0;
+      for (uint32_t i = g0; i < g1; ++i) {
+        const uint32_t b = f.fk_list[i], pb = parent[b];
+        const uint32_t pl = root[pb] ? 0xffffu : local[pb];
+        if (!root[pb] && pl == kInvalid) f.fk_small = false;  // not expected: trees are never split
+        f.fk_meta[i] = (depth[b] << 16) | (pl & 0xffffu);
+      }
+      for (uint32_t i = g0; i < g1; ++i) local[f.fk_list[i]] = kInvalid;
+    }
+  }
   // pointer_jump_rounds (device.rs:2103-2111).
   uint32_t covered = 1;
   while (covered < f.max_depth) {
@@ -481,6 +503,10 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
   d.fk_list = fk_list;
   d.fk_offset = fk_offset;
   d.fk_groups = f.fk_offset.empty() ? 0u : (uint32_t)f.fk_offset.size() - 1u;
+  uint32_t* fk_meta;
+  TRY(alloc_upload(p, &fk_meta, f.fk_meta.empty() ? std::vector<uint32_t>{0u} : f.fk_meta, error));
+  d.fk_meta = fk_meta;
+  d.fk_small = f.fk_small ? 1u : 0u;
   d.mech_state = mech_state;
   TRY(alloc_upload(p, &level_offset, f.level_offset, error));
   TRY(alloc_upload(p, &blist, f.blist, error));

@@ -63,6 +63,17 @@ def test_hinge_chain_one_tick_matches_oracle():
     np.testing.assert_allclose(gc, oc, rtol=0, atol=POSE_TOL)
 
 
+def test_chain_longer_than_one_publication_group_matches_oracle():
+    # 300 links make one tree of more than 256 bodies, which the per-group
+    # publication cannot hold: the single-workgroup publication runs instead.
+    scene = A.hinge_chain(links=300, length=0.05)
+    (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 2)
+    assert gf == of == 0
+    np.testing.assert_allclose(gp, op, rtol=0, atol=POSE_TOL)
+    np.testing.assert_allclose(gv, ov, rtol=0, atol=POSE_TOL)
+    np.testing.assert_allclose(gc, oc, rtol=0, atol=POSE_TOL)
+
+
 def test_hinge_chain_swings_like_the_oracle():
     scene = A.hinge_chain(links=6, length=0.4)
     (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 60)

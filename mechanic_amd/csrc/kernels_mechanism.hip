@@ -19,6 +19,7 @@
 //   serial PCG adds them -- and the PCG dot products are accumulated in FP64
 //   with a fixed reduction tree.
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.h"
 #include "mh_math.h"
@@ -507,6 +508,14 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
   return __hiloint2double(hi, lo);
 }
 
+// A value every lane holds the same bits of, as a wavefront-uniform one (so
+// branches on it stay scalar).
+__device__ __forceinline__ double uniform_f64(double x) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+
 // Sum over a whole wavefront (every lane active), the same bits in every lane:
 // within each row of 16 lanes, four DPP rotations by 8, 4, 2 and 1 (each adds a
 // pair that both of its lanes form, so the row's lanes agree at every step),
@@ -516,7 +525,7 @@ __device__ __forceinline__ double wave_sum_rows(double v) {
   v = v + dpp_f64<0x124>(v);  // row_ror:4
   v = v + dpp_f64<0x122>(v);  // row_ror:2
   v = v + dpp_f64<0x121>(v);  // row_ror:1
-  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+  return uniform_f64((readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48)));
 }
 
 struct Endpoint {
@@ -1161,7 +1170,12 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
   // With the dense J^T J (one wavefront, nq <= 64) lane l keeps coordinate l's
   // vectors in registers; only the direction is shared, through LDS, for the
   // product, whose row sums run in coordinate order as in the general form.
-  auto pcg_wave = [&](double rz) {
+  auto pcg_wave = [&](auto max_row, double rz) {
+    // kRow > 0: lane l holds row l of J^T J in registers (nq <= kRow; rows
+    // past nq are zeros, whose products add nothing: the sum never is -0.0)
+    // and takes each direction entry from its lane. kRow == 0: rows and the
+    // direction are read from LDS.
+    constexpr uint32_t kRow = decltype(max_row)::value;
     const bool own = t < nq;
     double x = 0.0, r = 0.0, z = 0.0, p = 0.0, dg = 0.0;
     if (own) {
@@ -1170,28 +1184,39 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
       p = L.pp[t];
       dg = L.pdg[t];
     }
+    rz = uniform_f64(rz);
     const double rz_stop = (double)d.closure_tol * (double)d.closure_tol * rz;
     double beta = 0.0;
     double* shared_dir = L.pop;
     const double* row = L.dA + t;
+    double arow[kRow > 0 ? kRow : 1];
+    if constexpr (kRow > 0) {
+#pragma unroll
+      for (uint32_t m = 0; m < kRow; ++m) arow[m] = own && m < nq ? row[m * NQ] : 0.0;
+    }
     for (uint32_t it = 0; it < d.closure_iters; ++it) {
       if (d.closure_tol > 0.0f && rz <= rz_stop) break;
       const double dir = it == 0 ? p : z + beta * p;
-      if (own) shared_dir[t] = dir;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
       double op = 0.0;
-      if (own) {
-        uint32_t m = 0;
-        for (; m + 4u <= nq; m += 4u) {
-          const double a0 = row[m * NQ], a1 = row[(m + 1u) * NQ], a2 = row[(m + 2u) * NQ], a3 = row[(m + 3u) * NQ];
-          const double d0 = shared_dir[m], d1 = shared_dir[m + 1u], d2 = shared_dir[m + 2u], d3 = shared_dir[m + 3u];
-          op = op + a0 * d0;
-          op = op + a1 * d1;
-          op = op + a2 * d2;
-          op = op + a3 * d3;
+      if constexpr (kRow > 0) {
+#pragma unroll
+        for (uint32_t m = 0; m < kRow; ++m) op = op + arow[m] * readlane_f64(dir, (int)m);
+      } else {
+        if (own) shared_dir[t] = dir;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (own) {
+          uint32_t m = 0;
+          for (; m + 4u <= nq; m += 4u) {
+            const double a0 = row[m * NQ], a1 = row[(m + 1u) * NQ], a2 = row[(m + 2u) * NQ], a3 = row[(m + 3u) * NQ];
+            const double d0 = shared_dir[m], d1 = shared_dir[m + 1u], d2 = shared_dir[m + 2u], d3 = shared_dir[m + 3u];
+            op = op + a0 * d0;
+            op = op + a1 * d1;
+            op = op + a2 * d2;
+            op = op + a3 * d3;
+          }
+          for (; m < nq; ++m) op = op + row[m * NQ] * shared_dir[m];
         }
-        for (; m < nq; ++m) op = op + row[m * NQ] * shared_dir[m];
       }
       double dad_part = 0.0;
       if (own) dad_part += dir * op;
@@ -1208,8 +1233,10 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
       const double rz_next = wave_sum_rows(rz_next_part);
       beta = fabs(rz) > 1.0e-20 ? rz_next / rz : 0.0;
       rz = rz_next;
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      __builtin_amdgcn_wave_barrier();  // every lane has read the direction before the next is stored
+      if constexpr (kRow == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // every lane has read the direction before the next is stored
+      }
     }
     if (own) L.px[t] = x;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1218,7 +1245,9 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
   auto pcg = [&](double rz) {
     if constexpr (kT == 64) {
       if (dense) {
-        pcg_wave(rz);
+        if (NQ <= 16u) pcg_wave(std::integral_constant<uint32_t, 16>{}, rz);
+        else if (NQ <= 32u) pcg_wave(std::integral_constant<uint32_t, 32>{}, rz);
+        else pcg_wave(std::integral_constant<uint32_t, 0>{}, rz);
         return;
       }
     }

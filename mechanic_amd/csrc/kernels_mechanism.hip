@@ -1407,26 +1407,42 @@ __global__ __launch_bounds__(256) void k_publish_groups(Dev d) {
     }
     s_rot[t] = rot;
   }
-  __syncthreads();
   const uint32_t depth = meta >> 16, pl = meta & 0xffffu;
   const bool local = pl != 0xffffu;
+  // What this kernel does not write, loaded before the level loop: the
+  // coordinate speed, and a root parent's rotation and velocities.
+  float speed = 0.0f;
+  float4 root_rot = make_float4(0.f, 0.f, 0.f, 1.f);
+  v3 root_ang = mk(0.0f, 0.0f, 0.0f), root_lin = root_ang;
+  if (own) {
+    speed = d.coords[r.coordinate].y;
+    if (!local) {
+      root_rot = d.rot[m.parent];
+      root_ang = xyz(d.ang[m.parent]);
+      root_lin = xyz(d.lin[m.parent]);
+    }
+  }
+  __syncthreads();
+  // reconstruct_body_velocities (articulated.wgsl:423-456): the joint axis and
+  // both anchors need only the published rotations, so every body forms them
+  // at once; the velocities then follow the tree one depth level at a time.
+  v3 axis = mk(0.0f, 0.0f, 0.0f), parent_anchor = axis, child_anchor = axis;
+  if (own) {
+    const float4 rp = local ? s_rot[pl] : root_rot, rb = s_rot[t];
+    if (m.direction != 0u) {
+      axis = -normalize_v(quat_rotate(rp, r.axis_b));
+      parent_anchor = quat_rotate(rp, r.anchor_b);
+      child_anchor = quat_rotate(rb, r.anchor_a);
+    } else {
+      axis = normalize_v(quat_rotate(rp, r.axis_a));
+      parent_anchor = quat_rotate(rp, r.anchor_a);
+      child_anchor = quat_rotate(rb, r.anchor_b);
+    }
+  }
   for (uint32_t level = 1; level <= d.level_count; ++level) {
     if (own && depth == level) {
-      // reconstruct_body_velocities (articulated.wgsl:423-456).
-      const float4 rp = local ? s_rot[pl] : d.rot[m.parent], rb = s_rot[t];
-      v3 axis, parent_anchor, child_anchor;
-      if (m.direction != 0u) {
-        axis = -normalize_v(quat_rotate(rp, r.axis_b));
-        parent_anchor = quat_rotate(rp, r.anchor_b);
-        child_anchor = quat_rotate(rb, r.anchor_a);
-      } else {
-        axis = normalize_v(quat_rotate(rp, r.axis_a));
-        parent_anchor = quat_rotate(rp, r.anchor_a);
-        child_anchor = quat_rotate(rb, r.anchor_b);
-      }
-      const float speed = d.coords[r.coordinate].y;
-      const v3 parent_ang = local ? xyz(s_ang[pl]) : xyz(d.ang[m.parent]);
-      const v3 parent_lin = local ? xyz(s_lin[pl]) : xyz(d.lin[m.parent]);
+      const v3 parent_ang = local ? xyz(s_ang[pl]) : root_ang;
+      const v3 parent_lin = local ? xyz(s_lin[pl]) : root_lin;
       const v3 angular = parent_ang + axis * speed;
       const v3 anchor_velocity = parent_lin + cross(parent_ang, parent_anchor);
       const v3 linear = anchor_velocity - cross(angular, child_anchor);

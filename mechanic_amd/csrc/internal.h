@@ -20,6 +20,8 @@ constexpr uint32_t kSideTile = 2048;  // bodies per tile of the contact-side off
 // accumulate through the atomic overflow path.
 constexpr uint32_t kMaxSlots = 32;      // contact-side delta slots per body (body-major)
 constexpr uint32_t kSlotOverflow = 31;  // slot code of a side past the body's slots (atomics instead)
+constexpr uint32_t kIslandBodies = 256u;    // bodies of one fused bearing-projection group (one per thread)
+constexpr uint32_t kIslandBearings = 51u;   // bearing rows of one such group (5 axis items each, one per thread)
 constexpr uint32_t kSerialBearingLimit = 64u;    // device.rs:19: serial sweeps up to this many bearings
 constexpr uint32_t kSerialPassMultiplier = 12u;  // device.rs:20: their passes per solver iteration
 constexpr uint32_t kRadixTile = 1024;  // keys per radix-sort workgroup
@@ -199,6 +201,14 @@ struct Dev {
   uint32_t fk_small;          // every group holds at most 256 bodies (publication per group)
   uint32_t* mech_state;       // [0]: this tick's closure Newton loop has converged
   const uint32_t* blist;      // bodies referenced by bearings
+  // Bearing islands in groups for the fused state-array projection passes
+  // (bi_groups 0: one launch pair per pass instead).
+  uint32_t bi_groups;
+  const uint32_t* bi_offset;  // bi_groups + 1 offsets into bi_bearing / bi_side
+  const uint32_t* bi_bearing; // bearing rows, grouped
+  const uint32_t* bi_side;    // group-local body of side a | of side b << 16
+  const uint32_t* bi_boff;    // bi_groups + 1 offsets into bi_body
+  const uint32_t* bi_body;    // group bodies; bit 31: this group writes the body back
   const uint32_t* closure_rows;  // bearing row of each closure
   const uint32_t* cpath_offset;  // 2 * closure_count + 1 offsets: branch a, branch b of each closure
   const uint32_t* cpath;         // coordinates along each branch, walking to the root
@@ -301,6 +311,8 @@ void launch_targets(hipStream_t s, const Dev& d);
 // solver records during the contact solve; `write_back` also stores them to the
 // state arrays.
 void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back);
+// `passes` projection passes on the state arrays (in_contact 0, write_back 1).
+void launch_bearing_passes(hipStream_t s, const Dev& d, uint32_t passes);
 // The serial small-mechanism sweeps after the parallel warm-start pass:
 // passes - 1 contact sweeps, each bearing sweep after the warm start and after
 // every contact sweep, then the final velocities and accumulated impulses.

@@ -232,6 +232,130 @@ __global__ __launch_bounds__(256) void k_bearing_apply(Dev d, int in_contact, in
   }
 }
 
+// world_inverse_inertia (articulated.wgsl:97-105) of a body whose rotation and
+// mass rows are held in registers (the arithmetic of wii).
+__device__ __forceinline__ v3 wii_of(float4 rotation, const float4 (&m)[4], v3 v) {
+  const v3 local = quat_rotate(quat_conj(rotation), v);
+  const v3 r = (xyz(m[1]) * local.x + xyz(m[2]) * local.y) + xyz(m[3]) * local.z;
+  return quat_rotate(rotation, r);
+}
+
+// `passes` passes of project_bearing_velocities and apply_velocity_deltas
+// (articulated.wgsl:281-326) on the state arrays, one workgroup per group of
+// bearing islands with one (row, axis) item per thread: the group's bodies
+// keep their velocities and Q20 sums in LDS from pass to pass. Rotations do not
+// change during the passes, so each item forms its arms, axis and denominator
+// once; each pass then computes what one lane of k_bearing_project computes
+// from the same values, the sums are the same wrapping integer sums and the
+// update is k_bearing_apply's: the result is the one of launching the two
+// kernels per pass.
+__global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes) {
+  __shared__ float4 s_lin[kIslandBodies], s_ang[kIslandBodies];
+  __shared__ int32_t s_q[kIslandBodies][6];
+  const uint32_t g = blockIdx.x, t = threadIdx.x;
+  const uint32_t r0 = d.bi_offset[g], rows = d.bi_offset[g + 1] - r0;
+  const uint32_t b0 = d.bi_boff[g], bodies = d.bi_boff[g + 1] - b0;
+  uint32_t body = 0, owner = 0;
+  if (t < bodies) {
+    const uint32_t w = d.bi_body[b0 + t];
+    body = w & 0x7fffffffu;
+    owner = w >> 31;
+    s_lin[t] = d.lin[body];
+    s_ang[t] = d.ang[body];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s_q[t][k] = 0;
+  }
+  // This thread's item: axis `part` of row j (parts 0..2 linear x, y, z; 3, 4
+  // the two angular tangents).
+  const bool item = t < 5u * rows;
+  const uint32_t j = t / 5u, part = t % 5u;
+  uint32_t la = 0, lb = 0;
+  bool live = false;
+  float4 rot_a = make_float4(0.f, 0.f, 0.f, 1.f), rot_b = rot_a;
+  float4 ma[4], mb[4];
+  float inv_a = 0.0f, inv_b = 0.0f, denominator = 1.0f;
+  v3 arm_a = mk(0.0f, 0.0f, 0.0f), arm_b = arm_a, direction = arm_a, inverse_a = arm_a, inverse_b = arm_a;
+  if (item) {
+    const BearingRow r = load_bearing(d, d.bi_bearing[r0 + j]);
+    const uint32_t side = d.bi_side[r0 + j];
+    la = side & 0xffffu;
+    lb = side >> 16;
+    rot_a = d.rot[r.a];
+    rot_b = d.rot[r.b];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ma[k] = d.masses[4 * (size_t)r.a + k];
+      mb[k] = d.masses[4 * (size_t)r.b + k];
+    }
+    inv_a = ma[0].x;
+    inv_b = mb[0].x;
+    if (part < 3u) {
+      // solve_linear_axis (articulated.wgsl:118-143), the part before the impulse.
+      arm_a = quat_rotate(rot_a, r.anchor_a);
+      arm_b = quat_rotate(rot_b, r.anchor_b);
+      direction = part == 0u ? mk(1.0f, 0.0f, 0.0f) : part == 1u ? mk(0.0f, 1.0f, 0.0f) : mk(0.0f, 0.0f, 1.0f);
+      const v3 angular_a = cross(arm_a, direction);
+      const v3 angular_b = cross(arm_b, direction);
+      denominator = ((inv_a + inv_b) + dot(angular_a, wii_of(rot_a, ma, angular_a))) +
+                    dot(angular_b, wii_of(rot_b, mb, angular_b));
+    } else {
+      // solve_angular_axis (articulated.wgsl:145-155), the part before the impulse.
+      const v3 axis_a = normalize_v(quat_rotate(rot_a, r.axis_a));
+      const v3 axis_b = normalize_v(quat_rotate(rot_b, r.axis_b));
+      const v3 hinge = normalize_v(axis_a + axis_b);
+      const v3 helper = fabsf(hinge.x) > 0.8f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+      const v3 tangent_a = normalize_v(cross(hinge, helper));
+      direction = part == 3u ? tangent_a : cross(hinge, tangent_a);
+      inverse_a = wii_of(rot_a, ma, direction);
+      inverse_b = wii_of(rot_b, mb, direction);
+      denominator = dot(direction, inverse_a + inverse_b);
+    }
+    live = !(denominator <= 1.0e-12f);
+  }
+  __syncthreads();
+  for (uint32_t pass = 0; pass < passes; ++pass) {
+    if (item && live) {
+      QSum qa = {{0, 0, 0, 0, 0, 0}}, qb = {{0, 0, 0, 0, 0, 0}};
+      const v3 ang_a = xyz(s_ang[la]), ang_b = xyz(s_ang[lb]);
+      if (part < 3u) {
+        const v3 velocity_a = xyz(s_lin[la]) + cross(ang_a, arm_a);
+        const v3 velocity_b = xyz(s_lin[lb]) + cross(ang_b, arm_b);
+        const v3 relative = velocity_b - velocity_a;
+        const v3 impulse = direction * ((kBearingRelaxation * dot(relative, direction)) / denominator);
+        qsum_add(qa, impulse * inv_a, wii_of(rot_a, ma, cross(arm_a, impulse)));
+        qsum_add(qb, (-impulse) * inv_b, wii_of(rot_b, mb, cross(arm_b, -impulse)));
+      } else {
+        const float impulse = (kBearingRelaxation * dot(ang_b - ang_a, direction)) / denominator;
+        qsum_add(qa, mk(0.0f, 0.0f, 0.0f), inverse_a * impulse);
+        qsum_add(qb, mk(0.0f, 0.0f, 0.0f), (-inverse_b) * impulse);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        atomicAdd(&s_q[la][k], qa.v[k]);
+        atomicAdd(&s_q[lb][k], qb.v[k]);
+      }
+    }
+    __syncthreads();
+    if (t < bodies) {
+      int32_t q[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        q[k] = s_q[t][k];
+        s_q[t][k] = 0;
+      }
+      const v3 linear_delta = mk((float)q[0], (float)q[1], (float)q[2]) / kFixedScaleM;
+      const v3 angular_delta = mk((float)q[3], (float)q[4], (float)q[5]) / kFixedScaleM;
+      s_lin[t] = f4(xyz(s_lin[t]) + linear_delta, 0.0f);
+      s_ang[t] = f4(xyz(s_ang[t]) + angular_delta, 0.0f);
+    }
+    __syncthreads();
+  }
+  if (t < bodies && owner) {
+    d.lin[body] = s_lin[t];
+    d.ang[body] = s_ang[t];
+  }
+}
+
 // permitted_axis / permitted_speed / stabilized_speed (articulated.wgsl:328-355).
 __device__ __forceinline__ v3 permitted_axis(const Dev& d, const MechBody& m, const BearingRow& r) {
   const float4 rp = d.rot[m.parent];
@@ -1784,6 +1908,15 @@ void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int 
   if (d.bearing_count == 0) return;
   k_bearing_project<<<blocks_for(d.bearing_count * kBearingLanes), 256, 0, s>>>(d, in_contact);
   k_bearing_apply<<<blocks_for(d.bearing_body_count), 256, 0, s>>>(d, in_contact, write_back);
+}
+
+void launch_bearing_passes(hipStream_t s, const Dev& d, uint32_t passes) {
+  if (d.bearing_count == 0) return;
+  if (d.bi_groups) {
+    k_bearing_islands<<<d.bi_groups, 256, 0, s>>>(d, passes);
+    return;
+  }
+  for (uint32_t i = 0; i < passes; ++i) launch_bearing_projection(s, d, 0, 1);
 }
 
 void launch_serial_solve(hipStream_t s, const Dev& d, uint32_t passes) {

@@ -125,7 +125,7 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uin
   if (mech) {
     // encode_mechanism_passes (device.rs:963-1048).
     const uint32_t iterations = d.solver_iterations ? d.solver_iterations : 1u;
-    for (uint32_t i = 0; i < iterations; ++i) launch_bearing_projection(s, d, 0, 1);
+    launch_bearing_passes(s, d, iterations);
     launch_advance_coordinates(s, d);
     launch_mechanism_pose(s, d, d.closure_count > 0 ? 1 : 0);
   }
@@ -158,7 +158,7 @@ void encode_tick(mh_physics* p, hipStream_t s, hipEvent_t* ev, int stop = 0, uin
       // encode_post_contact_mechanism (device.rs:1128-1155).
       if (!d.has_dynamic_root) {
         const uint32_t iterations = d.solver_iterations ? d.solver_iterations : 1u;
-        for (uint32_t i = 0; i < iterations; ++i) launch_bearing_projection(s, d, 0, 1);
+        launch_bearing_passes(s, d, iterations);
       }
       launch_capture_coordinates(s, d);
       if (!d.has_dynamic_root) launch_reconstruct(s, d);
@@ -208,6 +208,12 @@ struct Forest {
       coord_body, fk_list, fk_offset, fk_meta;
   uint32_t max_depth = 0, jump_rounds = 0;
   bool fk_small = false;
+  // Bearing islands packed into groups for the fused state-array projection
+  // passes: per group its bearing rows (bi_offset), each row's two bodies as
+  // group-local indices (bi_side: a | b << 16), and its bodies (bi_boff,
+  // bi_body; bit 31: this group writes the body back). Empty when some group
+  // would exceed kIslandBodies bodies or kIslandBearings rows.
+  std::vector<uint32_t> bi_offset, bi_bearing, bi_side, bi_boff, bi_body;
   bool has_dynamic_root = false;
   // Closure components (closures linked by shared coordinates), each with its
   // path bodies, coordinates and closures in local numbering (see Dev).
@@ -306,6 +312,113 @@ void build_components(const mh_creation* c, Forest& f, const std::vector<uint32_
   }
 }
 
+// Bodies joined by bearings through bodies that move. A body with zero inverse
+// mass and inverse inertia gets no velocity delta from any bearing (its Q20
+// terms are all zero), so it joins nothing: each group that reads it holds a
+// copy, which stays bit-identical to the others, and one of them writes it.
+void build_islands(const mh_creation* c, Forest& f) {
+  const uint32_t n = c->body_count, nb = c->bearing_count;
+  auto still = [&](uint32_t b) {
+    const mh_mass& m = c->masses[b];
+    if (m.inverse_mass[0] != 0.0f) return false;
+    for (int k = 0; k < 3; ++k)
+      if (m.inverse_inertia_x[k] != 0.0f || m.inverse_inertia_y[k] != 0.0f || m.inverse_inertia_z[k] != 0.0f)
+        return false;
+    return true;
+  };
+  std::vector<uint32_t> up(n);
+  for (uint32_t b = 0; b < n; ++b) up[b] = b;
+  auto find = [&](uint32_t x) {
+    while (up[x] != x) x = up[x] = up[up[x]];
+    return x;
+  };
+  std::vector<uint8_t> moving(n, 0);
+  for (uint32_t b = 0; b < n; ++b) moving[b] = still(b) ? 0 : 1;
+  for (uint32_t i = 0; i < nb; ++i) {
+    const uint32_t a = c->bearings[i].metadata[0], b = c->bearings[i].metadata[1];
+    if (moving[a] && moving[b]) up[find(a)] = find(b);
+  }
+  // Island key of a row: its moving body's root, or the row itself when both are still.
+  std::vector<std::vector<uint32_t>> rows_of;
+  std::vector<uint32_t> island_of(n, kInvalid);
+  std::vector<uint32_t> key(nb);
+  for (uint32_t i = 0; i < nb; ++i) {
+    const uint32_t a = c->bearings[i].metadata[0], b = c->bearings[i].metadata[1];
+    uint32_t k;
+    if (moving[a] || moving[b]) {
+      const uint32_t r = find(moving[a] ? a : b);
+      if (island_of[r] == kInvalid) {
+        island_of[r] = (uint32_t)rows_of.size();
+        rows_of.emplace_back();
+      }
+      k = island_of[r];
+    } else {
+      k = (uint32_t)rows_of.size();
+      rows_of.emplace_back();
+    }
+    rows_of[k].push_back(i);
+  }
+  std::vector<uint32_t> local(n, kInvalid);
+  std::vector<uint8_t> written(n, 0);
+  std::vector<uint32_t> group_bodies;
+  f.bi_offset.assign(1, 0u);
+  f.bi_boff.assign(1, 0u);
+  auto close_group = [&]() {
+    for (uint32_t b : group_bodies) {
+      const uint32_t owner = written[b] ? 0u : 0x80000000u;
+      written[b] = 1;
+      f.bi_body.push_back(b | owner);
+      local[b] = kInvalid;
+    }
+    group_bodies.clear();
+    f.bi_offset.push_back((uint32_t)f.bi_bearing.size());
+    f.bi_boff.push_back((uint32_t)f.bi_body.size());
+  };
+  // Bodies of `rows` not yet in the open group (all of them when `all`).
+  std::vector<uint8_t> mark(n, 0);
+  auto count_new = [&](const std::vector<uint32_t>& rows, bool all) {
+    uint32_t count = 0;
+    for (uint32_t i : rows)
+      for (int side = 0; side < 2; ++side) {
+        const uint32_t b = c->bearings[i].metadata[side];
+        if ((all || local[b] == kInvalid) && !mark[b]) {
+          mark[b] = 1;
+          ++count;
+        }
+      }
+    for (uint32_t i : rows)
+      for (int side = 0; side < 2; ++side) mark[c->bearings[i].metadata[side]] = 0;
+    return count;
+  };
+  for (const std::vector<uint32_t>& rows : rows_of) {
+    if (count_new(rows, true) > kIslandBodies || rows.size() > kIslandBearings) {
+      f.bi_offset.clear();
+      f.bi_bearing.clear();
+      f.bi_side.clear();
+      f.bi_boff.clear();
+      f.bi_body.clear();
+      return;
+    }
+    const uint32_t extra = count_new(rows, false);
+    const uint32_t open_rows = (uint32_t)f.bi_bearing.size() - f.bi_offset.back();
+    if (group_bodies.size() + extra > kIslandBodies || open_rows + rows.size() > kIslandBearings) close_group();
+    for (uint32_t i : rows) {
+      uint32_t idx[2];
+      for (int side = 0; side < 2; ++side) {
+        const uint32_t b = c->bearings[i].metadata[side];
+        if (local[b] == kInvalid) {
+          local[b] = (uint32_t)group_bodies.size();
+          group_bodies.push_back(b);
+        }
+        idx[side] = local[b];
+      }
+      f.bi_bearing.push_back(i);
+      f.bi_side.push_back(idx[0] | idx[1] << 16);
+    }
+  }
+  if (f.bi_bearing.size() > f.bi_offset.back()) close_group();
+}
+
 int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
   const uint32_t n = c->body_count, nb = c->bearing_count, nq = c->coordinate_count;
   if (!c->bearings || !c->mechanism_bodies || (n > 0 && !c->preorder))
@@ -394,6 +507,7 @@ int build_forest(const mh_creation* c, Forest& f, mh_error_detail* error) {
   for (uint32_t i = 0; i < nb; ++i) touched[c->bearings[i].metadata[0]] = touched[c->bearings[i].metadata[1]] = 1;
   for (uint32_t b = 0; b < n; ++b)
     if (touched[b]) f.blist.push_back(b);
+  build_islands(c, f);
   // Coordinates and their child bodies.
   f.coord_body.assign(nq, kInvalid);
   for (uint32_t b = 0; b < n; ++b)
@@ -510,6 +624,20 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
   d.mech_state = mech_state;
   TRY(alloc_upload(p, &level_offset, f.level_offset, error));
   TRY(alloc_upload(p, &blist, f.blist, error));
+  d.bi_groups = f.bi_offset.size() > 1 ? (uint32_t)f.bi_offset.size() - 1u : 0u;
+  if (d.bi_groups) {
+    uint32_t *bi_offset, *bi_bearing, *bi_side, *bi_boff, *bi_body;
+    TRY(alloc_upload(p, &bi_offset, f.bi_offset, error));
+    TRY(alloc_upload(p, &bi_bearing, f.bi_bearing, error));
+    TRY(alloc_upload(p, &bi_side, f.bi_side, error));
+    TRY(alloc_upload(p, &bi_boff, f.bi_boff, error));
+    TRY(alloc_upload(p, &bi_body, f.bi_body, error));
+    d.bi_offset = bi_offset;
+    d.bi_bearing = bi_bearing;
+    d.bi_side = bi_side;
+    d.bi_boff = bi_boff;
+    d.bi_body = bi_body;
+  }
   TRY(alloc_upload(p, &closure_rows, f.closure_rows, error));
   TRY(alloc_upload(p, &cpath_offset, f.cpath_offset, error));
   TRY(alloc_upload(p, &cpath, f.cpath, error));

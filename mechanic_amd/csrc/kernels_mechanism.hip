@@ -249,7 +249,7 @@ __device__ __forceinline__ v3 wii_of(float4 rotation, const float4 (&m)[4], v3 v
 // from the same values, the sums are the same wrapping integer sums and the
 // update is k_bearing_apply's: the result is the one of launching the two
 // kernels per pass.
-__global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes) {
+__global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes, int in_contact, int write_back) {
   __shared__ float4 s_lin[kIslandBodies], s_ang[kIslandBodies];
   __shared__ int32_t s_q[kIslandBodies][6];
   const uint32_t g = blockIdx.x, t = threadIdx.x;
@@ -260,8 +260,10 @@ __global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes)
     const uint32_t w = d.bi_body[b0 + t];
     body = w & 0x7fffffffu;
     owner = w >> 31;
-    s_lin[t] = d.lin[body];
-    s_ang[t] = d.ang[body];
+    // Velocities: the state arrays, or the solver records (whose fourth words
+    // the update keeps) while the contact solve runs.
+    s_lin[t] = in_contact ? d.sbody[4 * (size_t)body] : d.lin[body];
+    s_ang[t] = in_contact ? d.sbody[4 * (size_t)body + 1] : d.ang[body];
 #pragma unroll
     for (int k = 0; k < 6; ++k) s_q[t][k] = 0;
   }
@@ -345,14 +347,20 @@ __global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes)
       }
       const v3 linear_delta = mk((float)q[0], (float)q[1], (float)q[2]) / kFixedScaleM;
       const v3 angular_delta = mk((float)q[3], (float)q[4], (float)q[5]) / kFixedScaleM;
-      s_lin[t] = f4(xyz(s_lin[t]) + linear_delta, 0.0f);
-      s_ang[t] = f4(xyz(s_ang[t]) + angular_delta, 0.0f);
+      s_lin[t] = f4(xyz(s_lin[t]) + linear_delta, in_contact ? s_lin[t].w : 0.0f);
+      s_ang[t] = f4(xyz(s_ang[t]) + angular_delta, in_contact ? s_ang[t].w : 0.0f);
     }
     __syncthreads();
   }
   if (t < bodies && owner) {
-    d.lin[body] = s_lin[t];
-    d.ang[body] = s_ang[t];
+    if (in_contact) {
+      d.sbody[4 * (size_t)body] = s_lin[t];
+      d.sbody[4 * (size_t)body + 1] = s_ang[t];
+    }
+    if (!in_contact || write_back) {
+      d.lin[body] = f4(xyz(s_lin[t]), 0.0f);
+      d.ang[body] = f4(xyz(s_ang[t]), 0.0f);
+    }
   }
 }
 
@@ -1906,6 +1914,10 @@ __global__ __launch_bounds__(256) void k_serial_finish(Dev d) {
 
 void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back) {
   if (d.bearing_count == 0) return;
+  if (d.bi_groups) {
+    k_bearing_islands<<<d.bi_groups, 256, 0, s>>>(d, 1u, in_contact, write_back);
+    return;
+  }
   k_bearing_project<<<blocks_for(d.bearing_count * kBearingLanes), 256, 0, s>>>(d, in_contact);
   k_bearing_apply<<<blocks_for(d.bearing_body_count), 256, 0, s>>>(d, in_contact, write_back);
 }
@@ -1913,7 +1925,7 @@ void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int 
 void launch_bearing_passes(hipStream_t s, const Dev& d, uint32_t passes) {
   if (d.bearing_count == 0) return;
   if (d.bi_groups) {
-    k_bearing_islands<<<d.bi_groups, 256, 0, s>>>(d, passes);
+    k_bearing_islands<<<d.bi_groups, 256, 0, s>>>(d, passes, 0, 1);
     return;
   }
   for (uint32_t i = 0; i < passes; ++i) launch_bearing_projection(s, d, 0, 1);

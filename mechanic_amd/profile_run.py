@@ -2,6 +2,10 @@
 
     python -m mechanic_amd.profile_run pile_100k --ticks 20 [--settle 300]
     python -m mechanic_amd.profile_run articulated --ticks 20 [--closures config2|reference]
+    python -m mechanic_amd.profile_run batch --ticks 20
+
+(batch: the 256 scenes of 64..4,096 cuboids in one handle, with the cached
+launch shape for its size.)
 
 Captured graph launches hide the individual kernels from some tracers, so this
 driver dispatches every kernel directly (use_graph off). It prints one line
@@ -23,7 +27,7 @@ CONFIG2_CLOSURES = dict(closure_components=True, closure_pcg_tolerance=1e-6, clo
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=("pile_100k", "pile_1k", "articulated"))
+    ap.add_argument("workload", choices=("pile_100k", "pile_1k", "articulated", "batch"))
     ap.add_argument("--ticks", type=int, default=20)
     ap.add_argument("--settle", type=int, default=0, help="untimed ticks first (pile: 300 for the settled state)")
     ap.add_argument("--closures", choices=("config2", "reference"), default="config2")
@@ -35,10 +39,23 @@ def main(argv=None) -> int:
         extra = CONFIG2_CLOSURES if args.closures == "config2" else {}
         cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8, mechanism_self_collisions=False,
                                   use_graph=False, **extra)
+    elif args.workload == "batch":
+        from mechanic_amd import autotune
+        from mechanic_amd.scenes import batch_scenes
+
+        scenes = batch_scenes()
+        cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8, use_graph=False)
     else:
         scene = mh.named_pile(args.workload)
         cfg = mh.GpuPhysicsConfig(friction=0.6, solver_iterations=8, use_graph=False)
-    with mh.GpuPhysics.new_with_config(scene, cfg) as gpu:
+    if args.workload == "batch":
+        handle = mh.GpuPhysics.new_batch(scenes, cfg)
+        tuning = autotune.cached_tuning(sum(sc.body_count for sc in scenes))
+        if tuning:
+            handle.set_tuning(**tuning)
+    else:
+        handle = mh.GpuPhysics.new_with_config(scene, cfg)
+    with handle as gpu:
         last = args.settle + args.ticks
         for tick in range(1, last + 1):
             gpu.dispatch_tick(tick)

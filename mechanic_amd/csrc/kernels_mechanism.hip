@@ -1377,7 +1377,9 @@ __global__ __launch_bounds__(kT) void k_closure_components(Dev d) {
   auto pcg = [&](double rz) {
     if constexpr (kT == 64) {
       if (dense) {
-        if (NQ <= 16u) pcg_wave(std::integral_constant<uint32_t, 16>{}, rz);
+        if (NQ <= 8u) pcg_wave(std::integral_constant<uint32_t, 8>{}, rz);
+        else if (NQ <= 16u) pcg_wave(std::integral_constant<uint32_t, 16>{}, rz);
+        else if (NQ <= 24u) pcg_wave(std::integral_constant<uint32_t, 24>{}, rz);
         else if (NQ <= 32u) pcg_wave(std::integral_constant<uint32_t, 32>{}, rz);
         else pcg_wave(std::integral_constant<uint32_t, 0>{}, rz);
         return;

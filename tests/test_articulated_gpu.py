@@ -74,6 +74,35 @@ def test_chain_longer_than_one_publication_group_matches_oracle():
     np.testing.assert_allclose(gc, oc, rtol=0, atol=POSE_TOL)
 
 
+def fan_of_chains(chains=12, links=6, length=0.3, height=3.0):
+    """One static beam along z with `chains` hinge chains hanging off it, each
+    `links` cuboids along +x, hinged about z, 0.4 m apart along the beam."""
+    centers, half, static, specs = [(0.0, height, 0.2 * (chains - 1))], [(0.05, 0.05, 0.2 * chains)], [True], []
+    for c in range(chains):
+        z = 0.4 * c
+        first = len(centers)
+        for k in range(links):
+            centers.append(((k + 0.5) * length, height, z))
+            half.append((length * 0.5, 0.05, 0.05))
+            static.append(False)
+            specs.append(A.BearingSpec(0 if k == 0 else first + k - 1, first + k, (k * length, height, z),
+                                       (0.0, 0.0, 1.0)))
+    scene = A._cuboids(centers, half, static)
+    return A.attach(scene, A.compile_mechanisms(np.asarray(centers, np.float32), np.asarray(static), specs))
+
+
+def test_chains_sharing_a_static_body_across_bearing_groups_match_oracle():
+    # 72 rows on one static beam: the bearing islands (one per chain, the beam
+    # joins nothing) fill two groups of at most 51 rows, both holding the beam.
+    scene = fan_of_chains()
+    (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 3)
+    assert gf == of == 0
+    np.testing.assert_allclose(gp, op, rtol=0, atol=POSE_TOL)
+    np.testing.assert_allclose(gv, ov, rtol=0, atol=POSE_TOL)
+    np.testing.assert_allclose(gc, oc, rtol=0, atol=POSE_TOL)
+    assert np.all(gv[0] == 0.0)  # the beam never moves
+
+
 def test_hinge_chain_swings_like_the_oracle():
     scene = A.hinge_chain(links=6, length=0.4)
     (gp, gv, gc, gf), (op, ov, oc, of) = run_both(scene, 60)

@@ -209,6 +209,7 @@ struct Dev {
   const uint32_t* bi_side;    // group-local body of side a | of side b << 16
   const uint32_t* bi_boff;    // bi_groups + 1 offsets into bi_body
   const uint32_t* bi_body;    // group bodies; bit 31: this group writes the body back
+  const uint8_t* bi_mask;     // per body: 1 when an island group holds it
   const uint32_t* closure_rows;  // bearing row of each closure
   const uint32_t* cpath_offset;  // 2 * closure_count + 1 offsets: branch a, branch b of each closure
   const uint32_t* cpath;         // coordinates along each branch, walking to the root
@@ -313,6 +314,9 @@ void launch_targets(hipStream_t s, const Dev& d);
 void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int write_back);
 // `passes` projection passes on the state arrays (in_contact 0, write_back 1).
 void launch_bearing_passes(hipStream_t s, const Dev& d, uint32_t passes);
+// A contact pass's velocity update fused with its bearing projection when the
+// bearings form island groups; false (nothing launched) otherwise.
+bool launch_apply_islands(hipStream_t s, const Dev& d, uint32_t apply_blocks, int last, uint64_t* clock);
 // The serial small-mechanism sweeps after the parallel warm-start pass:
 // passes - 1 contact sweeps, each bearing sweep after the warm start and after
 // every contact sweep, then the final velocities and accumulated impulses.

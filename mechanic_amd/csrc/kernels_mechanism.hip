@@ -249,10 +249,17 @@ __device__ __forceinline__ v3 wii_of(float4 rotation, const float4 (&m)[4], v3 v
 // from the same values, the sums are the same wrapping integer sums and the
 // update is k_bearing_apply's: the result is the one of launching the two
 // kernels per pass.
-__global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes, int in_contact, int write_back) {
+//
+// With `apply_first` (inside the contact solve, in the launch of the pass's
+// velocity update) the group first adds each of its bodies' contact deltas of
+// the pass, as solve_apply does: the same wrapping sums of its slots and
+// overflow words, one thread per body; only the group that writes a body back
+// clears its overflow words (a still body's are zero).
+__device__ __forceinline__ void island_group(const Dev& d, uint32_t g, uint32_t passes, int in_contact,
+                                             int write_back, bool apply_first) {
   __shared__ float4 s_lin[kIslandBodies], s_ang[kIslandBodies];
   __shared__ int32_t s_q[kIslandBodies][6];
-  const uint32_t g = blockIdx.x, t = threadIdx.x;
+  const uint32_t t = threadIdx.x;
   const uint32_t r0 = d.bi_offset[g], rows = d.bi_offset[g + 1] - r0;
   const uint32_t b0 = d.bi_boff[g], bodies = d.bi_boff[g + 1] - b0;
   uint32_t body = 0, owner = 0;
@@ -262,8 +269,38 @@ __global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes,
     owner = w >> 31;
     // Velocities: the state arrays, or the solver records (whose fourth words
     // the update keeps) while the contact solve runs.
-    s_lin[t] = in_contact ? d.sbody[4 * (size_t)body] : d.lin[body];
-    s_ang[t] = in_contact ? d.sbody[4 * (size_t)body + 1] : d.ang[body];
+    float4 lin = in_contact ? d.sbody[4 * (size_t)body] : d.lin[body];
+    float4 ang = in_contact ? d.sbody[4 * (size_t)body + 1] : d.ang[body];
+    if (apply_first && *reinterpret_cast<volatile uint32_t*>(&d.diag[tick_sv(d) & 1][kDiagActive]) != 0u) {
+      const uint32_t count = d.slot_count[body];
+      const uint32_t n = count < kSlotOverflow ? count : kSlotOverflow;
+      uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+      for (uint32_t k = 0; k < n; ++k) {
+        const int4* e = d.cdelta + 2 * ((size_t)k * d.body_count + body);
+        const int4 x = e[0];
+        const int2 y = *reinterpret_cast<const int2*>(e + 1);
+        acc[0] += (uint32_t)x.x;
+        acc[1] += (uint32_t)x.y;
+        acc[2] += (uint32_t)x.z;
+        acc[3] += (uint32_t)x.w;
+        acc[4] += (uint32_t)y.x;
+        acc[5] += (uint32_t)y.y;
+      }
+      if (count > kSlotOverflow) {
+        int32_t* p = d.deltas + 6 * (size_t)body;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          acc[c] += (uint32_t)p[c];
+          if (owner) p[c] = 0;
+        }
+      }
+      const v3 lin_delta = mk((float)(int32_t)acc[0], (float)(int32_t)acc[1], (float)(int32_t)acc[2]) / kFixedScale;
+      const v3 ang_delta = mk((float)(int32_t)acc[3], (float)(int32_t)acc[4], (float)(int32_t)acc[5]) / kFixedScale;
+      lin = f4(xyz(lin) + lin_delta, lin.w);
+      ang = f4(xyz(ang) + ang_delta, ang.w);
+    }
+    s_lin[t] = lin;
+    s_ang[t] = ang;
 #pragma unroll
     for (int k = 0; k < 6; ++k) s_q[t][k] = 0;
   }
@@ -362,6 +399,22 @@ __global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes,
       d.ang[body] = f4(xyz(s_ang[t]), 0.0f);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes, int in_contact, int write_back) {
+  island_group(d, blockIdx.x, passes, in_contact, write_back, false);
+}
+
+// A contact pass's velocity update and bearing projection in one launch:
+// blocks [0, apply_blocks) update the bodies no island group holds, as k_apply
+// does; each later block is an island group that updates its own bodies and
+// then projects its bearings (one pass on the solver records).
+__global__ __launch_bounds__(256) void k_apply_islands(Dev d, uint32_t apply_blocks, int last, uint64_t* clock) {
+  if (blockIdx.x < apply_blocks) {
+    apply_contact_deltas(d, last, clock, d.bi_mask, blockIdx.x, apply_blocks);
+    return;
+  }
+  island_group(d, blockIdx.x - apply_blocks, 1u, 1, last, true);
 }
 
 // permitted_axis / permitted_speed / stabilized_speed (articulated.wgsl:328-355).
@@ -1922,6 +1975,12 @@ void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int 
   }
   k_bearing_project<<<blocks_for(d.bearing_count * kBearingLanes), 256, 0, s>>>(d, in_contact);
   k_bearing_apply<<<blocks_for(d.bearing_body_count), 256, 0, s>>>(d, in_contact, write_back);
+}
+
+bool launch_apply_islands(hipStream_t s, const Dev& d, uint32_t apply_blocks, int last, uint64_t* clock) {
+  if (!d.bi_groups || !d.bi_mask || apply_blocks == 0) return false;
+  k_apply_islands<<<apply_blocks + d.bi_groups, 256, 0, s>>>(d, apply_blocks, last, clock);
+  return true;
 }
 
 void launch_bearing_passes(hipStream_t s, const Dev& d, uint32_t passes) {

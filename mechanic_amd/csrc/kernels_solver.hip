@@ -654,86 +654,9 @@ __global__ __launch_bounds__(256) void k_copy_records(const float4* __restrict__
 // order) and adds them to its velocities. The last pass also writes
 // the body velocities back to the state arrays.
 //
-// kApplyLanes adjacent lanes share a body: each loads every kApplyLanes-th of
-// its slots, all loads in flight at once, and shuffles combine the partial
-// sums. The slot loads of a body cost one memory latency instead of one per slot.
-constexpr uint32_t kApplyLanes = 4;
-constexpr uint32_t kApplyPerLane = (kSlotOverflow + kApplyLanes - 1) / kApplyLanes;
-
 __global__ __launch_bounds__(256) void k_apply(Dev d, int last, uint64_t* clock, const uint32_t* gate) {
   if (gate && *reinterpret_cast<const volatile uint32_t*>(gate) == 0u) return;
-  if (clock && threadIdx.x == 0) clock[2 * blockIdx.x] = wall_clock64();
-  const uint64_t tick64 = tick_sv(d);
-  uint32_t* diag = d.diag[tick64 & 1];
-  if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
-  const XcdSplit xs = xcd_split(d.body_count);
-  const uint32_t per_block = blockDim.x / kApplyLanes;
-  const uint32_t xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
-  const uint32_t blocks = (gridDim.x - xcd + kXcds - 1) / kXcds;
-  const uint32_t part = threadIdx.x % kApplyLanes;
-  for (uint32_t b0 = xs.begin + local * per_block; b0 < xs.end; b0 += blocks * per_block) {
-    const uint32_t b = b0 + threadIdx.x / kApplyLanes;
-    const bool valid = b < xs.end;
-    uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
-    const uint32_t count = valid ? d.slot_count[b] : 0u;
-    const uint32_t n = count < kSlotOverflow ? count : kSlotOverflow;
-    // The body's velocity words, loaded alongside its slots.
-    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-    if (valid && part == 0u) {
-      v0 = d.sbody[4 * (size_t)b];
-      v1 = d.sbody[4 * (size_t)b + 1];
-    }
-    int4 x[kApplyPerLane];
-    int2 y[kApplyPerLane];
-#pragma unroll
-    for (uint32_t j = 0; j < kApplyPerLane; ++j) {
-      const uint32_t k = part + j * kApplyLanes;
-      x[j] = make_int4(0, 0, 0, 0);
-      y[j] = make_int2(0, 0);
-      if (k < n) {
-        const int4* e = d.cdelta + 2 * ((size_t)k * d.body_count + b);
-        x[j] = e[0];
-        y[j] = *reinterpret_cast<const int2*>(e + 1);
-      }
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < kApplyPerLane; ++j) {
-      acc[0] += (uint32_t)x[j].x;
-      acc[1] += (uint32_t)x[j].y;
-      acc[2] += (uint32_t)x[j].z;
-      acc[3] += (uint32_t)x[j].w;
-      acc[4] += (uint32_t)y[j].x;
-      acc[5] += (uint32_t)y[j].y;
-    }
-#pragma unroll
-    for (uint32_t m = 1; m < kApplyLanes; m <<= 1)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) acc[c] += (uint32_t)__shfl_xor((int)acc[c], (int)m);
-    if (!valid || part != 0u) continue;
-    if (count > kSlotOverflow) {
-      int32_t* p = d.deltas + 6 * (size_t)b;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        acc[c] += (uint32_t)p[c];
-        p[c] = 0;
-      }
-    }
-    const v3 lin_delta = mk((float)(int32_t)acc[0], (float)(int32_t)acc[1], (float)(int32_t)acc[2]) / kFixedScale;
-    const v3 ang_delta = mk((float)(int32_t)acc[3], (float)(int32_t)acc[4], (float)(int32_t)acc[5]) / kFixedScale;
-    const v3 lin = xyz(v0) + lin_delta;
-    const v3 ang = xyz(v1) + ang_delta;
-    float4* r = d.sbody + 4 * (size_t)b;
-    r[0] = f4(lin, v0.w);
-    r[1] = f4(ang, v1.w);
-    if (last) {
-      d.lin[b] = f4(lin, 0.0f);
-      d.ang[b] = f4(ang, 0.0f);
-    }
-  }
-  if (clock) {
-    __syncthreads();
-    if (threadIdx.x == 0) clock[2 * blockIdx.x + 1] = wall_clock64();
-  }
+  apply_contact_deltas(d, last, clock, nullptr, blockIdx.x, gridDim.x);
 }
 
 // Inputs of the learned warm start for one active contact, from the state the
@@ -1021,10 +944,12 @@ void launch_solve(hipStream_t s, const Dev& d, int bearings, uint64_t* clock, hi
         k_solve<true><<<sgrid, kBlock, 0, s>>>(d, 1, last, pass_clock, nullptr);
       else
         k_solve<false><<<sgrid, kBlock, 0, s>>>(d, 0, last, pass_clock, nullptr);
-      if (bgrid)
-        k_apply<<<bgrid, kBlock, 0, s>>>(
-            d, last, clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr, nullptr);
-      if (bearings) launch_bearing_projection(s, d, 1, last);
+      uint64_t* apply_clock = clock ? clock + 2 * ((size_t)iterations * sgrid + (size_t)it * bgrid) : nullptr;
+      // With bearing islands the pass's velocity update and bearing projection are one launch.
+      if (!(bearings && launch_apply_islands(s, d, bgrid, last, apply_clock))) {
+        if (bgrid) k_apply<<<bgrid, kBlock, 0, s>>>(d, last, apply_clock, nullptr);
+        if (bearings) launch_bearing_projection(s, d, 1, last);
+      }
     }
     if (fallback) {
       launch_residual(s, d);

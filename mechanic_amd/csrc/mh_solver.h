@@ -138,4 +138,91 @@ __device__ __forceinline__ void store_row(const Dev& d, uint32_t ai, const Solve
       r.body_a | (r.analytic ? kRowAnalytic : 0u) | (r.hit ? kRowHit : 0u);
 }
 
+// kApplyLanes adjacent lanes share a body: each loads every kApplyLanes-th of
+// its slots, all loads in flight at once, and shuffles combine the partial
+// sums. The slot loads of a body cost one memory latency instead of one per slot.
+constexpr uint32_t kApplyLanes = 4;
+constexpr uint32_t kApplyPerLane = (kSlotOverflow + kApplyLanes - 1) / kApplyLanes;
+
+// solve_apply of the contact solve (collision.wgsl:1050-1069) for the
+// bodies of `block` out of `nblocks` (the launch's first nblocks blocks; a
+// body whose `skip` byte is set is left to another part of the launch).
+__device__ __forceinline__ void apply_contact_deltas(const Dev& d, int last, uint64_t* clock, const uint8_t* skip,
+                                                     uint32_t block, uint32_t nblocks) {
+  if (clock && threadIdx.x == 0) clock[2 * block] = wall_clock64();
+  const uint64_t tick64 = tick_sv(d);
+  uint32_t* diag = d.diag[tick64 & 1];
+  if (*reinterpret_cast<volatile uint32_t*>(&diag[kDiagActive]) == 0u) return;
+  const uint32_t xcd = block % kXcds, local = block / kXcds;
+  const uint32_t blocks = (nblocks - xcd + kXcds - 1) / kXcds;
+  XcdSplit xs;
+  xs.begin = (uint32_t)(((uint64_t)d.body_count * xcd) / kXcds);
+  xs.end = (uint32_t)(((uint64_t)d.body_count * (xcd + 1)) / kXcds);
+  const uint32_t per_block = blockDim.x / kApplyLanes;
+  const uint32_t part = threadIdx.x % kApplyLanes;
+  for (uint32_t b0 = xs.begin + local * per_block; b0 < xs.end; b0 += blocks * per_block) {
+    const uint32_t b = b0 + threadIdx.x / kApplyLanes;
+    const bool valid = b < xs.end && !(skip && skip[b]);
+    uint32_t acc[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    const uint32_t count = valid ? d.slot_count[b] : 0u;
+    const uint32_t n = count < kSlotOverflow ? count : kSlotOverflow;
+    // The body's velocity words, loaded alongside its slots.
+    float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+    if (valid && part == 0u) {
+      v0 = d.sbody[4 * (size_t)b];
+      v1 = d.sbody[4 * (size_t)b + 1];
+    }
+    int4 x[kApplyPerLane];
+    int2 y[kApplyPerLane];
+#pragma unroll
+    for (uint32_t j = 0; j < kApplyPerLane; ++j) {
+      const uint32_t k = part + j * kApplyLanes;
+      x[j] = make_int4(0, 0, 0, 0);
+      y[j] = make_int2(0, 0);
+      if (k < n) {
+        const int4* e = d.cdelta + 2 * ((size_t)k * d.body_count + b);
+        x[j] = e[0];
+        y[j] = *reinterpret_cast<const int2*>(e + 1);
+      }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kApplyPerLane; ++j) {
+      acc[0] += (uint32_t)x[j].x;
+      acc[1] += (uint32_t)x[j].y;
+      acc[2] += (uint32_t)x[j].z;
+      acc[3] += (uint32_t)x[j].w;
+      acc[4] += (uint32_t)y[j].x;
+      acc[5] += (uint32_t)y[j].y;
+    }
+#pragma unroll
+    for (uint32_t m = 1; m < kApplyLanes; m <<= 1)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc[c] += (uint32_t)__shfl_xor((int)acc[c], (int)m);
+    if (!valid || part != 0u) continue;
+    if (count > kSlotOverflow) {
+      int32_t* p = d.deltas + 6 * (size_t)b;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        acc[c] += (uint32_t)p[c];
+        p[c] = 0;
+      }
+    }
+    const v3 lin_delta = mk((float)(int32_t)acc[0], (float)(int32_t)acc[1], (float)(int32_t)acc[2]) / kFixedScale;
+    const v3 ang_delta = mk((float)(int32_t)acc[3], (float)(int32_t)acc[4], (float)(int32_t)acc[5]) / kFixedScale;
+    const v3 lin = xyz(v0) + lin_delta;
+    const v3 ang = xyz(v1) + ang_delta;
+    float4* r = d.sbody + 4 * (size_t)b;
+    r[0] = f4(lin, v0.w);
+    r[1] = f4(ang, v1.w);
+    if (last) {
+      d.lin[b] = f4(lin, 0.0f);
+      d.ang[b] = f4(ang, 0.0f);
+    }
+  }
+  if (clock) {
+    __syncthreads();
+    if (threadIdx.x == 0) clock[2 * block + 1] = wall_clock64();
+  }
+}
+
 }  // namespace mh

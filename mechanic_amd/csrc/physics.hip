@@ -637,6 +637,11 @@ int upload_forest(mh_physics* p, const mh_creation* c, const Forest& f, mh_error
     d.bi_side = bi_side;
     d.bi_boff = bi_boff;
     d.bi_body = bi_body;
+    std::vector<uint8_t> mask(n, 0);
+    for (uint32_t w : f.bi_body) mask[w & 0x7fffffffu] = 1;
+    uint8_t* bi_mask;
+    TRY(alloc_upload(p, &bi_mask, mask, error));
+    d.bi_mask = bi_mask;
   }
   TRY(alloc_upload(p, &closure_rows, f.closure_rows, error));
   TRY(alloc_upload(p, &cpath_offset, f.cpath_offset, error));

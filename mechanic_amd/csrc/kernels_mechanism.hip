@@ -405,16 +405,19 @@ __global__ __launch_bounds__(256) void k_bearing_islands(Dev d, uint32_t passes,
   island_group(d, blockIdx.x, passes, in_contact, write_back, false);
 }
 
-// A contact pass's velocity update and bearing projection in one launch:
-// blocks [0, apply_blocks) update the bodies no island group holds, as k_apply
-// does; each later block is an island group that updates its own bodies and
-// then projects its bearings (one pass on the solver records).
+// A contact pass's velocity update and bearing projection in one launch: the
+// first blocks are the island groups (dispatched first, so they overlap the
+// rest), each updating its own bodies and then projecting its bearings (one
+// pass on the solver records), padded to whole rounds of the XCDs; the other
+// blocks update the bodies no island group holds, as k_apply does, with the
+// same block-to-XCD mapping as k_apply.
 __global__ __launch_bounds__(256) void k_apply_islands(Dev d, uint32_t apply_blocks, int last, uint64_t* clock) {
-  if (blockIdx.x < apply_blocks) {
-    apply_contact_deltas(d, last, clock, d.bi_mask, blockIdx.x, apply_blocks);
+  const uint32_t lead = (d.bi_groups + kXcds - 1) / kXcds * kXcds;
+  if (blockIdx.x < lead) {
+    if (blockIdx.x < d.bi_groups) island_group(d, blockIdx.x, 1u, 1, last, true);
     return;
   }
-  island_group(d, blockIdx.x - apply_blocks, 1u, 1, last, true);
+  apply_contact_deltas(d, last, clock, d.bi_mask, blockIdx.x - lead, apply_blocks);
 }
 
 // permitted_axis / permitted_speed / stabilized_speed (articulated.wgsl:328-355).
@@ -1979,7 +1982,8 @@ void launch_bearing_projection(hipStream_t s, const Dev& d, int in_contact, int 
 
 bool launch_apply_islands(hipStream_t s, const Dev& d, uint32_t apply_blocks, int last, uint64_t* clock) {
   if (!d.bi_groups || !d.bi_mask || apply_blocks == 0) return false;
-  k_apply_islands<<<apply_blocks + d.bi_groups, 256, 0, s>>>(d, apply_blocks, last, clock);
+  const uint32_t lead = (d.bi_groups + kXcds - 1) / kXcds * kXcds;
+  k_apply_islands<<<lead + apply_blocks, 256, 0, s>>>(d, apply_blocks, last, clock);
   return true;
 }
 

@@ -197,12 +197,16 @@ def cpu_baseline(scene, friction, iterations, state):
                                   "threads_1": out["one"]["phase_ms"]}}
 
 
-def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain, tuning=None, use_graph=True):
+def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain, tuning=None, use_graph=True,
+                       plain_state=None, plain_counts=None):
     """Learned warm start against the reference's cache warm start on the same
     pile: trained on 2M logged contacts of seeds 10-19 (32 solver passes), the
     smallest pass count whose mean contact residual over ticks 1..300 does not
     exceed the plain 8-pass one, with the mean and maximum penetration of both
-    runs, and its steps/s under the headline protocol (settled window)."""
+    runs, and its steps/s under the headline protocol (settled window). A pile
+    stepped with fewer passes settles into another state (fewer pairs), so the
+    last tick's counts of both windows are reported, and the learned rate is
+    also measured from the plain run's own settled state (`plain_state`)."""
     import numpy as np
     from mechanic_amd import learned as L
 
@@ -262,15 +266,43 @@ def learned_warm_start(mh, scene, friction, plain_iterations, steps, value_plain
         gpu.dispatch_ticks(SETTLE_TICKS + 1, steps)
         gpu.synchronize()
         rate = steps / (time.perf_counter() - t2)
-        flags = gpu.read_last_tick().error_flags
+        rb = gpu.read_last_tick()
         fired = gpu.read_learned_stats()["fallback_ticks"] - before
         gpu.close()
-        return rate, flags, fired
+        learned_rate.last = {"pairs": rb.pair_count, "active_contacts": rb.active_contact_count}
+        return rate, rb.error_flags, fired
 
     v, flags, _ = learned_rate({})
     out.update({"iteration_ratio": round(plain_iterations / chosen, 3), "steps_per_s": round(v, 2),
                 "steps_per_s_plain": round(value_plain, 2), "speedup": round(v / value_plain, 3),
-                "error_flags": flags})
+                "error_flags": flags,
+                "window_last_tick": {"learned": learned_rate.last, "plain": plain_counts}})
+    if plain_state is not None:
+        # The same workload as the headline's: the plain run's settled state
+        # restored into a learned handle, 5 ticks to form its contacts, then
+        # `steps` timed ticks.
+        data = np.load(plain_state)
+        cfg = mh.GpuPhysicsConfig(friction=friction, solver_iterations=chosen, use_graph=use_graph,
+                                  warm_start=mh._lib.WARM_START_LEARNED)
+        gpu = mh.GpuPhysics.new_with_config(scene, cfg)
+        if tuning:
+            gpu.set_tuning(**tuning)
+        gpu.set_mlp(net)
+        gpu.write_bodies(data["poses"], data["velocities"])
+        first = int(data["tick"]) + 1
+        gpu.dispatch_ticks(first, 5)
+        gpu.synchronize()
+        t3 = time.perf_counter()
+        gpu.dispatch_ticks(first + 5, steps)
+        gpu.synchronize()
+        v_same = steps / (time.perf_counter() - t3)
+        rb = gpu.read_last_tick()
+        gpu.close()
+        out["from_plain_state"] = {"steps_per_s": round(v_same, 2), "speedup": round(v_same / value_plain, 3),
+                                   "ticks": f"{first + 5}..{first + 4 + steps} from the plain run's tick "
+                                            f"{first - 1} state",
+                                   "last_tick": {"pairs": rb.pair_count, "active_contacts": rb.active_contact_count},
+                                   "error_flags": rb.error_flags}
     # The fallback: a tick whose residual after the learned passes exceeds the
     # plain run's mean runs `chosen` more passes (judged on the device).
     limit = float(plain["rms"].mean())
@@ -555,8 +587,9 @@ def main():
     run600_value, run600_rb = timed_run(make, 600, tuned["tuning"])
 
     progress(f"headline {value:.1f} steps/s; falling and 600-step windows done")
-    learned = None if args.no_learned else learned_warm_start(mh, scene, friction, iterations, steps, value,
-                                                                  tuned["tuning"], use_graph)
+    learned = None if args.no_learned else learned_warm_start(
+        mh, scene, friction, iterations, steps, value, tuned["tuning"], use_graph, plain_state=state_path,
+        plain_counts={"pairs": readback.pair_count, "active_contacts": readback.active_contact_count})
     progress("learned warm start done")
     batch = None if args.no_batched else batched(mh, steps)
     progress("batched handles done")
